@@ -1,0 +1,1348 @@
+// gle_engine.hip -- device side of libgle: instances, kernels, instance-layer C ABI.
+//
+// Device data layout (per instance, all in HBM):
+//   tips   u8  [N][Ppad]           nucleotide 4-bit masks (1..15) or state (S = missing)
+//   cla    T   [slots][Ppad][R][Sp] the reference's pattern-major layout (condlike.h:20-43);
+//                                   Sp = S except codon (61 -> 64)
+//   u      i32 [slots][Ppad]       per-pattern rescale counts (FP64: natural-log units
+//                                   exactly as the reference; FP32: powers of two)
+//   pm64   f64 [pmats][3][R][S][S] P, P', P'' (row = parent state, model.cpp:1271-1358)
+//   pm32   f32 [pmats][R][S][S]    FP32 copy of P for the FP32-bulk CLA kernels
+//   counts i32 [Ppad], cst i32 [Ppad] (constant-pattern state/mask, -1 otherwise)
+// Ppad rounds P up to a multiple of 256 so every 256-thread tile is whole.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "gle_internal.h"
+
+namespace gle {
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_last_error;
+int set_error(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+#define GLE_HIP(call)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return set_error(GLE_ERR_DEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                       __FILE__, __LINE__);                                                    \
+  } while (0)
+
+const RescaleTables &rescale_tables() {
+  static RescaleTables t = [] {
+    RescaleTables r;
+    r.rescale_below = std::exp(-24.0);
+    r.reduce_below = 1.0e-190;
+    r.bail_below = 1.0e-250;
+    const double maxmult = 1.0 / r.bail_below;
+    for (int i = 0; i < kRescaleLen; i++) {
+      r.incr[i] = i * 7 - int(std::log(r.rescale_below));
+      r.thresh[i] = std::exp(double(-r.incr[i]));
+      r.mult[i] = std::min(std::exp(double(r.incr[i])), maxmult);
+    }
+    return r;
+  }();
+  return t;
+}
+
+__constant__ double c_thresh[kRescaleLen];
+__constant__ double c_mult[kRescaleLen];
+__constant__ int c_incr[kRescaleLen];
+__constant__ double c_rescale_below, c_reduce_below, c_bail_below;
+
+// --------------------------------------------------------------- instance
+struct Instance {
+  int N = 0, P = 0, Ppad = 0, S = 0, Sp = 0, R = 0, n_eig = 0, n_slots = 0, n_pm = 0, prec = 0;
+  hipStream_t stream = nullptr;
+  unsigned char *d_tips = nullptr;
+  int *d_counts = nullptr, *d_cst = nullptr, *d_u = nullptr, *d_flags = nullptr, *d_eig_of_rate = nullptr;
+  void *d_cla = nullptr;
+  double *d_pm64 = nullptr;
+  float *d_pm32 = nullptr;
+  double *d_V = nullptr, *d_Vinv = nullptr, *d_lam = nullptr, *d_model = nullptr;  // model: pi[S] w[R] mult[R]
+  // staging
+  DevClaOp *d_ops = nullptr;
+  int ops_cap = 0;
+  DevEdgeOp *d_eops = nullptr;
+  int eops_cap = 0;
+  double *d_part = nullptr;
+  size_t part_cap = 0;
+  double *d_eres = nullptr;
+  int eres_cap = 0;
+  double *d_site = nullptr;
+  int *d_pmidx = nullptr;
+  double *d_pmt = nullptr;
+  int pm_cap = 0;
+  double *h_stage = nullptr;  // pinned
+  size_t h_stage_cap = 0;
+  double pinv = 0.0;
+  int pinv_in_model = 0, last_const = -1;
+  // kernel timing (HIP events on this instance's stream): 0 CLA, 1 edge, 2 pmat
+  bool timing = false;
+  struct KStat {
+    long long launches = 0;
+    double ms = 0.0, bytes = 0.0;
+  } kst[3];
+  std::vector<hipEvent_t> ev_free;
+  struct Pending {
+    hipEvent_t a, b;
+    int kind;
+    double bytes;
+  };
+  std::vector<Pending> pending;
+  size_t slot_elems() const { return size_t(Ppad) * R * Sp; }
+  size_t elem_bytes() const { return prec == GLE_PREC_FP64 ? 8 : 4; }
+};
+
+static std::mutex g_mu;
+static std::vector<std::unique_ptr<Instance>> g_instances;
+static int g_device = -1;
+
+Instance *get_instance(int h) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (h < 0 || h >= int(g_instances.size())) return nullptr;
+  return g_instances[h].get();
+}
+int instance_states(const Instance *I) { return I->S; }
+int instance_rates(const Instance *I) { return I->R; }
+int instance_patterns(const Instance *I) { return I->P; }
+int instance_tips(const Instance *I) { return I->N; }
+int instance_precision(const Instance *I) { return I->prec; }
+int instance_cla_slots(const Instance *I) { return I->n_slots; }
+int instance_pmat_slots(const Instance *I) { return I->n_pm; }
+
+static int ensure_device() {
+  if (g_device >= 0) return GLE_SUCCESS;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
+    return set_error(GLE_ERR_DEVICE, "no HIP device visible: the GARLI engine runs only on a GPU");
+  GLE_HIP(hipSetDevice(0));
+  g_device = 0;
+  const RescaleTables &t = rescale_tables();
+  GLE_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_thresh), t.thresh, sizeof t.thresh));
+  GLE_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_mult), t.mult, sizeof t.mult));
+  GLE_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_incr), t.incr, sizeof t.incr));
+  GLE_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_rescale_below), &t.rescale_below, sizeof(double)));
+  GLE_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_reduce_below), &t.reduce_below, sizeof(double)));
+  GLE_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_bail_below), &t.bail_below, sizeof(double)));
+  return GLE_SUCCESS;
+}
+
+template <typename X>
+static int grow(X *&ptr, int &cap, int need) {
+  if (need <= cap) return GLE_SUCCESS;
+  if (ptr) GLE_HIP(hipFree(ptr));
+  ptr = nullptr;
+  int c = std::max(need, 2 * cap);
+  GLE_HIP(hipMalloc(&ptr, sizeof(X) * size_t(c)));
+  cap = c;
+  return GLE_SUCCESS;
+}
+
+// HIP-event timing of individual launches; bytes are algorithmic bytes.
+static hipEvent_t take_event(Instance *I) {
+  if (!I->ev_free.empty()) {
+    hipEvent_t e = I->ev_free.back();
+    I->ev_free.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+static void timed_begin(Instance *I, int kind, double bytes) {
+  if (!I->timing) return;
+  Instance::Pending p{take_event(I), take_event(I), kind, bytes};
+  (void)hipEventRecord(p.a, I->stream);
+  I->pending.push_back(p);
+}
+static void timed_end(Instance *I) {
+  if (!I->timing || I->pending.empty()) return;
+  (void)hipEventRecord(I->pending.back().b, I->stream);
+}
+static void timed_collect(Instance *I) {  // call after the stream drained
+  for (auto &p : I->pending) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      I->kst[p.kind].launches++;
+      I->kst[p.kind].ms += ms;
+      I->kst[p.kind].bytes += p.bytes;
+    }
+    I->ev_free.push_back(p.a);
+    I->ev_free.push_back(p.b);
+  }
+  I->pending.clear();
+}
+
+// Algorithmic bytes of one CLA op / edge op per pattern (true S, not padded).
+static double cla_op_bytes(const Instance *I, const DevClaOp &o) {
+  const double rsb = double(I->R) * I->S * I->elem_bytes();
+  const int internal = (o.c1 >= 0) + (o.c2 >= 0);
+  const double per = internal == 2 ? 3 * rsb + 12 : internal == 1 ? 2 * rsb + 1 + 8 : rsb + 2 + 4;
+  return per * I->P;
+}
+static double edge_op_bytes(const Instance *I, const DevEdgeOp &o) {
+  const double rsb = double(I->R) * I->S * I->elem_bytes();
+  const double per = o.child >= 0 ? 2 * rsb + 8 + 4 : rsb + 1 + 4 + 4;
+  return per * I->P;
+}
+
+// =====================================================================
+// Device helpers
+// =====================================================================
+
+template <int R>
+struct Lanes {  // lanes per pattern: next power of two >= R
+  static constexpr int value = R <= 1 ? 1 : R <= 2 ? 2 : R <= 4 ? 4 : 8;
+};
+
+template <typename T>
+__device__ __forceinline__ T group_max(T v, int RP) {
+  for (int o = 1; o < RP; o <<= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+// Reference-order dot products for one P row.  FP64 paths run with
+// contraction off so the arithmetic is the reference's mul-then-add.
+__device__ __forceinline__ double dot4_paired(const double *p, const double *x) {
+#pragma clang fp contract(off)
+  return ((p[0] * x[0]) + (p[1] * x[1])) + ((p[2] * x[2]) + (p[3] * x[3]));
+}
+__device__ __forceinline__ double dot4_seq(const double *p, const double *x) {
+#pragma clang fp contract(off)
+  return p[0] * x[0] + p[1] * x[1] + p[2] * x[2] + p[3] * x[3];
+}
+__device__ __forceinline__ float dot4_paired(const float *p, const float *x) {
+  return __builtin_fmaf(p[0], x[0], p[1] * x[1]) + __builtin_fmaf(p[2], x[2], p[3] * x[3]);
+}
+__device__ __forceinline__ float dot4_seq(const float *p, const float *x) { return dot4_paired(p, x); }
+
+__device__ __forceinline__ double mul_nc(double a, double b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+__device__ __forceinline__ double add_nc(double a, double b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+
+template <typename T>
+__device__ __forceinline__ void load4(const T *src, T *v) {
+  if constexpr (sizeof(T) == 4) {
+    float4 a = *reinterpret_cast<const float4 *>(src);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  } else {
+    double2 a = reinterpret_cast<const double2 *>(src)[0], b = reinterpret_cast<const double2 *>(src)[1];
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+  }
+}
+template <typename T>
+__device__ __forceinline__ void store4(T *dst, const T *v) {
+  if constexpr (sizeof(T) == 4) {
+    *reinterpret_cast<float4 *>(dst) = make_float4(v[0], v[1], v[2], v[3]);
+  } else {
+    reinterpret_cast<double2 *>(dst)[0] = make_double2(v[0], v[1]);
+    reinterpret_cast<double2 *>(dst)[1] = make_double2(v[2], v[3]);
+  }
+}
+
+// Tip factor of a nucleotide mask for parent state i: P[i][t] for one state,
+// 1 for total ambiguity, and the ascending-order sum for partial ambiguity
+// (tree.cpp:7941-8047, 8387-8446; the stream lists states in ascending order).
+template <typename T>
+__device__ __forceinline__ T nuc_tip_factor(const T *prow, int m) {
+  if (m == 15) return T(1);
+  T acc = T(0);
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+    if (m & (1 << s)) acc = acc + prow[s];
+  return acc;
+}
+
+// Rescale one pattern slice held by the RP lanes of a pattern group.
+// FP64 (mode 0): RescaleRateHet (tree.cpp:3370-3490) table scheme, natural-log counts.
+// FP32 (mode 1): exact power-of-two scaling when the slice max drops below 2^-32.
+template <typename T, int W>
+__device__ __forceinline__ int rescale_slice(T *v, int RP, int *flags, bool group_live) {
+  T m = v[0];
+#pragma unroll
+  for (int q = 1; q < W; q++) m = fmax(m, v[q]);
+  m = group_max(m, RP);  // every lane of the group takes part in the shuffle
+  if (!group_live) return 0;  // padding patterns beyond P
+  int add = 0;
+  if constexpr (sizeof(T) == 8) {
+    if (m < c_rescale_below) {
+      if (m < c_reduce_below) atomicOr(flags, m < c_bail_below ? 3 : 1);
+      int idx = 0;
+      while (idx + 1 < kRescaleLen && c_thresh[idx + 1] > m) idx++;
+      const double mult = c_mult[idx];
+#pragma unroll
+      for (int q = 0; q < W; q++) v[q] = mul_nc(v[q], mult);
+      add = c_incr[idx];
+    }
+  } else {
+    if (m < 0x1p-32f && m > 0.0f) {
+      const int e = -ilogbf(m);  // m * 2^e in [1, 2)
+#pragma unroll
+      for (int q = 0; q < W; q++) v[q] = ldexpf(v[q], e);
+      add = e;
+    }
+  }
+  return add;
+}
+
+// =====================================================================
+// P(t), P', P'' (model.cpp:1271-1358, 1469-1597) -- FP64
+// grid (n, R); V/Vinv of the category's eigensystem staged in LDS.
+// =====================================================================
+__global__ __launch_bounds__(256) void k_pmat(int S, int R, const int *__restrict__ idx,
+                                              const double *__restrict__ tt, const int *__restrict__ eig_of_rate,
+                                              const double *__restrict__ rate_mult, const double *__restrict__ V,
+                                              const double *__restrict__ Vinv, const double *__restrict__ lam,
+                                              double *__restrict__ pm64, float *__restrict__ pm32, int derivs) {
+#pragma clang fp contract(off)
+  extern __shared__ double sm[];
+  const int SS = S * S, op = blockIdx.x, r = blockIdx.y;
+  const int e = eig_of_rate[r];
+  double *sV = sm, *sW = sm + SS, *ex = sm + 2 * SS, *e1 = ex + S, *e2 = e1 + S;
+  for (int q = threadIdx.x; q < SS; q += blockDim.x) {
+    sV[q] = V[size_t(e) * SS + q];
+    sW[q] = Vinv[size_t(e) * SS + q];
+  }
+  const double t = tt[op], mult = rate_mult[r];
+  for (int k = threadIdx.x; k < S; k += blockDim.x) {
+    const double s = lam[size_t(e) * S + k] * mult;
+    ex[k] = exp(s * t);
+    e1[k] = s * ex[k];
+    e2[k] = s * e1[k];
+  }
+  __syncthreads();
+  const int slot = idx[op];
+  double *P = pm64 + (size_t(slot) * 3 * R + r) * SS;
+  double *D1 = pm64 + (size_t(slot) * 3 * R + R + r) * SS;
+  double *D2 = pm64 + (size_t(slot) * 3 * R + 2 * R + r) * SS;
+  float *P32 = pm32 + (size_t(slot) * R + r) * SS;
+  for (int q = threadIdx.x; q < SS; q += blockDim.x) {
+    const int i = q / S, j = q - i * S;
+    double sp = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int k = 0; k < S; k++) {
+      const double x = sV[i * S + k] * sW[k * S + j];
+      sp += x * ex[k];
+      if (derivs) {
+        s1 += x * e1[k];
+        s2 += x * e2[k];
+      }
+    }
+    const double pv = sp > 0.0 ? sp : 0.0;
+    P[q] = pv;
+    P32[q] = float(pv);
+    if (derivs) {
+      D1[q] = s1;
+      D2[q] = s2;
+    }
+  }
+}
+
+// =====================================================================
+// Nucleotide CLA update (S = 4): one thread per (pattern, rate)
+// =====================================================================
+template <typename T>
+struct NucClaArgs {
+  T *cla;
+  int *u;
+  const unsigned char *tips;
+  const T *pm;          // P in precision T
+  size_t pm_stride;     // elements between pmat slots
+  size_t slot_stride;   // Ppad*R*4
+  int Ppad, P;
+  int *flags;
+};
+
+template <typename T, int R>
+__global__ __launch_bounds__(256) void k_cla_nuc(NucClaArgs<T> a, const DevClaOp *__restrict__ ops, int fused_n) {
+  constexpr int RP = Lanes<R>::value;
+  const int r = threadIdx.x % RP;
+  const int p = blockIdx.x * (256 / RP) + threadIdx.x / RP;
+  const bool live = (p < a.P) && (r < R);
+  const int rr = r < R ? r : R - 1;
+  const int first = fused_n ? 0 : blockIdx.y, last = fused_n ? fused_n : blockIdx.y + 1;
+  for (int oi = first; oi < last; ++oi) {
+    const DevClaOp op = ops[oi];
+    T v1[4], v2[4], P1[16], P2[16], x[4], out[4];
+    const T *pm1 = a.pm + op.pm1 * a.pm_stride + rr * 16;
+    const T *pm2 = a.pm + op.pm2 * a.pm_stride + rr * 16;
+#pragma unroll
+    for (int q = 0; q < 16; q += 4) {
+      load4(pm1 + q, P1 + q);
+      load4(pm2 + q, P2 + q);
+    }
+    int u = 0;
+    const size_t off = (size_t(p) * R + rr) * 4;
+    if (live) {
+      // child 1
+      if (op.c1 >= 0) {
+        load4(a.cla + op.c1 * a.slot_stride + off, x);
+        const bool paired = (R == 4) && (op.c2 >= 0 || __popc(a.tips[size_t(-op.c2 - 1) * a.Ppad + p]) == 1);
+#pragma unroll
+        for (int i = 0; i < 4; i++) v1[i] = paired ? dot4_paired(P1 + 4 * i, x) : dot4_seq(P1 + 4 * i, x);
+        u += a.u[op.c1 * size_t(a.Ppad) + p];
+      } else {
+        const int m = a.tips[size_t(-op.c1 - 1) * a.Ppad + p];
+#pragma unroll
+        for (int i = 0; i < 4; i++) v1[i] = nuc_tip_factor(P1 + 4 * i, m);
+      }
+      // child 2
+      if (op.c2 >= 0) {
+        load4(a.cla + op.c2 * a.slot_stride + off, x);
+        const bool paired = (R == 4) && (op.c1 >= 0 || __popc(a.tips[size_t(-op.c1 - 1) * a.Ppad + p]) == 1);
+#pragma unroll
+        for (int i = 0; i < 4; i++) v2[i] = paired ? dot4_paired(P2 + 4 * i, x) : dot4_seq(P2 + 4 * i, x);
+        u += a.u[op.c2 * size_t(a.Ppad) + p];
+      } else {
+        const int m = a.tips[size_t(-op.c2 - 1) * a.Ppad + p];
+#pragma unroll
+        for (int i = 0; i < 4; i++) v2[i] = nuc_tip_factor(P2 + 4 * i, m);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        if constexpr (sizeof(T) == 8) out[i] = mul_nc(v1[i], v2[i]);
+        else out[i] = v1[i] * v2[i];
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; i++) out[i] = T(0);
+    }
+    if (op.flags & kOpRescale) u += rescale_slice<T, 4>(out, RP, a.flags, p < a.P);
+    if (live) {
+      store4(a.cla + op.dest * a.slot_stride + off, out);
+      if (r == 0) a.u[op.dest * size_t(a.Ppad) + p] = u;
+    }
+  }
+}
+
+// =====================================================================
+// General-S CLA update (amino acids, codons): one thread per (pattern, rate),
+// P of the op staged in LDS.  tree.cpp:7843-7910, 8082-8160, 8537-8599.
+// =====================================================================
+template <typename T>
+struct NsClaArgs {
+  T *cla;
+  int *u;
+  const unsigned char *tips;
+  const T *pm;
+  size_t pm_stride, slot_stride;
+  int Ppad, P, R, Sp;
+  int *flags;
+};
+
+template <typename T, int S>
+__global__ __launch_bounds__(256) void k_cla_ns(NsClaArgs<T> a, const DevClaOp *__restrict__ ops, int fused_n) {
+  // P staged in LDS for S <= 32; larger state spaces read P through the caches.
+  constexpr bool kLds = S <= 32;
+  extern __shared__ unsigned char smem_raw[];
+  T *sP = reinterpret_cast<T *>(smem_raw);  // [2][R][S][S]
+  const int R = a.R, RP = R <= 1 ? 1 : R <= 2 ? 2 : R <= 4 ? 4 : 8;
+  const int r = threadIdx.x % RP;
+  const int p = blockIdx.x * (256 / RP) + threadIdx.x / RP;
+  const bool live = (p < a.P) && (r < R);
+  const int rr = r < R ? r : R - 1;
+  const int first = fused_n ? 0 : blockIdx.y, last = fused_n ? fused_n : blockIdx.y + 1;
+  const int RSS = R * S * S;
+  for (int oi = first; oi < last; ++oi) {
+    const DevClaOp op = ops[oi];
+    const T *P1, *P2;
+    if constexpr (kLds) {
+      if (oi != first) __syncthreads();
+      for (int q = threadIdx.x; q < RSS; q += blockDim.x) {
+        sP[q] = a.pm[op.pm1 * a.pm_stride + q];
+        sP[RSS + q] = a.pm[op.pm2 * a.pm_stride + q];
+      }
+      __syncthreads();
+      P1 = sP + rr * S * S;
+      P2 = sP + RSS + rr * S * S;
+    } else {
+      P1 = a.pm + op.pm1 * a.pm_stride + rr * S * S;
+      P2 = a.pm + op.pm2 * a.pm_stride + rr * S * S;
+    }
+    T x1[S], x2[S], out[S];
+    int t1 = S, t2 = S, u = 0;
+    const size_t off = (size_t(p) * R + rr) * a.Sp;
+    if (live) {
+      if (op.c1 >= 0) {
+#pragma unroll
+        for (int q = 0; q < S; q++) x1[q] = a.cla[op.c1 * a.slot_stride + off + q];
+        u += a.u[op.c1 * size_t(a.Ppad) + p];
+      } else
+        t1 = a.tips[size_t(-op.c1 - 1) * a.Ppad + p];
+      if (op.c2 >= 0) {
+#pragma unroll
+        for (int q = 0; q < S; q++) x2[q] = a.cla[op.c2 * a.slot_stride + off + q];
+        u += a.u[op.c2 * size_t(a.Ppad) + p];
+      } else
+        t2 = a.tips[size_t(-op.c2 - 1) * a.Ppad + p];
+#pragma unroll 4
+      for (int f = 0; f < S; f++) {
+        T L1, R1;
+        if (op.c1 >= 0) {
+          L1 = T(0);
+#pragma unroll
+          for (int q = 0; q < S; q++) {
+            if constexpr (sizeof(T) == 8) L1 = add_nc(L1, mul_nc(P1[f * S + q], x1[q]));
+            else L1 = __builtin_fmaf(P1[f * S + q], x1[q], L1);
+          }
+        } else
+          L1 = t1 < S ? P1[f * S + t1] : T(1);
+        if (op.c2 >= 0) {
+          R1 = T(0);
+#pragma unroll
+          for (int q = 0; q < S; q++) {
+            if constexpr (sizeof(T) == 8) R1 = add_nc(R1, mul_nc(P2[f * S + q], x2[q]));
+            else R1 = __builtin_fmaf(P2[f * S + q], x2[q], R1);
+          }
+        } else
+          R1 = t2 < S ? P2[f * S + t2] : T(1);
+        if constexpr (sizeof(T) == 8) out[f] = mul_nc(L1, R1);
+        else out[f] = L1 * R1;
+      }
+    } else {
+#pragma unroll
+      for (int f = 0; f < S; f++) out[f] = T(0);
+    }
+    if (op.flags & kOpRescale) u += rescale_slice<T, S>(out, RP, a.flags, p < a.P);
+    if (live) {
+#pragma unroll
+      for (int f = 0; f < S; f++) a.cla[op.dest * a.slot_stride + off + f] = out[f];
+      if (r == 0) a.u[op.dest * size_t(a.Ppad) + p] = u;
+    }
+  }
+}
+
+// =====================================================================
+// Edge integration + derivatives, FP64 arithmetic after the CLA loads.
+// One thread per (pattern, rate); the rate-0 lane combines the rates in
+// order, applies the invariant-site term and the log, and the block reduces
+// (count*lnL, count*d1, count*d2) into per-block partials.
+// =====================================================================
+template <typename T>
+struct EdgeArgs {
+  const T *cla;
+  const int *u;
+  const unsigned char *tips;
+  const double *pm64;  // [slot][3][R][S][S]
+  const double *model; // pi[S], w[R]
+  const int *counts, *cst;
+  double pinv;
+  int pinv_in_model;
+  size_t slot_stride;
+  int Ppad, P, R, S, Sp;
+  double *partials;  // [op][nblk][3]
+  double *site;      // per-pattern lnL for op 0, or null
+};
+
+__device__ __forceinline__ void block_reduce3(double a, double b, double c, double *out) {
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_down(a, o);
+    b += __shfl_down(b, o);
+    c += __shfl_down(c, o);
+  }
+  __shared__ double red[3][8];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) {
+    red[0][w] = a;
+    red[1][w] = b;
+    red[2][w] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int k = 0; k < int(blockDim.x >> 6); k++) {
+      s0 += red[0][k];
+      s1 += red[1][k];
+      s2 += red[2][k];
+    }
+    out[0] = s0;
+    out[1] = s1;
+    out[2] = s2;
+  }
+}
+
+// Combine one site: L/D1/D2 are the rate-summed, frequency-weighted values.
+// FP64 form: natural-log counts as tree.cpp:6118-6139 / optimization.cpp:3166-3187.
+// FP32 form: power-of-two counts; the invariant term is added at true scale.
+template <bool POW2>
+__device__ __forceinline__ void site_terms(double L, double D1, double D2, int u, double pinv_term, bool nstate,
+                                           int count, double &lnl, double &g1, double &g2, double &ul) {
+#pragma clang fp contract(off)
+  double sL, r1, r2;
+  if (!POW2) {
+    sL = L;
+    if (pinv_term > 0.0) sL = (u == 0) ? L + pinv_term : L + pinv_term * exp(double(u));
+    ul = log(sL) - double(u);
+    r1 = D1 / sL;
+    r2 = (D2 / sL) - r1 * r1;
+  } else {
+    if (pinv_term > 0.0) {
+      sL = ldexp(L, -u) + pinv_term;
+      ul = log(sL);
+      r1 = ldexp(D1, -u) / sL;
+      r2 = (ldexp(D2, -u) / sL) - r1 * r1;
+    } else {
+      ul = log(L) - double(u) * 0.69314718055994530942;
+      r1 = D1 / L;
+      r2 = (D2 / L) - r1 * r1;
+    }
+  }
+  if (nstate && !(ul < 0.0)) {  // NState scoring skips non-negative site lnL (tree.cpp:6473)
+    lnl = g1 = g2 = 0.0;
+    return;
+  }
+  lnl = double(count) * ul;
+  g1 = double(count) * r1;
+  g2 = double(count) * r2;
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(256) void k_edge_nuc(EdgeArgs<T> a, const DevEdgeOp *__restrict__ ops) {
+#pragma clang fp contract(off)
+  constexpr int RP = Lanes<R>::value;
+  const int r = threadIdx.x % RP;
+  const int p = blockIdx.x * (256 / RP) + threadIdx.x / RP;
+  const bool live = (p < a.P) && (r < R);
+  const int rr = r < R ? r : R - 1;
+  const DevEdgeOp op = ops[blockIdx.y];
+  const double *pi = a.model, *w = a.model + 4;
+  const double *Pm = a.pm64 + (size_t(op.pm) * 3 * R + rr) * 16;
+  const double *D1m = Pm + size_t(R) * 16, *D2m = Pm + size_t(2 * R) * 16;
+  double L[4] = {0, 0, 0, 0}, G1[4] = {0, 0, 0, 0}, G2[4] = {0, 0, 0, 0};
+  int u = 0;
+  const size_t off = (size_t(p) * R + rr) * 4;
+  if (live) {
+    T xt[4];
+    load4(a.cla + op.partial * a.slot_stride + off, xt);
+    const double x[4] = {double(xt[0]), double(xt[1]), double(xt[2]), double(xt[3])};
+    const double wr = w[rr];
+    u = a.u[op.partial * size_t(a.Ppad) + p];
+    if (op.child >= 0) {
+      T yt[4];
+      load4(a.cla + op.child * a.slot_stride + off, yt);
+      const double y[4] = {double(yt[0]), double(yt[1]), double(yt[2]), double(yt[3])};
+      u += a.u[op.child * size_t(a.Ppad) + p];
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+        L[s] = dot4_seq(Pm + 4 * s, y) * x[s] * wr;
+        if (op.derivs) {
+          G1[s] = dot4_seq(D1m + 4 * s, y) * x[s] * wr;
+          G2[s] = dot4_seq(D2m + 4 * s, y) * x[s] * wr;
+        }
+      }
+    } else {
+      const int m = a.tips[size_t(-op.child - 1) * a.Ppad + p];
+      if (m == 15) {
+#pragma unroll
+        for (int s = 0; s < 4; s++) L[s] = x[s] * wr;
+      } else {
+#pragma unroll
+        for (int st = 0; st < 4; st++) {
+          if (!(m & (1 << st))) continue;
+#pragma unroll
+          for (int s = 0; s < 4; s++) {
+            L[s] += Pm[4 * s + st] * x[s] * wr;
+            if (op.derivs) {
+              G1[s] += D1m[4 * s + st] * x[s] * wr;
+              G2[s] += D2m[4 * s + st] * x[s] * wr;
+            }
+          }
+        }
+      }
+    }
+  }
+  // rate-0 lane sums the categories in order
+  double sL[4], s1[4], s2[4];
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    sL[s] = L[s];
+    s1[s] = G1[s];
+    s2[s] = G2[s];
+  }
+#pragma unroll
+  for (int k = 1; k < RP; k++) {
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const double l = __shfl(L[s], (threadIdx.x & ~(RP - 1)) + k);
+      const double g1 = __shfl(G1[s], (threadIdx.x & ~(RP - 1)) + k);
+      const double g2 = __shfl(G2[s], (threadIdx.x & ~(RP - 1)) + k);
+      if (k < R) {
+        sL[s] += l;
+        s1[s] += g1;
+        s2[s] += g2;
+      }
+    }
+  }
+  double lnl = 0, g1 = 0, g2 = 0;
+  if (live && r == 0) {
+    const double Lt = sL[0] * pi[0] + sL[1] * pi[1] + sL[2] * pi[2] + sL[3] * pi[3];
+    const double D1 = s1[0] * pi[0] + s1[1] * pi[1] + s1[2] * pi[2] + s1[3] * pi[3];
+    const double D2 = s2[0] * pi[0] + s2[1] * pi[1] + s2[2] * pi[2] + s2[3] * pi[3];
+    double pt = 0.0;
+    const int cs = a.cst[p];
+    if (a.pinv_in_model && cs >= 0) {
+      double btot = 0.0;
+      for (int s = 0; s < 4; s++)
+        if (cs & (1 << s)) btot += pi[s];
+      pt = a.pinv * btot;
+    }
+    double ul;
+    site_terms<sizeof(T) == 4>(Lt, D1, D2, u, pt, false, a.counts[p], lnl, g1, g2, ul);
+    if (a.site && blockIdx.y == 0) a.site[p] = ul;
+  }
+  block_reduce3(lnl, g1, g2, a.partials + (size_t(blockIdx.y) * gridDim.x + blockIdx.x) * 3);
+}
+
+template <typename T, int S>
+__global__ __launch_bounds__(256) void k_edge_ns(EdgeArgs<T> a, const DevEdgeOp *__restrict__ ops) {
+#pragma clang fp contract(off)
+  const int R = a.R, RP = R <= 1 ? 1 : R <= 2 ? 2 : R <= 4 ? 4 : 8;
+  const int r = threadIdx.x % RP;
+  const int p = blockIdx.x * (256 / RP) + threadIdx.x / RP;
+  const bool live = (p < a.P) && (r < R);
+  const int rr = r < R ? r : R - 1;
+  const DevEdgeOp op = ops[blockIdx.y];
+  const double *pi = a.model, *w = a.model + S;
+  const double *Pm = a.pm64 + (size_t(op.pm) * 3 * R + rr) * S * S;
+  const double *D1m = Pm + size_t(R) * S * S, *D2m = Pm + size_t(2 * R) * S * S;
+  double rL = 0, r1 = 0, r2 = 0;
+  int u = 0;
+  const size_t off = (size_t(p) * R + rr) * a.Sp;
+  if (live) {
+    const T *x = a.cla + op.partial * a.slot_stride + off;
+    u = a.u[op.partial * size_t(a.Ppad) + p];
+    if (op.child >= 0) {
+      const T *y = a.cla + op.child * a.slot_stride + off;
+      u += a.u[op.child * size_t(a.Ppad) + p];
+      double yv[S];
+#pragma unroll
+      for (int q = 0; q < S; q++) yv[q] = double(y[q]);
+      for (int f = 0; f < S; f++) {
+        double tL = 0, t1 = 0, t2 = 0;
+#pragma unroll
+        for (int q = 0; q < S; q++) {
+          tL += Pm[f * S + q] * yv[q];
+          if (op.derivs) {
+            t1 += D1m[f * S + q] * yv[q];
+            t2 += D2m[f * S + q] * yv[q];
+          }
+        }
+        const double xf = double(x[f]);
+        rL += tL * xf * pi[f];
+        r1 += t1 * xf * pi[f];
+        r2 += t2 * xf * pi[f];
+      }
+    } else {
+      const int t = a.tips[size_t(-op.child - 1) * a.Ppad + p];
+      for (int f = 0; f < S; f++) {
+        const double xf = double(x[f]);
+        if (t < S) {
+          rL += Pm[f * S + t] * xf * pi[f];
+          if (op.derivs) {
+            r1 += D1m[f * S + t] * xf * pi[f];
+            r2 += D2m[f * S + t] * xf * pi[f];
+          }
+        } else
+          rL += xf * pi[f];
+      }
+    }
+  }
+  double sL = 0, s1 = 0, s2 = 0;
+  for (int k = 0; k < RP; k++) {
+    const double l = __shfl(rL, (threadIdx.x & ~(RP - 1)) + k);
+    const double g1 = __shfl(r1, (threadIdx.x & ~(RP - 1)) + k);
+    const double g2 = __shfl(r2, (threadIdx.x & ~(RP - 1)) + k);
+    if (k < R) {
+      sL += l * w[k];
+      s1 += g1 * w[k];
+      s2 += g2 * w[k];
+    }
+  }
+  double lnl = 0, g1 = 0, g2 = 0;
+  if (live && r == 0) {
+    double pt = 0.0;
+    const int cs = a.cst[p];
+    if (a.pinv_in_model && cs >= 0 && cs < S) pt = a.pinv * pi[cs];
+    double ul;
+    site_terms<sizeof(T) == 4>(sL, s1, s2, u, pt, true, a.counts[p], lnl, g1, g2, ul);
+    if (a.site && blockIdx.y == 0) a.site[p] = ul;
+  }
+  block_reduce3(lnl, g1, g2, a.partials + (size_t(blockIdx.y) * gridDim.x + blockIdx.x) * 3);
+}
+
+// Deterministic per-op reduction of block partials: one 64-lane wave per op.
+__global__ __launch_bounds__(64) void k_edge_reduce(const double *__restrict__ part, int nblk,
+                                                    double *__restrict__ out) {
+  const int op = blockIdx.x, l = threadIdx.x;
+  double a = 0, b = 0, c = 0;
+  for (int k = l; k < nblk; k += 64) {
+    const double *q = part + (size_t(op) * nblk + k) * 3;
+    a += q[0];
+    b += q[1];
+    c += q[2];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_down(a, o);
+    b += __shfl_down(b, o);
+    c += __shfl_down(c, o);
+  }
+  if (l == 0) {
+    out[3 * op] = a;
+    out[3 * op + 1] = b;
+    out[3 * op + 2] = c;
+  }
+}
+
+// =====================================================================
+// Host-side launchers
+// =====================================================================
+static int lanes_for(int R) { return R <= 1 ? 1 : R <= 2 ? 2 : R <= 4 ? 4 : 8; }
+static int blocks_for(const Instance *I) { return I->Ppad / (256 / lanes_for(I->R)); }
+
+bool engine_supports_fused(const Instance *I) { return I->S == 4 || I->S == 20; }
+
+int engine_update_pmats(Instance *I, const int *idx, const double *t, int n, int derivs) {
+  if (n <= 0) return GLE_SUCCESS;
+  for (int i = 0; i < n; i++)
+    if (idx[i] < 0 || idx[i] >= I->n_pm || !(t[i] >= 0.0))
+      return set_error(GLE_ERR_ARG, "pmat op %d: slot %d / length %g out of range", i, idx[i], t[i]);
+  if (n > I->pm_cap) {
+    if (I->d_pmidx) GLE_HIP(hipFree(I->d_pmidx));
+    if (I->d_pmt) GLE_HIP(hipFree(I->d_pmt));
+    I->d_pmidx = nullptr;
+    I->d_pmt = nullptr;
+    const int cap = std::max(n, 2 * I->pm_cap);
+    GLE_HIP(hipMalloc(&I->d_pmidx, sizeof(int) * cap));
+    GLE_HIP(hipMalloc(&I->d_pmt, sizeof(double) * cap));
+    I->pm_cap = cap;
+  }
+  GLE_HIP(hipMemcpyAsync(I->d_pmidx, idx, sizeof(int) * n, hipMemcpyHostToDevice, I->stream));
+  GLE_HIP(hipMemcpyAsync(I->d_pmt, t, sizeof(double) * n, hipMemcpyHostToDevice, I->stream));
+  const int S = I->S;
+  const size_t shmem = sizeof(double) * (2 * S * S + 3 * S);
+  timed_begin(I, 2, double(n) * I->R * S * S * 8.0 * (derivs ? 3 : 1));
+  hipLaunchKernelGGL(k_pmat, dim3(n, I->R), dim3(256), shmem, I->stream, S, I->R, I->d_pmidx, I->d_pmt,
+                     I->d_eig_of_rate, I->d_model + S + I->R, I->d_V, I->d_Vinv, I->d_lam, I->d_pm64, I->d_pm32,
+                     derivs);
+  timed_end(I);
+  GLE_HIP(hipGetLastError());
+  // the staging buffers are reused by the next call only after this stream drains
+  GLE_HIP(hipStreamSynchronize(I->stream));
+  timed_collect(I);
+  return GLE_SUCCESS;
+}
+
+template <typename T, int R>
+static void launch_nuc(Instance *I, const DevClaOp *d_ops, int n, bool fused, int nblk) {
+  NucClaArgs<T> a;
+  a.cla = reinterpret_cast<T *>(I->d_cla);
+  a.u = I->d_u;
+  a.tips = I->d_tips;
+  a.pm = sizeof(T) == 8 ? reinterpret_cast<const T *>(I->d_pm64) : reinterpret_cast<const T *>(I->d_pm32);
+  a.pm_stride = sizeof(T) == 8 ? size_t(3) * R * 16 : size_t(R) * 16;
+  a.slot_stride = I->slot_elems();
+  a.Ppad = I->Ppad;
+  a.P = I->P;
+  a.flags = I->d_flags;
+  hipLaunchKernelGGL((k_cla_nuc<T, R>), dim3(nblk, fused ? 1 : n), dim3(256), 0, I->stream, a, d_ops,
+                     fused ? n : 0);
+}
+
+template <typename T>
+static int launch_nuc_r(Instance *I, const DevClaOp *d_ops, int n, bool fused, int nblk) {
+  switch (I->R) {
+    case 1: launch_nuc<T, 1>(I, d_ops, n, fused, nblk); break;
+    case 2: launch_nuc<T, 2>(I, d_ops, n, fused, nblk); break;
+    case 3: launch_nuc<T, 3>(I, d_ops, n, fused, nblk); break;
+    case 4: launch_nuc<T, 4>(I, d_ops, n, fused, nblk); break;
+    case 5: launch_nuc<T, 5>(I, d_ops, n, fused, nblk); break;
+    case 6: launch_nuc<T, 6>(I, d_ops, n, fused, nblk); break;
+    case 7: launch_nuc<T, 7>(I, d_ops, n, fused, nblk); break;
+    case 8: launch_nuc<T, 8>(I, d_ops, n, fused, nblk); break;
+    default: return set_error(GLE_ERR_UNSUPPORTED, "rate categories must be 1..8");
+  }
+  return GLE_SUCCESS;
+}
+
+template <typename T, int S>
+static void launch_ns(Instance *I, const DevClaOp *d_ops, int n, bool fused, int nblk) {
+  NsClaArgs<T> a;
+  a.cla = reinterpret_cast<T *>(I->d_cla);
+  a.u = I->d_u;
+  a.tips = I->d_tips;
+  a.pm = sizeof(T) == 8 ? reinterpret_cast<const T *>(I->d_pm64) : reinterpret_cast<const T *>(I->d_pm32);
+  a.pm_stride = sizeof(T) == 8 ? size_t(3) * I->R * S * S : size_t(I->R) * S * S;
+  a.slot_stride = I->slot_elems();
+  a.Ppad = I->Ppad;
+  a.P = I->P;
+  a.R = I->R;
+  a.Sp = I->Sp;
+  a.flags = I->d_flags;
+  const size_t shmem = S <= 32 ? sizeof(T) * 2 * I->R * S * S : 0;
+  hipLaunchKernelGGL((k_cla_ns<T, S>), dim3(nblk, fused ? 1 : n), dim3(256), shmem, I->stream, a, d_ops,
+                     fused ? n : 0);
+}
+
+template <typename T>
+static int launch_cla(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
+  const int nblk = blocks_for(I);
+  switch (I->S) {
+    case 4: return launch_nuc_r<T>(I, d_ops, n, fused, nblk);
+    case 20: launch_ns<T, 20>(I, d_ops, n, fused, nblk); return GLE_SUCCESS;
+    case 61: launch_ns<T, 61>(I, d_ops, n, fused, nblk); return GLE_SUCCESS;
+    default: return set_error(GLE_ERR_UNSUPPORTED, "state count %d has no kernel", I->S);
+  }
+}
+
+int engine_update_partials(Instance *I, const DevClaOp *ops, int n_ops, const int *level_start, int n_levels,
+                           bool fused) {
+  if (n_ops <= 0) return GLE_SUCCESS;
+  for (int i = 0; i < n_ops; i++) {
+    const DevClaOp &o = ops[i];
+    auto bad_child = [&](int c) { return c >= I->n_slots || (c < 0 && -c - 1 >= I->N); };
+    if (o.dest < 0 || o.dest >= I->n_slots || bad_child(o.c1) || bad_child(o.c2) || o.pm1 < 0 ||
+        o.pm1 >= I->n_pm || o.pm2 < 0 || o.pm2 >= I->n_pm || o.dest == o.c1 || o.dest == o.c2)
+      return set_error(GLE_ERR_ARG, "cla op %d out of range (dest %d c1 %d c2 %d pm %d %d)", i, o.dest, o.c1,
+                       o.c2, o.pm1, o.pm2);
+  }
+  int rc = grow(I->d_ops, I->ops_cap, n_ops);
+  if (rc) return rc;
+  GLE_HIP(hipMemcpyAsync(I->d_ops, ops, sizeof(DevClaOp) * n_ops, hipMemcpyHostToDevice, I->stream));
+  auto bytes_of = [&](int b, int e) {
+    double t = 0.0;
+    for (int i = b; i < e; i++) t += cla_op_bytes(I, ops[i]);
+    return t;
+  };
+  if (fused && engine_supports_fused(I)) {
+    timed_begin(I, 0, bytes_of(0, n_ops));
+    rc = I->prec == GLE_PREC_FP64 ? launch_cla<double>(I, I->d_ops, n_ops, true)
+                                  : launch_cla<float>(I, I->d_ops, n_ops, true);
+    timed_end(I);
+    if (rc) return rc;
+  } else {
+    const int nl = n_levels > 0 ? n_levels : n_ops;
+    for (int l = 0; l < nl; l++) {
+      const int b = n_levels > 0 ? level_start[l] : l, e = n_levels > 0 ? level_start[l + 1] : l + 1;
+      if (e <= b) continue;
+      timed_begin(I, 0, bytes_of(b, e));
+      rc = I->prec == GLE_PREC_FP64 ? launch_cla<double>(I, I->d_ops + b, e - b, false)
+                                    : launch_cla<float>(I, I->d_ops + b, e - b, false);
+      timed_end(I);
+      if (rc) return rc;
+    }
+  }
+  GLE_HIP(hipGetLastError());
+  GLE_HIP(hipStreamSynchronize(I->stream));
+  timed_collect(I);
+  return GLE_SUCCESS;
+}
+
+template <typename T>
+static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk) {
+  EdgeArgs<T> a;
+  a.cla = reinterpret_cast<const T *>(I->d_cla);
+  a.u = I->d_u;
+  a.tips = I->d_tips;
+  a.pm64 = I->d_pm64;
+  a.model = I->d_model;
+  a.counts = I->d_counts;
+  a.cst = I->d_cst;
+  a.pinv = I->pinv;
+  a.pinv_in_model = I->pinv_in_model;
+  a.slot_stride = I->slot_elems();
+  a.Ppad = I->Ppad;
+  a.P = I->P;
+  a.R = I->R;
+  a.S = I->S;
+  a.Sp = I->Sp;
+  a.partials = I->d_part;
+  a.site = I->d_site;
+  dim3 grid(nblk, n);
+  if (I->S == 4) {
+    switch (I->R) {
+      case 1: hipLaunchKernelGGL((k_edge_nuc<T, 1>), grid, dim3(256), 0, I->stream, a, d_ops); break;
+      case 2: hipLaunchKernelGGL((k_edge_nuc<T, 2>), grid, dim3(256), 0, I->stream, a, d_ops); break;
+      case 3: hipLaunchKernelGGL((k_edge_nuc<T, 3>), grid, dim3(256), 0, I->stream, a, d_ops); break;
+      case 4: hipLaunchKernelGGL((k_edge_nuc<T, 4>), grid, dim3(256), 0, I->stream, a, d_ops); break;
+      case 5: hipLaunchKernelGGL((k_edge_nuc<T, 5>), grid, dim3(256), 0, I->stream, a, d_ops); break;
+      case 6: hipLaunchKernelGGL((k_edge_nuc<T, 6>), grid, dim3(256), 0, I->stream, a, d_ops); break;
+      case 7: hipLaunchKernelGGL((k_edge_nuc<T, 7>), grid, dim3(256), 0, I->stream, a, d_ops); break;
+      case 8: hipLaunchKernelGGL((k_edge_nuc<T, 8>), grid, dim3(256), 0, I->stream, a, d_ops); break;
+      default: return set_error(GLE_ERR_UNSUPPORTED, "rate categories must be 1..8");
+    }
+  } else if (I->S == 20) {
+    hipLaunchKernelGGL((k_edge_ns<T, 20>), grid, dim3(256), 0, I->stream, a, d_ops);
+  } else if (I->S == 61) {
+    hipLaunchKernelGGL((k_edge_ns<T, 61>), grid, dim3(256), 0, I->stream, a, d_ops);
+  } else
+    return set_error(GLE_ERR_UNSUPPORTED, "state count %d has no kernel", I->S);
+  return GLE_SUCCESS;
+}
+
+int engine_edges(Instance *I, const DevEdgeOp *ops, int n, double *out, double *site_lnl) {
+  if (n <= 0) return GLE_SUCCESS;
+  for (int i = 0; i < n; i++) {
+    const DevEdgeOp &o = ops[i];
+    if (o.partial < 0 || o.partial >= I->n_slots || o.child >= I->n_slots || (o.child < 0 && -o.child - 1 >= I->N) ||
+        o.pm < 0 || o.pm >= I->n_pm || o.child == o.partial)
+      return set_error(GLE_ERR_ARG, "edge op %d out of range", i);
+  }
+  const int nblk = blocks_for(I);
+  int rc = grow(I->d_eops, I->eops_cap, n);
+  if (rc) return rc;
+  rc = grow(I->d_eres, I->eres_cap, 3 * n);
+  if (rc) return rc;
+  const size_t need = size_t(n) * nblk * 3;
+  if (need > I->part_cap) {
+    if (I->d_part) GLE_HIP(hipFree(I->d_part));
+    GLE_HIP(hipMalloc(&I->d_part, sizeof(double) * need));
+    I->part_cap = need;
+  }
+  GLE_HIP(hipMemcpyAsync(I->d_eops, ops, sizeof(DevEdgeOp) * n, hipMemcpyHostToDevice, I->stream));
+  double *site_save = I->d_site;
+  if (!site_lnl) I->d_site = nullptr;
+  double eb = 0.0;
+  for (int i = 0; i < n; i++) eb += edge_op_bytes(I, ops[i]);
+  timed_begin(I, 1, eb);
+  rc = I->prec == GLE_PREC_FP64 ? launch_edges<double>(I, I->d_eops, n, nblk)
+                                : launch_edges<float>(I, I->d_eops, n, nblk);
+  timed_end(I);
+  I->d_site = site_save;
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_edge_reduce, dim3(n), dim3(64), 0, I->stream, I->d_part, nblk, I->d_eres);
+  GLE_HIP(hipGetLastError());
+  GLE_HIP(hipMemcpyAsync(out, I->d_eres, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, I->stream));
+  if (site_lnl)
+    GLE_HIP(hipMemcpyAsync(site_lnl, I->d_site, sizeof(double) * I->P, hipMemcpyDeviceToHost, I->stream));
+  GLE_HIP(hipStreamSynchronize(I->stream));
+  timed_collect(I);
+  return GLE_SUCCESS;
+}
+
+int engine_flags(Instance *I, int *flags, int reset) {
+  GLE_HIP(hipMemcpyAsync(flags, I->d_flags, sizeof(int), hipMemcpyDeviceToHost, I->stream));
+  GLE_HIP(hipStreamSynchronize(I->stream));
+  if (reset) {
+    const int z = 0;
+    GLE_HIP(hipMemcpyAsync(I->d_flags, &z, sizeof(int), hipMemcpyHostToDevice, I->stream));
+    GLE_HIP(hipStreamSynchronize(I->stream));
+  }
+  return GLE_SUCCESS;
+}
+
+int engine_get_partials(Instance *I, int slot, double *out, int *u) {
+  if (slot < 0 || slot >= I->n_slots) return set_error(GLE_ERR_ARG, "slot %d out of range", slot);
+  const size_t W = size_t(I->P) * I->R * I->Sp;
+  std::vector<unsigned char> tmp(W * I->elem_bytes());
+  const char *base = reinterpret_cast<const char *>(I->d_cla) + slot * I->slot_elems() * I->elem_bytes();
+  GLE_HIP(hipMemcpy(tmp.data(), base, tmp.size(), hipMemcpyDeviceToHost));
+  GLE_HIP(hipMemcpy(u, I->d_u + size_t(slot) * I->Ppad, sizeof(int) * I->P, hipMemcpyDeviceToHost));
+  for (int p = 0; p < I->P; p++)
+    for (int r = 0; r < I->R; r++)
+      for (int s = 0; s < I->S; s++) {
+        const size_t src = (size_t(p) * I->R + r) * I->Sp + s, dst = (size_t(p) * I->R + r) * I->S + s;
+        out[dst] = I->prec == GLE_PREC_FP64 ? reinterpret_cast<const double *>(tmp.data())[src]
+                                            : double(reinterpret_cast<const float *>(tmp.data())[src]);
+      }
+  return GLE_SUCCESS;
+}
+
+int engine_sync(Instance *I) {
+  GLE_HIP(hipStreamSynchronize(I->stream));
+  return GLE_SUCCESS;
+}
+
+static int destroy(Instance *I) {
+  void *ptrs[] = {I->d_tips, I->d_counts, I->d_cst, I->d_u, I->d_flags, I->d_eig_of_rate, I->d_cla,
+                  I->d_pm64, I->d_pm32,  I->d_V,   I->d_Vinv, I->d_lam, I->d_model, I->d_ops,
+                  I->d_eops, I->d_part,  I->d_eres, I->d_site, I->d_pmidx, I->d_pmt};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (I->h_stage) (void)hipHostFree(I->h_stage);
+  for (auto &p : I->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (hipEvent_t e : I->ev_free) (void)hipEventDestroy(e);
+  if (I->stream) (void)hipStreamDestroy(I->stream);
+  return GLE_SUCCESS;
+}
+
+}  // namespace gle
+
+// =====================================================================
+// C ABI: instance layer
+// =====================================================================
+using namespace gle;
+
+extern "C" const char *gle_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int gle_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+extern "C" int gle_init(int device) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
+    return set_error(GLE_ERR_DEVICE, "no HIP device visible: the GARLI engine runs only on a GPU");
+  if (device < 0 || device >= n) return set_error(GLE_ERR_ARG, "device %d out of range (%d visible)", device, n);
+  GLE_HIP(hipSetDevice(device));
+  g_device = -1;
+  int rc = ensure_device();
+  if (rc) return rc;
+  g_device = device;
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_create_instance(const gle_instance_desc *d, int *handle) {
+  if (!d || !handle) return set_error(GLE_ERR_ARG, "null argument");
+  if (d->n_tips < 2 || d->n_patterns < 1 || d->n_states < 2 || d->n_states > 64 || d->n_rate_cats < 1 ||
+      d->n_rate_cats > 8 || d->n_eigen < 1 || d->n_eigen > d->n_rate_cats || d->n_cla_slots < 1 ||
+      d->n_pmat_slots < 1 || (d->precision != GLE_PREC_FP64 && d->precision != GLE_PREC_FP32))
+    return set_error(GLE_ERR_ARG, "invalid instance description");
+  if (d->n_states != 4 && d->n_states != 20 && d->n_states != 61)
+    return set_error(GLE_ERR_UNSUPPORTED, "state count %d: kernels exist for 4, 20 and 61", d->n_states);
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc = ensure_device();
+    if (rc) return rc;
+  }
+  auto I = std::make_unique<Instance>();
+  I->N = d->n_tips;
+  I->P = d->n_patterns;
+  I->Ppad = (d->n_patterns + 255) / 256 * 256;
+  I->S = d->n_states;
+  I->Sp = d->n_states == 61 ? 64 : d->n_states;
+  I->R = d->n_rate_cats;
+  I->n_eig = d->n_eigen;
+  I->n_slots = d->n_cla_slots;
+  I->n_pm = d->n_pmat_slots;
+  I->prec = d->precision;
+  const int S = I->S, R = I->R;
+  auto fail = [&](int rc) {
+    destroy(I.get());
+    return rc;
+  };
+#define GLE_ALLOC(ptr, bytes)                                                                         \
+  do {                                                                                                \
+    if (hipMalloc(&(ptr), (bytes)) != hipSuccess)                                                     \
+      return fail(set_error(GLE_ERR_NOMEM, "device allocation of %zu bytes failed", size_t(bytes))); \
+  } while (0)
+  if (hipStreamCreateWithFlags(&I->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(set_error(GLE_ERR_DEVICE, "stream creation failed"));
+  GLE_ALLOC(I->d_tips, size_t(I->N) * I->Ppad);
+  GLE_ALLOC(I->d_counts, sizeof(int) * I->Ppad);
+  GLE_ALLOC(I->d_cst, sizeof(int) * I->Ppad);
+  GLE_ALLOC(I->d_u, sizeof(int) * size_t(I->n_slots) * I->Ppad);
+  GLE_ALLOC(I->d_flags, sizeof(int));
+  GLE_ALLOC(I->d_eig_of_rate, sizeof(int) * R);
+  GLE_ALLOC(I->d_cla, I->slot_elems() * I->elem_bytes() * I->n_slots);
+  GLE_ALLOC(I->d_pm64, sizeof(double) * size_t(I->n_pm) * 3 * R * S * S);
+  GLE_ALLOC(I->d_pm32, sizeof(float) * size_t(I->n_pm) * R * S * S);
+  GLE_ALLOC(I->d_V, sizeof(double) * size_t(I->n_eig) * S * S);
+  GLE_ALLOC(I->d_Vinv, sizeof(double) * size_t(I->n_eig) * S * S);
+  GLE_ALLOC(I->d_lam, sizeof(double) * size_t(I->n_eig) * S);
+  GLE_ALLOC(I->d_model, sizeof(double) * (S + 2 * R));
+  GLE_ALLOC(I->d_site, sizeof(double) * I->Ppad);
+#undef GLE_ALLOC
+  // zero everything the kernels may read before the caller sets it
+  if (hipMemset(I->d_tips, 15, size_t(I->N) * I->Ppad) != hipSuccess ||
+      hipMemset(I->d_counts, 0, sizeof(int) * I->Ppad) != hipSuccess ||
+      hipMemset(I->d_cst, 0xff, sizeof(int) * I->Ppad) != hipSuccess ||
+      hipMemset(I->d_u, 0, sizeof(int) * size_t(I->n_slots) * I->Ppad) != hipSuccess ||
+      hipMemset(I->d_flags, 0, sizeof(int)) != hipSuccess ||
+      hipMemset(I->d_cla, 0, I->slot_elems() * I->elem_bytes() * I->n_slots) != hipSuccess ||
+      hipMemset(I->d_pm64, 0, sizeof(double) * size_t(I->n_pm) * 3 * R * S * S) != hipSuccess ||
+      hipMemset(I->d_pm32, 0, sizeof(float) * size_t(I->n_pm) * R * S * S) != hipSuccess)
+    return fail(set_error(GLE_ERR_DEVICE, "device memset failed"));
+  std::vector<int> eor(R, 0);
+  std::vector<double> model(S + 2 * R, 0.0);
+  for (int s = 0; s < S; s++) model[s] = 1.0 / S;
+  for (int r = 0; r < R; r++) {
+    model[S + r] = 1.0 / R;
+    model[S + R + r] = 1.0;
+  }
+  if (hipMemcpy(I->d_eig_of_rate, eor.data(), sizeof(int) * R, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(I->d_model, model.data(), sizeof(double) * model.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(set_error(GLE_ERR_DEVICE, "device upload failed"));
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_instances.push_back(std::move(I));
+  *handle = int(g_instances.size()) - 1;
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_destroy_instance(int h) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (h < 0 || h >= int(g_instances.size()) || !g_instances[h]) return set_error(GLE_ERR_ARG, "bad instance");
+  destroy(g_instances[h].get());
+  g_instances[h].reset();
+  return GLE_SUCCESS;
+}
+
+#define GET_INSTANCE(h)                                                  \
+  Instance *I = get_instance(h);                                         \
+  if (!I) return set_error(GLE_ERR_ARG, "bad instance handle %d", (h));
+
+extern "C" int gle_set_tip_ambig_stream(int h, int tip, const signed char *stream, int len) {
+  GET_INSTANCE(h);
+  if (I->S != 4) return set_error(GLE_ERR_ARG, "ambiguity streams are nucleotide data");
+  if (tip < 0 || tip >= I->N || !stream) return set_error(GLE_ERR_ARG, "bad tip");
+  std::vector<unsigned char> m(I->Ppad, 15);
+  int used = ambig_string_to_masks(stream, len, I->P, m.data());
+  if (used < 0) return set_error(GLE_ERR_ARG, "malformed ambiguity stream for tip %d", tip);
+  GLE_HIP(hipMemcpy(I->d_tips + size_t(tip) * I->Ppad, m.data(), I->Ppad, hipMemcpyHostToDevice));
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_set_tip_states(int h, int tip, const unsigned char *states) {
+  GET_INSTANCE(h);
+  if (tip < 0 || tip >= I->N || !states) return set_error(GLE_ERR_ARG, "bad tip");
+  std::vector<unsigned char> m(I->Ppad, (unsigned char)(I->S == 4 ? 15 : I->S));
+  for (int p = 0; p < I->P; p++) {
+    const int v = states[p];
+    if ((I->S == 4 && (v < 1 || v > 15)) || (I->S != 4 && v > I->S))
+      return set_error(GLE_ERR_ARG, "tip %d pattern %d: code %d out of range", tip, p, v);
+    m[p] = (unsigned char)v;
+  }
+  GLE_HIP(hipMemcpy(I->d_tips + size_t(tip) * I->Ppad, m.data(), I->Ppad, hipMemcpyHostToDevice));
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_set_patterns(int h, const int *counts, int last_const, const int *cst) {
+  GET_INSTANCE(h);
+  if (!counts || last_const < -1 || last_const >= I->P || (last_const >= 0 && !cst))
+    return set_error(GLE_ERR_ARG, "bad pattern info");
+  std::vector<int> c(I->Ppad, 0), k(I->Ppad, -1);
+  for (int p = 0; p < I->P; p++) {
+    if (counts[p] < 0) return set_error(GLE_ERR_ARG, "negative count");
+    c[p] = counts[p];
+  }
+  for (int p = 0; p <= last_const; p++) k[p] = cst[p];
+  GLE_HIP(hipMemcpy(I->d_counts, c.data(), sizeof(int) * I->Ppad, hipMemcpyHostToDevice));
+  GLE_HIP(hipMemcpy(I->d_cst, k.data(), sizeof(int) * I->Ppad, hipMemcpyHostToDevice));
+  I->last_const = last_const;
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_set_model(int h, const double *pi, const double *w, double pinv, int pinv_in_model) {
+  GET_INSTANCE(h);
+  if (!pi || !w || pinv < 0.0 || pinv >= 1.0) return set_error(GLE_ERR_ARG, "bad model");
+  std::vector<double> m(I->S + I->R);
+  for (int s = 0; s < I->S; s++) m[s] = pi[s];
+  for (int r = 0; r < I->R; r++) m[I->S + r] = w[r];
+  GLE_HIP(hipMemcpy(I->d_model, m.data(), sizeof(double) * m.size(), hipMemcpyHostToDevice));
+  I->pinv = pinv;
+  I->pinv_in_model = pinv_in_model ? 1 : 0;
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_set_eigen(int h, int e, const double *V, const double *Vinv, const double *lam) {
+  GET_INSTANCE(h);
+  if (e < 0 || e >= I->n_eig || !V || !Vinv || !lam) return set_error(GLE_ERR_ARG, "bad eigensystem");
+  const size_t SS = size_t(I->S) * I->S;
+  GLE_HIP(hipMemcpy(I->d_V + e * SS, V, sizeof(double) * SS, hipMemcpyHostToDevice));
+  GLE_HIP(hipMemcpy(I->d_Vinv + e * SS, Vinv, sizeof(double) * SS, hipMemcpyHostToDevice));
+  GLE_HIP(hipMemcpy(I->d_lam + size_t(e) * I->S, lam, sizeof(double) * I->S, hipMemcpyHostToDevice));
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_set_category_rates(int h, const int *eig_of_rate, const double *mult) {
+  GET_INSTANCE(h);
+  if (!eig_of_rate || !mult) return set_error(GLE_ERR_ARG, "null argument");
+  for (int r = 0; r < I->R; r++)
+    if (eig_of_rate[r] < 0 || eig_of_rate[r] >= I->n_eig || !(mult[r] >= 0.0))
+      return set_error(GLE_ERR_ARG, "bad category %d", r);
+  GLE_HIP(hipMemcpy(I->d_eig_of_rate, eig_of_rate, sizeof(int) * I->R, hipMemcpyHostToDevice));
+  GLE_HIP(hipMemcpy(I->d_model + I->S + I->R, mult, sizeof(double) * I->R, hipMemcpyHostToDevice));
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_update_transition_matrices(int h, const int *idx, const double *t, int n, int derivs) {
+  GET_INSTANCE(h);
+  if (n < 0 || (n > 0 && (!idx || !t))) return set_error(GLE_ERR_ARG, "bad pmat op list");
+  return engine_update_pmats(I, idx, t, n, derivs);
+}
+
+extern "C" int gle_get_transition_matrices(int h, int pm, double *out) {
+  GET_INSTANCE(h);
+  if (pm < 0 || pm >= I->n_pm || !out) return set_error(GLE_ERR_ARG, "bad pmat slot");
+  const size_t n = size_t(3) * I->R * I->S * I->S;
+  GLE_HIP(hipMemcpy(out, I->d_pm64 + pm * n, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_update_partials(int h, const gle_cla_op *ops, int n, const int *level_start, int n_levels) {
+  GET_INSTANCE(h);
+  if (n < 0 || (n > 0 && !ops) || (n_levels > 0 && !level_start)) return set_error(GLE_ERR_ARG, "bad op list");
+  if (n_levels > 0 && (level_start[0] != 0 || level_start[n_levels] != n))
+    return set_error(GLE_ERR_ARG, "level ranges must cover the op list");
+  std::vector<DevClaOp> d(n);
+  for (int i = 0; i < n; i++)
+    d[i] = {ops[i].dest, ops[i].child1, ops[i].pmat1, ops[i].child2, ops[i].pmat2, ops[i].rescale ? kOpRescale : 0};
+  return engine_update_partials(I, d.data(), n, level_start, n_levels, false);
+}
+
+extern "C" int gle_get_partials(int h, int slot, double *out, int *u) {
+  GET_INSTANCE(h);
+  if (!out || !u) return set_error(GLE_ERR_ARG, "null output");
+  return engine_get_partials(I, slot, out, u);
+}
+
+extern "C" int gle_edge_likelihoods(int h, const gle_edge_op *ops, int n, double *out, double *site) {
+  GET_INSTANCE(h);
+  if (n < 0 || (n > 0 && (!ops || !out))) return set_error(GLE_ERR_ARG, "bad edge op list");
+  std::vector<DevEdgeOp> d(n);
+  for (int i = 0; i < n; i++) d[i] = {ops[i].partial, ops[i].child, ops[i].pmat, ops[i].derivs ? 1 : 0};
+  return engine_edges(I, d.data(), n, out, site);
+}
+
+extern "C" int gle_get_rescale_flags(int h, int *flags, int reset) {
+  GET_INSTANCE(h);
+  if (!flags) return set_error(GLE_ERR_ARG, "null flags");
+  return engine_flags(I, flags, reset);
+}
+
+extern "C" int gle_set_timing(int h, int enable) {
+  GET_INSTANCE(h);
+  I->timing = enable != 0;
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_get_kernel_stats(int h, int kind, long long *launches, double *ms, double *bytes, int reset) {
+  GET_INSTANCE(h);
+  if (kind < 0 || kind > 2 || !launches || !ms || !bytes) return set_error(GLE_ERR_ARG, "bad stats request");
+  *launches = I->kst[kind].launches;
+  *ms = I->kst[kind].ms;
+  *bytes = I->kst[kind].bytes;
+  if (reset) I->kst[kind] = Instance::KStat();
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_synchronize(int h) {
+  GET_INSTANCE(h);
+  return engine_sync(I);
+}

@@ -1,0 +1,64 @@
+// gle_internal.h -- shared declarations of libgle (not installed).
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "gle.h"
+
+namespace gle {
+
+int set_error(int code, const char *fmt, ...);
+
+// host model/data helpers (gle_host_model.cpp)
+int reversible_eigen(int n, const double *Q, const double *pi, double *V, double *Vinv, double *lam);
+int discrete_gamma(int K, double alpha, double *rates);
+int nuc_datum(char c);
+int aa_datum(char c);
+int make_ambig_string(const unsigned char *masks, int n, signed char *out);
+int ambig_string_to_masks(const signed char *st, int len, int n, unsigned char *masks);
+int compress_patterns(int ntax, int nsite, int ms, const unsigned char *matrix, unsigned char *out, int *counts,
+                      int *const_states, int *last_const, int *site_to_pattern);
+
+// Rescale constants of the FP64 reference (trunk/src/tree.cpp:159-172).
+constexpr int kRescaleLen = 90;
+struct RescaleTables {
+  double thresh[kRescaleLen];
+  double mult[kRescaleLen];
+  int incr[kRescaleLen];
+  double rescale_below, reduce_below, bail_below;
+};
+const RescaleTables &rescale_tables();
+
+// One CLA update as the device sees it (child < 0 means tip -(child+1)).
+struct DevClaOp {
+  int dest, c1, pm1, c2, pm2, flags;
+};
+// One edge integration.
+struct DevEdgeOp {
+  int partial, child, pm, derivs;
+};
+enum : int { kOpRescale = 1 };
+
+// Engine entry points used by the tree and serving layers (gle_engine.hip).
+struct Instance;
+Instance *get_instance(int h);
+int instance_states(const Instance *);
+int instance_rates(const Instance *);
+int instance_patterns(const Instance *);
+int instance_tips(const Instance *);
+int instance_precision(const Instance *);
+int instance_cla_slots(const Instance *);
+int instance_pmat_slots(const Instance *);
+int engine_update_pmats(Instance *, const int *idx, const double *t, int n, int derivs);
+// ops in level order; level_start[n_levels+1]; fused = one launch for all ops
+int engine_update_partials(Instance *, const DevClaOp *ops, int n_ops, const int *level_start, int n_levels,
+                           bool fused);
+int engine_edges(Instance *, const DevEdgeOp *ops, int n, double *out, double *site_lnl);
+int engine_flags(Instance *, int *flags, int reset);
+int engine_get_partials(Instance *, int slot, double *out, int *u);
+int engine_sync(Instance *);
+bool engine_supports_fused(const Instance *);
+
+}  // namespace gle

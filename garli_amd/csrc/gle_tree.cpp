@@ -1,0 +1,662 @@
+// gle_tree.cpp -- GARLI's tree-scoring surface on top of the device engine.
+//
+// A Tree owns the directional CLAs of an unrooted binary tree: for an internal
+// node a and a neighbour b, X(a|b) is the conditional likelihood of the part of
+// the tree on a's side of edge (a,b).  GARLI names these DOWN (b = anc),
+// UPLEFT (b = left) and UPRIGHT (b = right) (treenode.h:45-47); the root's DOWN
+// excludes its middle child (tree.cpp:3643-3647).  CLAs live in device slots
+// shared between trees by reference count, the ClaManager scheme
+// (clamanager.h:96-441): copying a tree shares every clean CLA, and a CLA is
+// recomputed into a fresh slot when it turns dirty (copy-on-write).
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "gle_internal.h"
+
+namespace gle {
+
+// ---------------------------------------------------------------- slot pool
+struct ClaPool {  // one per instance
+  std::vector<int> refs;
+  std::vector<int> free_slots;
+  std::vector<int> rank;  // rescale rank of the CLA held in the slot
+  std::vector<int> pm_free;
+};
+static std::mutex g_tree_mu;
+static std::map<const Instance *, std::unique_ptr<ClaPool>> g_pools;
+
+static ClaPool *pool_for(Instance *I) {
+  auto &p = g_pools[I];
+  if (!p) {
+    p = std::make_unique<ClaPool>();
+    const int n = instance_cla_slots(I);
+    p->refs.assign(n, 0);
+    p->rank.assign(n, 0);
+    for (int s = n - 1; s >= 0; s--) p->free_slots.push_back(s);
+    for (int s = instance_pmat_slots(I) - 1; s >= 0; s--) p->pm_free.push_back(s);
+  }
+  return p.get();
+}
+
+struct Tree {
+  Instance *I = nullptr;
+  ClaPool *pool = nullptr;
+  int N = 0, nn = 0;
+  std::vector<int> parent;
+  std::vector<double> blen;
+  std::vector<std::array<int, 3>> nb;  // anc first (if any), then children ascending
+  std::vector<int> nnb;
+  std::vector<std::array<int, 3>> slot;  // CLA slot of X(a|nb[a][k]), -1 = dirty
+  std::vector<int> pm;                   // pmat slot per node's parent edge
+  std::vector<double> pm_len;            // length the slot holds (NaN = stale)
+  std::vector<char> pm_has_derivs;
+  int rescale_every = 16;
+  int schedule = 0;
+  double lnl = 0.0;
+  std::vector<double> site;
+  // scratch
+  std::vector<int> level;
+  std::vector<DevClaOp> ops;
+  std::vector<int> op_level;
+};
+
+static std::vector<std::unique_ptr<Tree>> g_trees;
+
+static Tree *get_tree(int h) {
+  if (h < 0 || h >= int(g_trees.size())) return nullptr;
+  return g_trees[h].get();
+}
+
+static bool is_tip(const Tree &t, int n) { return n >= 1 && n <= t.N; }
+static int nb_index(const Tree &t, int a, int b) {
+  for (int k = 0; k < t.nnb[a]; k++)
+    if (t.nb[a][k] == b) return k;
+  return -1;
+}
+static int edge_node(const Tree &t, int a, int b) { return t.parent[a] == b ? a : b; }
+
+static void release(Tree &t, int a, int k) {
+  int &s = t.slot[a][k];
+  if (s < 0) return;
+  if (--t.pool->refs[s] == 0) t.pool->free_slots.push_back(s);
+  s = -1;
+}
+static void dirty_all(Tree &t) {
+  for (int a = 0; a < t.nn; a++)
+    if (!is_tip(t, a))
+      for (int k = 0; k < t.nnb[a]; k++) release(t, a, k);
+}
+// SweepDirtynessOverTree: every X(x|y) whose side holds edge (n, parent(n))
+static void dirty_from(Tree &t, int from, int came) {
+  for (int j = 0; j < t.nnb[from]; j++) {
+    const int nbr = t.nb[from][j];
+    if (nbr == came) continue;
+    if (!is_tip(t, from)) release(t, from, j);
+    dirty_from(t, nbr, from);
+  }
+}
+static void dirty_edge(Tree &t, int n) {
+  const int a = t.parent[n];
+  dirty_from(t, n, a);
+  dirty_from(t, a, n);
+}
+
+static int build_tree(Tree &t, Instance *I, const int *parent, const double *blen) {
+  const int N = instance_tips(I), nn = 2 * N - 2;
+  t.I = I;
+  t.pool = pool_for(I);
+  t.N = N;
+  t.nn = nn;
+  t.parent.assign(parent, parent + nn);
+  t.blen.assign(blen, blen + nn);
+  t.nb.assign(nn, {-1, -1, -1});
+  t.nnb.assign(nn, 0);
+  if (parent[0] != -1) return set_error(GLE_ERR_ARG, "node 0 must be the root (parent -1)");
+  for (int n = 1; n < nn; n++) {
+    const int a = parent[n];
+    if (a < 0 || a >= nn || a == n || (a >= 1 && a <= N))
+      return set_error(GLE_ERR_ARG, "node %d: bad parent %d", n, a);
+    if (!(blen[n] >= 0.0)) return set_error(GLE_ERR_ARG, "node %d: bad branch length", n);
+    t.nb[n][t.nnb[n]++] = a;
+  }
+  for (int n = 1; n < nn; n++) {
+    const int a = parent[n];
+    if (t.nnb[a] >= 3) return set_error(GLE_ERR_ARG, "node %d has too many children", a);
+    t.nb[a][t.nnb[a]++] = n;
+  }
+  for (int n = 0; n < nn; n++) {
+    const int want = is_tip(t, n) ? 1 : 3;
+    if (t.nnb[n] != want) return set_error(GLE_ERR_ARG, "node %d has degree %d, expected %d", n, t.nnb[n], want);
+  }
+  // every node must reach the root (no cycles)
+  for (int n = 1; n < nn; n++) {
+    int x = n, steps = 0;
+    while (x != 0 && steps <= nn) {
+      x = parent[x];
+      steps++;
+    }
+    if (x != 0) return set_error(GLE_ERR_ARG, "node %d does not reach the root", n);
+  }
+  t.slot.assign(nn, {-1, -1, -1});
+  if (int(t.pool->pm_free.size()) < nn) return set_error(GLE_ERR_NOMEM, "out of transition-matrix slots");
+  t.pm.assign(nn, -1);
+  for (int n = 1; n < nn; n++) {
+    t.pm[n] = t.pool->pm_free.back();
+    t.pool->pm_free.pop_back();
+  }
+  t.pm_len.assign(nn, std::nan(""));
+  t.pm_has_derivs.assign(nn, 0);
+  t.rescale_every = instance_precision(I) == GLE_PREC_FP64 ? 16 : 1;
+  t.level.assign(size_t(nn) * 3, -1);
+  return GLE_SUCCESS;
+}
+
+// ------------------------------------------------------ op-list generation
+
+// Collect the ops that make X(a|b) clean (post-order), recording levels.
+// Returns the level of X(a|b) (-1 for tips / already clean CLAs).
+static int collect(Tree &t, int a, int b, std::vector<std::array<int, 3>> &want) {
+  if (is_tip(t, a)) return -1;
+  const int k = nb_index(t, a, b);
+  if (t.slot[a][k] >= 0) return -1;
+  int &lv = t.level[size_t(a) * 3 + k];
+  if (lv >= 0) return lv;
+  int lmax = -1;
+  for (int j = 0; j < t.nnb[a]; j++) {
+    const int c = t.nb[a][j];
+    if (c == b) continue;
+    lmax = std::max(lmax, collect(t, c, a, want));
+  }
+  lv = lmax + 1;
+  want.push_back({a, k, lv});
+  return lv;
+}
+
+// Pmat bookkeeping: make sure the given edges hold P (and P', P'' if derivs).
+static int refresh_pmats(Tree &t, const std::vector<int> &edges, bool derivs) {
+  std::vector<int> idx, nodes;
+  std::vector<double> len;
+  for (int n : edges) {
+    const bool ok = t.pm_len[n] == t.blen[n] && (!derivs || t.pm_has_derivs[n]);
+    if (ok || std::find(nodes.begin(), nodes.end(), n) != nodes.end()) continue;
+    nodes.push_back(n);
+    idx.push_back(t.pm[n]);
+    len.push_back(t.blen[n]);
+  }
+  if (idx.empty()) return GLE_SUCCESS;
+  int rc = engine_update_pmats(t.I, idx.data(), len.data(), int(idx.size()), derivs ? 1 : 0);
+  if (rc) return rc;
+  for (int n : nodes) {
+    t.pm_len[n] = t.blen[n];
+    t.pm_has_derivs[n] = derivs ? 1 : 0;
+  }
+  return GLE_SUCCESS;
+}
+
+// Compute the listed directional CLAs (already in dependency order with levels).
+static int run_ops(Tree &t, std::vector<std::array<int, 3>> &want) {
+  if (want.empty()) return GLE_SUCCESS;
+  std::stable_sort(want.begin(), want.end(), [](const auto &x, const auto &y) { return x[2] < y[2]; });
+  std::vector<int> edges;
+  for (auto &w : want) {
+    const int a = w[0], k = w[1], b = t.nb[a][k];
+    for (int j = 0; j < t.nnb[a]; j++)
+      if (t.nb[a][j] != b) edges.push_back(edge_node(t, a, t.nb[a][j]));
+  }
+  int rc = refresh_pmats(t, edges, false);
+  if (rc) return rc;
+  std::vector<DevClaOp> ops;
+  std::vector<int> level_start;
+  ops.reserve(want.size());
+  int cur = -1;
+  for (auto &w : want) {
+    const int a = w[0], k = w[1], b = t.nb[a][k];
+    int x = -1, y = -1;
+    for (int j = 0; j < t.nnb[a]; j++) {
+      const int c = t.nb[a][j];
+      if (c == b) continue;
+      if (x < 0) x = c;
+      else y = c;
+    }
+    auto child_ref = [&](int c, int &rank) {
+      if (is_tip(t, c)) {
+        rank = 0;
+        return -c;  // tip index c-1 encoded as -(c-1)-1
+      }
+      const int s = t.slot[c][nb_index(t, c, a)];
+      rank = t.pool->rank[s];
+      return s;
+    };
+    int rx = 0, ry = 0;
+    DevClaOp op;
+    op.c1 = child_ref(x, rx);
+    op.c2 = child_ref(y, ry);
+    op.pm1 = t.pm[edge_node(t, a, x)];
+    op.pm2 = t.pm[edge_node(t, a, y)];
+    // rescaleRank: II 2+rL+rR, IT r+2, TT 2 (tree.cpp:7840, 8534, 8079)
+    int rank = 2 + rx + ry;
+    op.flags = 0;
+    if (rank >= t.rescale_every) {
+      op.flags = kOpRescale;
+      rank = 0;
+    }
+    if (t.pool->free_slots.empty()) return set_error(GLE_ERR_NOMEM, "out of CLA slots");
+    const int s = t.pool->free_slots.back();
+    t.pool->free_slots.pop_back();
+    t.pool->refs[s] = 1;
+    t.pool->rank[s] = rank;
+    t.slot[a][k] = s;
+    op.dest = s;
+    if (w[2] != cur) {
+      level_start.push_back(int(ops.size()));
+      cur = w[2];
+    }
+    ops.push_back(op);
+  }
+  level_start.push_back(int(ops.size()));
+  for (auto &w : want) t.level[size_t(w[0]) * 3 + w[1]] = -1;
+  const bool fused = t.schedule == 1 && engine_supports_fused(t.I);
+  return engine_update_partials(t.I, ops.data(), int(ops.size()), level_start.data(), int(level_start.size()) - 1,
+                                fused);
+}
+
+static int child_code(Tree &t, int c, int a) {
+  if (is_tip(t, c)) return -c;
+  return t.slot[c][nb_index(t, c, a)];
+}
+
+// One attempt of an edge evaluation.  Returns GLE_RESCALE_RETRY when the
+// reference would have thrown from RescaleRateHet.
+static int edge_attempt(Tree &t, int pa, int pb, bool derivs, double *res, double *site) {
+  std::vector<std::array<int, 3>> want;
+  collect(t, pa, pb, want);
+  collect(t, pb, pa, want);
+  int flags = 0, rc = engine_flags(t.I, &flags, 1);
+  if (rc) return rc;
+  rc = run_ops(t, want);
+  if (rc) return rc;
+  const int en = edge_node(t, pa, pb);
+  rc = refresh_pmats(t, {en}, derivs);
+  if (rc) return rc;
+  DevEdgeOp op;
+  op.partial = t.slot[pa][nb_index(t, pa, pb)];
+  op.child = child_code(t, pb, pa);
+  op.pm = t.pm[en];
+  op.derivs = derivs ? 1 : 0;
+  rc = engine_edges(t.I, &op, 1, res, site);
+  if (rc) return rc;
+  rc = engine_flags(t.I, &flags, 1);
+  if (rc) return rc;
+  if (instance_precision(t.I) == GLE_PREC_FP64) {
+    const bool thrown = ((flags & GLE_FLAG_REDUCE_RESCALE) && t.rescale_every > 2) ||
+                        ((flags & GLE_FLAG_BAIL_OUT) && t.rescale_every <= 2);
+    if (thrown) return GLE_RESCALE_RETRY;
+  }
+  return GLE_SUCCESS;
+}
+
+// Retry loop of Tree::Score / BranchLike (tree.cpp:4077-4090)
+static int edge_eval(Tree &t, int pa, int pb, bool derivs, double *res, double *site) {
+  for (;;) {
+    const int rc = edge_attempt(t, pa, pb, derivs, res, site);
+    if (rc != GLE_RESCALE_RETRY) return rc;
+    dirty_all(t);
+    t.rescale_every -= 2;
+    if (t.rescale_every < 2)
+      return set_error(GLE_ERR_RESCALE_FATAL, "Problem with rescaling during tree scoring.");
+  }
+}
+
+// ConditionalLikelihoodRateHet(ROOT, root) partial/child choice (tree.cpp:3686-3768)
+static void root_edge(Tree &t, int &pa, int &pb) {
+  const int left = t.nb[0][0], middle = t.nb[0][1], right = t.nb[0][2];
+  pa = 0;
+  if (t.slot[0][0] >= 0) pb = left;
+  else if (t.slot[0][2] >= 0) pb = right;
+  else pb = middle;
+}
+
+static int tree_score(Tree &t, double *lnl) {
+  int pa, pb;
+  root_edge(t, pa, pb);
+  double res[3];
+  t.site.resize(instance_patterns(t.I));
+  const int rc = edge_eval(t, pa, pb, false, res, t.site.data());
+  if (rc) return rc;
+  t.lnl = res[0];
+  *lnl = res[0];
+  return GLE_SUCCESS;
+}
+
+static int tree_derivs(Tree &t, int nd, double *lnl, double *d1, double *d2) {
+  if (nd <= 0 || nd >= t.nn) return set_error(GLE_ERR_ARG, "node %d has no parent edge", nd);
+  double res[3];
+  const int rc = edge_eval(t, t.parent[nd], nd, true, res, nullptr);
+  if (rc) return rc;
+  t.lnl = res[0];
+  *lnl = res[0];
+  *d1 = res[1];
+  *d2 = res[2];
+  return GLE_SUCCESS;
+}
+
+static void set_blen(Tree &t, int nd, double v) {
+  if (t.blen[nd] == v) return;
+  t.blen[nd] = v;
+  dirty_edge(t, nd);
+}
+
+// NewtonRaphsonOptimizeBranchLength (optimization.cpp:1359-1972), FP64 branch,
+// without the debugging output paths.
+static int newton(Tree &t, int nd, double prec, double *improve) {
+  const double min_b = 1e-8, max_b = 100.0;
+  double total = 0.0, v = t.blen[nd], v_prev = v, knownMin = min_b, knownMax = max_b, initialL = 0.0;
+  int iter = 0, negProposalNum = 0;
+  *improve = 0.0;
+  for (;;) {
+    double lnL, d1, d2;
+    int rc = tree_derivs(t, nd, &lnL, &d1, &d2);
+    if (rc) return rc;
+    if (iter == 0) initialL = lnL;
+    const double dlen = t.blen[nd];
+    const double estDeltaNR = -d1 / d2;
+    const double estScoreDelta = d1 * estDeltaNR + (d2 * estDeltaNR * estDeltaNR * 0.5);
+    if (d1 <= 0.0 && dlen < knownMax) knownMax = dlen;
+    else if (d1 > 0.0 && dlen > knownMin) knownMin = dlen;
+    auto done = [&]() {
+      *improve = total;
+      return GLE_SUCCESS;
+    };
+    if (d2 >= 0.0) {  // curvature wrong for NR
+      double proposed;
+      if (d1 <= 0.0) {
+        if (std::fabs(dlen - min_b) < 1.0e-8) return done();
+        if (knownMin == min_b) {
+          if (dlen <= 1.0e-4) proposed = min_b;
+          else if (dlen <= 0.005) proposed = 1.0e-4;
+          else if (dlen <= 0.05) proposed = dlen * 0.1;
+          else proposed = dlen * 0.25;
+        } else
+          proposed = (knownMin + dlen) * 0.5;
+        if (iter > 0 || proposed == min_b) {
+          const double estImp = d1 * (proposed - dlen) + (d2 * (proposed - dlen) * (proposed - dlen) * 0.5);
+          if (estImp < prec && lnL >= initialL - 1.0e-4) return done();
+        }
+      } else {
+        if (knownMax == max_b) proposed = dlen < 0.1 ? dlen * 10.0 : std::min(dlen * 2.0, max_b);
+        else proposed = (knownMax + dlen) * 0.5;
+        if (iter > 0) {
+          const double estImp = d1 * (proposed - dlen) + (d2 * (proposed - dlen) * (proposed - dlen) * 0.5);
+          if (estImp < prec && lnL >= initialL - 1.0e-4) return done();
+        }
+      }
+      v = proposed;
+      total += prec;
+    } else {
+      if (d1 < 0.0 && std::fabs(dlen - min_b) < 1.0e-8) return done();
+      if (d1 > 0.0 && std::fabs(dlen - max_b) < 1.0e-8) return done();
+      if (estScoreDelta < prec && (iter == 0 || lnL + ((iter < 10 ? 1 : iter) * 1.0e-7) >= initialL)) return done();
+      bool noNR = false;
+      if (iter > 10) {
+        if (d1 > 0.0) {
+          if (std::fabs(knownMax - max_b) < 1e-8) {
+            if (iter > 20 && dlen > max_b / 2.0) v = max_b;
+            else if (v < 0.2) v = std::min((v + max_b) * 0.5, v * 5.0);
+            else v = std::min((v + max_b) * 0.5, v * 2.0);
+            noNR = true;
+          } else if ((iter - 20) > 0 && (iter - 20) % 10 == 0)
+            v = std::min((v + knownMax) * 0.5, v * 100.0);
+        } else if (std::fabs(knownMin - min_b) < 1e-8) {
+          v = iter > 20 ? min_b : (v + min_b) * 0.5;
+          noNR = true;
+        }
+      }
+      if (!noNR) v += estDeltaNR;
+      if (v <= knownMin) {
+        negProposalNum++;
+        if (knownMin == min_b) {
+          const double dm = min_b - dlen, sdm = dm * d1 + (dm * dm * d2 * 0.5);
+          if (sdm < prec) return done();
+          if (negProposalNum == 1 && dlen > 1e-4 && v_prev != 1e-4) {
+            v = dlen < .005 ? 1e-4 : (dlen < 0.05 ? dlen * 0.1 : dlen * 0.25);
+            const double delta = v - dlen;
+            total += delta * d1 + (delta * delta * d2 * 0.5);
+          } else {
+            v = min_b;
+            total += sdm;
+          }
+        } else {
+          const double proposed = (knownMin + dlen) * 0.5, dm = proposed - dlen;
+          const double sdm = dm * d1 + (dm * dm * d2 * 0.5);
+          if (sdm < prec) return done();
+          v = proposed;
+          total += sdm;
+        }
+      } else if (v >= knownMax) {
+        if (knownMax == max_b) {
+          const double dm = max_b - dlen, sdm = dm * d1 + (dm * dm * d2 * 0.5);
+          if (sdm < prec) return done();
+          v = max_b;
+          total += sdm;
+        } else {
+          const double proposed = (knownMax + dlen) * 0.5, dm = proposed - dlen;
+          const double sdm = dm * d1 + (dm * dm * d2 * 0.5);
+          if (sdm < prec) return done();
+          v = proposed;
+          total += sdm;
+        }
+      } else
+        total += estScoreDelta;
+    }
+    set_blen(t, nd, v);
+    v_prev = v;
+    if (++iter > 100) return done();
+  }
+}
+
+static void preorder(const Tree &t, int n, std::vector<int> &out) {
+  out.push_back(n);
+  for (int j = 0; j < t.nnb[n]; j++)
+    if (t.nb[n][j] != t.parent[n]) preorder(t, t.nb[n][j], out);
+}
+
+static int optimize_all(Tree &t, double prec, double *improve, double *lnl) {
+  std::vector<int> order;
+  for (int j = 0; j < t.nnb[0]; j++) preorder(t, t.nb[0][j], order);
+  double total = 0.0;
+  for (int n : order) {
+    double imp = 0.0;
+    const int rc = newton(t, n, prec, &imp);
+    if (rc) return rc;
+    total += imp;
+  }
+  *improve = total;
+  dirty_all(t);
+  return tree_score(t, lnl);
+}
+
+// Every directional CLA, then (lnL, d1, d2) on every edge in one batched launch.
+static int sweep(Tree &t, double *out) {
+  for (int attempt = 0;; attempt++) {
+    dirty_all(t);
+    std::vector<std::array<int, 3>> want;
+    for (int a = 0; a < t.nn; a++)
+      if (!is_tip(t, a))
+        for (int k = 0; k < t.nnb[a]; k++) collect(t, a, t.nb[a][k], want);
+    int flags = 0, rc = engine_flags(t.I, &flags, 1);
+    if (rc) return rc;
+    rc = run_ops(t, want);
+    if (rc) return rc;
+    std::vector<int> edges;
+    for (int n = 1; n < t.nn; n++) edges.push_back(n);
+    rc = refresh_pmats(t, edges, true);
+    if (rc) return rc;
+    std::vector<DevEdgeOp> ops(t.nn - 1);
+    for (int n = 1; n < t.nn; n++) {
+      const int a = t.parent[n];
+      ops[n - 1] = {t.slot[a][nb_index(t, a, n)], child_code(t, n, a), t.pm[n], 1};
+    }
+    rc = engine_edges(t.I, ops.data(), int(ops.size()), out, nullptr);
+    if (rc) return rc;
+    rc = engine_flags(t.I, &flags, 1);
+    if (rc) return rc;
+    const bool thrown = instance_precision(t.I) == GLE_PREC_FP64 &&
+                        (((flags & GLE_FLAG_REDUCE_RESCALE) && t.rescale_every > 2) ||
+                         ((flags & GLE_FLAG_BAIL_OUT) && t.rescale_every <= 2));
+    if (!thrown) return GLE_SUCCESS;
+    t.rescale_every -= 2;
+    if (t.rescale_every < 2) return set_error(GLE_ERR_RESCALE_FATAL, "Problem with rescaling during sweep.");
+  }
+}
+
+static void destroy_tree(Tree &t) {
+  dirty_all(t);
+  for (int n = 1; n < t.nn; n++)
+    if (t.pm[n] >= 0) t.pool->pm_free.push_back(t.pm[n]);
+}
+
+}  // namespace gle
+
+using namespace gle;
+
+#define GET_TREE(h)                                                   \
+  std::lock_guard<std::mutex> lk_(g_tree_mu);                         \
+  Tree *T = get_tree(h);                                              \
+  if (!T) return set_error(GLE_ERR_ARG, "bad tree handle %d", (h));
+
+extern "C" int gle_tree_create(int inst, const int *parent, const double *blen, int *tree) {
+  if (!parent || !blen || !tree) return set_error(GLE_ERR_ARG, "null argument");
+  Instance *I = get_instance(inst);
+  if (!I) return set_error(GLE_ERR_ARG, "bad instance handle %d", inst);
+  std::lock_guard<std::mutex> lk(g_tree_mu);
+  auto t = std::make_unique<Tree>();
+  const int rc = build_tree(*t, I, parent, blen);
+  if (rc) {
+    if (!t->pm.empty()) destroy_tree(*t);
+    return rc;
+  }
+  g_trees.push_back(std::move(t));
+  *tree = int(g_trees.size()) - 1;
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_tree_copy(int h, int *out) {
+  if (!out) return set_error(GLE_ERR_ARG, "null argument");
+  GET_TREE(h);
+  auto t = std::make_unique<Tree>();
+  const int rc = build_tree(*t, T->I, T->parent.data(), T->blen.data());
+  if (rc) {
+    if (!t->pm.empty()) destroy_tree(*t);
+    return rc;
+  }
+  for (int a = 0; a < T->nn; a++)
+    for (int k = 0; k < 3; k++) {
+      const int s = T->slot[a][k];
+      if (s >= 0) {
+        t->slot[a][k] = s;
+        T->pool->refs[s]++;
+      }
+    }
+  t->rescale_every = T->rescale_every;
+  t->schedule = T->schedule;
+  g_trees.push_back(std::move(t));
+  *out = int(g_trees.size()) - 1;
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_tree_destroy(int h) {
+  GET_TREE(h);
+  destroy_tree(*T);
+  g_trees[h].reset();
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_tree_set_branch_length(int h, int node, double v) {
+  GET_TREE(h);
+  if (node <= 0 || node >= T->nn || !(v >= 1e-8 && v <= 100.0))
+    return set_error(GLE_ERR_ARG, "bad node %d or length %g (allowed [1e-8, 100])", node, v);
+  set_blen(*T, node, v);
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_tree_get_branch_lengths(int h, double *out) {
+  if (!out) return set_error(GLE_ERR_ARG, "null output");
+  GET_TREE(h);
+  std::copy(T->blen.begin(), T->blen.end(), out);
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_tree_invalidate(int h) {
+  GET_TREE(h);
+  dirty_all(*T);
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_tree_score(int h, double *lnl) {
+  if (!lnl) return set_error(GLE_ERR_ARG, "null output");
+  GET_TREE(h);
+  return tree_score(*T, lnl);
+}
+
+extern "C" int gle_tree_site_likelihoods(int h, double *out) {
+  if (!out) return set_error(GLE_ERR_ARG, "null output");
+  GET_TREE(h);
+  if (T->site.empty()) return set_error(GLE_ERR_STATE, "score the tree first");
+  std::copy(T->site.begin(), T->site.end(), out);
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_tree_derivatives(int h, int node, double *lnl, double *d1, double *d2) {
+  if (!lnl || !d1 || !d2) return set_error(GLE_ERR_ARG, "null output");
+  GET_TREE(h);
+  return tree_derivs(*T, node, lnl, d1, d2);
+}
+
+extern "C" int gle_tree_optimize_all_branches(int h, double prec, double *improve, double *lnl) {
+  if (!improve || !lnl || !(prec > 0.0)) return set_error(GLE_ERR_ARG, "bad argument");
+  GET_TREE(h);
+  return optimize_all(*T, prec, improve, lnl);
+}
+
+extern "C" int gle_tree_sweep(int h, double *out) {
+  if (!out) return set_error(GLE_ERR_ARG, "null output");
+  GET_TREE(h);
+  return sweep(*T, out);
+}
+
+extern "C" int gle_tree_get_cla(int h, int a, int b, double *out, int *u) {
+  if (!out || !u) return set_error(GLE_ERR_ARG, "null output");
+  GET_TREE(h);
+  if (a < 0 || a >= T->nn || is_tip(*T, a)) return set_error(GLE_ERR_ARG, "node %d is not internal", a);
+  const int k = nb_index(*T, a, b);
+  if (k < 0) return set_error(GLE_ERR_ARG, "%d is not a neighbour of %d", b, a);
+  std::vector<std::array<int, 3>> want;
+  collect(*T, a, b, want);
+  int rc = run_ops(*T, want);
+  if (rc) return rc;
+  return engine_get_partials(T->I, T->slot[a][k], out, u);
+}
+
+extern "C" int gle_tree_rescale_every(int h) {
+  GET_TREE(h);
+  return T->rescale_every;
+}
+
+extern "C" int gle_tree_set_rescale_every(int h, int every) {
+  GET_TREE(h);
+  if (every < 2) return set_error(GLE_ERR_ARG, "rescaleEvery must be >= 2");
+  T->rescale_every = every;
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_tree_set_schedule(int h, int mode) {
+  GET_TREE(h);
+  if (mode != 0 && mode != 1) return set_error(GLE_ERR_ARG, "schedule must be 0 or 1");
+  T->schedule = mode;
+  return GLE_SUCCESS;
+}

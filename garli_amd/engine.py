@@ -1,0 +1,401 @@
+"""ctypes bindings of libgle (include/gle.h).
+
+The classes mirror the two C-ABI layers: :class:`Instance` is the op-list layer
+(GARLI's CalculationManager seam, trunk/src/calculationmanager.h:39-123) and
+:class:`Tree` is GARLI's tree-scoring surface (Tree::Score, tree.cpp:4003;
+Tree::CalcDerivativesRateHet, optimization.cpp:1099; Tree::SetBranchLength,
+tree.h:425; Tree::OptimizeAllBranches, optimization.cpp:97).
+
+There is no CPU fallback: if libgle.so is missing, or no GPU is visible, the
+calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgle.so")
+
+PREC_FP64 = 0
+PREC_FP32 = 1
+SUCCESS = 0
+RESCALE_RETRY = 1
+
+_lib = None
+
+
+class GleError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("libgle error %d: %s" % (code, msg))
+        self.code = code
+
+
+class ClaOp(C.Structure):
+    _fields_ = [("dest", C.c_int), ("child1", C.c_int), ("pmat1", C.c_int),
+                ("child2", C.c_int), ("pmat2", C.c_int), ("rescale", C.c_int)]
+
+
+class EdgeOp(C.Structure):
+    _fields_ = [("partial", C.c_int), ("child", C.c_int), ("pmat", C.c_int), ("derivs", C.c_int)]
+
+
+class InstanceDesc(C.Structure):
+    _fields_ = [("n_tips", C.c_int), ("n_patterns", C.c_int), ("n_states", C.c_int),
+                ("n_rate_cats", C.c_int), ("n_eigen", C.c_int), ("n_cla_slots", C.c_int),
+                ("n_pmat_slots", C.c_int), ("precision", C.c_int)]
+
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+_U8 = C.POINTER(C.c_uint8)
+_S8 = C.POINTER(C.c_int8)
+
+# every symbol include/gle.h declares, with its argument types
+SIGNATURES = {
+    "gle_last_error": ([], C.c_char_p),
+    "gle_init": ([C.c_int], C.c_int),
+    "gle_device_count": ([], C.c_int),
+    "gle_model_gtr": ([_D, _D, _D, _D, _D], C.c_int),
+    "gle_model_aa": ([_D, _D, _D, _D, _D], C.c_int),
+    "gle_model_codon": ([_D, C.c_double, _D, _D, C.c_int, _D, _D, _D], C.c_int),
+    "gle_discrete_gamma": ([C.c_int, C.c_double, _D], C.c_int),
+    "gle_nuc_char_to_datum": ([C.c_char], C.c_int),
+    "gle_aa_char_to_datum": ([C.c_char], C.c_int),
+    "gle_make_ambig_string": ([_U8, C.c_int, _S8], C.c_int),
+    "gle_ambig_string_to_masks": ([_S8, C.c_int, C.c_int, _U8], C.c_int),
+    "gle_compress_patterns": ([C.c_int, C.c_int, C.c_int, _U8, _U8, _I, _I, _I, _I], C.c_int),
+    "gle_create_instance": ([C.POINTER(InstanceDesc), _I], C.c_int),
+    "gle_destroy_instance": ([C.c_int], C.c_int),
+    "gle_set_tip_ambig_stream": ([C.c_int, C.c_int, _S8, C.c_int], C.c_int),
+    "gle_set_tip_states": ([C.c_int, C.c_int, _U8], C.c_int),
+    "gle_set_patterns": ([C.c_int, _I, C.c_int, _I], C.c_int),
+    "gle_set_model": ([C.c_int, _D, _D, C.c_double, C.c_int], C.c_int),
+    "gle_set_eigen": ([C.c_int, C.c_int, _D, _D, _D], C.c_int),
+    "gle_set_category_rates": ([C.c_int, _I, _D], C.c_int),
+    "gle_update_transition_matrices": ([C.c_int, _I, _D, C.c_int, C.c_int], C.c_int),
+    "gle_get_transition_matrices": ([C.c_int, C.c_int, _D], C.c_int),
+    "gle_update_partials": ([C.c_int, C.POINTER(ClaOp), C.c_int, _I, C.c_int], C.c_int),
+    "gle_get_partials": ([C.c_int, C.c_int, _D, _I], C.c_int),
+    "gle_edge_likelihoods": ([C.c_int, C.POINTER(EdgeOp), C.c_int, _D, _D], C.c_int),
+    "gle_get_rescale_flags": ([C.c_int, _I, C.c_int], C.c_int),
+    "gle_synchronize": ([C.c_int], C.c_int),
+    "gle_set_timing": ([C.c_int, C.c_int], C.c_int),
+    "gle_get_kernel_stats": ([C.c_int, C.c_int, C.POINTER(C.c_longlong), _D, _D, C.c_int], C.c_int),
+    "gle_tree_create": ([C.c_int, _I, _D, _I], C.c_int),
+    "gle_tree_copy": ([C.c_int, _I], C.c_int),
+    "gle_tree_destroy": ([C.c_int], C.c_int),
+    "gle_tree_set_branch_length": ([C.c_int, C.c_int, C.c_double], C.c_int),
+    "gle_tree_get_branch_lengths": ([C.c_int, _D], C.c_int),
+    "gle_tree_invalidate": ([C.c_int], C.c_int),
+    "gle_tree_score": ([C.c_int, _D], C.c_int),
+    "gle_tree_site_likelihoods": ([C.c_int, _D], C.c_int),
+    "gle_tree_derivatives": ([C.c_int, C.c_int, _D, _D, _D], C.c_int),
+    "gle_tree_optimize_all_branches": ([C.c_int, C.c_double, _D, _D], C.c_int),
+    "gle_tree_sweep": ([C.c_int, _D], C.c_int),
+    "gle_tree_get_cla": ([C.c_int, C.c_int, C.c_int, _D, _I], C.c_int),
+    "gle_tree_rescale_every": ([C.c_int], C.c_int),
+    "gle_tree_set_rescale_every": ([C.c_int, C.c_int], C.c_int),
+    "gle_tree_set_schedule": ([C.c_int, C.c_int], C.c_int),
+}
+
+
+def lib():
+    """Load libgle.so (built by garli_amd.build); raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libgle.so not built: run `python -m garli_amd.build` "
+                               "(the engine has no CPU fallback)")
+        l = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(l, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = l
+    return _lib
+
+
+def _check(rc, allow=()):
+    if rc < 0 or (rc > 0 and rc not in allow):
+        raise GleError(rc, lib().gle_last_error().decode())
+    return rc
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def init(device=0):
+    _check(lib().gle_init(device))
+
+
+def device_count():
+    return lib().gle_device_count()
+
+
+# ------------------------------------------------------------ model helpers
+def model_gtr(pi, rel):
+    V = np.zeros(16)
+    W = np.zeros(16)
+    lam = np.zeros(4)
+    pi, rel = _d(pi), _d(rel)
+    _check(lib().gle_model_gtr(_p(pi, C.c_double), _p(rel, C.c_double), _p(V, C.c_double),
+                               _p(W, C.c_double), _p(lam, C.c_double)))
+    return V.reshape(4, 4), W.reshape(4, 4), lam
+
+
+def model_aa(pi, rel190):
+    V = np.zeros(400)
+    W = np.zeros(400)
+    lam = np.zeros(20)
+    pi, rel = _d(pi), _d(rel190)
+    _check(lib().gle_model_aa(_p(pi, C.c_double), _p(rel, C.c_double), _p(V, C.c_double),
+                              _p(W, C.c_double), _p(lam, C.c_double)))
+    return V.reshape(20, 20), W.reshape(20, 20), lam
+
+
+def model_codon(pi61, kappa, omega, omega_prob):
+    nw = len(omega)
+    V = np.zeros(nw * 3721)
+    W = np.zeros(nw * 3721)
+    lam = np.zeros(nw * 61)
+    pi, om, op = _d(pi61), _d(omega), _d(omega_prob)
+    _check(lib().gle_model_codon(_p(pi, C.c_double), float(kappa), _p(om, C.c_double), _p(op, C.c_double), nw,
+                                 _p(V, C.c_double), _p(W, C.c_double), _p(lam, C.c_double)))
+    return V.reshape(nw, 61, 61), W.reshape(nw, 61, 61), lam.reshape(nw, 61)
+
+
+def discrete_gamma(ncat, alpha):
+    r = np.zeros(ncat)
+    _check(lib().gle_discrete_gamma(ncat, float(alpha), _p(r, C.c_double)))
+    return r
+
+
+def nuc_char_to_datum(ch):
+    return lib().gle_nuc_char_to_datum(ch.encode() if isinstance(ch, str) else ch)
+
+
+def aa_char_to_datum(ch):
+    return lib().gle_aa_char_to_datum(ch.encode() if isinstance(ch, str) else ch)
+
+
+def make_ambig_string(masks):
+    m = np.ascontiguousarray(masks, dtype=np.uint8)
+    out = np.zeros(5 * len(m) + 1, dtype=np.int8)
+    n = _check(lib().gle_make_ambig_string(_p(m, C.c_uint8), len(m), _p(out, C.c_int8)), allow=range(1 << 30))
+    return out[:n].copy()
+
+
+def ambig_string_to_masks(stream, n):
+    s = np.ascontiguousarray(stream, dtype=np.int8)
+    out = np.zeros(n, dtype=np.uint8)
+    _check(lib().gle_ambig_string_to_masks(_p(s, C.c_int8), len(s), n, _p(out, C.c_uint8)), allow=range(1 << 30))
+    return out
+
+
+def compress_patterns(matrix, max_states):
+    """PatternManager::ProcessPatterns: returns (patterns [ntax][P], counts,
+    const_states, last_const, site_to_pattern)."""
+    m = np.ascontiguousarray(matrix, dtype=np.uint8)
+    ntax, nsite = m.shape
+    out = np.zeros((ntax, nsite), dtype=np.uint8)
+    counts = np.zeros(nsite, dtype=np.int32)
+    cs = np.zeros(nsite, dtype=np.int32)
+    lc = np.zeros(1, dtype=np.int32)
+    s2p = np.zeros(nsite, dtype=np.int32)
+    P = _check(lib().gle_compress_patterns(ntax, nsite, max_states, _p(m, C.c_uint8), _p(out, C.c_uint8),
+                                           _p(counts, C.c_int), _p(cs, C.c_int), _p(lc, C.c_int),
+                                           _p(s2p, C.c_int)), allow=range(1 << 30))
+    pats = out.reshape(-1)[: ntax * P].reshape(ntax, P).copy()
+    return pats, counts[:P].copy(), cs[:P].copy(), int(lc[0]), s2p
+
+
+# ----------------------------------------------------------------- instance
+class Instance:
+    """One data partition on the device (a BEAGLE-style instance)."""
+
+    def __init__(self, n_tips, n_patterns, n_states, n_rates, n_eigen=1, n_cla_slots=None,
+                 n_pmat_slots=None, precision=PREC_FP64):
+        self.n_tips, self.n_patterns, self.n_states, self.n_rates = n_tips, n_patterns, n_states, n_rates
+        self.precision = precision
+        if n_cla_slots is None:
+            n_cla_slots = 3 * (n_tips - 2) + 8
+        if n_pmat_slots is None:
+            n_pmat_slots = 2 * n_tips + 8
+        d = InstanceDesc(n_tips, n_patterns, n_states, n_rates, n_eigen, n_cla_slots, n_pmat_slots, precision)
+        h = C.c_int(-1)
+        _check(lib().gle_create_instance(C.byref(d), C.byref(h)))
+        self.h = h.value
+
+    def close(self):
+        if self.h is not None and self.h >= 0:
+            lib().gle_destroy_instance(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tip_states(self, tip, states):
+        s = np.ascontiguousarray(states, dtype=np.uint8)
+        assert len(s) == self.n_patterns
+        _check(lib().gle_set_tip_states(self.h, tip, _p(s, C.c_uint8)))
+
+    def set_tip_ambig_stream(self, tip, stream):
+        s = np.ascontiguousarray(stream, dtype=np.int8)
+        _check(lib().gle_set_tip_ambig_stream(self.h, tip, _p(s, C.c_int8), len(s)))
+
+    def set_patterns(self, counts, last_const, const_states):
+        c = np.ascontiguousarray(counts, dtype=np.int32)
+        k = np.ascontiguousarray(const_states, dtype=np.int32)
+        _check(lib().gle_set_patterns(self.h, _p(c, C.c_int), int(last_const), _p(k, C.c_int)))
+
+    def set_model(self, pi, rate_probs, pinv=0.0, pinv_in_model=False):
+        pi, w = _d(pi), _d(rate_probs)
+        _check(lib().gle_set_model(self.h, _p(pi, C.c_double), _p(w, C.c_double), float(pinv), int(pinv_in_model)))
+
+    def set_eigen(self, e, V, Vinv, lam):
+        V, W, l = _d(V), _d(Vinv), _d(lam)
+        _check(lib().gle_set_eigen(self.h, e, _p(V, C.c_double), _p(W, C.c_double), _p(l, C.c_double)))
+
+    def set_category_rates(self, eig_of_rate, mult):
+        e = np.ascontiguousarray(eig_of_rate, dtype=np.int32)
+        m = _d(mult)
+        _check(lib().gle_set_category_rates(self.h, _p(e, C.c_int), _p(m, C.c_double)))
+
+    def update_transition_matrices(self, idx, t, derivs=False):
+        i = np.ascontiguousarray(idx, dtype=np.int32)
+        tt = _d(t)
+        _check(lib().gle_update_transition_matrices(self.h, _p(i, C.c_int), _p(tt, C.c_double), len(i), int(derivs)))
+
+    def get_transition_matrices(self, slot):
+        S, R = self.n_states, self.n_rates
+        out = np.zeros(3 * R * S * S)
+        _check(lib().gle_get_transition_matrices(self.h, slot, _p(out, C.c_double)))
+        return out.reshape(3, R, S, S)
+
+    def update_partials(self, ops, level_start=None):
+        arr = (ClaOp * len(ops))(*[ClaOp(*o) for o in ops])
+        if level_start is None:
+            _check(lib().gle_update_partials(self.h, arr, len(ops), None, 0))
+        else:
+            ls = np.ascontiguousarray(level_start, dtype=np.int32)
+            _check(lib().gle_update_partials(self.h, arr, len(ops), _p(ls, C.c_int), len(ls) - 1))
+
+    def get_partials(self, slot):
+        out = np.zeros(self.n_patterns * self.n_rates * self.n_states)
+        u = np.zeros(self.n_patterns, dtype=np.int32)
+        _check(lib().gle_get_partials(self.h, slot, _p(out, C.c_double), _p(u, C.c_int)))
+        return out.reshape(self.n_patterns, self.n_rates, self.n_states), u
+
+    def edge_likelihoods(self, ops, want_sites=False):
+        arr = (EdgeOp * len(ops))(*[EdgeOp(*o) for o in ops])
+        out = np.zeros(3 * len(ops))
+        site = np.zeros(self.n_patterns) if want_sites else None
+        _check(lib().gle_edge_likelihoods(self.h, arr, len(ops), _p(out, C.c_double),
+                                          _p(site, C.c_double) if want_sites else None))
+        return (out.reshape(-1, 3), site) if want_sites else out.reshape(-1, 3)
+
+    def set_timing(self, enable=True):
+        _check(lib().gle_set_timing(self.h, int(enable)))
+
+    def kernel_stats(self, kind, reset=False):
+        """(launches, total ms, algorithmic bytes) for kind 0 CLA, 1 edge, 2 P(t)."""
+        n, ms, b = C.c_longlong(0), C.c_double(0), C.c_double(0)
+        _check(lib().gle_get_kernel_stats(self.h, kind, C.byref(n), C.byref(ms), C.byref(b), int(reset)))
+        return n.value, ms.value, b.value
+
+    def rescale_flags(self, reset=True):
+        f = C.c_int(0)
+        _check(lib().gle_get_rescale_flags(self.h, C.byref(f), int(reset)))
+        return f.value
+
+
+class Tree:
+    """GARLI's Tree scoring surface over an :class:`Instance`."""
+
+    def __init__(self, inst, parent=None, blen=None, _handle=None):
+        self.inst = inst
+        if _handle is not None:
+            self.h = _handle
+            return
+        par = np.ascontiguousarray(parent, dtype=np.int32)
+        bl = _d(blen)
+        assert len(par) == 2 * inst.n_tips - 2 and len(bl) == len(par)
+        h = C.c_int(-1)
+        _check(lib().gle_tree_create(inst.h, _p(par, C.c_int), _p(bl, C.c_double), C.byref(h)))
+        self.h = h.value
+
+    def copy(self):
+        h = C.c_int(-1)
+        _check(lib().gle_tree_copy(self.h, C.byref(h)))
+        return Tree(self.inst, _handle=h.value)
+
+    def close(self):
+        if self.h is not None:
+            lib().gle_tree_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def invalidate(self):
+        _check(lib().gle_tree_invalidate(self.h))
+
+    def score(self):
+        v = C.c_double(0)
+        _check(lib().gle_tree_score(self.h, C.byref(v)))
+        return v.value
+
+    def site_likelihoods(self):
+        out = np.zeros(self.inst.n_patterns)
+        _check(lib().gle_tree_site_likelihoods(self.h, _p(out, C.c_double)))
+        return out
+
+    def derivatives(self, node):
+        a, b, c = C.c_double(0), C.c_double(0), C.c_double(0)
+        _check(lib().gle_tree_derivatives(self.h, node, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def set_branch_length(self, node, v):
+        _check(lib().gle_tree_set_branch_length(self.h, node, float(v)))
+
+    def branch_lengths(self):
+        out = np.zeros(2 * self.inst.n_tips - 2)
+        _check(lib().gle_tree_get_branch_lengths(self.h, _p(out, C.c_double)))
+        return out
+
+    def optimize_all_branches(self, precision):
+        imp, lnl = C.c_double(0), C.c_double(0)
+        _check(lib().gle_tree_optimize_all_branches(self.h, float(precision), C.byref(imp), C.byref(lnl)))
+        return imp.value, lnl.value
+
+    def sweep(self):
+        out = np.zeros(3 * (2 * self.inst.n_tips - 3))
+        _check(lib().gle_tree_sweep(self.h, _p(out, C.c_double)))
+        return out.reshape(-1, 3)
+
+    def get_cla(self, a, b):
+        out = np.zeros(self.inst.n_patterns * self.inst.n_rates * self.inst.n_states)
+        u = np.zeros(self.inst.n_patterns, dtype=np.int32)
+        _check(lib().gle_tree_get_cla(self.h, a, b, _p(out, C.c_double), _p(u, C.c_int)))
+        return out.reshape(self.inst.n_patterns, self.inst.n_rates, self.inst.n_states), u
+
+    def rescale_every(self):
+        return lib().gle_tree_rescale_every(self.h)
+
+    def set_rescale_every(self, every):
+        _check(lib().gle_tree_set_rescale_every(self.h, int(every)))
+
+    def set_schedule(self, mode):
+        _check(lib().gle_tree_set_schedule(self.h, int(mode)))

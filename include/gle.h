@@ -1,0 +1,194 @@
+/*
+ * gle.h -- C ABI of the MI355X GARLI likelihood engine (libgle.so).
+ *
+ * The engine replaces GARLI's likelihood back ends (the native CPU kernels in
+ * trunk/src/tree.cpp and optimization.cpp, and the BEAGLE seam in
+ * trunk/src/calculationmanager.cpp / subsetcalculationmanager.cpp).  Every
+ * entry point takes plain pointers and sizes; results are FP64.
+ *
+ * Two layers:
+ *   1. instance layer: op lists in the shape of GARLI's CalculationManager
+ *      (ClaOperation / TransMatOperation / ScoringOperation,
+ *      trunk/src/calculationmanager.h:39-123);
+ *   2. tree layer: GARLI's tree-scoring surface (Tree::Score,
+ *      Tree::CalcDerivativesRateHet, Tree::SetBranchLength,
+ *      Tree::OptimizeAllBranches), with ClaManager-style CLA caching.
+ * A serving layer batches scoring requests from many trees.
+ *
+ * Return codes: GLE_SUCCESS (0), GLE_RESCALE_RETRY (1, the reference's
+ * throw(1) from RescaleRateHet, trunk/src/tree.cpp:3436-3449), negative values
+ * are errors; gle_last_error() describes the last failure on this thread.
+ */
+#ifndef GLE_H
+#define GLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GLE_SUCCESS 0
+#define GLE_RESCALE_RETRY 1
+#define GLE_ERR_ARG (-1)
+#define GLE_ERR_DEVICE (-2)
+#define GLE_ERR_NOMEM (-3)
+#define GLE_ERR_RESCALE_FATAL (-4)
+#define GLE_ERR_UNSUPPORTED (-5)
+#define GLE_ERR_STATE (-6)
+
+#define GLE_PREC_FP64 0 /* drop-in: FP64 CLAs, reference rescale tables */
+#define GLE_PREC_FP32 1 /* bulk: FP32 CLAs, exact power-of-two rescale counts */
+
+/* bit flags reported by gle_get_rescale_flags */
+#define GLE_FLAG_REDUCE_RESCALE 1 /* some site max < 1e-190 (FP64) */
+#define GLE_FLAG_BAIL_OUT 2       /* some site max < 1e-250 (FP64) */
+
+const char *gle_last_error(void);
+/* Select the HIP device (default 0).  Fails with GLE_ERR_DEVICE if none. */
+int gle_init(int device);
+/* Number of visible HIP devices (0 on a host without GPU). */
+int gle_device_count(void);
+
+/* ---------------------------------------------------------------------------
+ * Model and data helpers (host-side, FP64).  They replace Model::UpdateQMat
+ * (model.cpp:233-294, 442-687, 735-780), Model::CalcEigenStuff (:827-856),
+ * Model::DiscreteGamma (:2170-2201), NucleotideData/AminoacidData::CharToDatum
+ * (sequencedata.cpp:672-780), NucleotideData::MakeAmbigStrings (:278-362) and
+ * PatternManager::ProcessPatterns (datamatr.cpp:288-375).
+ * ------------------------------------------------------------------------- */
+/* GTR: rel = {AC, AG, AT, CG, CT} (GT = 1).  V, Vinv: 4x4 row-major; lam scaled
+ * by the branch-length normaliser (model.cpp:845-846). */
+int gle_model_gtr(const double *pi, const double *rel, double *V, double *Vinv, double *lam);
+/* AA with 190 exchangeabilities in upper-triangle row order (model.cpp:747-760). */
+int gle_model_aa(const double *pi, const double *rel190, double *V, double *Vinv, double *lam);
+/* GY94 codon (61 sense codons, standard code), kappa on transitions, one
+ * eigensystem per omega class: V, Vinv [n_omega][61*61], lam [n_omega][61]. */
+int gle_model_codon(const double *pi61, double kappa, const double *omega, const double *omega_prob,
+                    int n_omega, double *V, double *Vinv, double *lam);
+/* Mean-of-category discrete gamma rates (Yang 1994) as the reference computes them. */
+int gle_discrete_gamma(int n_cat, double alpha, double *rates);
+int gle_nuc_char_to_datum(char c); /* 1..15 bitmask, -1 unknown */
+int gle_aa_char_to_datum(char c);  /* 0..19, 20 = missing, -1 unknown */
+/* reference nucleotide ambiguity stream from 4-bit masks; out holds >= 5*n bytes */
+int gle_make_ambig_string(const unsigned char *masks, int n, signed char *out);
+/* inverse: stream -> masks (1..15); returns bytes consumed or <0 */
+int gle_ambig_string_to_masks(const signed char *stream, int stream_len, int n, unsigned char *masks);
+/* Pattern compression; same contract as PatternManager::ProcessPatterns:
+ * matrix[tax*nsite+site]; out[tax*P+p]; returns P. */
+int gle_compress_patterns(int ntax, int nsite, int max_states, const unsigned char *matrix,
+                          unsigned char *out, int *counts, int *const_states, int *last_const,
+                          int *site_to_pattern);
+
+/* ---------------------------------------------------------------------------
+ * Instance layer.
+ * ------------------------------------------------------------------------- */
+typedef struct gle_instance_desc {
+  int n_tips;       /* taxa */
+  int n_patterns;   /* compressed patterns */
+  int n_states;     /* 4, 20, 61, or any S <= 64 */
+  int n_rate_cats;  /* gamma categories or omega classes (GARLI NRateCats) */
+  int n_eigen;      /* eigensystems (1, or n_omega for codon M3) */
+  int n_cla_slots;  /* directional CLA buffers */
+  int n_pmat_slots; /* transition-matrix slots (each holds P, P', P'') */
+  int precision;    /* GLE_PREC_FP64 or GLE_PREC_FP32 */
+} gle_instance_desc;
+
+/* replaces CalculationManager/ClaManager allocation (calculationmanager.cpp:268-289,
+ * condlike.cpp:122-146) */
+int gle_create_instance(const gle_instance_desc *desc, int *handle);
+int gle_destroy_instance(int handle);
+/* Tip data in the reference's own encodings: a nucleotide ambiguity stream
+ * (sequencedata.cpp:278-362), or for other data one state per pattern with
+ * n_states meaning missing (sequencedata.h:614,677). */
+int gle_set_tip_ambig_stream(int h, int tip, const signed char *stream, int stream_len);
+int gle_set_tip_states(int h, int tip, const unsigned char *states);
+/* counts per pattern; constant-pattern states (nuc bitmask / state index) for
+ * patterns 0..last_const (DataPartition::GetCounts/LastConstant/GetConstStates) */
+int gle_set_patterns(int h, const int *counts, int last_const, const int *const_states);
+/* state freqs [S], category weights [R] (Model::GetRateProbs, model.h:2051-2064), pinv */
+int gle_set_model(int h, const double *pi, const double *rate_probs, double pinv, int pinv_in_model);
+/* eigensystem e: V, Vinv [S*S], lam [S] (already scaled) */
+int gle_set_eigen(int h, int e, const double *V, const double *Vinv, const double *lam);
+/* per category: which eigensystem and the multiplier on lam
+ * (rateMult/(1-pinv) with pinv in the model, rateMult otherwise, 1 for omega classes;
+ * model.cpp:1474-1491) */
+int gle_set_category_rates(int h, const int *eig_of_rate, const double *rate_mult);
+
+/* Model::CalcPmats / CalcDerivatives (model.cpp:866-916, 1271-1358) on the GPU
+ * in FP64: for each i, slot pmat_idx[i] <- P(t[i]) [and P', P'' if with_derivs]. */
+int gle_update_transition_matrices(int h, const int *pmat_idx, const double *t, int n, int with_derivs);
+/* Copy P (and P', P'' if derivs) of one slot back: out [3][R][S][S] */
+int gle_get_transition_matrices(int h, int pmat, double *out);
+
+/* One CLA update (ClaOperation; UpdateCLAs tree.cpp:3905-4001).
+ * child >= 0: CLA slot; child < 0: tip index -(child+1).  rescale != 0 applies
+ * RescaleRateHet[NState] (tree.cpp:3370-3598) after the product. */
+typedef struct gle_cla_op {
+  int dest, child1, pmat1, child2, pmat2, rescale;
+} gle_cla_op;
+/* ops sorted by dependency level; level_start[n_levels+1] gives the ranges
+ * (BlockingOperationsSet, calculationmanager.h:118-123).  n_levels = 0 means
+ * one op per level. */
+int gle_update_partials(int h, const gle_cla_op *ops, int n_ops, const int *level_start, int n_levels);
+/* debugging/parity: one CLA slot back in the reference layout [P][R][S] + u[P] */
+int gle_get_partials(int h, int slot, double *out, int *u);
+
+/* Edge integration (ScoringOperation; GetScorePartial* tree.cpp:5597-6500 and
+ * GetDerivsPartial* optimization.cpp:2412-3613).  child >= 0 CLA slot, < 0 tip.
+ * out[3*i] = {lnL, d1, d2} (d1, d2 zero unless derivs).  site_lnl, if non-NULL,
+ * receives the per-pattern unscaled lnL of op 0. */
+typedef struct gle_edge_op {
+  int partial, child, pmat, derivs;
+} gle_edge_op;
+int gle_edge_likelihoods(int h, const gle_edge_op *ops, int n, double *out, double *site_lnl);
+/* rescale flags accumulated since the last reset (see GLE_FLAG_*) */
+int gle_get_rescale_flags(int h, int *flags, int reset);
+int gle_synchronize(int h);
+/* Per-launch HIP-event timing on the instance's stream.  kind: 0 CLA update
+ * kernels, 1 edge kernels, 2 P(t) kernel.  bytes are algorithmic HBM bytes
+ * (true S, not padded).  No reference counterpart: the reference's Profiler
+ * buckets (tree.cpp:41-47, utility.h:240-323) time whole CPU kernels instead. */
+int gle_set_timing(int h, int enable);
+int gle_get_kernel_stats(int h, int kind, long long *launches, double *ms, double *bytes, int reset);
+
+/* ---------------------------------------------------------------------------
+ * Tree layer: GARLI's scoring surface over one instance.
+ * Nodes 0..2N-3: 0 = root (3 children), 1..N tips, N+1..2N-3 internal;
+ * parent[0] = -1, blen[n] = length of n's edge to its parent.  Children are
+ * ordered by ascending node number (left, [middle], right).
+ * ------------------------------------------------------------------------- */
+int gle_tree_create(int inst, const int *parent, const double *blen, int *tree);
+/* new tree sharing every clean CLA with the source (Tree::CopyClaIndeces,
+ * tree.cpp:9209; copy-on-write through holder refcounts, clamanager.h:324-347) */
+int gle_tree_copy(int tree, int *out);
+int gle_tree_destroy(int tree);
+/* Tree::SetBranchLength (tree.h:425) + SweepDirtynessOverTree (tree.cpp:4247) */
+int gle_tree_set_branch_length(int tree, int node, double blen);
+int gle_tree_get_branch_lengths(int tree, double *blen);
+/* Tree::MakeAllNodesDirty (tree.cpp): every CLA of this tree recomputed on next use */
+int gle_tree_invalidate(int tree);
+/* Tree::Score (tree.cpp:4003-4094), with the rescale retry; *lnl = Tree::lnL */
+int gle_tree_score(int tree, double *lnl);
+/* per-pattern site lnL of the last score (OutputSiteLikelihoods source) */
+int gle_tree_site_likelihoods(int tree, double *site_lnl);
+/* Tree::CalcDerivativesRateHet(nd->anc, nd) (optimization.cpp:1099-1194) */
+int gle_tree_derivatives(int tree, int node, double *lnl, double *d1, double *d2);
+/* Tree::OptimizeAllBranches (optimization.cpp:97-105, 1359-1972) */
+int gle_tree_optimize_all_branches(int tree, double precision, double *improve, double *lnl);
+/* Whole-tree sweep: every directional CLA, then (lnL, d1, d2) on every edge
+ * (edge of node n at out[3*(n-1)]).  The batched form of the reference's
+ * per-edge derivative calls. */
+int gle_tree_sweep(int tree, double *edge_out);
+/* directional CLA X(a|b) in the reference layout (parity/debug) */
+int gle_tree_get_cla(int tree, int a, int b, double *out, int *u);
+/* current rescaleEvery (16 at creation; lowered by 2 per retry) */
+int gle_tree_rescale_every(int tree);
+/* override rescaleEvery (tests force the rescale-retry path from the host) */
+int gle_tree_set_rescale_every(int tree, int every);
+/* Launch schedule for sweeps: 0 = one launch per dependency level,
+ * 1 = one fused launch for the whole op list (pattern-local kernels only). */
+int gle_tree_set_schedule(int tree, int mode);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

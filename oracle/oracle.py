@@ -1,0 +1,303 @@
+"""ctypes front end of the FP64 CPU oracle (oracle/gle_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker, never by the engine.
+Build with `make -C oracle` (or __graft_entry__.build()).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_libs = {}
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+
+
+class OraCtx(C.Structure):
+    _fields_ = [("ntax", C.c_int), ("npat", C.c_int), ("nstates", C.c_int), ("nrates", C.c_int),
+                ("pi", _D), ("rate_probs", _D), ("pinv", C.c_double), ("pinv_in_model", C.c_int),
+                ("counts", _I), ("last_const", C.c_int), ("const_states", _I),
+                ("n_eigen", C.c_int), ("V", _D), ("Vinv", _D), ("lam", _D),
+                ("eig_of_rate", _I), ("rate_mult", _D),
+                ("nuc_tip", C.POINTER(C.POINTER(C.c_int8))), ("ns_tip", C.POINTER(C.POINTER(C.c_uint8))),
+                ("rescale_every", C.c_int), ("min_brlen", C.c_double), ("max_brlen", C.c_double)]
+
+
+class OraTree(C.Structure):
+    _fields_ = [("ntax", C.c_int), ("parent", _I), ("blen", _D)]
+
+
+def lib(fast=False):
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    if name not in _libs:
+        path = os.path.join(HERE, name)
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        l = C.CDLL(path)
+        l.ora_tree_score.argtypes = [C.POINTER(OraCtx), C.POINTER(OraTree), _I, _D, _D]
+        l.ora_tree_derivs.argtypes = [C.POINTER(OraCtx), C.POINTER(OraTree), C.c_int, _I, _D, _D, _D]
+        l.ora_tree_sweep.argtypes = [C.POINTER(OraCtx), C.POINTER(OraTree), _I, _D]
+        l.ora_tree_cla.argtypes = [C.POINTER(OraCtx), C.POINTER(OraTree), C.c_int, C.c_int, _D, _I]
+        l.ora_tree_optimize_all.argtypes = [C.POINTER(OraCtx), C.POINTER(OraTree), C.c_double, _I, _D]
+        l.ora_tree_optimize_all.restype = C.c_double
+        l.ora_pmats.argtypes = [C.POINTER(OraCtx), C.c_double, _D, _D, _D]
+        l.ora_pmat.argtypes = [C.c_int, _D, _D, _D, C.c_double, C.c_double, _D, _D, _D]
+        l.ora_eigen.argtypes = [C.c_int, _D, _D, _D, _D, _D]
+        l.ora_qmat_gtr.argtypes = [_D, _D, _D, _D]
+        l.ora_qmat_aa.argtypes = [_D, _D, _D, _D]
+        l.ora_qmat_codon.argtypes = [_D, C.c_double, _D, _D, C.c_int, _D, _D]
+        l.ora_codon_lookup.argtypes = [_I, _I]
+        l.ora_discrete_gamma.argtypes = [C.c_int, C.c_double, _D]
+        for f in ("ora_lngamma", "ora_point_normal"):
+            getattr(l, f).argtypes = [C.c_double]
+            getattr(l, f).restype = C.c_double
+        l.ora_incomplete_gamma.argtypes = [C.c_double, C.c_double, C.c_double]
+        l.ora_incomplete_gamma.restype = C.c_double
+        l.ora_point_chi2.argtypes = [C.c_double, C.c_double]
+        l.ora_point_chi2.restype = C.c_double
+        l.ora_nuc_char_to_datum.argtypes = [C.c_char]
+        l.ora_aa_char_to_datum.argtypes = [C.c_char]
+        l.ora_make_ambig_string.argtypes = [C.POINTER(C.c_uint8), C.c_int, C.POINTER(C.c_int8)]
+        l.ora_compress_patterns.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint8),
+                                            C.POINTER(C.c_uint8), _I, _I, _I, _I]
+        _libs[name] = l
+    return _libs[name]
+
+
+def _p(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+# ------------------------------------------------------------- model pieces
+def qmat_gtr(pi, rel):
+    Q = np.zeros(16)
+    bm = C.c_double()
+    pi, rel = np.ascontiguousarray(pi, float), np.ascontiguousarray(rel, float)
+    lib().ora_qmat_gtr(_p(pi), _p(rel), _p(Q), C.byref(bm))
+    return Q.reshape(4, 4), bm.value
+
+
+def qmat_aa(pi, rel):
+    Q = np.zeros(400)
+    bm = C.c_double()
+    pi, rel = np.ascontiguousarray(pi, float), np.ascontiguousarray(rel, float)
+    lib().ora_qmat_aa(_p(pi), _p(rel), _p(Q), C.byref(bm))
+    return Q.reshape(20, 20), bm.value
+
+
+def qmat_codon(pi, kappa, omegas, oprobs):
+    nw = len(omegas)
+    Q = np.zeros(nw * 3721)
+    bm = C.c_double()
+    pi = np.ascontiguousarray(pi, float)
+    om, op = np.ascontiguousarray(omegas, float), np.ascontiguousarray(oprobs, float)
+    lib().ora_qmat_codon(_p(pi), float(kappa), _p(om), _p(op), nw, _p(Q), C.byref(bm))
+    return Q.reshape(nw, 61, 61), bm.value
+
+
+def codon_lookup():
+    lk = np.zeros(61 * 61, dtype=np.int32)
+    aa = np.zeros(61, dtype=np.int32)
+    lib().ora_codon_lookup(_p(lk, C.c_int), _p(aa, C.c_int))
+    return lk.reshape(61, 61), aa
+
+
+def eigen(Q, pi):
+    n = len(pi)
+    Qc = np.ascontiguousarray(Q, float).reshape(-1)
+    V, W, lam = np.zeros(n * n), np.zeros(n * n), np.zeros(n)
+    rc = lib().ora_eigen(n, _p(Qc), _p(np.ascontiguousarray(pi, float)), _p(V), _p(W), _p(lam))
+    assert rc == 0
+    return V.reshape(n, n), W.reshape(n, n), lam
+
+
+def pmat(V, W, lam, mult, t, derivs=True):
+    n = len(lam)
+    P, D1, D2 = np.zeros(n * n), np.zeros(n * n), np.zeros(n * n)
+    lib().ora_pmat(n, _p(np.ascontiguousarray(V, float)), _p(np.ascontiguousarray(W, float)),
+                   _p(np.ascontiguousarray(lam, float)), float(mult), float(t), _p(P), _p(D1), _p(D2))
+    return P.reshape(n, n), D1.reshape(n, n), D2.reshape(n, n)
+
+
+def discrete_gamma(k, alpha):
+    r = np.zeros(k)
+    lib().ora_discrete_gamma(k, float(alpha), _p(r))
+    return r
+
+
+def make_ambig_string(masks):
+    m = np.ascontiguousarray(masks, dtype=np.uint8)
+    out = np.zeros(5 * len(m) + 1, dtype=np.int8)
+    n = lib().ora_make_ambig_string(_p(m, C.c_uint8), len(m), _p(out, C.c_int8))
+    return out[:n].copy()
+
+
+def compress_patterns(matrix, max_states):
+    m = np.ascontiguousarray(matrix, dtype=np.uint8)
+    ntax, nsite = m.shape
+    out = np.zeros(ntax * max(nsite, 1), dtype=np.uint8)
+    counts, cs, s2p = np.zeros(nsite, np.int32), np.zeros(nsite, np.int32), np.zeros(nsite, np.int32)
+    lc = C.c_int()
+    P = lib().ora_compress_patterns(ntax, nsite, max_states, _p(m, C.c_uint8), _p(out, C.c_uint8),
+                                    _p(counts, C.c_int), _p(cs, C.c_int), C.byref(lc), _p(s2p, C.c_int))
+    return out[: ntax * P].reshape(ntax, P).copy(), counts[:P].copy(), cs[:P].copy(), lc.value, s2p
+
+
+# ------------------------------------------------------------ scoring ctx
+class Oracle:
+    """Oracle view of a garli_amd.sim.Dataset.
+
+    own_model=True rebuilds Q, the eigensystem and the gamma rates with the
+    oracle's own routines from the model parameters in ds.meta (independent of
+    the engine's host helpers); otherwise the dataset's eigensystems are used.
+    """
+
+    def __init__(self, ds, own_model=True, fast=False, patterns=None):
+        self.l = lib(fast)
+        self.ds = ds
+        tips = ds.tips if patterns is None else ds.tips[:, patterns]
+        counts = ds.counts if patterns is None else ds.counts[patterns]
+        self.P = tips.shape[1]
+        S, R = ds.n_states, ds.n_rates
+        self.S, self.R = S, R
+        self.pi = np.ascontiguousarray(ds.pi, float)
+        self.w = np.ascontiguousarray(ds.rate_probs, float)
+        self.counts = np.ascontiguousarray(counts, np.int32)
+        if patterns is None:
+            self.cs = np.ascontiguousarray(ds.const_states, np.int32)
+            lc = ds.last_const
+        else:  # keep the constant prefix only if the sample keeps it contiguous
+            sel = np.asarray(patterns)
+            isconst = sel <= ds.last_const
+            k = 0
+            while k < len(sel) and isconst[k]:
+                k += 1
+            lc = k - 1
+            self.cs = np.ascontiguousarray(ds.const_states[sel], np.int32)
+        if own_model:
+            eig, mult = self._own_model(ds)
+        else:
+            eig, mult = ds.eig, ds.rate_mult
+        self.V = np.ascontiguousarray(np.stack([np.asarray(e[0]).reshape(S, S) for e in eig]), float)
+        self.W = np.ascontiguousarray(np.stack([np.asarray(e[1]).reshape(S, S) for e in eig]), float)
+        self.lam = np.ascontiguousarray(np.stack([np.asarray(e[2]).reshape(S) for e in eig]), float)
+        self.eor = np.ascontiguousarray(ds.eig_of_rate, np.int32)
+        self.mult = np.ascontiguousarray(mult, float)
+        if S == 4:
+            self.streams = [make_ambig_string(tips[t]) for t in range(ds.n_tips)]
+            arr = (C.POINTER(C.c_int8) * ds.n_tips)(*[_p(s, C.c_int8) for s in self.streams])
+            self._nuc, self._ns = arr, None
+        else:
+            self.states = [np.ascontiguousarray(tips[t], np.uint8) for t in range(ds.n_tips)]
+            arr = (C.POINTER(C.c_uint8) * ds.n_tips)(*[_p(s, C.c_uint8) for s in self.states])
+            self._nuc, self._ns = None, arr
+        c = OraCtx()
+        c.ntax, c.npat, c.nstates, c.nrates = ds.n_tips, self.P, S, R
+        c.pi, c.rate_probs = _p(self.pi), _p(self.w)
+        c.pinv, c.pinv_in_model = ds.pinv, int(ds.pinv_in_model)
+        c.counts, c.last_const, c.const_states = _p(self.counts, C.c_int), lc, _p(self.cs, C.c_int)
+        c.n_eigen, c.V, c.Vinv, c.lam = len(eig), _p(self.V), _p(self.W), _p(self.lam)
+        c.eig_of_rate, c.rate_mult = _p(self.eor, C.c_int), _p(self.mult)
+        if self._nuc is not None:
+            c.nuc_tip = C.cast(self._nuc, C.POINTER(C.POINTER(C.c_int8)))
+        else:
+            c.ns_tip = C.cast(self._ns, C.POINTER(C.POINTER(C.c_uint8)))
+        c.rescale_every, c.min_brlen, c.max_brlen = 16, 1e-8, 100.0
+        self.ctx = c
+
+    def _own_model(self, ds):
+        m = ds.meta
+        if ds.n_states == 4:
+            Q, bm = qmat_gtr(ds.pi, m["rel"])
+            V, W, lam = eigen(Q, ds.pi)
+            eig = [(V, W, lam * bm)]
+            rates = discrete_gamma(ds.n_rates, m["alpha"]) if ds.n_rates > 1 else np.ones(1)
+            mult = rates / (1.0 - ds.pinv) if ds.pinv_in_model else rates
+        elif ds.n_states == 20:
+            Q, bm = qmat_aa(ds.pi, m["rel"])
+            V, W, lam = eigen(Q, ds.pi)
+            eig = [(V, W, lam * bm)]
+            mult = discrete_gamma(ds.n_rates, m["alpha"]) if ds.n_rates > 1 else np.ones(1)
+        else:
+            Qs, bm = qmat_codon(ds.pi, m["kappa"], m["omegas"], ds.rate_probs)
+            eig = []
+            for k in range(len(m["omegas"])):
+                V, W, lam = eigen(Qs[k], ds.pi)
+                eig.append((V, W, lam * bm))
+            mult = np.ones(len(m["omegas"]))
+        return eig, mult
+
+    def _tree(self, parent, blen):
+        self._par = np.ascontiguousarray(parent, np.int32)
+        self._bl = np.ascontiguousarray(blen, float).copy()
+        t = OraTree()
+        t.ntax, t.parent, t.blen = self.ds.n_tips, _p(self._par, C.c_int), _p(self._bl)
+        return t
+
+    def score(self, parent=None, blen=None, rescale_every=16, sites=False):
+        t = self._tree(self.ds.parent if parent is None else parent, self.ds.blen if blen is None else blen)
+        re = C.c_int(rescale_every)
+        lnl = C.c_double()
+        site = np.zeros(self.P)
+        rc = self.l.ora_tree_score(C.byref(self.ctx), C.byref(t), C.byref(re), C.byref(lnl), _p(site))
+        if rc:
+            raise RuntimeError("oracle rescale failure")
+        return (lnl.value, site, re.value) if sites else lnl.value
+
+    def derivs(self, node, parent=None, blen=None, rescale_every=16):
+        t = self._tree(self.ds.parent if parent is None else parent, self.ds.blen if blen is None else blen)
+        re = C.c_int(rescale_every)
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        rc = self.l.ora_tree_derivs(C.byref(self.ctx), C.byref(t), node, C.byref(re), C.byref(a), C.byref(b),
+                                    C.byref(c))
+        if rc:
+            raise RuntimeError("oracle rescale failure")
+        return a.value, b.value, c.value
+
+    def sweep(self, parent=None, blen=None, rescale_every=16):
+        t = self._tree(self.ds.parent if parent is None else parent, self.ds.blen if blen is None else blen)
+        re = C.c_int(rescale_every)
+        out = np.zeros(3 * (2 * self.ds.n_tips - 3))
+        rc = self.l.ora_tree_sweep(C.byref(self.ctx), C.byref(t), C.byref(re), _p(out))
+        if rc:
+            raise RuntimeError("oracle rescale failure")
+        return out.reshape(-1, 3)
+
+    def cla(self, a, b, parent=None, blen=None, rescale_every=16):
+        t = self._tree(self.ds.parent if parent is None else parent, self.ds.blen if blen is None else blen)
+        self.ctx.rescale_every = rescale_every
+        out = np.zeros(self.P * self.R * self.S)
+        u = np.zeros(self.P, np.int32)
+        rc = self.l.ora_tree_cla(C.byref(self.ctx), C.byref(t), a, b, _p(out), _p(u, C.c_int))
+        self.ctx.rescale_every = 16
+        assert rc >= 0
+        return out.reshape(self.P, self.R, self.S), u
+
+    def optimize_all(self, precision, parent=None, blen=None):
+        t = self._tree(self.ds.parent if parent is None else parent, self.ds.blen if blen is None else blen)
+        re = C.c_int(16)
+        lnl = C.c_double()
+        imp = self.l.ora_tree_optimize_all(C.byref(self.ctx), C.byref(t), float(precision), C.byref(re),
+                                           C.byref(lnl))
+        return imp, lnl.value, self._bl.copy()
+
+    def pmats(self, t):
+        n = self.R * self.S * self.S
+        P, D1, D2 = np.zeros(n), np.zeros(n), np.zeros(n)
+        self.l.ora_pmats(C.byref(self.ctx), float(t), _p(P), _p(D1), _p(D2))
+        sh = (self.R, self.S, self.S)
+        return P.reshape(sh), D1.reshape(sh), D2.reshape(sh)
+
+
+def ref_linalg():
+    """The reference's own linalg.cpp (oracle/_ref), or None where it was not built."""
+    path = os.path.join(HERE, "_ref", "libref_linalg.so")
+    if not os.path.exists(path):
+        return None
+    l = C.CDLL(path)
+    l.ref_eigen.argtypes = [C.c_int, _D, _D, _D, _D]
+    l.ref_pmat.argtypes = [C.c_int, _D, _D, _D, C.c_double, _D]
+    return l
