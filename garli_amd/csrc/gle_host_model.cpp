@@ -67,10 +67,12 @@ static int codon_aa(int c) {
   return (a == '*' || !p) ? 20 : int(p - order);
 }
 
-// model.cpp:296-687: one-position changes only; kappa on transitions, omega_w on
-// nonsynonymous changes; normaliser over the omega mixture.
-static void qmat_codon(const double *pi, double kappa, const double *omega, const double *oprob, int nw,
+// model.cpp:296-687: one-position changes only; the changed nucleotide pair's
+// exchangeability (rel6 = AC AG AT CG CT GT), omega_w on nonsynonymous changes;
+// normaliser over the omega mixture.
+static void qmat_codon(const double *pi, const double *rel6, const double *omega, const double *oprob, int nw,
                        double *Q, double *bm) {
+  static const int pair_index[4][4] = {{-1, 0, 1, 2}, {0, -1, 3, 4}, {1, 3, -1, 5}, {2, 4, 5, -1}};
   int sense[61], n = 0;
   for (int c = 0; c < 64; c++)
     if (codon_aa(c) != 20) sense[n++] = c;
@@ -88,8 +90,7 @@ static void qmat_codon(const double *pi, double kappa, const double *omega, cons
             if (xa != xb) { nd++; x = xa; y = xb; }
           }
           if (nd == 1) {
-            v = pi[j];
-            if (((x + y) & 1) == 0) v *= kappa;          // A<->G or C<->T
+            v = pi[j] * rel6[pair_index[x][y]];
             if (codon_aa(a) != codon_aa(b)) v *= omega[w];
           }
         }
@@ -455,12 +456,12 @@ extern "C" int gle_model_aa(const double *pi, const double *rel, double *V, doub
   return GLE_SUCCESS;
 }
 
-extern "C" int gle_model_codon(const double *pi, double kappa, const double *omega, const double *oprob, int nw,
-                               double *V, double *Vinv, double *lam) {
-  if (!pi || !omega || !oprob || nw < 1 || !V || !Vinv || !lam) return set_error(GLE_ERR_ARG, "bad argument");
+extern "C" int gle_model_codon(const double *pi, const double *rel6, const double *omega, const double *oprob,
+                               int nw, double *V, double *Vinv, double *lam) {
+  if (!pi || !rel6 || !omega || !oprob || nw < 1 || !V || !Vinv || !lam) return set_error(GLE_ERR_ARG, "bad argument");
   std::vector<double> Q(size_t(nw) * 61 * 61);
   double bm;
-  qmat_codon(pi, kappa, omega, oprob, nw, Q.data(), &bm);
+  qmat_codon(pi, rel6, omega, oprob, nw, Q.data(), &bm);
   for (int w = 0; w < nw; w++) {
     int rc = reversible_eigen(61, Q.data() + size_t(w) * 3721, pi, V + size_t(w) * 3721, Vinv + size_t(w) * 3721,
                               lam + w * 61);

@@ -58,7 +58,7 @@ SIGNATURES = {
     "gle_device_count": ([], C.c_int),
     "gle_model_gtr": ([_D, _D, _D, _D, _D], C.c_int),
     "gle_model_aa": ([_D, _D, _D, _D, _D], C.c_int),
-    "gle_model_codon": ([_D, C.c_double, _D, _D, C.c_int, _D, _D, _D], C.c_int),
+    "gle_model_codon": ([_D, _D, _D, _D, C.c_int, _D, _D, _D], C.c_int),
     "gle_discrete_gamma": ([C.c_int, C.c_double, _D], C.c_int),
     "gle_nuc_char_to_datum": ([C.c_char], C.c_int),
     "gle_aa_char_to_datum": ([C.c_char], C.c_int),
@@ -159,14 +159,15 @@ def model_aa(pi, rel190):
     return V.reshape(20, 20), W.reshape(20, 20), lam
 
 
-def model_codon(pi61, kappa, omega, omega_prob):
+def model_codon(pi61, rel6, omega, omega_prob):
+    """rel6 = (AC, AG, AT, CG, CT, GT); GY94 with kappa is (1, kappa, 1, 1, kappa, 1)."""
     nw = len(omega)
     V = np.zeros(nw * 3721)
     W = np.zeros(nw * 3721)
     lam = np.zeros(nw * 61)
-    pi, om, op = _d(pi61), _d(omega), _d(omega_prob)
-    _check(lib().gle_model_codon(_p(pi, C.c_double), float(kappa), _p(om, C.c_double), _p(op, C.c_double), nw,
-                                 _p(V, C.c_double), _p(W, C.c_double), _p(lam, C.c_double)))
+    pi, r6, om, op = _d(pi61), _d(rel6), _d(omega), _d(omega_prob)
+    _check(lib().gle_model_codon(_p(pi, C.c_double), _p(r6, C.c_double), _p(om, C.c_double), _p(op, C.c_double),
+                                 nw, _p(V, C.c_double), _p(W, C.c_double), _p(lam, C.c_double)))
     return V.reshape(nw, 61, 61), W.reshape(nw, 61, 61), lam.reshape(nw, 61)
 
 

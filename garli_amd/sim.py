@@ -235,13 +235,14 @@ def codon_m3(n_tips=32, n_sites=None, n_patterns=5000, mean_blen=0.1, kappa=2.5,
     pi = np.maximum(pi, 1e-5)
     pi /= pi.sum()
     op = np.full(len(omegas), 1.0 / len(omegas))
-    V, W, lam = E.model_codon(pi, kappa, np.asarray(omegas, float), op)
+    rel6 = np.array([1.0, kappa, 1.0, 1.0, kappa, 1.0])
+    V, W, lam = E.model_codon(pi, rel6, np.asarray(omegas, float), op)
     eig = [(V[k], W[k], lam[k]) for k in range(len(omegas))]
     eor = np.arange(len(omegas), dtype=np.int32)
     mult = np.ones(len(omegas))
     tips = _simulate_until(parent, blen, pi, eig, eor, mult, op, 0.0, n_tips, rng, n_sites, n_patterns,
                            lambda s: s.astype(np.uint8))
-    meta = dict(kappa=kappa, omegas=omegas, seed=seed)
+    meta = dict(kappa=kappa, rel6=rel6, omegas=omegas, seed=seed)
     return _finish(name, parent, blen, tips, 61, pi, op, 0.0, False, eig, eor, mult, meta)
 
 

@@ -60,9 +60,11 @@ int gle_device_count(void);
 int gle_model_gtr(const double *pi, const double *rel, double *V, double *Vinv, double *lam);
 /* AA with 190 exchangeabilities in upper-triangle row order (model.cpp:747-760). */
 int gle_model_aa(const double *pi, const double *rel190, double *V, double *Vinv, double *lam);
-/* GY94 codon (61 sense codons, standard code), kappa on transitions, one
- * eigensystem per omega class: V, Vinv [n_omega][61*61], lam [n_omega][61]. */
-int gle_model_codon(const double *pi61, double kappa, const double *omega, const double *omega_prob,
+/* Codon model (61 sense codons, standard code): single-nucleotide changes with
+ * exchangeabilities rel6 = {AC, AG, AT, CG, CT, GT} (GY94 with kappa is
+ * {1, kappa, 1, 1, kappa, 1}), omega on nonsynonymous changes; one eigensystem
+ * per omega class: V, Vinv [n_omega][61*61], lam [n_omega][61]. */
+int gle_model_codon(const double *pi61, const double *rel6, const double *omega, const double *omega_prob,
                     int n_omega, double *V, double *Vinv, double *lam);
 /* Mean-of-category discrete gamma rates (Yang 1994) as the reference computes them. */
 int gle_discrete_gamma(int n_cat, double alpha, double *rates);

@@ -111,8 +111,10 @@ void ora_codon_lookup(int *lookup, int *aa_of_codon61) {
   }
 }
 
-/* model.cpp:442-687 with nst = 2 (kappa on transitions) and one Q per omega class */
-void ora_qmat_codon(const double *pi, double kappa, const double *omega, const double *omega_prob,
+/* model.cpp:442-687: Q_ij = pi_j * r(pair changed) * (omega_w if nonsynonymous) for
+ * single-nucleotide changes; rel6 = {AC, AG, AT, CG, CT, GT} (nst = 6; nst = 2 is
+ * rel6 = {1, kappa, 1, 1, kappa, 1}); one Q per omega class. */
+void ora_qmat_codon(const double *pi, const double *rel6, const double *omega, const double *omega_prob,
                     int nw, double *Q, double *blen_mult) {
   const int n = 61;
   static int lookup[61 * 61];
@@ -124,7 +126,12 @@ void ora_qmat_codon(const double *pi, double kappa, const double *omega, const d
         int l = lookup[i * n + j];
         if (l == 0) { q[i * n + j] = 0.0; continue; }
         q[i * n + j] = pi[j];
-        if (l & 2) q[i * n + j] *= kappa;
+        if (l & 8) q[i * n + j] *= rel6[0];
+        else if (l & 16) q[i * n + j] *= rel6[1];
+        else if (l & 32) q[i * n + j] *= rel6[2];
+        else if (l & 64) q[i * n + j] *= rel6[3];
+        else if (l & 128) q[i * n + j] *= rel6[4];
+        else if (l & 256) q[i * n + j] *= rel6[5];
         if (l & 4) q[i * n + j] *= omega[w];
       }
   }

@@ -33,7 +33,7 @@ extern "C" {
 void ora_qmat_gtr(const double pi[4], const double rel[5], double *Q, double *blen_mult);
 void ora_qmat_aa(const double pi[20], const double *rel190, double *Q, double *blen_mult);
 void ora_codon_lookup(int *lookup61x61, int *aa_of_codon61);
-void ora_qmat_codon(const double *pi61, double kappa, const double *omega, const double *omega_prob,
+void ora_qmat_codon(const double *pi61, const double *rel6, const double *omega, const double *omega_prob,
                     int n_omega, double *Q /* n_omega*61*61 */, double *blen_mult);
 /* Reversible eigensystem of Q (symmetrised by pi, cyclic Jacobi).  Returns 0 on success. */
 int ora_eigen(int n, const double *Q, const double *pi, double *V, double *Vinv, double *lam);

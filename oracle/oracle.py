@@ -48,7 +48,7 @@ def lib(fast=False):
         l.ora_eigen.argtypes = [C.c_int, _D, _D, _D, _D, _D]
         l.ora_qmat_gtr.argtypes = [_D, _D, _D, _D]
         l.ora_qmat_aa.argtypes = [_D, _D, _D, _D]
-        l.ora_qmat_codon.argtypes = [_D, C.c_double, _D, _D, C.c_int, _D, _D]
+        l.ora_qmat_codon.argtypes = [_D, _D, _D, _D, C.c_int, _D, _D]
         l.ora_codon_lookup.argtypes = [_I, _I]
         l.ora_discrete_gamma.argtypes = [C.c_int, C.c_double, _D]
         for f in ("ora_lngamma", "ora_point_normal"):
@@ -88,13 +88,13 @@ def qmat_aa(pi, rel):
     return Q.reshape(20, 20), bm.value
 
 
-def qmat_codon(pi, kappa, omegas, oprobs):
+def qmat_codon(pi, rel6, omegas, oprobs):
     nw = len(omegas)
     Q = np.zeros(nw * 3721)
     bm = C.c_double()
-    pi = np.ascontiguousarray(pi, float)
+    pi, r6 = np.ascontiguousarray(pi, float), np.ascontiguousarray(rel6, float)
     om, op = np.ascontiguousarray(omegas, float), np.ascontiguousarray(oprobs, float)
-    lib().ora_qmat_codon(_p(pi), float(kappa), _p(om), _p(op), nw, _p(Q), C.byref(bm))
+    lib().ora_qmat_codon(_p(pi), _p(r6), _p(om), _p(op), nw, _p(Q), C.byref(bm))
     return Q.reshape(nw, 61, 61), bm.value
 
 
@@ -222,7 +222,7 @@ class Oracle:
             eig = [(V, W, lam * bm)]
             mult = discrete_gamma(ds.n_rates, m["alpha"]) if ds.n_rates > 1 else np.ones(1)
         else:
-            Qs, bm = qmat_codon(ds.pi, m["kappa"], m["omegas"], ds.rate_probs)
+            Qs, bm = qmat_codon(ds.pi, m["rel6"], m["omegas"], ds.rate_probs)
             eig = []
             for k in range(len(m["omegas"])):
                 V, W, lam = eigen(Qs[k], ds.pi)

@@ -1,0 +1,77 @@
+"""Extract the reference's own test data into a committed fixture.
+
+Reads (as data, never executed) garli/trunk/tests/data/ from the reference
+checkout: the 11-taxon alignments z.11x2178.nex / z.11x2178.AA.nex, the
+starting trees *.start with their [!GarliScore] and [!GarliModel] comments,
+and tests/data/expected.scr.  Writes tests/golden/garli_testdata.json.
+
+    python tests/golden/make_fixtures.py [/path/to/garli/trunk/tests/data]
+"""
+import json
+import os
+import re
+import sys
+
+DEFAULT = "/root/reference/garli/trunk/tests/data"
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "garli_testdata.json")
+
+
+def read_matrix(path):
+    txt = open(path).read()
+    m = re.search(r"\bmatrix\b(.*?);", txt, re.S | re.I)
+    names, seqs = [], {}
+    for line in m.group(1).splitlines():
+        line = line.strip()
+        if not line or line.startswith("["):
+            continue
+        parts = line.split()
+        name, seq = parts[0], "".join(parts[1:])
+        if name not in seqs:
+            names.append(name)
+            seqs[name] = ""
+        seqs[name] += seq
+    return names, [seqs[n] for n in names]
+
+
+def read_start(path):
+    txt = open(path).read()
+    tr = re.search(r"translate(.*?);", txt, re.S)
+    translate = {}
+    for item in tr.group(1).split(","):
+        k, v = item.split()
+        translate[int(k)] = v
+    t = re.search(r"tree\s+\S+\s*=\s*(.*?;)", txt, re.S).group(1)
+    score = re.search(r"GarliScore\s+(-?[0-9.]+)", t)
+    model = re.search(r"GarliModel\s+([^\]]*)\]", t)
+    newick = re.sub(r"\[.*?\]", "", t).strip()
+    return {"translate": translate, "newick": newick,
+            "score": float(score.group(1)) if score else None,
+            "model": model.group(1).strip() if model else None}
+
+
+def main():
+    src = sys.argv[1] if len(sys.argv) > 1 else DEFAULT
+    names, nuc = read_matrix(os.path.join(src, "z.11x2178.nex"))
+    anames, aa = read_matrix(os.path.join(src, "z.11x2178.AA.nex"))
+    starts = {}
+    for f in sorted(os.listdir(src)):
+        if f.endswith(".start"):
+            try:
+                starts[f[:-6]] = read_start(os.path.join(src, f))
+            except (AttributeError, ValueError):
+                pass
+    expected = {}
+    for line in open(os.path.join(src, "expected.scr")):
+        p = line.split()
+        if len(p) == 2:
+            expected[p[0]] = float(p[1])
+    out = {"source": "garli/trunk/tests/data (reference test fixtures)",
+           "taxa": names, "nuc_matrix": nuc, "aa_taxa": anames, "aa_matrix": aa,
+           "starts": starts, "expected_scr": expected}
+    with open(OUT, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print("wrote", OUT, len(nuc), "taxa x", len(nuc[0]), "nt;", len(starts), "start trees")
+
+
+if __name__ == "__main__":
+    main()
