@@ -7,7 +7,8 @@
 //   u      i32 [slots][Ppad]       per-pattern rescale counts (FP64: natural-log units
 //                                   exactly as the reference; FP32: powers of two)
 //   pm64   f64 [pmats][3][R][S][S] P, P', P'' (row = parent state, model.cpp:1271-1358)
-//   pm32   f32 [pmats][R][S][S]    FP32 copy of P for the FP32-bulk CLA kernels
+//   pm32   f32 [pmats][R][Sp][Sp]  FP32 copy of P for the FP32-bulk CLA kernels
+//                                   (codon: zero-padded to 64 x 64 for MFMA tiles)
 //   counts i32 [Ppad], cst i32 [Ppad] (constant-pattern state/mask, -1 otherwise)
 // Ppad rounds P up to a multiple of 256 so every 256-thread tile is whole.
 #include <hip/hip_runtime.h>
@@ -311,7 +312,7 @@ __device__ __forceinline__ int rescale_slice(T *v, int RP, int *flags, bool grou
 // P(t), P', P'' (model.cpp:1271-1358, 1469-1597) -- FP64
 // grid (n, R); V/Vinv of the category's eigensystem staged in LDS.
 // =====================================================================
-__global__ __launch_bounds__(256) void k_pmat(int S, int R, const int *__restrict__ idx,
+__global__ __launch_bounds__(256) void k_pmat(int S, int Sp, int R, const int *__restrict__ idx,
                                               const double *__restrict__ tt, const int *__restrict__ eig_of_rate,
                                               const double *__restrict__ rate_mult, const double *__restrict__ V,
                                               const double *__restrict__ Vinv, const double *__restrict__ lam,
@@ -337,7 +338,7 @@ __global__ __launch_bounds__(256) void k_pmat(int S, int R, const int *__restric
   double *P = pm64 + (size_t(slot) * 3 * R + r) * SS;
   double *D1 = pm64 + (size_t(slot) * 3 * R + R + r) * SS;
   double *D2 = pm64 + (size_t(slot) * 3 * R + 2 * R + r) * SS;
-  float *P32 = pm32 + (size_t(slot) * R + r) * SS;
+  float *P32 = pm32 + (size_t(slot) * R + r) * Sp * Sp;
   for (int q = threadIdx.x; q < SS; q += blockDim.x) {
     const int i = q / S, j = q - i * S;
     double sp = 0.0, s1 = 0.0, s2 = 0.0;
@@ -351,7 +352,7 @@ __global__ __launch_bounds__(256) void k_pmat(int S, int R, const int *__restric
     }
     const double pv = sp > 0.0 ? sp : 0.0;
     P[q] = pv;
-    P32[q] = float(pv);
+    P32[i * Sp + j] = float(pv);
     if (derivs) {
       D1[q] = s1;
       D2[q] = s2;
@@ -448,6 +449,7 @@ struct NsClaArgs {
   const T *pm;
   size_t pm_stride, slot_stride;
   int Ppad, P, R, Sp;
+  int prow;  // row stride of P: S for FP64 matrices, Sp for the padded FP32 copy
   int *flags;
 };
 
@@ -463,7 +465,7 @@ __global__ __launch_bounds__(256) void k_cla_ns(NsClaArgs<T> a, const DevClaOp *
   const bool live = (p < a.P) && (r < R);
   const int rr = r < R ? r : R - 1;
   const int first = fused_n ? 0 : blockIdx.y, last = fused_n ? fused_n : blockIdx.y + 1;
-  const int RSS = R * S * S;
+  const int PR = a.prow, RSS = R * PR * PR;
   for (int oi = first; oi < last; ++oi) {
     const DevClaOp op = ops[oi];
     const T *P1, *P2;
@@ -474,11 +476,11 @@ __global__ __launch_bounds__(256) void k_cla_ns(NsClaArgs<T> a, const DevClaOp *
         sP[RSS + q] = a.pm[op.pm2 * a.pm_stride + q];
       }
       __syncthreads();
-      P1 = sP + rr * S * S;
-      P2 = sP + RSS + rr * S * S;
+      P1 = sP + rr * PR * PR;
+      P2 = sP + RSS + rr * PR * PR;
     } else {
-      P1 = a.pm + op.pm1 * a.pm_stride + rr * S * S;
-      P2 = a.pm + op.pm2 * a.pm_stride + rr * S * S;
+      P1 = a.pm + op.pm1 * a.pm_stride + rr * PR * PR;
+      P2 = a.pm + op.pm2 * a.pm_stride + rr * PR * PR;
     }
     T x1[S], x2[S], out[S];
     int t1 = S, t2 = S, u = 0;
@@ -503,20 +505,20 @@ __global__ __launch_bounds__(256) void k_cla_ns(NsClaArgs<T> a, const DevClaOp *
           L1 = T(0);
 #pragma unroll
           for (int q = 0; q < S; q++) {
-            if constexpr (sizeof(T) == 8) L1 = add_nc(L1, mul_nc(P1[f * S + q], x1[q]));
-            else L1 = __builtin_fmaf(P1[f * S + q], x1[q], L1);
+            if constexpr (sizeof(T) == 8) L1 = add_nc(L1, mul_nc(P1[f * PR + q], x1[q]));
+            else L1 = __builtin_fmaf(P1[f * PR + q], x1[q], L1);
           }
         } else
-          L1 = t1 < S ? P1[f * S + t1] : T(1);
+          L1 = t1 < S ? P1[f * PR + t1] : T(1);
         if (op.c2 >= 0) {
           R1 = T(0);
 #pragma unroll
           for (int q = 0; q < S; q++) {
-            if constexpr (sizeof(T) == 8) R1 = add_nc(R1, mul_nc(P2[f * S + q], x2[q]));
-            else R1 = __builtin_fmaf(P2[f * S + q], x2[q], R1);
+            if constexpr (sizeof(T) == 8) R1 = add_nc(R1, mul_nc(P2[f * PR + q], x2[q]));
+            else R1 = __builtin_fmaf(P2[f * PR + q], x2[q], R1);
           }
         } else
-          R1 = t2 < S ? P2[f * S + t2] : T(1);
+          R1 = t2 < S ? P2[f * PR + t2] : T(1);
         if constexpr (sizeof(T) == 8) out[f] = mul_nc(L1, R1);
         else out[f] = L1 * R1;
       }
@@ -529,6 +531,130 @@ __global__ __launch_bounds__(256) void k_cla_ns(NsClaArgs<T> a, const DevClaOp *
 #pragma unroll
       for (int f = 0; f < S; f++) a.cla[op.dest * a.slot_stride + off + f] = out[f];
       if (r == 0) a.u[op.dest * size_t(a.Ppad) + p] = u;
+    }
+  }
+}
+
+
+// =====================================================================
+// Codon CLA update, FP32 bulk form, on MFMA (61 states padded to 64).
+// One wave owns 16 patterns of an op and all R categories.  For each
+// (category, child) the 64x64 contraction out[p][i] = sum_j P[i][j] x[p][j]
+// runs as 4 output tiles x 16 steps of v_mfma_f32_16x16x4_f32.  The K index
+// is permuted so that, at step kk, the four K rows of the instruction are
+// j = 16g + kk (g = lane/16): each lane's A and B fragments are then 16
+// contiguous floats of one CLA row / one P row (four 16-byte loads).
+// MFMA D layout: lane l, element v holds row 4*(l/16)+v (pattern), column l%16.
+// =====================================================================
+typedef float gle_floatx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void codon_child_f32(const NsClaArgs<float> &a, int c, int pm, int r, int R, int lo,
+                                                int g, int p0, gle_floatx4 acc[4], int &u_add) {
+  const float *P = a.pm + size_t(pm) * a.pm_stride + size_t(r) * 64 * 64;
+  if (c >= 0) {
+    float x[16];
+    const int p = p0 + lo;
+    if (p < a.P) {
+      const float4 *src = reinterpret_cast<const float4 *>(a.cla + size_t(c) * a.slot_stride +
+                                                           (size_t(p) * R + r) * 64 + 16 * g);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const float4 v = src[q];
+        x[4 * q] = v.x; x[4 * q + 1] = v.y; x[4 * q + 2] = v.z; x[4 * q + 3] = v.w;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 16; q++) x[q] = 0.0f;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      float b[16];
+      const float4 *ps = reinterpret_cast<const float4 *>(P + (16 * t + lo) * 64 + 16 * g);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const float4 v = ps[q];
+        b[4 * q] = v.x; b[4 * q + 1] = v.y; b[4 * q + 2] = v.z; b[4 * q + 3] = v.w;
+      }
+      gle_floatx4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int kk = 0; kk < 16; kk++) d = __builtin_amdgcn_mfma_f32_16x16x4f32(x[kk], b[kk], d, 0, 0, 0);
+      acc[t] = d;
+    }
+    if (r == 0) u_add = 1;
+  } else {
+    const unsigned char *tip = a.tips + size_t(-c - 1) * a.Ppad;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int p = p0 + 4 * g + v;
+      const int st = p < a.P ? tip[p] : 61;
+#pragma unroll
+      for (int t = 0; t < 4; t++) acc[t][v] = st < 61 ? P[(16 * t + lo) * 64 + st] : 1.0f;
+    }
+  }
+}
+
+template <int R>
+__global__ __launch_bounds__(64) void k_cla_codon_mfma(NsClaArgs<float> a, const DevClaOp *__restrict__ ops,
+                                                       int fused_n) {
+  const int l = threadIdx.x, lo = l & 15, g = l >> 4;
+  const int p0 = blockIdx.x * 16;
+  const int first = fused_n ? 0 : blockIdx.y, last = fused_n ? fused_n : blockIdx.y + 1;
+  for (int oi = first; oi < last; ++oi) {
+    if (oi != first) __syncthreads();  // make the previous op's stores visible to every lane
+    const DevClaOp op = ops[oi];
+    float out[R][4][4];
+    int u1 = 0, u2 = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      gle_floatx4 dl[4], dr[4];
+      codon_child_f32(a, op.c1, op.pm1, r, R, lo, g, p0, dl, u1);
+      codon_child_f32(a, op.c2, op.pm2, r, R, lo, g, p0, dr, u2);
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) out[r][t][v] = (16 * t + lo < 61) ? dl[t][v] * dr[t][v] : 0.0f;
+    }
+    int uv[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int p = p0 + 4 * g + v;
+      int u = 0;
+      if (p < a.P) {
+        if (op.c1 >= 0) u += a.u[size_t(op.c1) * a.Ppad + p];
+        if (op.c2 >= 0) u += a.u[size_t(op.c2) * a.Ppad + p];
+      }
+      uv[v] = u;
+    }
+    if (op.flags & kOpRescale) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        float m = 0.0f;
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+          for (int t = 0; t < 4; t++) m = fmaxf(m, out[r][t][v]);
+        for (int o = 1; o < 16; o <<= 1) m = fmaxf(m, __shfl_xor(m, o));
+        if (p0 + 4 * g + v < a.P && m < 0x1p-32f && m > 0.0f) {
+          const int e = -ilogbf(m);
+#pragma unroll
+          for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) out[r][t][v] = ldexpf(out[r][t][v], e);
+          uv[v] += e;
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int p = p0 + 4 * g + v;
+      if (p < a.P) {
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+          for (int t = 0; t < 4; t++)
+            a.cla[size_t(op.dest) * a.slot_stride + (size_t(p) * R + r) * 64 + 16 * t + lo] = out[r][t][v];
+        if (lo == 0) a.u[size_t(op.dest) * a.Ppad + p] = uv[v];
+      }
     }
   }
 }
@@ -842,7 +968,7 @@ int engine_update_pmats(Instance *I, const int *idx, const double *t, int n, int
   const int S = I->S;
   const size_t shmem = sizeof(double) * (2 * S * S + 3 * S);
   timed_begin(I, 2, double(n) * I->R * S * S * 8.0 * (derivs ? 3 : 1));
-  hipLaunchKernelGGL(k_pmat, dim3(n, I->R), dim3(256), shmem, I->stream, S, I->R, I->d_pmidx, I->d_pmt,
+  hipLaunchKernelGGL(k_pmat, dim3(n, I->R), dim3(256), shmem, I->stream, S, I->Sp, I->R, I->d_pmidx, I->d_pmt,
                      I->d_eig_of_rate, I->d_model + S + I->R, I->d_V, I->d_Vinv, I->d_lam, I->d_pm64, I->d_pm32,
                      derivs);
   timed_end(I);
@@ -892,21 +1018,49 @@ static void launch_ns(Instance *I, const DevClaOp *d_ops, int n, bool fused, int
   a.u = I->d_u;
   a.tips = I->d_tips;
   a.pm = sizeof(T) == 8 ? reinterpret_cast<const T *>(I->d_pm64) : reinterpret_cast<const T *>(I->d_pm32);
-  a.pm_stride = sizeof(T) == 8 ? size_t(3) * I->R * S * S : size_t(I->R) * S * S;
+  a.prow = sizeof(T) == 8 ? S : I->Sp;
+  a.pm_stride = sizeof(T) == 8 ? size_t(3) * I->R * S * S : size_t(I->R) * I->Sp * I->Sp;
   a.slot_stride = I->slot_elems();
   a.Ppad = I->Ppad;
   a.P = I->P;
   a.R = I->R;
   a.Sp = I->Sp;
   a.flags = I->d_flags;
-  const size_t shmem = S <= 32 ? sizeof(T) * 2 * I->R * S * S : 0;
+  const size_t shmem = S <= 32 ? sizeof(T) * 2 * I->R * a.prow * a.prow : 0;
   hipLaunchKernelGGL((k_cla_ns<T, S>), dim3(nblk, fused ? 1 : n), dim3(256), shmem, I->stream, a, d_ops,
                      fused ? n : 0);
+}
+
+static int launch_codon_mfma(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
+  NsClaArgs<float> a;
+  a.cla = reinterpret_cast<float *>(I->d_cla);
+  a.u = I->d_u;
+  a.tips = I->d_tips;
+  a.pm = I->d_pm32;
+  a.prow = 64;
+  a.pm_stride = size_t(I->R) * 64 * 64;
+  a.slot_stride = I->slot_elems();
+  a.Ppad = I->Ppad;
+  a.P = I->P;
+  a.R = I->R;
+  a.Sp = I->Sp;
+  a.flags = I->d_flags;
+  const dim3 grid(I->Ppad / 16, fused ? 1 : n), block(64);
+  const int fn = fused ? n : 0;
+  switch (I->R) {
+    case 1: hipLaunchKernelGGL(k_cla_codon_mfma<1>, grid, block, 0, I->stream, a, d_ops, fn); break;
+    case 2: hipLaunchKernelGGL(k_cla_codon_mfma<2>, grid, block, 0, I->stream, a, d_ops, fn); break;
+    case 3: hipLaunchKernelGGL(k_cla_codon_mfma<3>, grid, block, 0, I->stream, a, d_ops, fn); break;
+    case 4: hipLaunchKernelGGL(k_cla_codon_mfma<4>, grid, block, 0, I->stream, a, d_ops, fn); break;
+    default: return set_error(GLE_ERR_UNSUPPORTED, "codon MFMA path takes 1..4 categories");
+  }
+  return GLE_SUCCESS;
 }
 
 template <typename T>
 static int launch_cla(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
   const int nblk = blocks_for(I);
+  if (sizeof(T) == 4 && I->S == 61 && I->R <= 4) return launch_codon_mfma(I, d_ops, n, fused);
   switch (I->S) {
     case 4: return launch_nuc_r<T>(I, d_ops, n, fused, nblk);
     case 20: launch_ns<T, 20>(I, d_ops, n, fused, nblk); return GLE_SUCCESS;
@@ -1162,7 +1316,7 @@ extern "C" int gle_create_instance(const gle_instance_desc *d, int *handle) {
   GLE_ALLOC(I->d_eig_of_rate, sizeof(int) * R);
   GLE_ALLOC(I->d_cla, I->slot_elems() * I->elem_bytes() * I->n_slots);
   GLE_ALLOC(I->d_pm64, sizeof(double) * size_t(I->n_pm) * 3 * R * S * S);
-  GLE_ALLOC(I->d_pm32, sizeof(float) * size_t(I->n_pm) * R * S * S);
+  GLE_ALLOC(I->d_pm32, sizeof(float) * size_t(I->n_pm) * R * I->Sp * I->Sp);
   GLE_ALLOC(I->d_V, sizeof(double) * size_t(I->n_eig) * S * S);
   GLE_ALLOC(I->d_Vinv, sizeof(double) * size_t(I->n_eig) * S * S);
   GLE_ALLOC(I->d_lam, sizeof(double) * size_t(I->n_eig) * S);
@@ -1177,7 +1331,7 @@ extern "C" int gle_create_instance(const gle_instance_desc *d, int *handle) {
       hipMemset(I->d_flags, 0, sizeof(int)) != hipSuccess ||
       hipMemset(I->d_cla, 0, I->slot_elems() * I->elem_bytes() * I->n_slots) != hipSuccess ||
       hipMemset(I->d_pm64, 0, sizeof(double) * size_t(I->n_pm) * 3 * R * S * S) != hipSuccess ||
-      hipMemset(I->d_pm32, 0, sizeof(float) * size_t(I->n_pm) * R * S * S) != hipSuccess)
+      hipMemset(I->d_pm32, 0, sizeof(float) * size_t(I->n_pm) * R * I->Sp * I->Sp) != hipSuccess)
     return fail(set_error(GLE_ERR_DEVICE, "device memset failed"));
   std::vector<int> eor(R, 0);
   std::vector<double> model(S + 2 * R, 0.0);
