@@ -107,6 +107,7 @@ struct Instance {
     double bytes;
   };
   std::vector<Pending> pending;
+  std::mutex mu;  // one caller at a time per instance (staging buffers, stream)
   size_t slot_elems() const { return size_t(Ppad) * R * Sp; }
   size_t elem_bytes() const { return prec == GLE_PREC_FP64 ? 8 : 4; }
 };
@@ -949,6 +950,7 @@ static int blocks_for(const Instance *I) { return I->Ppad / (256 / lanes_for(I->
 bool engine_supports_fused(const Instance *I) { return I->S == 4 || I->S == 20; }
 
 int engine_update_pmats(Instance *I, const int *idx, const double *t, int n, int derivs) {
+  std::lock_guard<std::mutex> lk_(I->mu);
   if (n <= 0) return GLE_SUCCESS;
   for (int i = 0; i < n; i++)
     if (idx[i] < 0 || idx[i] >= I->n_pm || !(t[i] >= 0.0))
@@ -1071,6 +1073,7 @@ static int launch_cla(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
 
 int engine_update_partials(Instance *I, const DevClaOp *ops, int n_ops, const int *level_start, int n_levels,
                            bool fused) {
+  std::lock_guard<std::mutex> lk_(I->mu);
   if (n_ops <= 0) return GLE_SUCCESS;
   for (int i = 0; i < n_ops; i++) {
     const DevClaOp &o = ops[i];
@@ -1155,6 +1158,7 @@ static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk) {
 }
 
 int engine_edges(Instance *I, const DevEdgeOp *ops, int n, double *out, double *site_lnl) {
+  std::lock_guard<std::mutex> lk_(I->mu);
   if (n <= 0) return GLE_SUCCESS;
   for (int i = 0; i < n; i++) {
     const DevEdgeOp &o = ops[i];
@@ -1195,6 +1199,7 @@ int engine_edges(Instance *I, const DevEdgeOp *ops, int n, double *out, double *
 }
 
 int engine_flags(Instance *I, int *flags, int reset) {
+  std::lock_guard<std::mutex> lk_(I->mu);
   GLE_HIP(hipMemcpyAsync(flags, I->d_flags, sizeof(int), hipMemcpyDeviceToHost, I->stream));
   GLE_HIP(hipStreamSynchronize(I->stream));
   if (reset) {
@@ -1206,6 +1211,7 @@ int engine_flags(Instance *I, int *flags, int reset) {
 }
 
 int engine_get_partials(Instance *I, int slot, double *out, int *u) {
+  std::lock_guard<std::mutex> lk_(I->mu);
   if (slot < 0 || slot >= I->n_slots) return set_error(GLE_ERR_ARG, "slot %d out of range", slot);
   const size_t W = size_t(I->P) * I->R * I->Sp;
   std::vector<unsigned char> tmp(W * I->elem_bytes());
@@ -1223,6 +1229,7 @@ int engine_get_partials(Instance *I, int slot, double *out, int *u) {
 }
 
 int engine_sync(Instance *I) {
+  std::lock_guard<std::mutex> lk_(I->mu);
   GLE_HIP(hipStreamSynchronize(I->stream));
   return GLE_SUCCESS;
 }

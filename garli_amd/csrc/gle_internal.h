@@ -2,6 +2,7 @@
 #pragma once
 #include <cstddef>
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -60,5 +61,17 @@ int engine_flags(Instance *, int *flags, int reset);
 int engine_get_partials(Instance *, int slot, double *out, int *u);
 int engine_sync(Instance *);
 bool engine_supports_fused(const Instance *);
+
+// Slot allocator shared by the tree and serving layers (one per instance):
+// CLA slots with reference counts and the rescale rank of the CLA each holds,
+// and transition-matrix slots.  Guarded by pool_mutex().
+struct ClaPool {
+  std::vector<int> refs;
+  std::vector<int> free_slots;
+  std::vector<int> rank;
+  std::vector<int> pm_free;
+};
+ClaPool *pool_for(Instance *I);
+std::mutex &pool_mutex();
 
 }  // namespace gle

@@ -1,0 +1,610 @@
+// gle_serve.cpp -- continuous-batching scoring service for a GA population.
+//
+// GARLI scores its population one individual at a time (population.cpp:4555);
+// each individual is a parent tree plus a topology move and a branch-length
+// pass (Individual::Mutate, individual.cpp:127-222).  The service takes such
+// requests from any number of client threads and runs them together:
+//
+//  * CLA cache.  Every directional CLA X(a|b) is keyed by a 128-bit signature
+//    of the subtree on a's side (tip identities, topology, exact branch-length
+//    bits).  Any request that needs a subtree some earlier request computed --
+//    the unmodified part of a perturbed parent, or a subtree shared by other
+//    individuals -- reuses the device slot.  This generalises ClaManager's
+//    holder sharing (clamanager.h:96-441): a changed branch changes the
+//    signatures of exactly the CLAs SweepDirtynessOverTree would dirty.
+//  * P(t) cache keyed by branch-length bits (P, P', P'' per slot).
+//  * Admission by HBM budget: a request is admitted when its working set fits
+//    in the instance's CLA slots alongside the requests already running;
+//    least-recently-used cache entries are evicted on demand.
+//  * Continuous batching: each tick gathers the next step of every admitted
+//    request (a score, or one Newton iteration of a branch-length pass),
+//    dedupes shared CLAs, and issues one P(t) launch, one CLA launch per
+//    dependency level and one edge launch for all of them.  Requests that
+//    arrive while a tick runs join the next tick.
+// Per-request results do not depend on what else is in the batch: every CLA
+// and every edge reduction is computed independently and deterministically.
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "gle_internal.h"
+#include "gle_newton.h"
+
+namespace gle {
+
+struct Sig {
+  uint64_t a = 0, b = 0;
+  bool operator==(const Sig &o) const { return a == o.a && b == o.b; }
+};
+struct SigHash {
+  size_t operator()(const Sig &s) const { return size_t(s.a ^ (s.b * 0x9E3779B97F4A7C15ull)); }
+};
+
+static inline uint64_t mix64(uint64_t x) {  // SplitMix64 finaliser
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+static inline uint64_t dbits(double v) {
+  uint64_t u;
+  std::memcpy(&u, &v, sizeof u);
+  return u;
+}
+static Sig tip_sig(int tip) {
+  Sig s;
+  s.a = mix64(0x7469700000000000ull + uint64_t(tip));
+  s.b = mix64(s.a ^ 0x5eed5eed5eed5eedull);
+  return s;
+}
+// order-independent combination of two (child signature, branch length) pairs
+static Sig combine(Sig x, double bx, Sig y, double by) {
+  uint64_t hx = mix64(x.a ^ mix64(dbits(bx))), hy = mix64(y.a ^ mix64(dbits(by)));
+  uint64_t gx = mix64(x.b ^ mix64(dbits(bx) + 1)), gy = mix64(y.b ^ mix64(dbits(by) + 1));
+  if (hx > hy || (hx == hy && gx > gy)) {
+    std::swap(hx, hy);
+    std::swap(gx, gy);
+  }
+  Sig s;
+  s.a = mix64(hx * 3 + mix64(hy));
+  s.b = mix64(gx * 5 + mix64(gy ^ 0xabcdef));
+  return s;
+}
+
+struct CacheEntry {
+  int slot = -1, rank = 0;
+  uint64_t last = 0;
+};
+
+struct ServeInstance {
+  Instance *I = nullptr;
+  int rescale_every = 16;
+  std::unordered_map<Sig, CacheEntry, SigHash> cla;
+  std::unordered_map<uint64_t, CacheEntry> pm;  // blen bits -> P, P', P'' slot
+  int reserved = 0;                              // working-set slots of admitted requests
+};
+
+enum { kReqScore = 0, kReqOptimize = 1 };
+
+struct Req {
+  int id = 0, inst = 0, kind = 0, N = 0, nn = 0;
+  double prec = 0.0;
+  std::vector<int> parent;
+  std::vector<double> blen;
+  std::vector<std::array<int, 3>> nb;
+  std::vector<int> nnb;
+  std::vector<int> order;
+  size_t bi = 0;
+  NewtonBranch newton;
+  bool scoring = true;  // final score phase
+  double lnl = 0.0, improve = 0.0;
+  int rc = GLE_SUCCESS, need = 0;
+  bool done = false;      // set by the worker while it owns the request
+  bool finished = false;  // published under Server::mu once the worker let go
+  std::chrono::steady_clock::time_point t_submit, t_done;
+  // per-tick scratch
+  std::vector<std::array<Sig, 3>> sig;
+  int edge_a = 0, edge_b = 0;
+  bool derivs = false;
+};
+
+struct Server {
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Req *> pending;
+  std::vector<Req *> active;
+  std::unordered_map<int, std::unique_ptr<Req>> reqs;
+  std::map<int, ServeInstance> insts;
+  std::thread worker;
+  bool stop = false;
+  int next_id = 1;
+  uint64_t tick = 0;
+  std::vector<double> latency_ms;
+  long long batched_requests = 0, ticks = 0;
+};
+
+static std::mutex g_servers_mu;
+static std::vector<std::unique_ptr<Server>> g_servers;
+
+static int tree_topology(Req &r) {
+  r.nn = 2 * r.N - 2;
+  r.nb.assign(r.nn, {-1, -1, -1});
+  r.nnb.assign(r.nn, 0);
+  if (int(r.parent.size()) != r.nn || r.parent[0] != -1) return set_error(GLE_ERR_ARG, "bad parent array");
+  for (int n = 1; n < r.nn; n++) {
+    const int a = r.parent[n];
+    if (a < 0 || a >= r.nn || a == n || (a >= 1 && a <= r.N)) return set_error(GLE_ERR_ARG, "node %d: bad parent", n);
+    if (!(r.blen[n] >= 0.0)) return set_error(GLE_ERR_ARG, "node %d: bad length", n);
+    r.nb[n][r.nnb[n]++] = a;
+  }
+  for (int n = 1; n < r.nn; n++) {
+    const int a = r.parent[n];
+    if (r.nnb[a] >= 3) return set_error(GLE_ERR_ARG, "node %d has too many neighbours", a);
+    r.nb[a][r.nnb[a]++] = n;
+  }
+  for (int n = 0; n < r.nn; n++)
+    if (r.nnb[n] != ((n >= 1 && n <= r.N) ? 1 : 3)) return set_error(GLE_ERR_ARG, "node %d: wrong degree", n);
+  for (int n = 1; n < r.nn; n++) {
+    int x = n, steps = 0;
+    while (x != 0 && steps++ <= r.nn) x = r.parent[x];
+    if (x != 0) return set_error(GLE_ERR_ARG, "node %d does not reach the root", n);
+  }
+  return GLE_SUCCESS;
+}
+
+static bool req_tip(const Req &r, int n) { return n >= 1 && n <= r.N; }
+static int req_nbk(const Req &r, int a, int b) {
+  for (int k = 0; k < r.nnb[a]; k++)
+    if (r.nb[a][k] == b) return k;
+  return -1;
+}
+static double req_len(const Req &r, int a, int b) { return r.parent[a] == b ? r.blen[a] : r.blen[b]; }
+
+// signature of X(a|b), memoised in r.sig (a.a == 0 && a.b == 0 means unset)
+static Sig req_sig(Req &r, int a, int b) {
+  if (req_tip(r, a)) return tip_sig(a - 1);
+  const int k = req_nbk(r, a, b);
+  Sig &s = r.sig[a][k];
+  if (s.a || s.b) return s;
+  int x = -1, y = -1;
+  for (int j = 0; j < r.nnb[a]; j++) {
+    const int c = r.nb[a][j];
+    if (c == b) continue;
+    if (x < 0) x = c;
+    else y = c;
+  }
+  s = combine(req_sig(r, x, a), req_len(r, a, x), req_sig(r, y, a), req_len(r, a, y));
+  if (!s.a && !s.b) s.a = 1;
+  return s;
+}
+
+// ------------------------------------------------------------ one tick
+struct TickNew {
+  int slot, rank, level;
+};
+
+struct TickBuilder {
+  ServeInstance *si;
+  ClaPool *pool;
+  uint64_t tick;
+  std::unordered_map<Sig, TickNew, SigHash> fresh;
+  std::vector<std::pair<int, DevClaOp>> ops;  // (level, op)
+  std::vector<int> pm_idx;
+  std::vector<double> pm_len;
+  std::unordered_map<uint64_t, int> pm_fresh;
+  int err = GLE_SUCCESS;
+
+  int take_slot() {
+    if (pool->free_slots.empty()) {  // evict the least recently used entry not used this tick
+      auto victim = si->cla.end();
+      for (auto it = si->cla.begin(); it != si->cla.end(); ++it)
+        if (it->second.last < tick && (victim == si->cla.end() || it->second.last < victim->second.last))
+          victim = it;
+      if (victim == si->cla.end()) return -1;
+      const int s = victim->second.slot;
+      if (--pool->refs[s] == 0) pool->free_slots.push_back(s);
+      si->cla.erase(victim);
+      if (pool->free_slots.empty()) return -1;
+    }
+    const int s = pool->free_slots.back();
+    pool->free_slots.pop_back();
+    pool->refs[s] = 1;
+    return s;
+  }
+
+  int pm_slot(double len) {
+    const uint64_t key = dbits(len);
+    auto it = si->pm.find(key);
+    if (it != si->pm.end()) {
+      it->second.last = tick;
+      return it->second.slot;
+    }
+    auto f = pm_fresh.find(key);
+    if (f != pm_fresh.end()) return f->second;
+    if (pool->pm_free.empty()) {
+      auto victim = si->pm.end();
+      for (auto jt = si->pm.begin(); jt != si->pm.end(); ++jt)
+        if (jt->second.last < tick && (victim == si->pm.end() || jt->second.last < victim->second.last))
+          victim = jt;
+      if (victim == si->pm.end()) return -1;
+      pool->pm_free.push_back(victim->second.slot);
+      si->pm.erase(victim);
+    }
+    const int s = pool->pm_free.back();
+    pool->pm_free.pop_back();
+    pm_fresh[key] = s;
+    pm_idx.push_back(s);
+    pm_len.push_back(len);
+    return s;
+  }
+
+  // device reference of X(a|b): CLA slot, or tip code -(tip)-1; fills rank/level
+  int resolve(Req &r, int a, int b, int &rank, int &level) {
+    if (req_tip(r, a)) {
+      rank = 0;
+      level = -1;
+      return -a;
+    }
+    const Sig s = req_sig(r, a, b);
+    auto it = si->cla.find(s);
+    if (it != si->cla.end()) {
+      it->second.last = tick;
+      rank = it->second.rank;
+      level = -1;
+      return it->second.slot;
+    }
+    auto f = fresh.find(s);
+    if (f != fresh.end()) {
+      rank = f->second.rank;
+      level = f->second.level;
+      return f->second.slot;
+    }
+    int x = -1, y = -1;
+    for (int j = 0; j < r.nnb[a]; j++) {
+      const int c = r.nb[a][j];
+      if (c == b) continue;
+      if (x < 0) x = c;
+      else y = c;
+    }
+    int rx, ry, lx, ly;
+    DevClaOp op;
+    op.c1 = resolve(r, x, a, rx, lx);
+    op.c2 = resolve(r, y, a, ry, ly);
+    if (err) return 0;
+    op.pm1 = pm_slot(req_len(r, a, x));
+    op.pm2 = pm_slot(req_len(r, a, y));
+    const int slot = take_slot();
+    if (slot < 0 || op.pm1 < 0 || op.pm2 < 0) {
+      err = set_error(GLE_ERR_NOMEM, "serving cache exhausted: raise n_cla_slots / n_pmat_slots");
+      return 0;
+    }
+    rank = 2 + rx + ry;
+    op.flags = 0;
+    if (rank >= si->rescale_every) {
+      op.flags = kOpRescale;
+      rank = 0;
+    }
+    op.dest = slot;
+    level = std::max(lx, ly) + 1;
+    fresh[s] = {slot, rank, level};
+    ops.push_back({level, op});
+    return slot;
+  }
+};
+
+// Next evaluation of a request: (edge, derivs?).
+static void next_edge(Req &r) {
+  if (r.scoring) {
+    // Tree::Score on a fresh tree integrates across (root, middle child)
+    r.edge_a = 0;
+    r.edge_b = r.nb[0][1];
+    r.derivs = false;
+  } else {
+    const int nd = r.order[r.bi];
+    r.edge_a = r.parent[nd];
+    r.edge_b = nd;
+    r.derivs = true;
+  }
+}
+
+static void advance(Req &r, const double *res) {
+  if (r.scoring) {
+    r.lnl = res[0];
+    r.done = true;
+    return;
+  }
+  const int nd = r.order[r.bi];
+  double next = 0.0;
+  const NewtonBranch::Action act = r.newton.step(res[0], res[1], res[2], r.blen[nd], &next);
+  if (act != NewtonBranch::kFinish) r.blen[nd] = next;
+  if (act != NewtonBranch::kContinue) {
+    r.improve += r.newton.total;
+    if (++r.bi < r.order.size()) r.newton.start(r.blen[r.order[r.bi]], r.prec);
+    else r.scoring = true;
+  }
+}
+
+// Runs one batched step of every request in `batch` (all on instance si).
+static int run_tick(Server &sv, ServeInstance &si, std::vector<Req *> &batch) {
+  for (;;) {
+    std::vector<DevClaOp> ops;
+    std::vector<int> level_start;
+    std::vector<DevEdgeOp> eops;
+    std::vector<std::pair<Sig, TickNew>> fresh;
+    {
+      std::lock_guard<std::mutex> pk(pool_mutex());
+      TickBuilder tb{&si, pool_for(si.I), sv.tick, {}, {}, {}, {}, {}, GLE_SUCCESS};
+      for (Req *r : batch) {
+        r->sig.assign(r->nn, {Sig(), Sig(), Sig()});
+        next_edge(*r);
+        int ra, la, rb, lb;
+        DevEdgeOp e;
+        e.partial = tb.resolve(*r, r->edge_a, r->edge_b, ra, la);
+        e.child = tb.resolve(*r, r->edge_b, r->edge_a, rb, lb);
+        e.pm = tb.pm_slot(req_len(*r, r->edge_a, r->edge_b));
+        e.derivs = r->derivs ? 1 : 0;
+        if (tb.err || e.pm < 0) return tb.err ? tb.err : set_error(GLE_ERR_NOMEM, "out of pmat slots");
+        eops.push_back(e);
+      }
+      std::stable_sort(tb.ops.begin(), tb.ops.end(),
+                       [](const std::pair<int, DevClaOp> &x, const std::pair<int, DevClaOp> &y) {
+                         return x.first < y.first;
+                       });
+      int cur = -1;
+      for (auto &lo : tb.ops) {
+        if (lo.first != cur) {
+          level_start.push_back(int(ops.size()));
+          cur = lo.first;
+        }
+        ops.push_back(lo.second);
+      }
+      level_start.push_back(int(ops.size()));
+      for (auto &f : tb.fresh) fresh.push_back(f);
+      // P, P', P'' for new branch lengths (one launch)
+      int rc = engine_update_pmats(si.I, tb.pm_idx.data(), tb.pm_len.data(), int(tb.pm_idx.size()), 1);
+      if (rc) return rc;
+      for (size_t i = 0; i < tb.pm_idx.size(); i++) si.pm[dbits(tb.pm_len[i])] = {tb.pm_idx[i], 0, sv.tick};
+    }
+    int flags = 0;
+    int rc = engine_flags(si.I, &flags, 1);
+    if (rc) return rc;
+    rc = engine_update_partials(si.I, ops.data(), int(ops.size()), level_start.data(),
+                                int(level_start.size()) - 1, false);
+    if (rc) return rc;
+    std::vector<double> res(3 * eops.size());
+    rc = engine_edges(si.I, eops.data(), int(eops.size()), res.data(), nullptr);
+    if (rc) return rc;
+    rc = engine_flags(si.I, &flags, 1);
+    if (rc) return rc;
+    const bool thrown = instance_precision(si.I) == GLE_PREC_FP64 &&
+                        (((flags & GLE_FLAG_REDUCE_RESCALE) && si.rescale_every > 2) ||
+                         ((flags & GLE_FLAG_BAIL_OUT) && si.rescale_every <= 2));
+    std::lock_guard<std::mutex> pk(pool_mutex());
+    ClaPool *pool = pool_for(si.I);
+    if (thrown) {
+      // Tree::Score's retry: drop every cached CLA (they were built with the old
+      // rescaleEvery) and run the step again with rescaleEvery - 2.
+      for (auto &f : fresh)
+        if (--pool->refs[f.second.slot] == 0) pool->free_slots.push_back(f.second.slot);
+      for (auto &c : si.cla)
+        if (--pool->refs[c.second.slot] == 0) pool->free_slots.push_back(c.second.slot);
+      si.cla.clear();
+      si.rescale_every -= 2;
+      if (si.rescale_every < 2) return set_error(GLE_ERR_RESCALE_FATAL, "Problem with rescaling in serving.");
+      continue;
+    }
+    for (auto &f : fresh) si.cla[f.first] = {f.second.slot, f.second.rank, sv.tick};
+    for (size_t i = 0; i < batch.size(); i++) advance(*batch[i], &res[3 * i]);
+    return GLE_SUCCESS;
+  }
+}
+
+static int working_set(const Req &r) { return 2 * (r.N - 2) + 4; }
+
+static void worker_loop(Server *sv) {
+  for (;;) {
+    std::vector<Req *> work;
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      sv->cv_work.wait(lk, [&] { return sv->stop || !sv->pending.empty() || !sv->active.empty(); });
+      if (sv->stop) return;
+      // admission by HBM budget: FIFO while the working sets fit the instance's CLA slots
+      while (!sv->pending.empty()) {
+        Req *r = sv->pending.front();
+        ServeInstance &si = sv->insts[r->inst];
+        const int cap = instance_cla_slots(si.I);
+        if (si.reserved + r->need > cap && !sv->active.empty()) break;
+        si.reserved += r->need;
+        sv->pending.pop_front();
+        sv->active.push_back(r);
+      }
+      work = sv->active;
+      sv->tick++;
+    }
+    std::map<int, std::vector<Req *>> by_inst;
+    for (Req *r : work) by_inst[r->inst].push_back(r);
+    for (auto &kv : by_inst) {
+      ServeInstance &si = sv->insts[kv.first];
+      const int rc = run_tick(*sv, si, kv.second);
+      if (rc)
+        for (Req *r : kv.second) {
+          r->rc = rc;
+          r->done = true;
+        }
+    }
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      sv->ticks++;
+      sv->batched_requests += long(work.size());
+      std::vector<Req *> still;
+      const auto now = std::chrono::steady_clock::now();
+      for (Req *r : sv->active) {
+        if (r->done) {
+          r->t_done = now;
+          r->finished = true;
+          sv->insts[r->inst].reserved -= r->need;
+          sv->latency_ms.push_back(std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count());
+        } else
+          still.push_back(r);
+      }
+      sv->active.swap(still);
+    }
+    sv->cv_done.notify_all();
+  }
+}
+
+static Server *get_server(int h) {
+  std::lock_guard<std::mutex> lk(g_servers_mu);
+  if (h < 0 || h >= int(g_servers.size())) return nullptr;
+  return g_servers[h].get();
+}
+
+}  // namespace gle
+
+using namespace gle;
+
+extern "C" int gle_serve_create(int *server) {
+  if (!server) return set_error(GLE_ERR_ARG, "null argument");
+  auto sv = std::make_unique<Server>();
+  Server *raw = sv.get();
+  raw->worker = std::thread(worker_loop, raw);
+  std::lock_guard<std::mutex> lk(g_servers_mu);
+  g_servers.push_back(std::move(sv));
+  *server = int(g_servers.size()) - 1;
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_serve_destroy(int h) {
+  Server *sv = get_server(h);
+  if (!sv) return set_error(GLE_ERR_ARG, "bad server handle %d", h);
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->stop = true;
+  }
+  sv->cv_work.notify_all();
+  if (sv->worker.joinable()) sv->worker.join();
+  {
+    std::lock_guard<std::mutex> pk(pool_mutex());
+    for (auto &kv : sv->insts) {
+      ClaPool *pool = pool_for(kv.second.I);
+      for (auto &c : kv.second.cla)
+        if (--pool->refs[c.second.slot] == 0) pool->free_slots.push_back(c.second.slot);
+      for (auto &p : kv.second.pm) pool->pm_free.push_back(p.second.slot);
+    }
+  }
+  std::lock_guard<std::mutex> lk(g_servers_mu);
+  g_servers[h].reset();
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_serve_submit(int h, int inst, const int *parent, const double *blen, int kind, double precision,
+                                int *req) {
+  Server *sv = get_server(h);
+  if (!sv) return set_error(GLE_ERR_ARG, "bad server handle %d", h);
+  Instance *I = get_instance(inst);
+  if (!I || !parent || !blen || !req || (kind != kReqScore && kind != kReqOptimize) ||
+      (kind == kReqOptimize && !(precision > 0.0)))
+    return set_error(GLE_ERR_ARG, "bad request");
+  auto r = std::make_unique<Req>();
+  r->inst = inst;
+  r->kind = kind;
+  r->N = instance_tips(I);
+  r->prec = precision;
+  r->parent.assign(parent, parent + 2 * r->N - 2);
+  r->blen.assign(blen, blen + 2 * r->N - 2);
+  int rc = tree_topology(*r);
+  if (rc) return rc;
+  r->need = working_set(*r);
+  if (kind == kReqOptimize) {
+    // OptimizeAllBranches order: preorder from each root child (optimization.cpp:97-105)
+    std::vector<int> stack;
+    for (int j = r->nnb[0] - 1; j >= 0; j--) stack.push_back(r->nb[0][j]);
+    while (!stack.empty()) {
+      const int n = stack.back();
+      stack.pop_back();
+      r->order.push_back(n);
+      for (int j = r->nnb[n] - 1; j >= 0; j--)
+        if (r->nb[n][j] != r->parent[n]) stack.push_back(r->nb[n][j]);
+    }
+    r->scoring = false;
+    r->newton.start(r->blen[r->order[0]], precision);
+  }
+  r->t_submit = std::chrono::steady_clock::now();
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    ServeInstance &si = sv->insts[inst];
+    if (!si.I) {
+      si.I = I;
+      si.rescale_every = instance_precision(I) == GLE_PREC_FP64 ? 16 : 1;
+    }
+    if (r->need > instance_cla_slots(I))
+      return set_error(GLE_ERR_NOMEM, "request needs %d CLA slots, instance has %d", r->need,
+                       instance_cla_slots(I));
+    r->id = sv->next_id++;
+    *req = r->id;
+    sv->pending.push_back(r.get());
+    sv->reqs[r->id] = std::move(r);
+  }
+  sv->cv_work.notify_one();
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_serve_poll(int h, int req, int *done) {
+  Server *sv = get_server(h);
+  if (!sv || !done) return set_error(GLE_ERR_ARG, "bad argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  auto it = sv->reqs.find(req);
+  if (it == sv->reqs.end()) return set_error(GLE_ERR_ARG, "unknown request %d", req);
+  *done = it->second->finished ? 1 : 0;
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_serve_wait(int h, int req, double *lnl, double *blen_out, double *latency_ms) {
+  Server *sv = get_server(h);
+  if (!sv || !lnl) return set_error(GLE_ERR_ARG, "bad argument");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  auto it = sv->reqs.find(req);
+  if (it == sv->reqs.end()) return set_error(GLE_ERR_ARG, "unknown request %d", req);
+  Req *r = it->second.get();
+  sv->cv_done.wait(lk, [&] { return r->finished; });
+  const int rc = r->rc;
+  *lnl = r->lnl;
+  if (blen_out) std::copy(r->blen.begin(), r->blen.end(), blen_out);
+  if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+  sv->reqs.erase(it);
+  return rc;
+}
+
+extern "C" int gle_serve_stats(int h, long long *completed, double *p50_ms, double *p99_ms, double *mean_batch,
+                               int reset) {
+  Server *sv = get_server(h);
+  if (!sv || !completed || !p50_ms || !p99_ms || !mean_batch) return set_error(GLE_ERR_ARG, "bad argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  std::vector<double> v = sv->latency_ms;
+  *completed = (long long)v.size();
+  std::sort(v.begin(), v.end());
+  auto pct = [&](double q) {
+    if (v.empty()) return 0.0;
+    const size_t i = std::min(v.size() - 1, size_t(std::ceil(q * double(v.size()))) - (q > 0 ? 1 : 0));
+    return v[i];
+  };
+  *p50_ms = pct(0.50);
+  *p99_ms = pct(0.99);
+  *mean_batch = sv->ticks ? double(sv->batched_requests) / double(sv->ticks) : 0.0;
+  if (reset) {
+    sv->latency_ms.clear();
+    sv->ticks = 0;
+    sv->batched_requests = 0;
+  }
+  return GLE_SUCCESS;
+}

@@ -17,20 +17,20 @@
 #include <vector>
 
 #include "gle_internal.h"
+#include "gle_newton.h"
 
 namespace gle {
 
 // ---------------------------------------------------------------- slot pool
-struct ClaPool {  // one per instance
-  std::vector<int> refs;
-  std::vector<int> free_slots;
-  std::vector<int> rank;  // rescale rank of the CLA held in the slot
-  std::vector<int> pm_free;
-};
-static std::mutex g_tree_mu;
+static std::recursive_mutex g_tree_rmu;
 static std::map<const Instance *, std::unique_ptr<ClaPool>> g_pools;
 
-static ClaPool *pool_for(Instance *I) {
+std::mutex &pool_mutex() {
+  static std::mutex m;
+  return m;
+}
+
+ClaPool *pool_for(Instance *I) {
   auto &p = g_pools[I];
   if (!p) {
     p = std::make_unique<ClaPool>();
@@ -351,112 +351,20 @@ static void set_blen(Tree &t, int nd, double v) {
   dirty_edge(t, nd);
 }
 
-// NewtonRaphsonOptimizeBranchLength (optimization.cpp:1359-1972), FP64 branch,
-// without the debugging output paths.
+// NewtonRaphsonOptimizeBranchLength (optimization.cpp:1359-1972) on one branch.
 static int newton(Tree &t, int nd, double prec, double *improve) {
-  const double min_b = 1e-8, max_b = 100.0;
-  double total = 0.0, v = t.blen[nd], v_prev = v, knownMin = min_b, knownMax = max_b, initialL = 0.0;
-  int iter = 0, negProposalNum = 0;
-  *improve = 0.0;
+  NewtonBranch nb;
+  nb.start(t.blen[nd], prec);
   for (;;) {
-    double lnL, d1, d2;
-    int rc = tree_derivs(t, nd, &lnL, &d1, &d2);
+    double lnL, d1, d2, next = 0.0;
+    const int rc = tree_derivs(t, nd, &lnL, &d1, &d2);
     if (rc) return rc;
-    if (iter == 0) initialL = lnL;
-    const double dlen = t.blen[nd];
-    const double estDeltaNR = -d1 / d2;
-    const double estScoreDelta = d1 * estDeltaNR + (d2 * estDeltaNR * estDeltaNR * 0.5);
-    if (d1 <= 0.0 && dlen < knownMax) knownMax = dlen;
-    else if (d1 > 0.0 && dlen > knownMin) knownMin = dlen;
-    auto done = [&]() {
-      *improve = total;
-      return GLE_SUCCESS;
-    };
-    if (d2 >= 0.0) {  // curvature wrong for NR
-      double proposed;
-      if (d1 <= 0.0) {
-        if (std::fabs(dlen - min_b) < 1.0e-8) return done();
-        if (knownMin == min_b) {
-          if (dlen <= 1.0e-4) proposed = min_b;
-          else if (dlen <= 0.005) proposed = 1.0e-4;
-          else if (dlen <= 0.05) proposed = dlen * 0.1;
-          else proposed = dlen * 0.25;
-        } else
-          proposed = (knownMin + dlen) * 0.5;
-        if (iter > 0 || proposed == min_b) {
-          const double estImp = d1 * (proposed - dlen) + (d2 * (proposed - dlen) * (proposed - dlen) * 0.5);
-          if (estImp < prec && lnL >= initialL - 1.0e-4) return done();
-        }
-      } else {
-        if (knownMax == max_b) proposed = dlen < 0.1 ? dlen * 10.0 : std::min(dlen * 2.0, max_b);
-        else proposed = (knownMax + dlen) * 0.5;
-        if (iter > 0) {
-          const double estImp = d1 * (proposed - dlen) + (d2 * (proposed - dlen) * (proposed - dlen) * 0.5);
-          if (estImp < prec && lnL >= initialL - 1.0e-4) return done();
-        }
-      }
-      v = proposed;
-      total += prec;
-    } else {
-      if (d1 < 0.0 && std::fabs(dlen - min_b) < 1.0e-8) return done();
-      if (d1 > 0.0 && std::fabs(dlen - max_b) < 1.0e-8) return done();
-      if (estScoreDelta < prec && (iter == 0 || lnL + ((iter < 10 ? 1 : iter) * 1.0e-7) >= initialL)) return done();
-      bool noNR = false;
-      if (iter > 10) {
-        if (d1 > 0.0) {
-          if (std::fabs(knownMax - max_b) < 1e-8) {
-            if (iter > 20 && dlen > max_b / 2.0) v = max_b;
-            else if (v < 0.2) v = std::min((v + max_b) * 0.5, v * 5.0);
-            else v = std::min((v + max_b) * 0.5, v * 2.0);
-            noNR = true;
-          } else if ((iter - 20) > 0 && (iter - 20) % 10 == 0)
-            v = std::min((v + knownMax) * 0.5, v * 100.0);
-        } else if (std::fabs(knownMin - min_b) < 1e-8) {
-          v = iter > 20 ? min_b : (v + min_b) * 0.5;
-          noNR = true;
-        }
-      }
-      if (!noNR) v += estDeltaNR;
-      if (v <= knownMin) {
-        negProposalNum++;
-        if (knownMin == min_b) {
-          const double dm = min_b - dlen, sdm = dm * d1 + (dm * dm * d2 * 0.5);
-          if (sdm < prec) return done();
-          if (negProposalNum == 1 && dlen > 1e-4 && v_prev != 1e-4) {
-            v = dlen < .005 ? 1e-4 : (dlen < 0.05 ? dlen * 0.1 : dlen * 0.25);
-            const double delta = v - dlen;
-            total += delta * d1 + (delta * delta * d2 * 0.5);
-          } else {
-            v = min_b;
-            total += sdm;
-          }
-        } else {
-          const double proposed = (knownMin + dlen) * 0.5, dm = proposed - dlen;
-          const double sdm = dm * d1 + (dm * dm * d2 * 0.5);
-          if (sdm < prec) return done();
-          v = proposed;
-          total += sdm;
-        }
-      } else if (v >= knownMax) {
-        if (knownMax == max_b) {
-          const double dm = max_b - dlen, sdm = dm * d1 + (dm * dm * d2 * 0.5);
-          if (sdm < prec) return done();
-          v = max_b;
-          total += sdm;
-        } else {
-          const double proposed = (knownMax + dlen) * 0.5, dm = proposed - dlen;
-          const double sdm = dm * d1 + (dm * dm * d2 * 0.5);
-          if (sdm < prec) return done();
-          v = proposed;
-          total += sdm;
-        }
-      } else
-        total += estScoreDelta;
-    }
-    set_blen(t, nd, v);
-    v_prev = v;
-    if (++iter > 100) return done();
+    const NewtonBranch::Action act = nb.step(lnL, d1, d2, t.blen[nd], &next);
+    if (act != NewtonBranch::kFinish) set_blen(t, nd, next);
+    if (act != NewtonBranch::kContinue) break;
   }
+  *improve = nb.total;
+  return GLE_SUCCESS;
 }
 
 static void preorder(const Tree &t, int n, std::vector<int> &out) {
@@ -525,7 +433,8 @@ static void destroy_tree(Tree &t) {
 using namespace gle;
 
 #define GET_TREE(h)                                                   \
-  std::lock_guard<std::mutex> lk_(g_tree_mu);                         \
+  std::lock_guard<std::recursive_mutex> lk_(g_tree_rmu);              \
+  std::lock_guard<std::mutex> pk_(pool_mutex());                      \
   Tree *T = get_tree(h);                                              \
   if (!T) return set_error(GLE_ERR_ARG, "bad tree handle %d", (h));
 
@@ -533,7 +442,8 @@ extern "C" int gle_tree_create(int inst, const int *parent, const double *blen, 
   if (!parent || !blen || !tree) return set_error(GLE_ERR_ARG, "null argument");
   Instance *I = get_instance(inst);
   if (!I) return set_error(GLE_ERR_ARG, "bad instance handle %d", inst);
-  std::lock_guard<std::mutex> lk(g_tree_mu);
+  std::lock_guard<std::recursive_mutex> lk(g_tree_rmu);
+  std::lock_guard<std::mutex> pk(pool_mutex());
   auto t = std::make_unique<Tree>();
   const int rc = build_tree(*t, I, parent, blen);
   if (rc) {

@@ -97,6 +97,12 @@ SIGNATURES = {
     "gle_tree_rescale_every": ([C.c_int], C.c_int),
     "gle_tree_set_rescale_every": ([C.c_int, C.c_int], C.c_int),
     "gle_tree_set_schedule": ([C.c_int, C.c_int], C.c_int),
+    "gle_serve_create": ([_I], C.c_int),
+    "gle_serve_destroy": ([C.c_int], C.c_int),
+    "gle_serve_submit": ([C.c_int, C.c_int, _I, _D, C.c_int, C.c_double, _I], C.c_int),
+    "gle_serve_poll": ([C.c_int, C.c_int, _I], C.c_int),
+    "gle_serve_wait": ([C.c_int, C.c_int, _D, _D, _D], C.c_int),
+    "gle_serve_stats": ([C.c_int, C.POINTER(C.c_longlong), _D, _D, _D, C.c_int], C.c_int),
 }
 
 
@@ -400,3 +406,53 @@ class Tree:
 
     def set_schedule(self, mode):
         _check(lib().gle_tree_set_schedule(self.h, int(mode)))
+
+
+REQ_SCORE = 0
+REQ_OPTIMIZE = 1
+
+
+class Server:
+    """Continuous-batching request service (gle_serve_*)."""
+
+    def __init__(self):
+        h = C.c_int(-1)
+        _check(lib().gle_serve_create(C.byref(h)))
+        self.h = h.value
+
+    def close(self):
+        if self.h is not None:
+            lib().gle_serve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, inst, parent, blen, kind=REQ_SCORE, precision=0.0):
+        par = np.ascontiguousarray(parent, dtype=np.int32)
+        bl = _d(blen)
+        rid = C.c_int(-1)
+        _check(lib().gle_serve_submit(self.h, inst.h, _p(par, C.c_int), _p(bl, C.c_double), int(kind),
+                                      float(precision), C.byref(rid)))
+        return rid.value
+
+    def poll(self, rid):
+        d = C.c_int(0)
+        _check(lib().gle_serve_poll(self.h, rid, C.byref(d)))
+        return bool(d.value)
+
+    def wait(self, rid, n_nodes=None):
+        lnl, lat = C.c_double(0), C.c_double(0)
+        bl = np.zeros(n_nodes) if n_nodes else None
+        _check(lib().gle_serve_wait(self.h, rid, C.byref(lnl), _p(bl, C.c_double) if n_nodes else None,
+                                    C.byref(lat)))
+        return (lnl.value, lat.value, bl) if n_nodes else (lnl.value, lat.value)
+
+    def stats(self, reset=False):
+        n = C.c_longlong(0)
+        p50, p99, mb = C.c_double(0), C.c_double(0), C.c_double(0)
+        _check(lib().gle_serve_stats(self.h, C.byref(n), C.byref(p50), C.byref(p99), C.byref(mb), int(reset)))
+        return {"completed": n.value, "p50_ms": p50.value, "p99_ms": p99.value, "mean_batch": mb.value}

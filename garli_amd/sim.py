@@ -249,3 +249,96 @@ def codon_m3(n_tips=32, n_sites=None, n_patterns=5000, mean_blen=0.1, kappa=2.5,
 def ambig_streams(ds):
     """Reference-format nucleotide ambiguity streams, one per taxon."""
     return [E.make_ambig_string(ds.tips[t]) for t in range(ds.n_tips)]
+
+
+# ------------------------------------------------------------ topology moves
+def _adjacency(parent, blen):
+    nn = len(parent)
+    adj = {n: {} for n in range(nn)}
+    for n in range(1, nn):
+        a = int(parent[n])
+        adj[n][a] = blen[n]
+        adj[a][n] = blen[n]
+    return adj
+
+
+def _reroot(adj, nn):
+    """Parent array rooted at node 0 (which must have degree 3)."""
+    parent = np.full(nn, -1, np.int32)
+    blen = np.zeros(nn)
+    seen = {0}
+    stack = [0]
+    while stack:
+        a = stack.pop()
+        for c, l in adj[a].items():
+            if c not in seen:
+                seen.add(c)
+                parent[c] = a
+                blen[c] = l
+                stack.append(c)
+    assert len(seen) == nn
+    return parent, blen
+
+
+def nni(parent, blen, rng, n_tips):
+    """Nearest-neighbour interchange across a random internal edge
+    (the move of Tree::NNIMutate, tree.cpp:6970)."""
+    adj = _adjacency(parent, blen)
+    internal = [n for n in adj if not (1 <= n <= n_tips)]
+    edges = [(u, v) for u in internal for v in adj[u] if v > u and not (1 <= v <= n_tips)]
+    u, v = edges[int(rng.integers(len(edges)))]
+    b = [x for x in adj[u] if x != v][int(rng.integers(2))]
+    c = [x for x in adj[v] if x != u][int(rng.integers(2))]
+    lb, lc = adj[u].pop(b), adj[v].pop(c)
+    del adj[b][u], adj[c][v]
+    adj[u][c] = adj[c][u] = lc
+    adj[v][b] = adj[b][v] = lb
+    return _reroot(adj, len(parent))
+
+
+def spr(parent, blen, rng, n_tips):
+    """Subtree prune and regraft (Tree::SPRMutate, tree.cpp:2571-2708): detach the
+    subtree below a random edge at its internal attachment node p, join p's other
+    two neighbours, and reinsert p at the midpoint of a random edge outside the
+    subtree."""
+    adj = _adjacency(parent, blen)
+    nn = len(parent)
+    for _ in range(100):
+        p = int(rng.choice([n for n in adj if not (1 <= n <= n_tips)]))
+        s = int(rng.choice(list(adj[p])))
+        x, y = [q for q in adj[p] if q != s]
+        # nodes of the pruned subtree (s's side of edge (p, s))
+        sub, stack = {s}, [s]
+        while stack:
+            a = stack.pop()
+            for q in adj[a]:
+                if q != p and q not in sub:
+                    sub.add(q)
+                    stack.append(q)
+        cand = [(a, q) for a in adj if a not in sub and a != p for q in adj[a]
+                if q not in sub and q != p and a < q and {a, q} != {x, y}]
+        if not cand:
+            continue
+        lx, ly = adj[p].pop(x), adj[p].pop(y)
+        del adj[x][p], adj[y][p]
+        adj[x][y] = adj[y][x] = lx + ly
+        cand = [(a, q) for (a, q) in cand if q in adj[a]]
+        if not cand:
+            return _reroot(_adjacency(parent, blen), nn)
+        a, q = cand[int(rng.integers(len(cand)))]
+        l = adj[a].pop(q)
+        del adj[q][a]
+        adj[a][p] = adj[p][a] = l / 2
+        adj[q][p] = adj[p][q] = l / 2
+        return _reroot(adj, nn)
+    return parent.copy(), blen.copy()
+
+
+def perturb(parent, blen, rng, n_tips, n_moves):
+    """1..n random NNI/SPR moves plus one branch-length change: a GA offspring."""
+    for _ in range(n_moves):
+        parent, blen = (nni if rng.random() < 0.5 else spr)(parent, blen, rng, n_tips)
+    blen = blen.copy()
+    k = int(rng.integers(1, len(blen)))
+    blen[k] = max(1e-8, blen[k] * float(rng.uniform(0.5, 1.5)))
+    return parent, blen

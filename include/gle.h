@@ -190,6 +190,27 @@ int gle_tree_set_rescale_every(int tree, int every);
  * 1 = one fused launch for the whole op list (pattern-local kernels only). */
 int gle_tree_set_schedule(int tree, int mode);
 
+/* ---------------------------------------------------------------------------
+ * Serving layer: continuous batching of scoring / branch-optimisation requests
+ * from a GA population (the reference scores individuals one at a time,
+ * population.cpp:4532-4589; Individual::CalcFitness, individual.cpp:224).
+ * CLAs are cached on the device by subtree signature and reused across
+ * requests; P(t) by branch length.  Requests run on the instance they name;
+ * an instance served here must not be used by other calls at the same time.
+ * ------------------------------------------------------------------------- */
+#define GLE_REQ_SCORE 0    /* Tree::Score of the given tree */
+#define GLE_REQ_OPTIMIZE 1 /* Tree::OptimizeAllBranches(precision), then Score */
+int gle_serve_create(int *server);
+int gle_serve_destroy(int server);
+/* parent/blen as in gle_tree_create; returns a request id */
+int gle_serve_submit(int server, int inst, const int *parent, const double *blen, int kind, double precision,
+                     int *req);
+int gle_serve_poll(int server, int req, int *done);
+/* blocks until done; blen_out (optional) receives the final branch lengths */
+int gle_serve_wait(int server, int req, double *lnl, double *blen_out, double *latency_ms);
+/* latency percentiles over completed requests; mean requests per batched step */
+int gle_serve_stats(int server, long long *completed, double *p50_ms, double *p99_ms, double *mean_batch, int reset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,86 @@
+"""Serving layer: continuous batching with a device CLA cache.
+
+Each result must be the same alone and in the mix (bit-identical), and equal
+to the tree layer's / oracle's answer for that tree."""
+import threading
+
+import numpy as np
+import pytest
+
+from garli_amd import engine as E
+from garli_amd import sim
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _population(ds, n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        out.append(sim.perturb(ds.parent, ds.blen, rng, ds.n_tips, int(rng.integers(1, 4))))
+    return out
+
+
+@pytest.mark.parametrize("prec", [E.PREC_FP64, E.PREC_FP32])
+def test_alone_equals_in_the_mix(gpu, prec):
+    ds = sim.gtr_gamma(n_tips=20, n_sites=1500, pinv=0.1, seed=101)
+    inst = ds.make_instance(prec, n_cla_slots=600, n_pmat_slots=600)
+    srv = E.Server()
+    pop = _population(ds, 24, 5)
+    # alone
+    alone = []
+    for p, b in pop[:6]:
+        rid = srv.submit(inst, p, b)
+        alone.append(srv.wait(rid)[0])
+    # in the mix: everything submitted at once, from several threads
+    ids = [None] * len(pop)
+
+    def sub(k):
+        ids[k] = srv.submit(inst, *pop[k])
+
+    th = [threading.Thread(target=sub, args=(k,)) for k in range(len(pop))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    mix = [srv.wait(i)[0] for i in ids]
+    for a, m in zip(alone, mix[:6]):
+        assert a == m
+    st = srv.stats()
+    assert st["completed"] == len(pop) + 6 and st["mean_batch"] > 1.0
+    o = O.Oracle(ds)
+    tol = 1e-9 if prec == E.PREC_FP64 else 1e-3
+    for (p, b), got in zip(pop[::4], mix[::4]):
+        ref = o.score(parent=p, blen=b)
+        assert abs(got - ref) < (tol * abs(ref) if prec == E.PREC_FP64 else tol)
+    srv.close()
+
+
+def test_optimize_request_matches_tree_layer(gpu):
+    ds = sim.gtr_gamma(n_tips=14, n_sites=800, seed=102)
+    inst = ds.make_instance(E.PREC_FP64, n_cla_slots=400, n_pmat_slots=800)
+    srv = E.Server()
+    rid = srv.submit(inst, ds.parent, ds.blen, kind=E.REQ_OPTIMIZE, precision=0.001)
+    lnl, _, bl = srv.wait(rid, n_nodes=2 * ds.n_tips - 2)
+    srv.close()
+    inst2 = ds.make_instance(E.PREC_FP64)
+    t = E.Tree(inst2, ds.parent, ds.blen)
+    _, tl = t.optimize_all_branches(0.001)
+    np.testing.assert_allclose(bl[1:], t.branch_lengths()[1:], rtol=1e-10)
+    assert abs(lnl - tl) < 1e-9 * abs(tl)
+
+
+def test_admission_under_small_budget(gpu):
+    """A cache smaller than the population's working set still serves every
+    request correctly by admitting a few at a time and evicting LRU entries."""
+    ds = sim.gtr_gamma(n_tips=16, n_sites=600, seed=103)
+    inst = ds.make_instance(E.PREC_FP32, n_cla_slots=2 * 14 + 4 + 10, n_pmat_slots=400)
+    srv = E.Server()
+    pop = _population(ds, 12, 7)
+    ids = [srv.submit(inst, p, b) for p, b in pop]
+    got = [srv.wait(i)[0] for i in ids]
+    o = O.Oracle(ds)
+    for (p, b), g in zip(pop, got):
+        assert abs(g - o.score(parent=p, blen=b)) < 1e-3
+    srv.close()
