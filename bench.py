@@ -45,6 +45,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle legs")
     ap.add_argument("--patterns", type=int, default=100000)
     ap.add_argument("--taxa", type=int, default=128)
+    ap.add_argument("--quick", action="store_true",
+                    help="sweeps only (for profiler runs): no evals, accuracy or CPU legs")
+    ap.add_argument("--calibrate", type=float, default=0.0,
+                    help="also run a plain copy of this many GiB (counter calibration)")
+    ap.add_argument("--workload", choices=["gtr", "aa", "codon", "serve"], default="gtr",
+                    help="gtr: the metric's configuration (default); aa / codon: the 20- and 61-state "
+                         "configurations; serve: Poisson-arrival GA requests through the serving layer")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this engine has no multi-GPU path "
@@ -53,6 +60,11 @@ def main():
 
     from garli_amd import engine as E
     from garli_amd import sim
+
+    if args.workload in ("aa", "codon"):
+        return bench_states(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     t0 = time.time()
     ds = sim.gtr_gamma(n_tips=args.taxa, n_sites=None, n_patterns=args.patterns, mean_blen=0.05, alpha=None,
@@ -94,6 +106,18 @@ def main():
     achieved = bytes_l / max(ms_l, 1e-12) / 1e6  # GB/s
     log("CLA kernel: %d launches, avg %.3f ms, %.1f GB/s; edge kernel avg %.3f ms %.1f GB/s; pmat avg %.3f ms" %
         (n_l, avg_ms, achieved, e_ms / max(e_l, 1), e_bytes / max(e_ms, 1e-12) / 1e6, p_ms / max(p_l, 1)))
+
+    copy = None
+    if args.calibrate > 0:
+        nbytes = int(args.calibrate * (1 << 30))
+        ms = E.copy_calibrate(nbytes, 10)
+        copy = {"bytes": nbytes, "ms": ms, "GB/s": 2 * nbytes / ms / 1e6}
+        log("copy calibration: %d bytes in %.3f ms -> %.1f GB/s (read+write)" % (nbytes, ms, copy["GB/s"]))
+    if args.quick:
+        print(json.dumps({"metric": METRIC, "value": site_updates, "unit": "CLA site-updates/s",
+                          "ms_per_step": ms_step, "cla_avg_launch_ms": avg_ms, "cla_GBs": achieved,
+                          "launches": n_l, "copy": copy}), flush=True)
+        return
 
     # full-tree lnL evaluations (MakeAllNodesDirty + Score), latency distribution
     lat = []
@@ -174,6 +198,174 @@ def main():
                      "bytes_per_site_update": 3 * 4 * 4 * (4 if prec == E.PREC_FP32 else 8) + 12},
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out), flush=True)
+
+
+def _population(ds, n, seed):
+    from garli_amd import sim
+    rng = np.random.default_rng(seed)
+    return [sim.perturb(ds.parent, ds.blen, rng, ds.n_tips, int(rng.integers(1, 4))) for _ in range(n)]
+
+
+def _eval_rates(ds, E, pop, prec):
+    """Per-request plain drop-in: a fresh tree per request, Tree::Score."""
+    inst = ds.make_instance(prec)
+    lat = []
+    for p, b in pop:
+        t = E.Tree(inst, p, b)
+        a = time.perf_counter()
+        t.score()
+        lat.append(time.perf_counter() - a)
+        t.close()
+    inst.close()
+    lat = np.array(lat) * 1000.0
+    return 1000.0 / lat.mean(), lat
+
+
+def _serve_rate(ds, E, pop, prec, slots):
+    inst = ds.make_instance(prec, n_cla_slots=slots, n_pmat_slots=4 * slots)
+    srv = E.Server()
+    a = time.perf_counter()
+    ids = [srv.submit(inst, p, b) for p, b in pop]
+    res = [srv.wait(i)[0] for i in ids]
+    dt = time.perf_counter() - a
+    st = srv.stats()
+    srv.close()
+    inst.close()
+    return len(pop) / dt, st, res
+
+
+def bench_states(args):
+    """AA (20 states, G4, 64 taxa, 20k patterns) or codon (GY94 M3, 32 taxa, 5k
+    patterns): sweeps in both forms, lnL evaluations per request (FP64 plain
+    drop-in) and batched through the serving layer (FP32 bulk)."""
+    from garli_amd import engine as E
+    from garli_amd import sim
+    from oracle import oracle as O
+    if args.workload == "aa":
+        ds = sim.aa_gamma(n_tips=64, n_patterns=20000, seed=3)
+        name, S = "amino-acid 20-state G4 (random exchangeabilities), 64 taxa, 20000 patterns", 20
+    else:
+        ds = sim.codon_m3(n_tips=32, n_patterns=5000, seed=4)
+        name, S = "codon GY94 M3 (omega 0.1/1/3, kappa 2.5), 32 taxa, 5000 patterns", 61
+    E.init(0)
+    N, P = ds.n_tips, ds.n_patterns
+    n_cla = 3 * (N - 2)
+    res = {}
+    sweeps = {}
+    for prec, tag in ((E.PREC_FP64, "fp64"), (E.PREC_FP32, "fp32")):
+        inst = ds.make_instance(prec)
+        t = E.Tree(inst, ds.parent, ds.blen)
+        t.set_schedule(0)
+        for _ in range(args.warmup):
+            t.sweep()
+        inst.set_timing(True)
+        inst.kernel_stats(0, reset=True)
+        a = time.perf_counter()
+        for _ in range(args.steps):
+            sw = t.sweep()
+        ms = 1000.0 * (time.perf_counter() - a) / args.steps
+        inst.set_timing(False)
+        n_l, ms_l, bytes_l = inst.kernel_stats(0)
+        sweeps[tag] = sw
+        res[tag] = {"ms_per_sweep": ms, "site_updates_per_s": n_cla * P / (ms / 1000.0),
+                    "cla_kernel_GBs": bytes_l / max(ms_l, 1e-12) / 1e6, "cla_kernel_ms_per_sweep": ms_l / args.steps}
+        t.close()
+        inst.close()
+        log("%s sweep %.3f ms, CLA kernels %.1f GB/s" % (tag, ms, res[tag]["cla_kernel_GBs"]))
+    if S == 61:
+        flops = 3 * (2 * 2 * 61 * 61) * n_cla * P  # useful codon CLA flops per sweep
+        res["fp32"]["codon_mfma_TFLOPs"] = flops / (res["fp32"]["cla_kernel_ms_per_sweep"] / 1000.0) / 1e12
+    pop = _population(ds, 64, 11)
+    e64, lat64 = _eval_rates(ds, E, pop, E.PREC_FP64)
+    e32, _ = _eval_rates(ds, E, pop, E.PREC_FP32)
+    s32, st, served = _serve_rate(ds, E, pop, E.PREC_FP32, 3 * (N - 2) * 16)
+    o = O.Oracle(ds)
+    ref = [o.score(parent=p, blen=b) for p, b in pop[:8]]
+    acc = {"serve_fp32_max_dlnl": float(max(abs(x - y) for x, y in zip(served[:8], ref))),
+           "sweep_fp32_vs_fp64_max_dlnl": float(np.max(np.abs(sweeps["fp32"][:, 0] - sweeps["fp64"][:, 0]))),
+           "sweep_fp64_vs_oracle_max_rel": float(np.max(np.abs(sweeps["fp64"][:, 0] - o.sweep()[:, 0]) /
+                                                        np.abs(sweeps["fp64"][:, 0])))}
+    log("evals/s: fp64 plain %.1f, fp32 plain %.1f, fp32 served %.1f (%.1fx)" % (e64, e32, s32, s32 / e64))
+    out = {"metric": METRIC, "value": s32, "unit": "lnL evals/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": res["fp32"]["ms_per_sweep"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded simulation)",
+           "config": {"workload": name, "taxa": N, "patterns": P, "states": S, "rate_cats": ds.n_rates,
+                      "requests": len(pop)},
+           "fp64_plain_evals_per_s": e64, "fp32_plain_evals_per_s": e32, "fp32_served_evals_per_s": s32,
+           "served_vs_fp64_plain": s32 / e64, "serve_stats": st, "sweeps": res, "accuracy": acc}
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(args):
+    """Config 5: a stream of GA requests with Poisson arrivals over uneven
+    instances (nucleotide and amino-acid, 16-256 taxa, 1k-100k patterns), each
+    a parent tree plus 1-3 NNI/SPR moves; FP32 bulk served vs FP64 plain
+    per-request drop-in."""
+    import threading
+    from garli_amd import engine as E
+    from garli_amd import sim
+    rng = np.random.default_rng(99)
+    E.init(0)
+    specs = [("gtr", 16, 1000), ("aa", 24, 2000), ("gtr", 64, 20000), ("aa", 48, 5000),
+             ("gtr", 128, 50000), ("gtr", 256, 100000), ("aa", 96, 10000), ("gtr", 32, 5000)]
+    dss = []
+    for k, (kind, n, p) in enumerate(specs):
+        if kind == "gtr":
+            dss.append(sim.gtr_gamma(n_tips=n, n_sites=None, n_patterns=p, mean_blen=0.05, pinv=0.1,
+                                     ambig_frac=0.0, seed=200 + k))
+        else:
+            dss.append(sim.aa_gamma(n_tips=n, n_patterns=p, seed=200 + k))
+    n_req = 256
+    which = rng.integers(len(dss), size=n_req)
+    reqs = []
+    for k in range(n_req):
+        ds = dss[which[k]]
+        p, b = sim.perturb(ds.parent, ds.blen, rng, ds.n_tips, int(rng.integers(1, 4)))
+        reqs.append((int(which[k]), p, b))
+    # FP64 plain per-request drop-in
+    insts64 = [ds.make_instance(E.PREC_FP64) for ds in dss]
+    a = time.perf_counter()
+    for w, p, b in reqs:
+        t = E.Tree(insts64[w], p, b)
+        t.score()
+        t.close()
+    t64 = time.perf_counter() - a
+    for i in insts64:
+        i.close()
+    # FP32 bulk served, Poisson arrivals at a rate above the plain path's capacity
+    budget = 16 << 30
+    insts = []
+    for ds in dss:
+        slot_bytes = ds.n_patterns * ds.n_rates * ds.n_states * 4
+        slots = max(2 * (ds.n_tips - 2) + 4, min(8 * (ds.n_tips - 2), budget // len(dss) // slot_bytes))
+        insts.append(ds.make_instance(E.PREC_FP32, n_cla_slots=int(slots), n_pmat_slots=int(8 * ds.n_tips)))
+    srv = E.Server()
+    rate = 4.0 * n_req / t64
+    ids = [None] * n_req
+
+    def producer():
+        for k, (w, p, b) in enumerate(reqs):
+            ids[k] = srv.submit(insts[w], p, b)
+            time.sleep(float(rng.exponential(1.0 / rate)))
+
+    a = time.perf_counter()
+    th = threading.Thread(target=producer)
+    th.start()
+    th.join()
+    lnls = [srv.wait(i)[0] for i in ids]
+    t32 = time.perf_counter() - a
+    st = srv.stats()
+    srv.close()
+    out = {"metric": METRIC, "value": n_req / t32, "unit": "lnL evals/s", "n_gpus": 1, "steps": n_req,
+           "warmup": 0, "ms_per_step": 1000.0 * t32 / n_req, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded simulations)",
+           "config": {"workload": "GA request stream, Poisson arrivals, %d requests over %d instances" %
+                      (n_req, len(dss)), "instances": [list(x) for x in specs], "arrival_rate_per_s": rate},
+           "fp64_plain_evals_per_s": n_req / t64, "served_vs_fp64_plain": t64 / t32,
+           "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_batch": st["mean_batch"]}
+    log("serve: %.1f evals/s (plain fp64 %.1f), p50 %.2f ms p99 %.2f ms, mean batch %.1f" %
+        (n_req / t32, n_req / t64, st["p50_ms"], st["p99_ms"], st["mean_batch"]))
     print(json.dumps(out), flush=True)
 
 

@@ -7,8 +7,8 @@
 //   u      i32 [slots][Ppad]       per-pattern rescale counts (FP64: natural-log units
 //                                   exactly as the reference; FP32: powers of two)
 //   pm64   f64 [pmats][3][R][S][S] P, P', P'' (row = parent state, model.cpp:1271-1358)
-//   pm32   f32 [pmats][R][Sp][Sp]  FP32 copy of P for the FP32-bulk CLA kernels
-//                                   (codon: zero-padded to 64 x 64 for MFMA tiles)
+//   pm32   f32 [pmats][2][R][Sp][Sp] P as an FP32 pair hi + lo (lo = P - hi) for the
+//                                   FP32-bulk CLA kernels (codon: zero-padded to 64 x 64)
 //   counts i32 [Ppad], cst i32 [Ppad] (constant-pattern state/mask, -1 otherwise)
 // Ppad rounds P up to a multiple of 256 so every 256-thread tile is whole.
 #include <hip/hip_runtime.h>
@@ -108,6 +108,7 @@ struct Instance {
   };
   std::vector<Pending> pending;
   std::mutex mu;  // one caller at a time per instance (staging buffers, stream)
+  std::unique_ptr<ClaPool> pool;  // slot allocator of the tree and serving layers
   size_t slot_elems() const { return size_t(Ppad) * R * Sp; }
   size_t elem_bytes() const { return prec == GLE_PREC_FP64 ? 8 : 4; }
 };
@@ -128,6 +129,23 @@ int instance_tips(const Instance *I) { return I->N; }
 int instance_precision(const Instance *I) { return I->prec; }
 int instance_cla_slots(const Instance *I) { return I->n_slots; }
 int instance_pmat_slots(const Instance *I) { return I->n_pm; }
+
+// Guarded by pool_mutex() (callers hold it).
+ClaPool *pool_for(Instance *I) {
+  if (!I->pool) {
+    I->pool = std::make_unique<ClaPool>();
+    I->pool->refs.assign(I->n_slots, 0);
+    I->pool->rank.assign(I->n_slots, 0);
+    for (int s = I->n_slots - 1; s >= 0; s--) I->pool->free_slots.push_back(s);
+    for (int s = I->n_pm - 1; s >= 0; s--) I->pool->pm_free.push_back(s);
+  }
+  return I->pool.get();
+}
+
+std::mutex &pool_mutex() {
+  static std::mutex m;
+  return m;
+}
 
 static int ensure_device() {
   if (g_device >= 0) return GLE_SUCCESS;
@@ -339,7 +357,7 @@ __global__ __launch_bounds__(256) void k_pmat(int S, int Sp, int R, const int *_
   double *P = pm64 + (size_t(slot) * 3 * R + r) * SS;
   double *D1 = pm64 + (size_t(slot) * 3 * R + R + r) * SS;
   double *D2 = pm64 + (size_t(slot) * 3 * R + 2 * R + r) * SS;
-  float *P32 = pm32 + (size_t(slot) * R + r) * Sp * Sp;
+  float *P32 = pm32 + (size_t(slot) * 2 * R + r) * Sp * Sp, *P32lo = P32 + size_t(R) * Sp * Sp;
   for (int q = threadIdx.x; q < SS; q += blockDim.x) {
     const int i = q / S, j = q - i * S;
     double sp = 0.0, s1 = 0.0, s2 = 0.0;
@@ -353,7 +371,9 @@ __global__ __launch_bounds__(256) void k_pmat(int S, int Sp, int R, const int *_
     }
     const double pv = sp > 0.0 ? sp : 0.0;
     P[q] = pv;
-    P32[i * Sp + j] = float(pv);
+    const float hi = float(pv);
+    P32[i * Sp + j] = hi;
+    P32lo[i * Sp + j] = float(pv - double(hi));
     if (derivs) {
       D1[q] = s1;
       D2[q] = s2;
@@ -369,12 +389,52 @@ struct NucClaArgs {
   T *cla;
   int *u;
   const unsigned char *tips;
-  const T *pm;          // P in precision T
+  const T *pm;          // P in precision T (FP32: hi part; the lo part follows at pm_lo)
   size_t pm_stride;     // elements between pmat slots
+  size_t pm_lo;         // offset of the lo part within a slot (FP32 only)
   size_t slot_stride;   // Ppad*R*4
   int Ppad, P;
   int *flags;
 };
+
+// FP32 form: P = hi + lo (both FP32).  A CLA child gives v = sum_j (hi_j + lo_j) x_j
+// with one final rounding; a tip child gives the exact pair (sum hi, sum lo), so
+// that rounding P to FP32 never enters the product as an error shared by every
+// site.  dest = h1*h2 + (h1*l2 + l1*h2).
+__device__ __forceinline__ void nuc_child_f32(const float *hi, const float *lo, const float *x, float *h, float *l) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const float *ph = hi + 4 * i, *pl = lo + 4 * i;
+    float acc = pl[0] * x[0];
+    acc = __builtin_fmaf(pl[1], x[1], acc);
+    acc = __builtin_fmaf(pl[2], x[2], acc);
+    acc = __builtin_fmaf(pl[3], x[3], acc);
+    acc = __builtin_fmaf(ph[3], x[3], acc);
+    acc = __builtin_fmaf(ph[2], x[2], acc);
+    acc = __builtin_fmaf(ph[1], x[1], acc);
+    h[i] = __builtin_fmaf(ph[0], x[0], acc);
+    l[i] = 0.0f;
+  }
+}
+__device__ __forceinline__ void nuc_tip_f32(const float *hi, const float *lo, int m, float *h, float *l) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    if (m == 15) {
+      h[i] = 1.0f;
+      l[i] = 0.0f;
+    } else {
+      float sh = 0.0f, sl = 0.0f;
+#pragma unroll
+      for (int s2 = 0; s2 < 4; s2++)
+        if (m & (1 << s2)) {
+          sh += hi[4 * i + s2];
+          sl += lo[4 * i + s2];
+        }
+      h[i] = sh;
+      l[i] = sl;
+    }
+  }
+}
 
 template <typename T, int R>
 __global__ __launch_bounds__(256) void k_cla_nuc(NucClaArgs<T> a, const DevClaOp *__restrict__ ops, int fused_n) {
@@ -386,7 +446,7 @@ __global__ __launch_bounds__(256) void k_cla_nuc(NucClaArgs<T> a, const DevClaOp
   const int first = fused_n ? 0 : blockIdx.y, last = fused_n ? fused_n : blockIdx.y + 1;
   for (int oi = first; oi < last; ++oi) {
     const DevClaOp op = ops[oi];
-    T v1[4], v2[4], P1[16], P2[16], x[4], out[4];
+    T P1[16], P2[16], x[4], out[4];
     const T *pm1 = a.pm + op.pm1 * a.pm_stride + rr * 16;
     const T *pm2 = a.pm + op.pm2 * a.pm_stride + rr * 16;
 #pragma unroll
@@ -397,34 +457,54 @@ __global__ __launch_bounds__(256) void k_cla_nuc(NucClaArgs<T> a, const DevClaOp
     int u = 0;
     const size_t off = (size_t(p) * R + rr) * 4;
     if (live) {
-      // child 1
-      if (op.c1 >= 0) {
-        load4(a.cla + op.c1 * a.slot_stride + off, x);
-        const bool paired = (R == 4) && (op.c2 >= 0 || __popc(a.tips[size_t(-op.c2 - 1) * a.Ppad + p]) == 1);
+      if constexpr (sizeof(T) == 8) {
+        T v1[4], v2[4];
+        if (op.c1 >= 0) {
+          load4(a.cla + op.c1 * a.slot_stride + off, x);
+          const bool paired = (R == 4) && (op.c2 >= 0 || __popc(a.tips[size_t(-op.c2 - 1) * a.Ppad + p]) == 1);
 #pragma unroll
-        for (int i = 0; i < 4; i++) v1[i] = paired ? dot4_paired(P1 + 4 * i, x) : dot4_seq(P1 + 4 * i, x);
-        u += a.u[op.c1 * size_t(a.Ppad) + p];
+          for (int i = 0; i < 4; i++) v1[i] = paired ? dot4_paired(P1 + 4 * i, x) : dot4_seq(P1 + 4 * i, x);
+          u += a.u[op.c1 * size_t(a.Ppad) + p];
+        } else {
+          const int m = a.tips[size_t(-op.c1 - 1) * a.Ppad + p];
+#pragma unroll
+          for (int i = 0; i < 4; i++) v1[i] = nuc_tip_factor(P1 + 4 * i, m);
+        }
+        if (op.c2 >= 0) {
+          load4(a.cla + op.c2 * a.slot_stride + off, x);
+          const bool paired = (R == 4) && (op.c1 >= 0 || __popc(a.tips[size_t(-op.c1 - 1) * a.Ppad + p]) == 1);
+#pragma unroll
+          for (int i = 0; i < 4; i++) v2[i] = paired ? dot4_paired(P2 + 4 * i, x) : dot4_seq(P2 + 4 * i, x);
+          u += a.u[op.c2 * size_t(a.Ppad) + p];
+        } else {
+          const int m = a.tips[size_t(-op.c2 - 1) * a.Ppad + p];
+#pragma unroll
+          for (int i = 0; i < 4; i++) v2[i] = nuc_tip_factor(P2 + 4 * i, m);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) out[i] = mul_nc(v1[i], v2[i]);
       } else {
-        const int m = a.tips[size_t(-op.c1 - 1) * a.Ppad + p];
+        float L1[16], L2[16], h1[4], l1[4], h2[4], l2[4];
+        const T *lo1 = pm1 + a.pm_lo, *lo2 = pm2 + a.pm_lo;
 #pragma unroll
-        for (int i = 0; i < 4; i++) v1[i] = nuc_tip_factor(P1 + 4 * i, m);
-      }
-      // child 2
-      if (op.c2 >= 0) {
-        load4(a.cla + op.c2 * a.slot_stride + off, x);
-        const bool paired = (R == 4) && (op.c1 >= 0 || __popc(a.tips[size_t(-op.c1 - 1) * a.Ppad + p]) == 1);
+        for (int q = 0; q < 16; q += 4) {
+          load4(lo1 + q, L1 + q);
+          load4(lo2 + q, L2 + q);
+        }
+        if (op.c1 >= 0) {
+          load4(a.cla + op.c1 * a.slot_stride + off, x);
+          nuc_child_f32(P1, L1, x, h1, l1);
+          u += a.u[op.c1 * size_t(a.Ppad) + p];
+        } else
+          nuc_tip_f32(P1, L1, a.tips[size_t(-op.c1 - 1) * a.Ppad + p], h1, l1);
+        if (op.c2 >= 0) {
+          load4(a.cla + op.c2 * a.slot_stride + off, x);
+          nuc_child_f32(P2, L2, x, h2, l2);
+          u += a.u[op.c2 * size_t(a.Ppad) + p];
+        } else
+          nuc_tip_f32(P2, L2, a.tips[size_t(-op.c2 - 1) * a.Ppad + p], h2, l2);
 #pragma unroll
-        for (int i = 0; i < 4; i++) v2[i] = paired ? dot4_paired(P2 + 4 * i, x) : dot4_seq(P2 + 4 * i, x);
-        u += a.u[op.c2 * size_t(a.Ppad) + p];
-      } else {
-        const int m = a.tips[size_t(-op.c2 - 1) * a.Ppad + p];
-#pragma unroll
-        for (int i = 0; i < 4; i++) v2[i] = nuc_tip_factor(P2 + 4 * i, m);
-      }
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        if constexpr (sizeof(T) == 8) out[i] = mul_nc(v1[i], v2[i]);
-        else out[i] = v1[i] * v2[i];
+        for (int i = 0; i < 4; i++) out[i] = __builtin_fmaf(h1[i], h2[i], __builtin_fmaf(h1[i], l2[i], l1[i] * h2[i]));
       }
     } else {
 #pragma unroll
@@ -450,38 +530,49 @@ struct NsClaArgs {
   const T *pm;
   size_t pm_stride, slot_stride;
   int Ppad, P, R, Sp;
-  int prow;  // row stride of P: S for FP64 matrices, Sp for the padded FP32 copy
+  int prow;       // row stride of P: S for FP64 matrices, Sp for the padded FP32 copy
+  size_t pm_lo;   // offset of the lo part of an FP32 slot
   int *flags;
 };
 
 template <typename T, int S>
 __global__ __launch_bounds__(256) void k_cla_ns(NsClaArgs<T> a, const DevClaOp *__restrict__ ops, int fused_n) {
   // P staged in LDS for S <= 32; larger state spaces read P through the caches.
+  // FP32: P is the pair hi + lo (lo at pm_lo) and tip factors stay exact pairs.
   constexpr bool kLds = S <= 32;
+  constexpr bool kF32 = sizeof(T) == 4;
   extern __shared__ unsigned char smem_raw[];
-  T *sP = reinterpret_cast<T *>(smem_raw);  // [2][R][S][S]
+  T *sP = reinterpret_cast<T *>(smem_raw);  // [2 children][hi, lo][R][PR][PR]
   const int R = a.R, RP = R <= 1 ? 1 : R <= 2 ? 2 : R <= 4 ? 4 : 8;
   const int r = threadIdx.x % RP;
   const int p = blockIdx.x * (256 / RP) + threadIdx.x / RP;
   const bool live = (p < a.P) && (r < R);
   const int rr = r < R ? r : R - 1;
   const int first = fused_n ? 0 : blockIdx.y, last = fused_n ? fused_n : blockIdx.y + 1;
-  const int PR = a.prow, RSS = R * PR * PR;
+  const int PR = a.prow, RSS = R * PR * PR, NP = kF32 ? 2 : 1;
   for (int oi = first; oi < last; ++oi) {
     const DevClaOp op = ops[oi];
-    const T *P1, *P2;
+    const T *P1, *P2, *Q1 = nullptr, *Q2 = nullptr;  // Q = lo parts (FP32)
     if constexpr (kLds) {
       if (oi != first) __syncthreads();
-      for (int q = threadIdx.x; q < RSS; q += blockDim.x) {
+      for (int q = threadIdx.x; q < NP * RSS; q += blockDim.x) {
         sP[q] = a.pm[op.pm1 * a.pm_stride + q];
-        sP[RSS + q] = a.pm[op.pm2 * a.pm_stride + q];
+        sP[NP * RSS + q] = a.pm[op.pm2 * a.pm_stride + q];
       }
       __syncthreads();
       P1 = sP + rr * PR * PR;
-      P2 = sP + RSS + rr * PR * PR;
+      P2 = sP + NP * RSS + rr * PR * PR;
+      if constexpr (kF32) {
+        Q1 = P1 + RSS;
+        Q2 = P2 + RSS;
+      }
     } else {
       P1 = a.pm + op.pm1 * a.pm_stride + rr * PR * PR;
       P2 = a.pm + op.pm2 * a.pm_stride + rr * PR * PR;
+      if constexpr (kF32) {
+        Q1 = P1 + a.pm_lo;
+        Q2 = P2 + a.pm_lo;
+      }
     }
     T x1[S], x2[S], out[S];
     int t1 = S, t2 = S, u = 0;
@@ -501,27 +592,49 @@ __global__ __launch_bounds__(256) void k_cla_ns(NsClaArgs<T> a, const DevClaOp *
         t2 = a.tips[size_t(-op.c2 - 1) * a.Ppad + p];
 #pragma unroll 4
       for (int f = 0; f < S; f++) {
-        T L1, R1;
-        if (op.c1 >= 0) {
-          L1 = T(0);
+        if constexpr (kF32) {
+          float h1, l1 = 0.0f, h2, l2 = 0.0f;
+          if (op.c1 >= 0) {
+            float acc = 0.0f;
 #pragma unroll
-          for (int q = 0; q < S; q++) {
-            if constexpr (sizeof(T) == 8) L1 = add_nc(L1, mul_nc(P1[f * PR + q], x1[q]));
-            else L1 = __builtin_fmaf(P1[f * PR + q], x1[q], L1);
-          }
-        } else
-          L1 = t1 < S ? P1[f * PR + t1] : T(1);
-        if (op.c2 >= 0) {
-          R1 = T(0);
+            for (int q = 0; q < S; q++) acc = __builtin_fmaf(Q1[f * PR + q], x1[q], acc);
 #pragma unroll
-          for (int q = 0; q < S; q++) {
-            if constexpr (sizeof(T) == 8) R1 = add_nc(R1, mul_nc(P2[f * PR + q], x2[q]));
-            else R1 = __builtin_fmaf(P2[f * PR + q], x2[q], R1);
-          }
-        } else
-          R1 = t2 < S ? P2[f * PR + t2] : T(1);
-        if constexpr (sizeof(T) == 8) out[f] = mul_nc(L1, R1);
-        else out[f] = L1 * R1;
+            for (int q = 0; q < S; q++) acc = __builtin_fmaf(P1[f * PR + q], x1[q], acc);
+            h1 = acc;
+          } else if (t1 < S) {
+            h1 = P1[f * PR + t1];
+            l1 = Q1[f * PR + t1];
+          } else
+            h1 = 1.0f;
+          if (op.c2 >= 0) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int q = 0; q < S; q++) acc = __builtin_fmaf(Q2[f * PR + q], x2[q], acc);
+#pragma unroll
+            for (int q = 0; q < S; q++) acc = __builtin_fmaf(P2[f * PR + q], x2[q], acc);
+            h2 = acc;
+          } else if (t2 < S) {
+            h2 = P2[f * PR + t2];
+            l2 = Q2[f * PR + t2];
+          } else
+            h2 = 1.0f;
+          out[f] = __builtin_fmaf(h1, h2, __builtin_fmaf(h1, l2, l1 * h2));
+        } else {
+          T L1, R1;
+          if (op.c1 >= 0) {
+            L1 = T(0);
+#pragma unroll
+            for (int q = 0; q < S; q++) L1 = add_nc(L1, mul_nc(P1[f * PR + q], x1[q]));
+          } else
+            L1 = t1 < S ? P1[f * PR + t1] : T(1);
+          if (op.c2 >= 0) {
+            R1 = T(0);
+#pragma unroll
+            for (int q = 0; q < S; q++) R1 = add_nc(R1, mul_nc(P2[f * PR + q], x2[q]));
+          } else
+            R1 = t2 < S ? P2[f * PR + t2] : T(1);
+          out[f] = mul_nc(L1, R1);
+        }
       }
     } else {
 #pragma unroll
@@ -535,7 +648,6 @@ __global__ __launch_bounds__(256) void k_cla_ns(NsClaArgs<T> a, const DevClaOp *
     }
   }
 }
-
 
 // =====================================================================
 // Codon CLA update, FP32 bulk form, on MFMA (61 states padded to 64).
@@ -744,88 +856,80 @@ __device__ __forceinline__ void site_terms(double L, double D1, double D2, int u
   g2 = double(count) * r2;
 }
 
+// One thread per pattern over all categories; P, P', P'' of the op in LDS.
+// Per state the categories accumulate in order, as GetScorePartial*RateHet /
+// GetDerivsPartial* do (tree.cpp:6208-6230, optimization.cpp:3123-3158).
 template <typename T, int R>
 __global__ __launch_bounds__(256) void k_edge_nuc(EdgeArgs<T> a, const DevEdgeOp *__restrict__ ops) {
 #pragma clang fp contract(off)
-  constexpr int RP = Lanes<R>::value;
-  const int r = threadIdx.x % RP;
-  const int p = blockIdx.x * (256 / RP) + threadIdx.x / RP;
-  const bool live = (p < a.P) && (r < R);
-  const int rr = r < R ? r : R - 1;
+  __shared__ double sP[3 * R * 16];
   const DevEdgeOp op = ops[blockIdx.y];
+  const int nmat = op.derivs ? 3 : 1;
+  for (int q = threadIdx.x; q < nmat * R * 16; q += blockDim.x) sP[q] = a.pm64[size_t(op.pm) * 3 * R * 16 + q];
+  __syncthreads();
+  const int p = blockIdx.x * 256 + threadIdx.x;
   const double *pi = a.model, *w = a.model + 4;
-  const double *Pm = a.pm64 + (size_t(op.pm) * 3 * R + rr) * 16;
-  const double *D1m = Pm + size_t(R) * 16, *D2m = Pm + size_t(2 * R) * 16;
-  double L[4] = {0, 0, 0, 0}, G1[4] = {0, 0, 0, 0}, G2[4] = {0, 0, 0, 0};
-  int u = 0;
-  const size_t off = (size_t(p) * R + rr) * 4;
-  if (live) {
-    T xt[4];
-    load4(a.cla + op.partial * a.slot_stride + off, xt);
-    const double x[4] = {double(xt[0]), double(xt[1]), double(xt[2]), double(xt[3])};
-    const double wr = w[rr];
-    u = a.u[op.partial * size_t(a.Ppad) + p];
+  double lnl = 0, g1 = 0, g2 = 0;
+  if (p < a.P) {
+    double L[4] = {0, 0, 0, 0}, G1[4] = {0, 0, 0, 0}, G2[4] = {0, 0, 0, 0};
+    int u = a.u[op.partial * size_t(a.Ppad) + p];
+    const T *xp = a.cla + op.partial * a.slot_stride + size_t(p) * R * 4;
     if (op.child >= 0) {
-      T yt[4];
-      load4(a.cla + op.child * a.slot_stride + off, yt);
-      const double y[4] = {double(yt[0]), double(yt[1]), double(yt[2]), double(yt[3])};
+      const T *yp = a.cla + op.child * a.slot_stride + size_t(p) * R * 4;
       u += a.u[op.child * size_t(a.Ppad) + p];
 #pragma unroll
-      for (int s = 0; s < 4; s++) {
-        L[s] = dot4_seq(Pm + 4 * s, y) * x[s] * wr;
-        if (op.derivs) {
-          G1[s] = dot4_seq(D1m + 4 * s, y) * x[s] * wr;
-          G2[s] = dot4_seq(D2m + 4 * s, y) * x[s] * wr;
+      for (int r = 0; r < R; r++) {
+        T xt[4], yt[4];
+        load4(xp + 4 * r, xt);
+        load4(yp + 4 * r, yt);
+        const double x[4] = {double(xt[0]), double(xt[1]), double(xt[2]), double(xt[3])};
+        const double y[4] = {double(yt[0]), double(yt[1]), double(yt[2]), double(yt[3])};
+        const double *Pm = sP + r * 16, *D1m = sP + (R + r) * 16, *D2m = sP + (2 * R + r) * 16;
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+          L[s] += dot4_seq(Pm + 4 * s, y) * x[s] * w[r];
+          if (op.derivs) {
+            G1[s] += dot4_seq(D1m + 4 * s, y) * x[s] * w[r];
+            G2[s] += dot4_seq(D2m + 4 * s, y) * x[s] * w[r];
+          }
         }
       }
     } else {
       const int m = a.tips[size_t(-op.child - 1) * a.Ppad + p];
-      if (m == 15) {
 #pragma unroll
-        for (int s = 0; s < 4; s++) L[s] = x[s] * wr;
-      } else {
+      for (int r = 0; r < R; r++) {
+        T xt[4];
+        load4(xp + 4 * r, xt);
+        const double x[4] = {double(xt[0]), double(xt[1]), double(xt[2]), double(xt[3])};
+        const double *Pm = sP + r * 16, *D1m = sP + (R + r) * 16, *D2m = sP + (2 * R + r) * 16;
+        if (m == 15) {
 #pragma unroll
-        for (int st = 0; st < 4; st++) {
-          if (!(m & (1 << st))) continue;
+          for (int s = 0; s < 4; s++) L[s] += x[s] * w[r];
+        } else {
 #pragma unroll
           for (int s = 0; s < 4; s++) {
-            L[s] += Pm[4 * s + st] * x[s] * wr;
+            double tl = 0, t1 = 0, t2 = 0;
+#pragma unroll
+            for (int st = 0; st < 4; st++)
+              if (m & (1 << st)) {
+                tl += Pm[4 * s + st];
+                if (op.derivs) {
+                  t1 += D1m[4 * s + st];
+                  t2 += D2m[4 * s + st];
+                }
+              }
+            L[s] += tl * x[s] * w[r];
             if (op.derivs) {
-              G1[s] += D1m[4 * s + st] * x[s] * wr;
-              G2[s] += D2m[4 * s + st] * x[s] * wr;
+              G1[s] += t1 * x[s] * w[r];
+              G2[s] += t2 * x[s] * w[r];
             }
           }
         }
       }
     }
-  }
-  // rate-0 lane sums the categories in order
-  double sL[4], s1[4], s2[4];
-#pragma unroll
-  for (int s = 0; s < 4; s++) {
-    sL[s] = L[s];
-    s1[s] = G1[s];
-    s2[s] = G2[s];
-  }
-#pragma unroll
-  for (int k = 1; k < RP; k++) {
-#pragma unroll
-    for (int s = 0; s < 4; s++) {
-      const double l = __shfl(L[s], (threadIdx.x & ~(RP - 1)) + k);
-      const double g1 = __shfl(G1[s], (threadIdx.x & ~(RP - 1)) + k);
-      const double g2 = __shfl(G2[s], (threadIdx.x & ~(RP - 1)) + k);
-      if (k < R) {
-        sL[s] += l;
-        s1[s] += g1;
-        s2[s] += g2;
-      }
-    }
-  }
-  double lnl = 0, g1 = 0, g2 = 0;
-  if (live && r == 0) {
-    const double Lt = sL[0] * pi[0] + sL[1] * pi[1] + sL[2] * pi[2] + sL[3] * pi[3];
-    const double D1 = s1[0] * pi[0] + s1[1] * pi[1] + s1[2] * pi[2] + s1[3] * pi[3];
-    const double D2 = s2[0] * pi[0] + s2[1] * pi[1] + s2[2] * pi[2] + s2[3] * pi[3];
+    const double Lt = L[0] * pi[0] + L[1] * pi[1] + L[2] * pi[2] + L[3] * pi[3];
+    const double D1 = G1[0] * pi[0] + G1[1] * pi[1] + G1[2] * pi[2] + G1[3] * pi[3];
+    const double D2 = G2[0] * pi[0] + G2[1] * pi[1] + G2[2] * pi[2] + G2[3] * pi[3];
     double pt = 0.0;
     const int cs = a.cst[p];
     if (a.pinv_in_model && cs >= 0) {
@@ -851,8 +955,19 @@ __global__ __launch_bounds__(256) void k_edge_ns(EdgeArgs<T> a, const DevEdgeOp 
   const int rr = r < R ? r : R - 1;
   const DevEdgeOp op = ops[blockIdx.y];
   const double *pi = a.model, *w = a.model + S;
-  const double *Pm = a.pm64 + (size_t(op.pm) * 3 * R + rr) * S * S;
-  const double *D1m = Pm + size_t(R) * S * S, *D2m = Pm + size_t(2 * R) * S * S;
+  const double *Pm, *D1m, *D2m;
+  if constexpr (S <= 32) {  // P, P', P'' of every category staged in LDS
+    extern __shared__ double sPm[];
+    const double *src = a.pm64 + size_t(op.pm) * 3 * R * S * S;
+    const int n = (op.derivs ? 3 : 1) * R * S * S;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) sPm[q] = src[q];
+    __syncthreads();
+    Pm = sPm + size_t(rr) * S * S;
+  } else {
+    Pm = a.pm64 + (size_t(op.pm) * 3 * R + rr) * S * S;
+  }
+  D1m = Pm + size_t(R) * S * S;
+  D2m = Pm + size_t(2 * R) * S * S;
   double rL = 0, r1 = 0, r2 = 0;
   int u = 0;
   const size_t off = (size_t(p) * R + rr) * a.Sp;
@@ -941,13 +1056,24 @@ __global__ __launch_bounds__(64) void k_edge_reduce(const double *__restrict__ p
   }
 }
 
+// Plain device copy of known size: calibrates the HBM byte counters
+// (FETCH_SIZE / WRITE_SIZE) against the CLA kernels' measured traffic.
+__global__ __launch_bounds__(256) void k_copy_calibrate(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                                        size_t n) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+
 // =====================================================================
 // Host-side launchers
 // =====================================================================
 static int lanes_for(int R) { return R <= 1 ? 1 : R <= 2 ? 2 : R <= 4 ? 4 : 8; }
 static int blocks_for(const Instance *I) { return I->Ppad / (256 / lanes_for(I->R)); }
 
-bool engine_supports_fused(const Instance *I) { return I->S == 4 || I->S == 20; }
+// Every CLA kernel can walk a whole topologically ordered op list in one launch:
+// a (pattern, rate) lane reads only what it wrote itself, and the kernels that
+// share data across lanes (LDS-staged P, the codon MFMA tiles) barrier per op.
+bool engine_supports_fused(const Instance *) { return true; }
 
 int engine_update_pmats(Instance *I, const int *idx, const double *t, int n, int derivs) {
   std::lock_guard<std::mutex> lk_(I->mu);
@@ -988,7 +1114,8 @@ static void launch_nuc(Instance *I, const DevClaOp *d_ops, int n, bool fused, in
   a.u = I->d_u;
   a.tips = I->d_tips;
   a.pm = sizeof(T) == 8 ? reinterpret_cast<const T *>(I->d_pm64) : reinterpret_cast<const T *>(I->d_pm32);
-  a.pm_stride = sizeof(T) == 8 ? size_t(3) * R * 16 : size_t(R) * 16;
+  a.pm_stride = sizeof(T) == 8 ? size_t(3) * R * 16 : size_t(2) * R * 16;
+  a.pm_lo = size_t(R) * 16;
   a.slot_stride = I->slot_elems();
   a.Ppad = I->Ppad;
   a.P = I->P;
@@ -1021,14 +1148,15 @@ static void launch_ns(Instance *I, const DevClaOp *d_ops, int n, bool fused, int
   a.tips = I->d_tips;
   a.pm = sizeof(T) == 8 ? reinterpret_cast<const T *>(I->d_pm64) : reinterpret_cast<const T *>(I->d_pm32);
   a.prow = sizeof(T) == 8 ? S : I->Sp;
-  a.pm_stride = sizeof(T) == 8 ? size_t(3) * I->R * S * S : size_t(I->R) * I->Sp * I->Sp;
+  a.pm_stride = sizeof(T) == 8 ? size_t(3) * I->R * S * S : size_t(2) * I->R * I->Sp * I->Sp;
+  a.pm_lo = size_t(I->R) * I->Sp * I->Sp;
   a.slot_stride = I->slot_elems();
   a.Ppad = I->Ppad;
   a.P = I->P;
   a.R = I->R;
   a.Sp = I->Sp;
   a.flags = I->d_flags;
-  const size_t shmem = S <= 32 ? sizeof(T) * 2 * I->R * a.prow * a.prow : 0;
+  const size_t shmem = S <= 32 ? sizeof(T) * 2 * (sizeof(T) == 4 ? 2 : 1) * I->R * a.prow * a.prow : 0;
   hipLaunchKernelGGL((k_cla_ns<T, S>), dim3(nblk, fused ? 1 : n), dim3(256), shmem, I->stream, a, d_ops,
                      fused ? n : 0);
 }
@@ -1040,7 +1168,8 @@ static int launch_codon_mfma(Instance *I, const DevClaOp *d_ops, int n, bool fus
   a.tips = I->d_tips;
   a.pm = I->d_pm32;
   a.prow = 64;
-  a.pm_stride = size_t(I->R) * 64 * 64;
+  a.pm_stride = size_t(2) * I->R * 64 * 64;
+  a.pm_lo = size_t(I->R) * 64 * 64;
   a.slot_stride = I->slot_elems();
   a.Ppad = I->Ppad;
   a.P = I->P;
@@ -1149,13 +1278,17 @@ static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk) {
       default: return set_error(GLE_ERR_UNSUPPORTED, "rate categories must be 1..8");
     }
   } else if (I->S == 20) {
-    hipLaunchKernelGGL((k_edge_ns<T, 20>), grid, dim3(256), 0, I->stream, a, d_ops);
+    const size_t shmem = sizeof(double) * 3 * I->R * 400;
+    if (shmem > 65536) return set_error(GLE_ERR_UNSUPPORTED, "20-state edge kernel takes at most 6 categories");
+    hipLaunchKernelGGL((k_edge_ns<T, 20>), grid, dim3(256), shmem, I->stream, a, d_ops);
   } else if (I->S == 61) {
     hipLaunchKernelGGL((k_edge_ns<T, 61>), grid, dim3(256), 0, I->stream, a, d_ops);
   } else
     return set_error(GLE_ERR_UNSUPPORTED, "state count %d has no kernel", I->S);
   return GLE_SUCCESS;
 }
+
+static int edge_blocks(const Instance *I) { return I->S == 4 ? I->Ppad / 256 : blocks_for(I); }
 
 int engine_edges(Instance *I, const DevEdgeOp *ops, int n, double *out, double *site_lnl) {
   std::lock_guard<std::mutex> lk_(I->mu);
@@ -1166,7 +1299,7 @@ int engine_edges(Instance *I, const DevEdgeOp *ops, int n, double *out, double *
         o.pm < 0 || o.pm >= I->n_pm || o.child == o.partial)
       return set_error(GLE_ERR_ARG, "edge op %d out of range", i);
   }
-  const int nblk = blocks_for(I);
+  const int nblk = edge_blocks(I);
   int rc = grow(I->d_eops, I->eops_cap, n);
   if (rc) return rc;
   rc = grow(I->d_eres, I->eres_cap, 3 * n);
@@ -1323,7 +1456,7 @@ extern "C" int gle_create_instance(const gle_instance_desc *d, int *handle) {
   GLE_ALLOC(I->d_eig_of_rate, sizeof(int) * R);
   GLE_ALLOC(I->d_cla, I->slot_elems() * I->elem_bytes() * I->n_slots);
   GLE_ALLOC(I->d_pm64, sizeof(double) * size_t(I->n_pm) * 3 * R * S * S);
-  GLE_ALLOC(I->d_pm32, sizeof(float) * size_t(I->n_pm) * R * I->Sp * I->Sp);
+  GLE_ALLOC(I->d_pm32, sizeof(float) * size_t(I->n_pm) * 2 * R * I->Sp * I->Sp);
   GLE_ALLOC(I->d_V, sizeof(double) * size_t(I->n_eig) * S * S);
   GLE_ALLOC(I->d_Vinv, sizeof(double) * size_t(I->n_eig) * S * S);
   GLE_ALLOC(I->d_lam, sizeof(double) * size_t(I->n_eig) * S);
@@ -1338,7 +1471,7 @@ extern "C" int gle_create_instance(const gle_instance_desc *d, int *handle) {
       hipMemset(I->d_flags, 0, sizeof(int)) != hipSuccess ||
       hipMemset(I->d_cla, 0, I->slot_elems() * I->elem_bytes() * I->n_slots) != hipSuccess ||
       hipMemset(I->d_pm64, 0, sizeof(double) * size_t(I->n_pm) * 3 * R * S * S) != hipSuccess ||
-      hipMemset(I->d_pm32, 0, sizeof(float) * size_t(I->n_pm) * R * I->Sp * I->Sp) != hipSuccess)
+      hipMemset(I->d_pm32, 0, sizeof(float) * size_t(I->n_pm) * 2 * R * I->Sp * I->Sp) != hipSuccess)
     return fail(set_error(GLE_ERR_DEVICE, "device memset failed"));
   std::vector<int> eor(R, 0);
   std::vector<double> model(S + 2 * R, 0.0);
@@ -1357,6 +1490,7 @@ extern "C" int gle_create_instance(const gle_instance_desc *d, int *handle) {
 }
 
 extern "C" int gle_destroy_instance(int h) {
+  std::lock_guard<std::mutex> pk(pool_mutex());
   std::lock_guard<std::mutex> lk(g_mu);
   if (h < 0 || h >= int(g_instances.size()) || !g_instances[h]) return set_error(GLE_ERR_ARG, "bad instance");
   destroy(g_instances[h].get());
@@ -1500,6 +1634,40 @@ extern "C" int gle_get_kernel_stats(int h, int kind, long long *launches, double
   *ms = I->kst[kind].ms;
   *bytes = I->kst[kind].bytes;
   if (reset) I->kst[kind] = Instance::KStat();
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_copy_calibrate(size_t bytes, int reps, double *ms_per_copy) {
+  if (bytes < 16 || reps < 1 || !ms_per_copy) return set_error(GLE_ERR_ARG, "bad calibration request");
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc = ensure_device();
+    if (rc) return rc;
+  }
+  const size_t n = bytes / 16;
+  float4 *a = nullptr, *b = nullptr;
+  GLE_HIP(hipMalloc(&a, n * 16));
+  if (hipMalloc(&b, n * 16) != hipSuccess) {
+    (void)hipFree(a);
+    return set_error(GLE_ERR_NOMEM, "calibration buffer");
+  }
+  (void)hipMemset(a, 0, n * 16);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  hipLaunchKernelGGL(k_copy_calibrate, dim3(4096), dim3(256), 0, nullptr, a, b, n);
+  (void)hipEventRecord(e0, nullptr);
+  for (int i = 0; i < reps; i++) hipLaunchKernelGGL(k_copy_calibrate, dim3(4096), dim3(256), 0, nullptr, a, b, n);
+  (void)hipEventRecord(e1, nullptr);
+  const hipError_t err = hipEventSynchronize(e1);
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  *ms_per_copy = double(ms) / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (err != hipSuccess) return set_error(GLE_ERR_DEVICE, "calibration copy failed: %s", hipGetErrorString(err));
   return GLE_SUCCESS;
 }
 

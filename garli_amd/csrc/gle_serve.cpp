@@ -87,6 +87,7 @@ struct CacheEntry {
 };
 
 struct ServeInstance {
+  int h = -1;
   Instance *I = nullptr;
   int rescale_every = 16;
   std::unordered_map<Sig, CacheEntry, SigHash> cla;
@@ -496,6 +497,7 @@ extern "C" int gle_serve_destroy(int h) {
   {
     std::lock_guard<std::mutex> pk(pool_mutex());
     for (auto &kv : sv->insts) {
+      if (get_instance(kv.second.h) != kv.second.I) continue;  // destroyed: its pool went with it
       ClaPool *pool = pool_for(kv.second.I);
       for (auto &c : kv.second.cla)
         if (--pool->refs[c.second.slot] == 0) pool->free_slots.push_back(c.second.slot);
@@ -544,6 +546,7 @@ extern "C" int gle_serve_submit(int h, int inst, const int *parent, const double
     std::lock_guard<std::mutex> lk(sv->mu);
     ServeInstance &si = sv->insts[inst];
     if (!si.I) {
+      si.h = inst;
       si.I = I;
       si.rescale_every = instance_precision(I) == GLE_PREC_FP64 ? 16 : 1;
     }

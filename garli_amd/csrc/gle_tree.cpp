@@ -21,29 +21,11 @@
 
 namespace gle {
 
-// ---------------------------------------------------------------- slot pool
+// The slot pool lives in the instance (pool_for, guarded by pool_mutex).
 static std::recursive_mutex g_tree_rmu;
-static std::map<const Instance *, std::unique_ptr<ClaPool>> g_pools;
-
-std::mutex &pool_mutex() {
-  static std::mutex m;
-  return m;
-}
-
-ClaPool *pool_for(Instance *I) {
-  auto &p = g_pools[I];
-  if (!p) {
-    p = std::make_unique<ClaPool>();
-    const int n = instance_cla_slots(I);
-    p->refs.assign(n, 0);
-    p->rank.assign(n, 0);
-    for (int s = n - 1; s >= 0; s--) p->free_slots.push_back(s);
-    for (int s = instance_pmat_slots(I) - 1; s >= 0; s--) p->pm_free.push_back(s);
-  }
-  return p.get();
-}
 
 struct Tree {
+  int inst_h = -1;  // the instance handle; trees of a destroyed instance are orphaned
   Instance *I = nullptr;
   ClaPool *pool = nullptr;
   int N = 0, nn = 0;
@@ -106,8 +88,9 @@ static void dirty_edge(Tree &t, int n) {
   dirty_from(t, a, n);
 }
 
-static int build_tree(Tree &t, Instance *I, const int *parent, const double *blen) {
+static int build_tree(Tree &t, int inst_h, Instance *I, const int *parent, const double *blen) {
   const int N = instance_tips(I), nn = 2 * N - 2;
+  t.inst_h = inst_h;
   t.I = I;
   t.pool = pool_for(I);
   t.N = N;
@@ -201,7 +184,11 @@ static int refresh_pmats(Tree &t, const std::vector<int> &edges, bool derivs) {
 // Compute the listed directional CLAs (already in dependency order with levels).
 static int run_ops(Tree &t, std::vector<std::array<int, 3>> &want) {
   if (want.empty()) return GLE_SUCCESS;
-  std::stable_sort(want.begin(), want.end(), [](const auto &x, const auto &y) { return x[2] < y[2]; });
+  const bool fused = t.schedule == 1 && engine_supports_fused(t.I);
+  // Level schedule: sort by dependency level.  Fused schedule: keep collect()'s
+  // depth-first post-order, so a CLA is consumed soon after it is written.
+  if (!fused)
+    std::stable_sort(want.begin(), want.end(), [](const auto &x, const auto &y) { return x[2] < y[2]; });
   std::vector<int> edges;
   for (auto &w : want) {
     const int a = w[0], k = w[1], b = t.nb[a][k];
@@ -260,7 +247,6 @@ static int run_ops(Tree &t, std::vector<std::array<int, 3>> &want) {
   }
   level_start.push_back(int(ops.size()));
   for (auto &w : want) t.level[size_t(w[0]) * 3 + w[1]] = -1;
-  const bool fused = t.schedule == 1 && engine_supports_fused(t.I);
   return engine_update_partials(t.I, ops.data(), int(ops.size()), level_start.data(), int(level_start.size()) - 1,
                                 fused);
 }
@@ -432,11 +418,13 @@ static void destroy_tree(Tree &t) {
 
 using namespace gle;
 
-#define GET_TREE(h)                                                   \
-  std::lock_guard<std::recursive_mutex> lk_(g_tree_rmu);              \
-  std::lock_guard<std::mutex> pk_(pool_mutex());                      \
-  Tree *T = get_tree(h);                                              \
-  if (!T) return set_error(GLE_ERR_ARG, "bad tree handle %d", (h));
+#define GET_TREE(h)                                                        \
+  std::lock_guard<std::recursive_mutex> lk_(g_tree_rmu);                   \
+  std::lock_guard<std::mutex> pk_(pool_mutex());                           \
+  Tree *T = get_tree(h);                                                   \
+  if (!T) return set_error(GLE_ERR_ARG, "bad tree handle %d", (h));        \
+  if (get_instance(T->inst_h) != T->I)                                     \
+    return set_error(GLE_ERR_STATE, "the tree's instance was destroyed");
 
 extern "C" int gle_tree_create(int inst, const int *parent, const double *blen, int *tree) {
   if (!parent || !blen || !tree) return set_error(GLE_ERR_ARG, "null argument");
@@ -445,7 +433,7 @@ extern "C" int gle_tree_create(int inst, const int *parent, const double *blen, 
   std::lock_guard<std::recursive_mutex> lk(g_tree_rmu);
   std::lock_guard<std::mutex> pk(pool_mutex());
   auto t = std::make_unique<Tree>();
-  const int rc = build_tree(*t, I, parent, blen);
+  const int rc = build_tree(*t, inst, I, parent, blen);
   if (rc) {
     if (!t->pm.empty()) destroy_tree(*t);
     return rc;
@@ -459,7 +447,7 @@ extern "C" int gle_tree_copy(int h, int *out) {
   if (!out) return set_error(GLE_ERR_ARG, "null argument");
   GET_TREE(h);
   auto t = std::make_unique<Tree>();
-  const int rc = build_tree(*t, T->I, T->parent.data(), T->blen.data());
+  const int rc = build_tree(*t, T->inst_h, T->I, T->parent.data(), T->blen.data());
   if (rc) {
     if (!t->pm.empty()) destroy_tree(*t);
     return rc;
@@ -480,8 +468,11 @@ extern "C" int gle_tree_copy(int h, int *out) {
 }
 
 extern "C" int gle_tree_destroy(int h) {
-  GET_TREE(h);
-  destroy_tree(*T);
+  std::lock_guard<std::recursive_mutex> lk(g_tree_rmu);
+  std::lock_guard<std::mutex> pk(pool_mutex());
+  Tree *T = get_tree(h);
+  if (!T) return set_error(GLE_ERR_ARG, "bad tree handle %d", h);
+  if (get_instance(T->inst_h) == T->I) destroy_tree(*T);  // else the pool went with the instance
   g_trees[h].reset();
   return GLE_SUCCESS;
 }

@@ -81,6 +81,7 @@ SIGNATURES = {
     "gle_get_rescale_flags": ([C.c_int, _I, C.c_int], C.c_int),
     "gle_synchronize": ([C.c_int], C.c_int),
     "gle_set_timing": ([C.c_int, C.c_int], C.c_int),
+    "gle_copy_calibrate": ([C.c_size_t, C.c_int, _D], C.c_int),
     "gle_get_kernel_stats": ([C.c_int, C.c_int, C.POINTER(C.c_longlong), _D, _D, C.c_int], C.c_int),
     "gle_tree_create": ([C.c_int, _I, _D, _I], C.c_int),
     "gle_tree_copy": ([C.c_int, _I], C.c_int),
@@ -142,6 +143,13 @@ def init(device=0):
 
 def device_count():
     return lib().gle_device_count()
+
+
+def copy_calibrate(nbytes, reps=10):
+    """Mean ms of a plain device copy of nbytes (reads nbytes, writes nbytes)."""
+    ms = C.c_double(0)
+    _check(lib().gle_copy_calibrate(int(nbytes), int(reps), C.byref(ms)))
+    return ms.value
 
 
 # ------------------------------------------------------------ model helpers

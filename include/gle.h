@@ -150,6 +150,10 @@ int gle_synchronize(int h);
  * (true S, not padded).  No reference counterpart: the reference's Profiler
  * buckets (tree.cpp:41-47, utility.h:240-323) time whole CPU kernels instead. */
 int gle_set_timing(int h, int enable);
+/* Plain device-to-device copy of `bytes` (read + write), `reps` times; returns
+ * the mean ms per copy.  No reference counterpart: it calibrates hardware
+ * byte counters and the achievable HBM bandwidth for the roofline. */
+int gle_copy_calibrate(size_t bytes, int reps, double *ms_per_copy);
 int gle_get_kernel_stats(int h, int kind, long long *launches, double *ms, double *bytes, int reset);
 
 /* ---------------------------------------------------------------------------
