@@ -78,6 +78,9 @@ def main():
     prec = E.PREC_FP32 if args.precision == "fp32" else E.PREC_FP64
 
     inst = ds.make_instance(prec)
+    n_heavy = inst.heavy_patterns()
+    if n_heavy:
+        log("%d patterns with count > 4 run in FP64 beside the FP32 bulk" % n_heavy)
     tree = E.Tree(inst, ds.parent, ds.blen)
     tree.set_schedule(1 if args.schedule == "fused" else 0)
 
@@ -94,7 +97,7 @@ def main():
 
     # roofline of the dominant kernel (CLA update), HIP events on the instance stream
     inst.set_timing(True)
-    for k in range(3):
+    for k in range(6):
         inst.kernel_stats(k, reset=True)
     for _ in range(max(3, args.steps // 2)):
         tree.sweep()
@@ -106,6 +109,11 @@ def main():
     achieved = bytes_l / max(ms_l, 1e-12) / 1e6  # GB/s
     log("CLA kernel: %d launches, avg %.3f ms, %.1f GB/s; edge kernel avg %.3f ms %.1f GB/s; pmat avg %.3f ms" %
         (n_l, avg_ms, achieved, e_ms / max(e_l, 1), e_bytes / max(e_ms, 1e-12) / 1e6, p_ms / max(p_l, 1)))
+    h_l, h_ms, _ = inst.kernel_stats(3)
+    he_l, he_ms, _ = inst.kernel_stats(4)
+    if h_l:
+        log("FP64 heavy part: CLA %d launches avg %.3f ms, edge avg %.3f ms (own stream)" %
+            (h_l, h_ms / h_l, he_ms / max(he_l, 1)))
 
     copy = None
     if args.calibrate > 0:
@@ -186,7 +194,8 @@ def main():
         "config": {"workload": "nucleotide GTR+I+G4 full-tree sweep, %d taxa, %d patterns" % (N, P),
                    "taxa": N, "patterns": P, "states": 4, "rate_cats": 4, "pinv": 0.2,
                    "alpha": round(ds.meta["alpha"], 4), "form": "fp32-bulk" if prec == E.PREC_FP32 else "fp64",
-                   "schedule": args.schedule, "cla_updates_per_step": n_cla, "edges_per_step": n_edges},
+                   "schedule": args.schedule, "cla_updates_per_step": n_cla, "edges_per_step": n_edges,
+                   "fp64_heavy_patterns": n_heavy},
         "lnl_evals_per_s": evals_per_s,
         "lnl_eval_p50_ms": float(np.percentile(lat, 50)),
         "lnl_eval_p99_ms": float(np.percentile(lat, 99)),

@@ -109,6 +109,22 @@ struct Instance {
   std::vector<Pending> pending;
   std::mutex mu;  // one caller at a time per instance (staging buffers, stream)
   std::unique_ptr<ClaPool> pool;  // slot allocator of the tree and serving layers
+  // FP32 instances: patterns whose count exceeds heavy_threshold run in an FP64
+  // sub-instance that receives the same slots, P(t) and op lists, so FP32
+  // rounding is never multiplied by a large pattern weight.
+  int heavy_threshold = 4;  // 0 keeps every pattern in FP32
+  std::unique_ptr<Instance> heavy;
+  int P_orig = 0;
+  std::vector<int> light_idx, heavy_idx;  // original pattern indices of each part
+  // host copies (the split is re-derived whenever the patterns change)
+  std::vector<unsigned char> h_tips;  // [N][P_orig]
+  std::vector<int> h_counts, h_cst;
+  int h_last_const = -1;
+  bool h_patterns = false;
+  std::vector<double> h_pi, h_w, h_V, h_Vinv, h_lam, h_mult;
+  std::vector<int> h_eor;
+  std::vector<char> h_eig_set;
+  bool h_model = false, h_rates = false;
   size_t slot_elems() const { return size_t(Ppad) * R * Sp; }
   size_t elem_bytes() const { return prec == GLE_PREC_FP64 ? 8 : 4; }
 };
@@ -124,7 +140,7 @@ Instance *get_instance(int h) {
 }
 int instance_states(const Instance *I) { return I->S; }
 int instance_rates(const Instance *I) { return I->R; }
-int instance_patterns(const Instance *I) { return I->P; }
+int instance_patterns(const Instance *I) { return I->P_orig; }
 int instance_tips(const Instance *I) { return I->N; }
 int instance_precision(const Instance *I) { return I->prec; }
 int instance_cla_slots(const Instance *I) { return I->n_slots; }
@@ -1076,6 +1092,10 @@ static int blocks_for(const Instance *I) { return I->Ppad / (256 / lanes_for(I->
 bool engine_supports_fused(const Instance *) { return true; }
 
 int engine_update_pmats(Instance *I, const int *idx, const double *t, int n, int derivs) {
+  if (I->heavy) {
+    const int rc = engine_update_pmats(I->heavy.get(), idx, t, n, derivs);
+    if (rc) return rc;
+  }
   std::lock_guard<std::mutex> lk_(I->mu);
   if (n <= 0) return GLE_SUCCESS;
   for (int i = 0; i < n; i++)
@@ -1202,6 +1222,10 @@ static int launch_cla(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
 
 int engine_update_partials(Instance *I, const DevClaOp *ops, int n_ops, const int *level_start, int n_levels,
                            bool fused) {
+  if (I->heavy) {
+    const int rc = engine_update_partials(I->heavy.get(), ops, n_ops, level_start, n_levels, fused);
+    if (rc) return rc;
+  }
   std::lock_guard<std::mutex> lk_(I->mu);
   if (n_ops <= 0) return GLE_SUCCESS;
   for (int i = 0; i < n_ops; i++) {
@@ -1290,7 +1314,7 @@ static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk) {
 
 static int edge_blocks(const Instance *I) { return I->S == 4 ? I->Ppad / 256 : blocks_for(I); }
 
-int engine_edges(Instance *I, const DevEdgeOp *ops, int n, double *out, double *site_lnl) {
+static int engine_edges_one(Instance *I, const DevEdgeOp *ops, int n, double *out, double *site_lnl) {
   std::lock_guard<std::mutex> lk_(I->mu);
   if (n <= 0) return GLE_SUCCESS;
   for (int i = 0; i < n; i++) {
@@ -1331,7 +1355,44 @@ int engine_edges(Instance *I, const DevEdgeOp *ops, int n, double *out, double *
   return GLE_SUCCESS;
 }
 
+// Edge results of an instance and its heavy-pattern part, in the caller's pattern order.
+int engine_edges(Instance *I, const DevEdgeOp *ops, int n, double *out, double *site_lnl) {
+  if (!I->heavy) {
+    if (I->light_idx.empty() || !site_lnl) return engine_edges_one(I, ops, n, out, site_lnl);
+  }
+  std::vector<double> site_l(site_lnl ? std::max(I->P, 1) : 0), site_h, out_h(3 * size_t(n), 0.0);
+  int rc = engine_edges_one(I, ops, n, out, site_lnl ? site_l.data() : nullptr);
+  if (rc) return rc;
+  if (I->heavy) {
+    site_h.resize(site_lnl ? std::max(I->heavy->P, 1) : 0);
+    rc = engine_edges_one(I->heavy.get(), ops, n, out_h.data(), site_lnl ? site_h.data() : nullptr);
+    if (rc) return rc;
+    for (int i = 0; i < 3 * n; i++) out[i] += out_h[i];
+  }
+  if (site_lnl) {
+    for (size_t k = 0; k < I->light_idx.size(); k++) site_lnl[I->light_idx[k]] = site_l[k];
+    for (size_t k = 0; k < I->heavy_idx.size(); k++) site_lnl[I->heavy_idx[k]] = site_h[k];
+  }
+  return GLE_SUCCESS;
+}
+
 int engine_flags(Instance *I, int *flags, int reset) {
+  if (I->heavy) {
+    int hf = 0;
+    const int rc = engine_flags(I->heavy.get(), &hf, reset);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk_(I->mu);
+    int f = 0;
+    GLE_HIP(hipMemcpyAsync(&f, I->d_flags, sizeof(int), hipMemcpyDeviceToHost, I->stream));
+    GLE_HIP(hipStreamSynchronize(I->stream));
+    if (reset) {
+      const int z = 0;
+      GLE_HIP(hipMemcpyAsync(I->d_flags, &z, sizeof(int), hipMemcpyHostToDevice, I->stream));
+      GLE_HIP(hipStreamSynchronize(I->stream));
+    }
+    *flags = f | hf;
+    return GLE_SUCCESS;
+  }
   std::lock_guard<std::mutex> lk_(I->mu);
   GLE_HIP(hipMemcpyAsync(flags, I->d_flags, sizeof(int), hipMemcpyDeviceToHost, I->stream));
   GLE_HIP(hipStreamSynchronize(I->stream));
@@ -1343,7 +1404,35 @@ int engine_flags(Instance *I, int *flags, int reset) {
   return GLE_SUCCESS;
 }
 
+static int engine_get_partials_one(Instance *I, int slot, double *out, int *u);
+
+// One CLA slot in the caller's pattern order.  In a split FP32 instance the
+// heavy patterns' counts are natural-log units (FP64 scheme), the others powers of two.
 int engine_get_partials(Instance *I, int slot, double *out, int *u) {
+  if (I->light_idx.empty() && !I->heavy) return engine_get_partials_one(I, slot, out, u);
+  const size_t W = size_t(I->R) * I->S;
+  std::vector<double> ol(std::max<size_t>(1, I->light_idx.size() * W)), oh;
+  std::vector<int> ul(std::max<size_t>(1, I->light_idx.size())), uh;
+  int rc = engine_get_partials_one(I, slot, ol.data(), ul.data());
+  if (rc) return rc;
+  if (I->heavy) {
+    oh.resize(std::max<size_t>(1, I->heavy_idx.size() * W));
+    uh.resize(std::max<size_t>(1, I->heavy_idx.size()));
+    rc = engine_get_partials_one(I->heavy.get(), slot, oh.data(), uh.data());
+    if (rc) return rc;
+  }
+  for (size_t k = 0; k < I->light_idx.size(); k++) {
+    std::copy(ol.begin() + k * W, ol.begin() + (k + 1) * W, out + size_t(I->light_idx[k]) * W);
+    u[I->light_idx[k]] = ul[k];
+  }
+  for (size_t k = 0; k < I->heavy_idx.size(); k++) {
+    std::copy(oh.begin() + k * W, oh.begin() + (k + 1) * W, out + size_t(I->heavy_idx[k]) * W);
+    u[I->heavy_idx[k]] = uh[k];
+  }
+  return GLE_SUCCESS;
+}
+
+static int engine_get_partials_one(Instance *I, int slot, double *out, int *u) {
   std::lock_guard<std::mutex> lk_(I->mu);
   if (slot < 0 || slot >= I->n_slots) return set_error(GLE_ERR_ARG, "slot %d out of range", slot);
   const size_t W = size_t(I->P) * I->R * I->Sp;
@@ -1362,12 +1451,20 @@ int engine_get_partials(Instance *I, int slot, double *out, int *u) {
 }
 
 int engine_sync(Instance *I) {
+  if (I->heavy) {
+    const int rc = engine_sync(I->heavy.get());
+    if (rc) return rc;
+  }
   std::lock_guard<std::mutex> lk_(I->mu);
   GLE_HIP(hipStreamSynchronize(I->stream));
   return GLE_SUCCESS;
 }
 
 static int destroy(Instance *I) {
+  if (I->heavy) {
+    destroy(I->heavy.get());
+    I->heavy.reset();
+  }
   void *ptrs[] = {I->d_tips, I->d_counts, I->d_cst, I->d_u, I->d_flags, I->d_eig_of_rate, I->d_cla,
                   I->d_pm64, I->d_pm32,  I->d_V,   I->d_Vinv, I->d_lam, I->d_model, I->d_ops,
                   I->d_eops, I->d_part,  I->d_eres, I->d_site, I->d_pmidx, I->d_pmt};
@@ -1412,22 +1509,11 @@ extern "C" int gle_init(int device) {
   return GLE_SUCCESS;
 }
 
-extern "C" int gle_create_instance(const gle_instance_desc *d, int *handle) {
-  if (!d || !handle) return set_error(GLE_ERR_ARG, "null argument");
-  if (d->n_tips < 2 || d->n_patterns < 1 || d->n_states < 2 || d->n_states > 64 || d->n_rate_cats < 1 ||
-      d->n_rate_cats > 8 || d->n_eigen < 1 || d->n_eigen > d->n_rate_cats || d->n_cla_slots < 1 ||
-      d->n_pmat_slots < 1 || (d->precision != GLE_PREC_FP64 && d->precision != GLE_PREC_FP32))
-    return set_error(GLE_ERR_ARG, "invalid instance description");
-  if (d->n_states != 4 && d->n_states != 20 && d->n_states != 61)
-    return set_error(GLE_ERR_UNSUPPORTED, "state count %d: kernels exist for 4, 20 and 61", d->n_states);
-  {
-    std::lock_guard<std::mutex> lk(g_mu);
-    int rc = ensure_device();
-    if (rc) return rc;
-  }
+static int create_internal(const gle_instance_desc *d, std::unique_ptr<Instance> &out) {
   auto I = std::make_unique<Instance>();
   I->N = d->n_tips;
   I->P = d->n_patterns;
+  I->P_orig = d->n_patterns;
   I->Ppad = (d->n_patterns + 255) / 256 * 256;
   I->S = d->n_states;
   I->Sp = d->n_states == 61 ? 64 : d->n_states;
@@ -1464,7 +1550,8 @@ extern "C" int gle_create_instance(const gle_instance_desc *d, int *handle) {
   GLE_ALLOC(I->d_site, sizeof(double) * I->Ppad);
 #undef GLE_ALLOC
   // zero everything the kernels may read before the caller sets it
-  if (hipMemset(I->d_tips, 15, size_t(I->N) * I->Ppad) != hipSuccess ||
+  const unsigned char missing = (unsigned char)(S == 4 ? 15 : S);
+  if (hipMemset(I->d_tips, missing, size_t(I->N) * I->Ppad) != hipSuccess ||
       hipMemset(I->d_counts, 0, sizeof(int) * I->Ppad) != hipSuccess ||
       hipMemset(I->d_cst, 0xff, sizeof(int) * I->Ppad) != hipSuccess ||
       hipMemset(I->d_u, 0, sizeof(int) * size_t(I->n_slots) * I->Ppad) != hipSuccess ||
@@ -1473,16 +1560,149 @@ extern "C" int gle_create_instance(const gle_instance_desc *d, int *handle) {
       hipMemset(I->d_pm64, 0, sizeof(double) * size_t(I->n_pm) * 3 * R * S * S) != hipSuccess ||
       hipMemset(I->d_pm32, 0, sizeof(float) * size_t(I->n_pm) * 2 * R * I->Sp * I->Sp) != hipSuccess)
     return fail(set_error(GLE_ERR_DEVICE, "device memset failed"));
-  std::vector<int> eor(R, 0);
+  I->h_tips.assign(size_t(I->N) * I->P_orig, missing);
+  I->h_pi.assign(S, 1.0 / S);
+  I->h_w.assign(R, 1.0 / R);
+  I->h_mult.assign(R, 1.0);
+  I->h_eor.assign(R, 0);
+  I->h_V.assign(size_t(I->n_eig) * S * S, 0.0);
+  I->h_Vinv.assign(size_t(I->n_eig) * S * S, 0.0);
+  I->h_lam.assign(size_t(I->n_eig) * S, 0.0);
+  I->h_eig_set.assign(I->n_eig, 0);
   std::vector<double> model(S + 2 * R, 0.0);
-  for (int s = 0; s < S; s++) model[s] = 1.0 / S;
+  for (int q = 0; q < S; q++) model[q] = 1.0 / S;
   for (int r = 0; r < R; r++) {
     model[S + r] = 1.0 / R;
     model[S + R + r] = 1.0;
   }
-  if (hipMemcpy(I->d_eig_of_rate, eor.data(), sizeof(int) * R, hipMemcpyHostToDevice) != hipSuccess ||
+  if (hipMemcpy(I->d_eig_of_rate, I->h_eor.data(), sizeof(int) * R, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(I->d_model, model.data(), sizeof(double) * model.size(), hipMemcpyHostToDevice) != hipSuccess)
     return fail(set_error(GLE_ERR_DEVICE, "device upload failed"));
+  out = std::move(I);
+  return GLE_SUCCESS;
+}
+
+// ---- pushing host state to a device instance (the main one or its heavy part)
+static int push_tip_row(Instance *X, const Instance *M, int tip, const std::vector<int> *idx) {
+  const unsigned char missing = (unsigned char)(X->S == 4 ? 15 : X->S);
+  std::vector<unsigned char> m(X->Ppad, missing);
+  const unsigned char *row = M->h_tips.data() + size_t(tip) * M->P_orig;
+  for (int k = 0; k < X->P; k++) m[k] = row[idx ? (*idx)[k] : k];
+  GLE_HIP(hipMemcpy(X->d_tips + size_t(tip) * X->Ppad, m.data(), X->Ppad, hipMemcpyHostToDevice));
+  return GLE_SUCCESS;
+}
+
+static int push_patterns(Instance *X, const Instance *M, const std::vector<int> *idx) {
+  std::vector<int> c(X->Ppad, 0), k(X->Ppad, -1);
+  for (int q = 0; q < X->P; q++) {
+    const int o = idx ? (*idx)[q] : q;
+    c[q] = M->h_counts[o];
+    k[q] = M->h_cst[o];
+  }
+  GLE_HIP(hipMemcpy(X->d_counts, c.data(), sizeof(int) * X->Ppad, hipMemcpyHostToDevice));
+  GLE_HIP(hipMemcpy(X->d_cst, k.data(), sizeof(int) * X->Ppad, hipMemcpyHostToDevice));
+  return GLE_SUCCESS;
+}
+
+static int push_model(Instance *X, const Instance *M) {
+  const int S = X->S, R = X->R;
+  std::vector<double> m(S + 2 * R);
+  for (int q = 0; q < S; q++) m[q] = M->h_pi[q];
+  for (int r = 0; r < R; r++) {
+    m[S + r] = M->h_w[r];
+    m[S + R + r] = M->h_mult[r];
+  }
+  GLE_HIP(hipMemcpy(X->d_model, m.data(), sizeof(double) * m.size(), hipMemcpyHostToDevice));
+  GLE_HIP(hipMemcpy(X->d_eig_of_rate, M->h_eor.data(), sizeof(int) * R, hipMemcpyHostToDevice));
+  GLE_HIP(hipMemcpy(X->d_V, M->h_V.data(), sizeof(double) * M->h_V.size(), hipMemcpyHostToDevice));
+  GLE_HIP(hipMemcpy(X->d_Vinv, M->h_Vinv.data(), sizeof(double) * M->h_Vinv.size(), hipMemcpyHostToDevice));
+  GLE_HIP(hipMemcpy(X->d_lam, M->h_lam.data(), sizeof(double) * M->h_lam.size(), hipMemcpyHostToDevice));
+  X->pinv = M->pinv;
+  X->pinv_in_model = M->pinv_in_model;
+  return GLE_SUCCESS;
+}
+
+// Split the patterns by weight (FP32 instances) and push tips/counts to both parts.
+static int apply_patterns(Instance *I) {
+  I->light_idx.clear();
+  I->heavy_idx.clear();
+  if (I->prec == GLE_PREC_FP32 && I->heavy_threshold > 0)
+    for (int q = 0; q < I->P_orig; q++)
+      (I->h_counts[q] > I->heavy_threshold ? I->heavy_idx : I->light_idx).push_back(q);
+  int rc;
+  if (I->heavy_idx.empty()) {
+    I->light_idx.clear();
+    if (I->heavy) {
+      destroy(I->heavy.get());
+      I->heavy.reset();
+    }
+    I->P = I->P_orig;
+    if ((rc = push_patterns(I, I, nullptr))) return rc;
+    for (int t = 0; t < I->N; t++)
+      if ((rc = push_tip_row(I, I, t, nullptr))) return rc;
+    return GLE_SUCCESS;
+  }
+  I->P = int(I->light_idx.size());
+  if (!I->heavy || I->heavy->P != int(I->heavy_idx.size())) {
+    if (I->heavy) {
+      destroy(I->heavy.get());
+      I->heavy.reset();
+    }
+    gle_instance_desc d;
+    d.n_tips = I->N;
+    d.n_patterns = int(I->heavy_idx.size());
+    d.n_states = I->S;
+    d.n_rate_cats = I->R;
+    d.n_eigen = I->n_eig;
+    d.n_cla_slots = I->n_slots;
+    d.n_pmat_slots = I->n_pm;
+    d.precision = GLE_PREC_FP64;
+    if ((rc = create_internal(&d, I->heavy))) return rc;
+    I->heavy->heavy_threshold = 0;
+    I->heavy->timing = I->timing;
+    // P(t) slots already filled for the main part stay valid for the new one
+    GLE_HIP(hipStreamSynchronize(I->stream));
+    GLE_HIP(hipMemcpy(I->heavy->d_pm64, I->d_pm64, sizeof(double) * size_t(I->n_pm) * 3 * I->R * I->S * I->S,
+                      hipMemcpyDeviceToDevice));
+  }
+  if ((rc = push_model(I->heavy.get(), I))) return rc;
+  if ((rc = push_patterns(I, I, &I->light_idx))) return rc;
+  if ((rc = push_patterns(I->heavy.get(), I, &I->heavy_idx))) return rc;
+  for (int t = 0; t < I->N; t++) {
+    if ((rc = push_tip_row(I, I, t, &I->light_idx))) return rc;
+    if ((rc = push_tip_row(I->heavy.get(), I, t, &I->heavy_idx))) return rc;
+  }
+  return GLE_SUCCESS;
+}
+
+static int push_tip(Instance *I, int tip) {
+  int rc = push_tip_row(I, I, tip, I->light_idx.empty() ? nullptr : &I->light_idx);
+  if (!rc && I->heavy) rc = push_tip_row(I->heavy.get(), I, tip, &I->heavy_idx);
+  return rc;
+}
+
+static int push_model_all(Instance *I) {
+  int rc = push_model(I, I);
+  if (!rc && I->heavy) rc = push_model(I->heavy.get(), I);
+  return rc;
+}
+
+extern "C" int gle_create_instance(const gle_instance_desc *d, int *handle) {
+  if (!d || !handle) return set_error(GLE_ERR_ARG, "null argument");
+  if (d->n_tips < 2 || d->n_patterns < 1 || d->n_states < 2 || d->n_states > 64 || d->n_rate_cats < 1 ||
+      d->n_rate_cats > 8 || d->n_eigen < 1 || d->n_eigen > d->n_rate_cats || d->n_cla_slots < 1 ||
+      d->n_pmat_slots < 1 || (d->precision != GLE_PREC_FP64 && d->precision != GLE_PREC_FP32))
+    return set_error(GLE_ERR_ARG, "invalid instance description");
+  if (d->n_states != 4 && d->n_states != 20 && d->n_states != 61)
+    return set_error(GLE_ERR_UNSUPPORTED, "state count %d: kernels exist for 4, 20 and 61", d->n_states);
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc = ensure_device();
+    if (rc) return rc;
+  }
+  std::unique_ptr<Instance> I;
+  const int rc = create_internal(d, I);
+  if (rc) return rc;
   std::lock_guard<std::mutex> lk(g_mu);
   g_instances.push_back(std::move(I));
   *handle = int(g_instances.size()) - 1;
@@ -1506,63 +1726,74 @@ extern "C" int gle_set_tip_ambig_stream(int h, int tip, const signed char *strea
   GET_INSTANCE(h);
   if (I->S != 4) return set_error(GLE_ERR_ARG, "ambiguity streams are nucleotide data");
   if (tip < 0 || tip >= I->N || !stream) return set_error(GLE_ERR_ARG, "bad tip");
-  std::vector<unsigned char> m(I->Ppad, 15);
-  int used = ambig_string_to_masks(stream, len, I->P, m.data());
+  std::vector<unsigned char> m(I->P_orig, 15);
+  int used = ambig_string_to_masks(stream, len, I->P_orig, m.data());
   if (used < 0) return set_error(GLE_ERR_ARG, "malformed ambiguity stream for tip %d", tip);
-  GLE_HIP(hipMemcpy(I->d_tips + size_t(tip) * I->Ppad, m.data(), I->Ppad, hipMemcpyHostToDevice));
-  return GLE_SUCCESS;
+  std::copy(m.begin(), m.end(), I->h_tips.begin() + size_t(tip) * I->P_orig);
+  return push_tip(I, tip);
 }
 
 extern "C" int gle_set_tip_states(int h, int tip, const unsigned char *states) {
   GET_INSTANCE(h);
   if (tip < 0 || tip >= I->N || !states) return set_error(GLE_ERR_ARG, "bad tip");
-  std::vector<unsigned char> m(I->Ppad, (unsigned char)(I->S == 4 ? 15 : I->S));
-  for (int p = 0; p < I->P; p++) {
-    const int v = states[p];
+  for (int q = 0; q < I->P_orig; q++) {
+    const int v = states[q];
     if ((I->S == 4 && (v < 1 || v > 15)) || (I->S != 4 && v > I->S))
-      return set_error(GLE_ERR_ARG, "tip %d pattern %d: code %d out of range", tip, p, v);
-    m[p] = (unsigned char)v;
+      return set_error(GLE_ERR_ARG, "tip %d pattern %d: code %d out of range", tip, q, v);
   }
-  GLE_HIP(hipMemcpy(I->d_tips + size_t(tip) * I->Ppad, m.data(), I->Ppad, hipMemcpyHostToDevice));
-  return GLE_SUCCESS;
+  std::copy(states, states + I->P_orig, I->h_tips.begin() + size_t(tip) * I->P_orig);
+  return push_tip(I, tip);
 }
 
 extern "C" int gle_set_patterns(int h, const int *counts, int last_const, const int *cst) {
   GET_INSTANCE(h);
-  if (!counts || last_const < -1 || last_const >= I->P || (last_const >= 0 && !cst))
+  if (!counts || last_const < -1 || last_const >= I->P_orig || (last_const >= 0 && !cst))
     return set_error(GLE_ERR_ARG, "bad pattern info");
-  std::vector<int> c(I->Ppad, 0), k(I->Ppad, -1);
-  for (int p = 0; p < I->P; p++) {
-    if (counts[p] < 0) return set_error(GLE_ERR_ARG, "negative count");
-    c[p] = counts[p];
-  }
-  for (int p = 0; p <= last_const; p++) k[p] = cst[p];
-  GLE_HIP(hipMemcpy(I->d_counts, c.data(), sizeof(int) * I->Ppad, hipMemcpyHostToDevice));
-  GLE_HIP(hipMemcpy(I->d_cst, k.data(), sizeof(int) * I->Ppad, hipMemcpyHostToDevice));
+  for (int q = 0; q < I->P_orig; q++)
+    if (counts[q] < 0) return set_error(GLE_ERR_ARG, "negative count");
+  I->h_counts.assign(counts, counts + I->P_orig);
+  I->h_cst.assign(I->P_orig, -1);
+  for (int q = 0; q <= last_const; q++) I->h_cst[q] = cst[q];
+  I->h_last_const = last_const;
+  I->h_patterns = true;
   I->last_const = last_const;
+  return apply_patterns(I);
+}
+
+extern "C" int gle_set_heavy_split(int h, int threshold) {
+  GET_INSTANCE(h);
+  if (threshold < 0) return set_error(GLE_ERR_ARG, "threshold must be >= 0");
+  I->heavy_threshold = threshold;
+  return I->h_patterns ? apply_patterns(I) : GLE_SUCCESS;
+}
+
+extern "C" int gle_get_heavy_split(int h, int *n_heavy) {
+  GET_INSTANCE(h);
+  if (!n_heavy) return set_error(GLE_ERR_ARG, "null output");
+  *n_heavy = int(I->heavy_idx.size());
   return GLE_SUCCESS;
 }
 
 extern "C" int gle_set_model(int h, const double *pi, const double *w, double pinv, int pinv_in_model) {
   GET_INSTANCE(h);
   if (!pi || !w || pinv < 0.0 || pinv >= 1.0) return set_error(GLE_ERR_ARG, "bad model");
-  std::vector<double> m(I->S + I->R);
-  for (int s = 0; s < I->S; s++) m[s] = pi[s];
-  for (int r = 0; r < I->R; r++) m[I->S + r] = w[r];
-  GLE_HIP(hipMemcpy(I->d_model, m.data(), sizeof(double) * m.size(), hipMemcpyHostToDevice));
+  I->h_pi.assign(pi, pi + I->S);
+  I->h_w.assign(w, w + I->R);
   I->pinv = pinv;
   I->pinv_in_model = pinv_in_model ? 1 : 0;
-  return GLE_SUCCESS;
+  I->h_model = true;
+  return push_model_all(I);
 }
 
 extern "C" int gle_set_eigen(int h, int e, const double *V, const double *Vinv, const double *lam) {
   GET_INSTANCE(h);
   if (e < 0 || e >= I->n_eig || !V || !Vinv || !lam) return set_error(GLE_ERR_ARG, "bad eigensystem");
   const size_t SS = size_t(I->S) * I->S;
-  GLE_HIP(hipMemcpy(I->d_V + e * SS, V, sizeof(double) * SS, hipMemcpyHostToDevice));
-  GLE_HIP(hipMemcpy(I->d_Vinv + e * SS, Vinv, sizeof(double) * SS, hipMemcpyHostToDevice));
-  GLE_HIP(hipMemcpy(I->d_lam + size_t(e) * I->S, lam, sizeof(double) * I->S, hipMemcpyHostToDevice));
-  return GLE_SUCCESS;
+  std::copy(V, V + SS, I->h_V.begin() + e * SS);
+  std::copy(Vinv, Vinv + SS, I->h_Vinv.begin() + e * SS);
+  std::copy(lam, lam + I->S, I->h_lam.begin() + size_t(e) * I->S);
+  I->h_eig_set[e] = 1;
+  return push_model_all(I);
 }
 
 extern "C" int gle_set_category_rates(int h, const int *eig_of_rate, const double *mult) {
@@ -1571,9 +1802,10 @@ extern "C" int gle_set_category_rates(int h, const int *eig_of_rate, const doubl
   for (int r = 0; r < I->R; r++)
     if (eig_of_rate[r] < 0 || eig_of_rate[r] >= I->n_eig || !(mult[r] >= 0.0))
       return set_error(GLE_ERR_ARG, "bad category %d", r);
-  GLE_HIP(hipMemcpy(I->d_eig_of_rate, eig_of_rate, sizeof(int) * I->R, hipMemcpyHostToDevice));
-  GLE_HIP(hipMemcpy(I->d_model + I->S + I->R, mult, sizeof(double) * I->R, hipMemcpyHostToDevice));
-  return GLE_SUCCESS;
+  I->h_eor.assign(eig_of_rate, eig_of_rate + I->R);
+  I->h_mult.assign(mult, mult + I->R);
+  I->h_rates = true;
+  return push_model_all(I);
 }
 
 extern "C" int gle_update_transition_matrices(int h, const int *idx, const double *t, int n, int derivs) {
@@ -1624,16 +1856,24 @@ extern "C" int gle_get_rescale_flags(int h, int *flags, int reset) {
 extern "C" int gle_set_timing(int h, int enable) {
   GET_INSTANCE(h);
   I->timing = enable != 0;
+  if (I->heavy) I->heavy->timing = I->timing;
   return GLE_SUCCESS;
 }
 
 extern "C" int gle_get_kernel_stats(int h, int kind, long long *launches, double *ms, double *bytes, int reset) {
   GET_INSTANCE(h);
-  if (kind < 0 || kind > 2 || !launches || !ms || !bytes) return set_error(GLE_ERR_ARG, "bad stats request");
-  *launches = I->kst[kind].launches;
-  *ms = I->kst[kind].ms;
-  *bytes = I->kst[kind].bytes;
-  if (reset) I->kst[kind] = Instance::KStat();
+  if (kind < 0 || kind > 5 || !launches || !ms || !bytes) return set_error(GLE_ERR_ARG, "bad stats request");
+  Instance *X = kind < 3 ? I : I->heavy.get();
+  const int k = kind % 3;
+  if (!X) {
+    *launches = 0;
+    *ms = *bytes = 0.0;
+    return GLE_SUCCESS;
+  }
+  *launches = X->kst[k].launches;
+  *ms = X->kst[k].ms;
+  *bytes = X->kst[k].bytes;
+  if (reset) X->kst[k] = Instance::KStat();
   return GLE_SUCCESS;
 }
 

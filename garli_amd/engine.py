@@ -71,6 +71,8 @@ SIGNATURES = {
     "gle_set_tip_states": ([C.c_int, C.c_int, _U8], C.c_int),
     "gle_set_patterns": ([C.c_int, _I, C.c_int, _I], C.c_int),
     "gle_set_model": ([C.c_int, _D, _D, C.c_double, C.c_int], C.c_int),
+    "gle_set_heavy_split": ([C.c_int, C.c_int], C.c_int),
+    "gle_get_heavy_split": ([C.c_int, _I], C.c_int),
     "gle_set_eigen": ([C.c_int, C.c_int, _D, _D, _D], C.c_int),
     "gle_set_category_rates": ([C.c_int, _I, _D], C.c_int),
     "gle_update_transition_matrices": ([C.c_int, _I, _D, C.c_int, C.c_int], C.c_int),
@@ -239,9 +241,9 @@ class Instance:
         self.n_tips, self.n_patterns, self.n_states, self.n_rates = n_tips, n_patterns, n_states, n_rates
         self.precision = precision
         if n_cla_slots is None:
-            n_cla_slots = 3 * (n_tips - 2) + 8
+            n_cla_slots = 6 * (n_tips - 2) + 16   # room for a tree and a copy-on-write copy
         if n_pmat_slots is None:
-            n_pmat_slots = 2 * n_tips + 8
+            n_pmat_slots = 8 * n_tips + 16
         d = InstanceDesc(n_tips, n_patterns, n_states, n_rates, n_eigen, n_cla_slots, n_pmat_slots, precision)
         h = C.c_int(-1)
         _check(lib().gle_create_instance(C.byref(d), C.byref(h)))
@@ -271,6 +273,15 @@ class Instance:
         c = np.ascontiguousarray(counts, dtype=np.int32)
         k = np.ascontiguousarray(const_states, dtype=np.int32)
         _check(lib().gle_set_patterns(self.h, _p(c, C.c_int), int(last_const), _p(k, C.c_int)))
+
+    def set_heavy_split(self, threshold):
+        """FP32 instances: patterns with count > threshold run in FP64 (0 = none)."""
+        _check(lib().gle_set_heavy_split(self.h, int(threshold)))
+
+    def heavy_patterns(self):
+        n = C.c_int(0)
+        _check(lib().gle_get_heavy_split(self.h, C.byref(n)))
+        return n.value
 
     def set_model(self, pi, rate_probs, pinv=0.0, pinv_in_model=False):
         pi, w = _d(pi), _d(rate_probs)

@@ -106,6 +106,12 @@ int gle_set_tip_states(int h, int tip, const unsigned char *states);
 /* counts per pattern; constant-pattern states (nuc bitmask / state index) for
  * patterns 0..last_const (DataPartition::GetCounts/LastConstant/GetConstStates) */
 int gle_set_patterns(int h, const int *counts, int last_const, const int *const_states);
+/* FP32 instances keep patterns whose count exceeds `threshold` (default 4) in an
+ * FP64 part that runs the same op lists, so FP32 rounding is never multiplied
+ * by a large pattern weight; 0 keeps every pattern in FP32.  No reference
+ * counterpart (the reference has one precision per build, defs.h:86-112). */
+int gle_set_heavy_split(int h, int threshold);
+int gle_get_heavy_split(int h, int *n_heavy);
 /* state freqs [S], category weights [R] (Model::GetRateProbs, model.h:2051-2064), pinv */
 int gle_set_model(int h, const double *pi, const double *rate_probs, double pinv, int pinv_in_model);
 /* eigensystem e: V, Vinv [S*S], lam [S] (already scaled) */
@@ -146,7 +152,8 @@ int gle_edge_likelihoods(int h, const gle_edge_op *ops, int n, double *out, doub
 int gle_get_rescale_flags(int h, int *flags, int reset);
 int gle_synchronize(int h);
 /* Per-launch HIP-event timing on the instance's stream.  kind: 0 CLA update
- * kernels, 1 edge kernels, 2 P(t) kernel.  bytes are algorithmic HBM bytes
+ * kernels, 1 edge kernels, 2 P(t) kernel; 3..5 the same for the FP64 part of
+ * a split FP32 instance.  bytes are algorithmic HBM bytes
  * (true S, not padded).  No reference counterpart: the reference's Profiler
  * buckets (tree.cpp:41-47, utility.h:240-323) time whole CPU kernels instead. */
 int gle_set_timing(int h, int enable);

@@ -38,7 +38,7 @@ def test_fp64_every_cla_and_count(gpu):
     ds = sim.gtr_gamma(n_tips=40, n_sites=600, mean_blen=0.6, seed=21)
     inst = ds.make_instance(E.PREC_FP64)
     t = E.Tree(inst, ds.parent, ds.blen)
-    o = O.Oracle(ds)
+    o = O.Oracle(ds, own_model=False)  # same eigensystem: element-wise comparison
     nn = 2 * ds.n_tips - 2
     nb = {n: [] for n in range(nn)}
     for n in range(1, nn):
@@ -101,6 +101,35 @@ def test_fp32_bulk_within_tolerance(gpu, mode):
         assert abs(sw[n - 1, 0] - rl) < 1e-3
         assert abs(sw[n - 1, 1] - r1) < 1e-3 * max(1.0, abs(r1))
         assert abs(sw[n - 1, 2] - r2) < 1e-3 * max(1.0, abs(r2))
+
+
+def test_fp32_heavy_pattern_split(gpu):
+    """Heavily weighted patterns run in FP64 inside an FP32 instance; per-site
+    results come back in the caller's pattern order whatever the split."""
+    ds = sim.gtr_gamma(n_tips=24, n_sites=40000, pinv=0.3, mean_blen=0.05, seed=52)
+    ref = O.Oracle(ds).score()
+    f64 = E.Tree(ds.make_instance(E.PREC_FP64), ds.parent, ds.blen)
+    site64 = f64.site_likelihoods()
+    inst = ds.make_instance(E.PREC_FP32)
+    n_heavy = inst.heavy_patterns()
+    assert 0 < n_heavy < ds.n_patterns
+    t = E.Tree(inst, ds.parent, ds.blen)
+    got = t.score()
+    assert abs(got - ref) < 1e-3
+    site = t.site_likelihoods()
+    heavy = ds.counts > 4
+    np.testing.assert_allclose(site[heavy], site64[heavy], rtol=1e-12)
+    np.testing.assert_allclose(site, site64, rtol=1e-5)
+    # switching the split off and on again re-partitions in place
+    inst.set_heavy_split(0)
+    assert inst.heavy_patterns() == 0
+    t.invalidate()
+    all32 = t.score()
+    assert abs(all32 - ref) < 0.1
+    inst.set_heavy_split(4)
+    assert inst.heavy_patterns() == n_heavy
+    t.invalidate()
+    assert t.score() == got
 
 
 def test_fused_schedule_equals_levels(gpu):
