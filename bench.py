@@ -40,7 +40,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
-    ap.add_argument("--schedule", choices=["levels", "fused"], default="fused")
+    ap.add_argument("--schedule", choices=["levels", "fused"], default="levels")
     ap.add_argument("--evals", type=int, default=60, help="timed full-tree lnL evaluations")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle legs")
     ap.add_argument("--patterns", type=int, default=100000)

@@ -20,6 +20,8 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "gle_internal.h"
@@ -113,6 +115,7 @@ struct Instance {
   // sub-instance that receives the same slots, P(t) and op lists, so FP32
   // rounding is never multiplied by a large pattern weight.
   int heavy_threshold = 4;  // 0 keeps every pattern in FP32
+  int fused_min_patterns = 1 << 19;  // smallest padded pattern count that runs fused walks
   std::unique_ptr<Instance> heavy;
   int P_orig = 0;
   std::vector<int> light_idx, heavy_idx;  // original pattern indices of each part
@@ -398,139 +401,158 @@ __global__ __launch_bounds__(256) void k_pmat(int S, int Sp, int R, const int *_
 }
 
 // =====================================================================
-// Nucleotide CLA update (S = 4): one thread per (pattern, rate)
+// Nucleotide CLA update (S = 4): one thread per pattern holding every rate
+// category, so the per-pattern rescale needs no cross-lane traffic.  The op's
+// P matrices are the same for the whole wave: with every pointer argument
+// __restrict__ they are fetched by scalar loads and enter the FMAs as SGPR
+// operands, leaving the vector memory path to CLA, tip and count traffic.
+// tree.cpp:7843-7910 (II), 7941-8047 (IT), 8387-8446 (TT).
 // =====================================================================
-template <typename T>
-struct NucClaArgs {
-  T *cla;
-  int *u;
-  const unsigned char *tips;
-  const T *pm;          // P in precision T (FP32: hi part; the lo part follows at pm_lo)
-  size_t pm_stride;     // elements between pmat slots
-  size_t pm_lo;         // offset of the lo part within a slot (FP32 only)
-  size_t slot_stride;   // Ppad*R*4
-  int Ppad, P;
-  int *flags;
-};
 
-// FP32 form: P = hi + lo (both FP32).  A CLA child gives v = sum_j (hi_j + lo_j) x_j
-// with one final rounding; a tip child gives the exact pair (sum hi, sum lo), so
-// that rounding P to FP32 never enters the product as an error shared by every
-// site.  dest = h1*h2 + (h1*l2 + l1*h2).
-__device__ __forceinline__ void nuc_child_f32(const float *hi, const float *lo, const float *x, float *h, float *l) {
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const float *ph = hi + 4 * i, *pl = lo + 4 * i;
-    float acc = pl[0] * x[0];
-    acc = __builtin_fmaf(pl[1], x[1], acc);
-    acc = __builtin_fmaf(pl[2], x[2], acc);
-    acc = __builtin_fmaf(pl[3], x[3], acc);
-    acc = __builtin_fmaf(ph[3], x[3], acc);
-    acc = __builtin_fmaf(ph[2], x[2], acc);
-    acc = __builtin_fmaf(ph[1], x[1], acc);
-    h[i] = __builtin_fmaf(ph[0], x[0], acc);
-    l[i] = 0.0f;
-  }
+// FP32 form: CLAs are stored in FP32 but every update computes in FP64 from the
+// FP64 P (FP64 FMA issues at the FP32 rate here), so the only FP32 rounding is
+// the store.  That store adds a dither uniform in [-ulp/2, ulp/2) of the FP32
+// result before rounding to nearest: the rounding is unbiased, and patterns
+// that carry equal values (every pattern below a cherry with the same two tip
+// states, for one) get independent errors.  A plain round-to-nearest gives
+// them all the same error, which then adds up linearly over the alignment.
+// The dither is a hash of the pattern index, the element and the value, so
+// results are deterministic and independent of slot assignment and batching.
+__device__ __forceinline__ unsigned mix32(unsigned h) {
+  h ^= h >> 16;
+  h *= 0x7feb352du;
+  h ^= h >> 15;
+  h *= 0x846ca68bu;
+  h ^= h >> 16;
+  return h;
 }
-__device__ __forceinline__ void nuc_tip_f32(const float *hi, const float *lo, int m, float *h, float *l) {
+__device__ __forceinline__ float round_dither(double x, unsigned key) {
+  if (x == 0.0) return 0.0f;
+  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(x));
+  const unsigned h = mix32(key * 0x9e3779b1u ^ unsigned(b >> 29));
+  const int e = __builtin_amdgcn_frexp_exp(x);  // x = m * 2^e, m in [0.5, 1)
+  const double ulp = __builtin_amdgcn_ldexp(1.0, e - 24 > -149 ? e - 24 : -149);
+  return float(x + double(int(h)) * 0x1p-32 * ulp);
+}
+// FP32-form rescale on FP64 values: exact power of two once the max drops below 2^-32.
+template <int W>
+__device__ __forceinline__ int rescale_pow2(double *v, double m) {
+  if (!(m < 0x1p-32 && m > 0.0)) return 0;
+  const int e = 1 - __builtin_amdgcn_frexp_exp(m);  // m * 2^e in [1, 2)
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    if (m == 15) {
-      h[i] = 1.0f;
-      l[i] = 0.0f;
-    } else {
-      float sh = 0.0f, sl = 0.0f;
-#pragma unroll
-      for (int s2 = 0; s2 < 4; s2++)
-        if (m & (1 << s2)) {
-          sh += hi[4 * i + s2];
-          sl += lo[4 * i + s2];
-        }
-      h[i] = sh;
-      l[i] = sl;
-    }
-  }
+  for (int q = 0; q < W; q++) v[q] = __builtin_amdgcn_ldexp(v[q], e);
+  return e;
 }
 
+// FP64 rescale of a pattern slice held by one thread (RescaleRateHet table scheme).
+template <typename T, int W>
+__device__ __forceinline__ int rescale_thread(T *v, int *flags) {
+  static_assert(sizeof(T) == 8, "FP64 scheme");
+  T m = v[0];
+#pragma unroll
+  for (int q = 1; q < W; q++) m = fmax(m, v[q]);
+  int add = 0;
+  if (m < c_rescale_below) {
+    if (m < c_reduce_below) atomicOr(flags, m < c_bail_below ? 3 : 1);
+    int idx = 0;
+    while (idx + 1 < kRescaleLen && c_thresh[idx + 1] > m) idx++;
+    const double mult = c_mult[idx];
+#pragma unroll
+    for (int q = 0; q < W; q++) v[q] = mul_nc(v[q], mult);
+    add = c_incr[idx];
+  }
+  return add;
+}
+
+// grid (Ppad/256, fused ? 1 : n_ops); fused_n > 0 walks ops[0..fused_n) in order.
+// pm: FP64 P slots [slot][3][R][4][4] (P, P', P'').
 template <typename T, int R>
-__global__ __launch_bounds__(256) void k_cla_nuc(NucClaArgs<T> a, const DevClaOp *__restrict__ ops, int fused_n) {
-  constexpr int RP = Lanes<R>::value;
-  const int r = threadIdx.x % RP;
-  const int p = blockIdx.x * (256 / RP) + threadIdx.x / RP;
-  const bool live = (p < a.P) && (r < R);
-  const int rr = r < R ? r : R - 1;
+__global__ __launch_bounds__(256) void k_cla_nuc(T *__restrict__ cla, int *__restrict__ ua,
+                                                 const unsigned char *__restrict__ tips,
+                                                 const double *__restrict__ pm, const DevClaOp *__restrict__ ops,
+                                                 int *__restrict__ flags, int Ppad, int P, int fused_n) {
+  constexpr int kPmStride = 3 * R * 16;
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= P) return;  // no cross-lane work below
+  const size_t slot = size_t(Ppad) * R * 4, off = size_t(p) * R * 4;
   const int first = fused_n ? 0 : blockIdx.y, last = fused_n ? fused_n : blockIdx.y + 1;
   for (int oi = first; oi < last; ++oi) {
     const DevClaOp op = ops[oi];
-    T P1[16], P2[16], x[4], out[4];
-    const T *pm1 = a.pm + op.pm1 * a.pm_stride + rr * 16;
-    const T *pm2 = a.pm + op.pm2 * a.pm_stride + rr * 16;
+    T x1[R][4], x2[R][4];
+    double out[R][4];
+    int m1 = 15, m2 = 15, u = 0;
+    if (op.c1 >= 0) {
 #pragma unroll
-    for (int q = 0; q < 16; q += 4) {
-      load4(pm1 + q, P1 + q);
-      load4(pm2 + q, P2 + q);
-    }
-    int u = 0;
-    const size_t off = (size_t(p) * R + rr) * 4;
-    if (live) {
+      for (int r = 0; r < R; r++) load4(cla + op.c1 * slot + off + 4 * r, x1[r]);
+      u += ua[op.c1 * size_t(Ppad) + p];
+    } else
+      m1 = tips[size_t(-op.c1 - 1) * Ppad + p];
+    if (op.c2 >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; r++) load4(cla + op.c2 * slot + off + 4 * r, x2[r]);
+      u += ua[op.c2 * size_t(Ppad) + p];
+    } else
+      m2 = tips[size_t(-op.c2 - 1) * Ppad + p];
+    const double *pm1 = pm + size_t(op.pm1) * kPmStride, *pm2 = pm + size_t(op.pm2) * kPmStride;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const double *P1 = pm1 + 16 * r, *P2 = pm2 + 16 * r;
       if constexpr (sizeof(T) == 8) {
-        T v1[4], v2[4];
-        if (op.c1 >= 0) {
-          load4(a.cla + op.c1 * a.slot_stride + off, x);
-          const bool paired = (R == 4) && (op.c2 >= 0 || __popc(a.tips[size_t(-op.c2 - 1) * a.Ppad + p]) == 1);
+        // reference summation order: paired for four categories when the other
+        // child is a CLA or an unambiguous tip
+        const bool paired1 = (R == 4) && (op.c2 >= 0 || __popc(m2) == 1);
+        const bool paired2 = (R == 4) && (op.c1 >= 0 || __popc(m1) == 1);
 #pragma unroll
-          for (int i = 0; i < 4; i++) v1[i] = paired ? dot4_paired(P1 + 4 * i, x) : dot4_seq(P1 + 4 * i, x);
-          u += a.u[op.c1 * size_t(a.Ppad) + p];
-        } else {
-          const int m = a.tips[size_t(-op.c1 - 1) * a.Ppad + p];
-#pragma unroll
-          for (int i = 0; i < 4; i++) v1[i] = nuc_tip_factor(P1 + 4 * i, m);
+        for (int i = 0; i < 4; i++) {
+          const double v1 = op.c1 >= 0 ? (paired1 ? dot4_paired(P1 + 4 * i, x1[r]) : dot4_seq(P1 + 4 * i, x1[r]))
+                                       : nuc_tip_factor(P1 + 4 * i, m1);
+          const double v2 = op.c2 >= 0 ? (paired2 ? dot4_paired(P2 + 4 * i, x2[r]) : dot4_seq(P2 + 4 * i, x2[r]))
+                                       : nuc_tip_factor(P2 + 4 * i, m2);
+          out[r][i] = mul_nc(v1, v2);
         }
-        if (op.c2 >= 0) {
-          load4(a.cla + op.c2 * a.slot_stride + off, x);
-          const bool paired = (R == 4) && (op.c1 >= 0 || __popc(a.tips[size_t(-op.c1 - 1) * a.Ppad + p]) == 1);
-#pragma unroll
-          for (int i = 0; i < 4; i++) v2[i] = paired ? dot4_paired(P2 + 4 * i, x) : dot4_seq(P2 + 4 * i, x);
-          u += a.u[op.c2 * size_t(a.Ppad) + p];
-        } else {
-          const int m = a.tips[size_t(-op.c2 - 1) * a.Ppad + p];
-#pragma unroll
-          for (int i = 0; i < 4; i++) v2[i] = nuc_tip_factor(P2 + 4 * i, m);
-        }
-#pragma unroll
-        for (int i = 0; i < 4; i++) out[i] = mul_nc(v1[i], v2[i]);
       } else {
-        float L1[16], L2[16], h1[4], l1[4], h2[4], l2[4];
-        const T *lo1 = pm1 + a.pm_lo, *lo2 = pm2 + a.pm_lo;
 #pragma unroll
-        for (int q = 0; q < 16; q += 4) {
-          load4(lo1 + q, L1 + q);
-          load4(lo2 + q, L2 + q);
+        for (int i = 0; i < 4; i++) {
+          double v1, v2;
+          if (op.c1 >= 0) {
+            const double *q = P1 + 4 * i;
+            v1 = __builtin_fma(q[0], double(x1[r][0]), __builtin_fma(q[1], double(x1[r][1]),
+                 __builtin_fma(q[2], double(x1[r][2]), q[3] * double(x1[r][3]))));
+          } else
+            v1 = nuc_tip_factor(P1 + 4 * i, m1);
+          if (op.c2 >= 0) {
+            const double *q = P2 + 4 * i;
+            v2 = __builtin_fma(q[0], double(x2[r][0]), __builtin_fma(q[1], double(x2[r][1]),
+                 __builtin_fma(q[2], double(x2[r][2]), q[3] * double(x2[r][3]))));
+          } else
+            v2 = nuc_tip_factor(P2 + 4 * i, m2);
+          out[r][i] = v1 * v2;
         }
-        if (op.c1 >= 0) {
-          load4(a.cla + op.c1 * a.slot_stride + off, x);
-          nuc_child_f32(P1, L1, x, h1, l1);
-          u += a.u[op.c1 * size_t(a.Ppad) + p];
-        } else
-          nuc_tip_f32(P1, L1, a.tips[size_t(-op.c1 - 1) * a.Ppad + p], h1, l1);
-        if (op.c2 >= 0) {
-          load4(a.cla + op.c2 * a.slot_stride + off, x);
-          nuc_child_f32(P2, L2, x, h2, l2);
-          u += a.u[op.c2 * size_t(a.Ppad) + p];
-        } else
-          nuc_tip_f32(P2, L2, a.tips[size_t(-op.c2 - 1) * a.Ppad + p], h2, l2);
-#pragma unroll
-        for (int i = 0; i < 4; i++) out[i] = __builtin_fmaf(h1[i], h2[i], __builtin_fmaf(h1[i], l2[i], l1[i] * h2[i]));
       }
-    } else {
+    }
+    if (op.flags & kOpRescale) {
+      if constexpr (sizeof(T) == 8) {
+        u += rescale_thread<double, 4 * R>(&out[0][0], flags);
+      } else {
+        double m = out[0][0];
 #pragma unroll
-      for (int i = 0; i < 4; i++) out[i] = T(0);
+        for (int q = 1; q < 4 * R; q++) m = fmax(m, (&out[0][0])[q]);
+        u += rescale_pow2<4 * R>(&out[0][0], m);
+      }
     }
-    if (op.flags & kOpRescale) u += rescale_slice<T, 4>(out, RP, a.flags, p < a.P);
-    if (live) {
-      store4(a.cla + op.dest * a.slot_stride + off, out);
-      if (r == 0) a.u[op.dest * size_t(a.Ppad) + p] = u;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      T o[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        if constexpr (sizeof(T) == 8)
+          o[i] = out[r][i];
+        else
+          o[i] = round_dither(out[r][i], unsigned(p) * 64u + unsigned(4 * r + i));
+      }
+      store4(cla + op.dest * slot + off + 4 * r, o);
     }
+    ua[op.dest * size_t(Ppad) + p] = u;
   }
 }
 
@@ -543,54 +565,48 @@ struct NsClaArgs {
   T *cla;
   int *u;
   const unsigned char *tips;
-  const T *pm;
-  size_t pm_stride, slot_stride;
+  const double *pm64;  // FP64 P slots [slot][3][R][S][S]
+  const float *pm32;   // padded FP32 P slots [slot][2][R][Sp][Sp] (codon MFMA path)
+  size_t slot_stride;
   int Ppad, P, R, Sp;
-  int prow;       // row stride of P: S for FP64 matrices, Sp for the padded FP32 copy
-  size_t pm_lo;   // offset of the lo part of an FP32 slot
   int *flags;
 };
 
+// FP64 arithmetic in both forms (the FP32 form stores with round_dither); the
+// FP64 form keeps the reference's mul-then-add order.
 template <typename T, int S>
 __global__ __launch_bounds__(256) void k_cla_ns(NsClaArgs<T> a, const DevClaOp *__restrict__ ops, int fused_n) {
   // P staged in LDS for S <= 32; larger state spaces read P through the caches.
-  // FP32: P is the pair hi + lo (lo at pm_lo) and tip factors stay exact pairs.
   constexpr bool kLds = S <= 32;
   constexpr bool kF32 = sizeof(T) == 4;
   extern __shared__ unsigned char smem_raw[];
-  T *sP = reinterpret_cast<T *>(smem_raw);  // [2 children][hi, lo][R][PR][PR]
+  double *sP = reinterpret_cast<double *>(smem_raw);  // [2 children][R][S][S]
   const int R = a.R, RP = R <= 1 ? 1 : R <= 2 ? 2 : R <= 4 ? 4 : 8;
   const int r = threadIdx.x % RP;
   const int p = blockIdx.x * (256 / RP) + threadIdx.x / RP;
   const bool live = (p < a.P) && (r < R);
   const int rr = r < R ? r : R - 1;
   const int first = fused_n ? 0 : blockIdx.y, last = fused_n ? fused_n : blockIdx.y + 1;
-  const int PR = a.prow, RSS = R * PR * PR, NP = kF32 ? 2 : 1;
+  const int RSS = R * S * S;
+  const size_t pm_stride = size_t(3) * RSS;
   for (int oi = first; oi < last; ++oi) {
     const DevClaOp op = ops[oi];
-    const T *P1, *P2, *Q1 = nullptr, *Q2 = nullptr;  // Q = lo parts (FP32)
+    const double *P1, *P2;
     if constexpr (kLds) {
       if (oi != first) __syncthreads();
-      for (int q = threadIdx.x; q < NP * RSS; q += blockDim.x) {
-        sP[q] = a.pm[op.pm1 * a.pm_stride + q];
-        sP[NP * RSS + q] = a.pm[op.pm2 * a.pm_stride + q];
+      for (int q = threadIdx.x; q < RSS; q += blockDim.x) {
+        sP[q] = a.pm64[op.pm1 * pm_stride + q];
+        sP[RSS + q] = a.pm64[op.pm2 * pm_stride + q];
       }
       __syncthreads();
-      P1 = sP + rr * PR * PR;
-      P2 = sP + NP * RSS + rr * PR * PR;
-      if constexpr (kF32) {
-        Q1 = P1 + RSS;
-        Q2 = P2 + RSS;
-      }
+      P1 = sP + rr * S * S;
+      P2 = sP + RSS + rr * S * S;
     } else {
-      P1 = a.pm + op.pm1 * a.pm_stride + rr * PR * PR;
-      P2 = a.pm + op.pm2 * a.pm_stride + rr * PR * PR;
-      if constexpr (kF32) {
-        Q1 = P1 + a.pm_lo;
-        Q2 = P2 + a.pm_lo;
-      }
+      P1 = a.pm64 + op.pm1 * pm_stride + rr * S * S;
+      P2 = a.pm64 + op.pm2 * pm_stride + rr * S * S;
     }
-    T x1[S], x2[S], out[S];
+    T x1[S], x2[S];
+    double out[S];
     int t1 = S, t2 = S, u = 0;
     const size_t off = (size_t(p) * R + rr) * a.Sp;
     if (live) {
@@ -608,59 +624,171 @@ __global__ __launch_bounds__(256) void k_cla_ns(NsClaArgs<T> a, const DevClaOp *
         t2 = a.tips[size_t(-op.c2 - 1) * a.Ppad + p];
 #pragma unroll 4
       for (int f = 0; f < S; f++) {
-        if constexpr (kF32) {
-          float h1, l1 = 0.0f, h2, l2 = 0.0f;
-          if (op.c1 >= 0) {
-            float acc = 0.0f;
+        double L1, R1;
+        if (op.c1 >= 0) {
+          L1 = 0.0;
+          if constexpr (kF32) {
 #pragma unroll
-            for (int q = 0; q < S; q++) acc = __builtin_fmaf(Q1[f * PR + q], x1[q], acc);
+            for (int q = 0; q < S; q++) L1 = __builtin_fma(P1[f * S + q], double(x1[q]), L1);
+          } else {
 #pragma unroll
-            for (int q = 0; q < S; q++) acc = __builtin_fmaf(P1[f * PR + q], x1[q], acc);
-            h1 = acc;
-          } else if (t1 < S) {
-            h1 = P1[f * PR + t1];
-            l1 = Q1[f * PR + t1];
-          } else
-            h1 = 1.0f;
-          if (op.c2 >= 0) {
-            float acc = 0.0f;
+            for (int q = 0; q < S; q++) L1 = add_nc(L1, mul_nc(P1[f * S + q], x1[q]));
+          }
+        } else
+          L1 = t1 < S ? P1[f * S + t1] : 1.0;
+        if (op.c2 >= 0) {
+          R1 = 0.0;
+          if constexpr (kF32) {
 #pragma unroll
-            for (int q = 0; q < S; q++) acc = __builtin_fmaf(Q2[f * PR + q], x2[q], acc);
+            for (int q = 0; q < S; q++) R1 = __builtin_fma(P2[f * S + q], double(x2[q]), R1);
+          } else {
 #pragma unroll
-            for (int q = 0; q < S; q++) acc = __builtin_fmaf(P2[f * PR + q], x2[q], acc);
-            h2 = acc;
-          } else if (t2 < S) {
-            h2 = P2[f * PR + t2];
-            l2 = Q2[f * PR + t2];
-          } else
-            h2 = 1.0f;
-          out[f] = __builtin_fmaf(h1, h2, __builtin_fmaf(h1, l2, l1 * h2));
-        } else {
-          T L1, R1;
-          if (op.c1 >= 0) {
-            L1 = T(0);
-#pragma unroll
-            for (int q = 0; q < S; q++) L1 = add_nc(L1, mul_nc(P1[f * PR + q], x1[q]));
-          } else
-            L1 = t1 < S ? P1[f * PR + t1] : T(1);
-          if (op.c2 >= 0) {
-            R1 = T(0);
-#pragma unroll
-            for (int q = 0; q < S; q++) R1 = add_nc(R1, mul_nc(P2[f * PR + q], x2[q]));
-          } else
-            R1 = t2 < S ? P2[f * PR + t2] : T(1);
-          out[f] = mul_nc(L1, R1);
-        }
+            for (int q = 0; q < S; q++) R1 = add_nc(R1, mul_nc(P2[f * S + q], x2[q]));
+          }
+        } else
+          R1 = t2 < S ? P2[f * S + t2] : 1.0;
+        out[f] = mul_nc(L1, R1);
       }
     } else {
 #pragma unroll
-      for (int f = 0; f < S; f++) out[f] = T(0);
+      for (int f = 0; f < S; f++) out[f] = 0.0;
     }
-    if (op.flags & kOpRescale) u += rescale_slice<T, S>(out, RP, a.flags, p < a.P);
+    if (op.flags & kOpRescale) {
+      if constexpr (kF32) {
+        double m = out[0];
+#pragma unroll
+        for (int q = 1; q < S; q++) m = fmax(m, out[q]);
+        m = group_max(m, RP);
+        if (p < a.P) u += rescale_pow2<S>(out, m);
+      } else
+        u += rescale_slice<double, S>(out, RP, a.flags, p < a.P);
+    }
     if (live) {
 #pragma unroll
-      for (int f = 0; f < S; f++) a.cla[op.dest * a.slot_stride + off + f] = out[f];
+      for (int f = 0; f < S; f++) {
+        if constexpr (kF32)
+          a.cla[op.dest * a.slot_stride + off + f] = round_dither(out[f], unsigned(p) * 512u + unsigned(rr * S + f));
+        else
+          a.cla[op.dest * a.slot_stride + off + f] = out[f];
+      }
       if (r == 0) a.u[op.dest * size_t(a.Ppad) + p] = u;
+    }
+  }
+}
+
+// =====================================================================
+// Amino-acid CLA update (S = 20).  A block is R waves over the same 64
+// patterns and wave r takes category r, so the op's P rows are wave-uniform:
+// scalar loads, SGPR operands, no LDS traffic for P.  The per-pattern rescale
+// takes the max over the R waves through a small LDS table.  Arithmetic as in
+// k_cla_ns (FP64; reference mul-then-add order for the FP64 form).
+// =====================================================================
+template <typename T, int R>
+__global__ __launch_bounds__(64 * R) void k_cla_aa(T *__restrict__ cla, int *__restrict__ ua,
+                                                    const unsigned char *__restrict__ tips,
+                                                    const double *__restrict__ pm, const DevClaOp *__restrict__ ops,
+                                                    int *__restrict__ flags, int Ppad, int P, int fused_n) {
+  constexpr int S = 20;
+  constexpr bool kF32 = sizeof(T) == 4;
+  __shared__ double smax[R][64];
+  const int lane = threadIdx.x & 63;
+  const int r = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+  const int p = blockIdx.x * 64 + lane;
+  const bool live = p < P;
+  const size_t slot = size_t(Ppad) * R * S, off = (size_t(p) * R + r) * S;
+  const int first = fused_n ? 0 : blockIdx.y, last = fused_n ? fused_n : blockIdx.y + 1;
+  for (int oi = first; oi < last; ++oi) {
+    if (oi != first) __syncthreads();  // smax of the previous op has been read
+    const DevClaOp op = ops[oi];
+    const double *P1 = pm + size_t(op.pm1) * 3 * R * S * S + r * S * S;
+    const double *P2 = pm + size_t(op.pm2) * 3 * R * S * S + r * S * S;
+    T x1[S], x2[S];
+    double out[S];
+    int t1 = S, t2 = S, u = 0;
+    if (live) {
+      if (op.c1 >= 0) {
+#pragma unroll
+        for (int q = 0; q < S; q += 4) load4(cla + op.c1 * slot + off + q, x1 + q);
+        u += ua[op.c1 * size_t(Ppad) + p];
+      } else
+        t1 = tips[size_t(-op.c1 - 1) * Ppad + p];
+      if (op.c2 >= 0) {
+#pragma unroll
+        for (int q = 0; q < S; q += 4) load4(cla + op.c2 * slot + off + q, x2 + q);
+        u += ua[op.c2 * size_t(Ppad) + p];
+      } else
+        t2 = tips[size_t(-op.c2 - 1) * Ppad + p];
+    } else {
+#pragma unroll
+      for (int q = 0; q < S; q++) x1[q] = x2[q] = T(0);
+    }
+    // one child at a time keeps the live registers to x1, x2 and out
+#pragma unroll
+    for (int f = 0; f < S; f++) {
+      double L1;
+      if (op.c1 >= 0) {
+        L1 = 0.0;
+#pragma unroll
+        for (int q = 0; q < S; q++) {
+          if constexpr (kF32)
+            L1 = __builtin_fma(P1[f * S + q], double(x1[q]), L1);
+          else
+            L1 = add_nc(L1, mul_nc(P1[f * S + q], x1[q]));
+        }
+      } else
+        L1 = t1 < S ? P1[f * S + t1] : 1.0;
+      out[f] = L1;
+    }
+#pragma unroll
+    for (int f = 0; f < S; f++) {
+      double R1;
+      if (op.c2 >= 0) {
+        R1 = 0.0;
+#pragma unroll
+        for (int q = 0; q < S; q++) {
+          if constexpr (kF32)
+            R1 = __builtin_fma(P2[f * S + q], double(x2[q]), R1);
+          else
+            R1 = add_nc(R1, mul_nc(P2[f * S + q], x2[q]));
+        }
+      } else
+        R1 = t2 < S ? P2[f * S + t2] : 1.0;
+      out[f] = mul_nc(out[f], R1);
+    }
+    if (op.flags & kOpRescale) {
+      double m = out[0];
+#pragma unroll
+      for (int f = 1; f < S; f++) m = fmax(m, out[f]);
+      smax[r][lane] = m;
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < R; k++) m = fmax(m, smax[k][lane]);
+      if (live) {
+        if constexpr (kF32) {
+          u += rescale_pow2<S>(out, m);
+        } else if (m < c_rescale_below) {
+          if (m < c_reduce_below && r == 0) atomicOr(flags, m < c_bail_below ? 3 : 1);
+          int idx = 0;
+          while (idx + 1 < kRescaleLen && c_thresh[idx + 1] > m) idx++;
+          const double mult = c_mult[idx];
+#pragma unroll
+          for (int f = 0; f < S; f++) out[f] = mul_nc(out[f], mult);
+          u += c_incr[idx];
+        }
+      }
+    }
+    if (live) {
+      T o[S];
+#pragma unroll
+      for (int f = 0; f < S; f++) {
+        if constexpr (kF32)
+          o[f] = round_dither(out[f], unsigned(p) * 512u + unsigned(r * S + f));
+        else
+          o[f] = out[f];
+      }
+#pragma unroll
+      for (int q = 0; q < S; q += 4) store4(cla + op.dest * slot + off + q, o + q);
+      if (r == 0) ua[op.dest * size_t(Ppad) + p] = u;
     }
   }
 }
@@ -679,7 +807,7 @@ typedef float gle_floatx4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ void codon_child_f32(const NsClaArgs<float> &a, int c, int pm, int r, int R, int lo,
                                                 int g, int p0, gle_floatx4 acc[4], int &u_add) {
-  const float *P = a.pm + size_t(pm) * a.pm_stride + size_t(r) * 64 * 64;
+  const float *P = a.pm32 + size_t(pm) * (2 * R * 64 * 64) + size_t(r) * 64 * 64;
   if (c >= 0) {
     float x[16];
     const int p = p0 + lo;
@@ -872,27 +1000,33 @@ __device__ __forceinline__ void site_terms(double L, double D1, double D2, int u
   g2 = double(count) * r2;
 }
 
-// One thread per pattern over all categories; P, P', P'' of the op in LDS.
-// Per state the categories accumulate in order, as GetScorePartial*RateHet /
-// GetDerivsPartial* do (tree.cpp:6208-6230, optimization.cpp:3123-3158).
+// One thread per pattern over all categories.  P, P', P'' of the op and the
+// model are wave-uniform and reach the FMAs through scalar loads (every pointer
+// argument is __restrict__).  Per state the categories accumulate in order, as
+// GetScorePartial*RateHet / GetDerivsPartial* do (tree.cpp:6208-6230,
+// optimization.cpp:3123-3158).  grid (Ppad/256, n_ops).
 template <typename T, int R>
-__global__ __launch_bounds__(256) void k_edge_nuc(EdgeArgs<T> a, const DevEdgeOp *__restrict__ ops) {
+__global__ __launch_bounds__(256) void k_edge_nuc(const T *__restrict__ cla, const int *__restrict__ ua,
+                                                  const unsigned char *__restrict__ tips,
+                                                  const double *__restrict__ pm64, const double *__restrict__ model,
+                                                  const int *__restrict__ counts, const int *__restrict__ cst,
+                                                  const DevEdgeOp *__restrict__ ops, double *__restrict__ partials,
+                                                  double *__restrict__ site, double pinv, int pinv_in_model,
+                                                  int Ppad, int P) {
 #pragma clang fp contract(off)
-  __shared__ double sP[3 * R * 16];
   const DevEdgeOp op = ops[blockIdx.y];
-  const int nmat = op.derivs ? 3 : 1;
-  for (int q = threadIdx.x; q < nmat * R * 16; q += blockDim.x) sP[q] = a.pm64[size_t(op.pm) * 3 * R * 16 + q];
-  __syncthreads();
+  const double *Pb = pm64 + size_t(op.pm) * 3 * R * 16;
+  const size_t slot = size_t(Ppad) * R * 4;
   const int p = blockIdx.x * 256 + threadIdx.x;
-  const double *pi = a.model, *w = a.model + 4;
+  const double *pi = model, *w = model + 4;
   double lnl = 0, g1 = 0, g2 = 0;
-  if (p < a.P) {
+  if (p < P) {
     double L[4] = {0, 0, 0, 0}, G1[4] = {0, 0, 0, 0}, G2[4] = {0, 0, 0, 0};
-    int u = a.u[op.partial * size_t(a.Ppad) + p];
-    const T *xp = a.cla + op.partial * a.slot_stride + size_t(p) * R * 4;
+    int u = ua[op.partial * size_t(Ppad) + p];
+    const T *xp = cla + op.partial * slot + size_t(p) * R * 4;
     if (op.child >= 0) {
-      const T *yp = a.cla + op.child * a.slot_stride + size_t(p) * R * 4;
-      u += a.u[op.child * size_t(a.Ppad) + p];
+      const T *yp = cla + op.child * slot + size_t(p) * R * 4;
+      u += ua[op.child * size_t(Ppad) + p];
 #pragma unroll
       for (int r = 0; r < R; r++) {
         T xt[4], yt[4];
@@ -900,7 +1034,7 @@ __global__ __launch_bounds__(256) void k_edge_nuc(EdgeArgs<T> a, const DevEdgeOp
         load4(yp + 4 * r, yt);
         const double x[4] = {double(xt[0]), double(xt[1]), double(xt[2]), double(xt[3])};
         const double y[4] = {double(yt[0]), double(yt[1]), double(yt[2]), double(yt[3])};
-        const double *Pm = sP + r * 16, *D1m = sP + (R + r) * 16, *D2m = sP + (2 * R + r) * 16;
+        const double *Pm = Pb + r * 16, *D1m = Pb + (R + r) * 16, *D2m = Pb + (2 * R + r) * 16;
 #pragma unroll
         for (int s = 0; s < 4; s++) {
           L[s] += dot4_seq(Pm + 4 * s, y) * x[s] * w[r];
@@ -911,13 +1045,13 @@ __global__ __launch_bounds__(256) void k_edge_nuc(EdgeArgs<T> a, const DevEdgeOp
         }
       }
     } else {
-      const int m = a.tips[size_t(-op.child - 1) * a.Ppad + p];
+      const int m = tips[size_t(-op.child - 1) * Ppad + p];
 #pragma unroll
       for (int r = 0; r < R; r++) {
         T xt[4];
         load4(xp + 4 * r, xt);
         const double x[4] = {double(xt[0]), double(xt[1]), double(xt[2]), double(xt[3])};
-        const double *Pm = sP + r * 16, *D1m = sP + (R + r) * 16, *D2m = sP + (2 * R + r) * 16;
+        const double *Pm = Pb + r * 16, *D1m = Pb + (R + r) * 16, *D2m = Pb + (2 * R + r) * 16;
         if (m == 15) {
 #pragma unroll
           for (int s = 0; s < 4; s++) L[s] += x[s] * w[r];
@@ -947,18 +1081,18 @@ __global__ __launch_bounds__(256) void k_edge_nuc(EdgeArgs<T> a, const DevEdgeOp
     const double D1 = G1[0] * pi[0] + G1[1] * pi[1] + G1[2] * pi[2] + G1[3] * pi[3];
     const double D2 = G2[0] * pi[0] + G2[1] * pi[1] + G2[2] * pi[2] + G2[3] * pi[3];
     double pt = 0.0;
-    const int cs = a.cst[p];
-    if (a.pinv_in_model && cs >= 0) {
+    const int cs = cst[p];
+    if (pinv_in_model && cs >= 0) {
       double btot = 0.0;
       for (int s = 0; s < 4; s++)
         if (cs & (1 << s)) btot += pi[s];
-      pt = a.pinv * btot;
+      pt = pinv * btot;
     }
     double ul;
-    site_terms<sizeof(T) == 4>(Lt, D1, D2, u, pt, false, a.counts[p], lnl, g1, g2, ul);
-    if (a.site && blockIdx.y == 0) a.site[p] = ul;
+    site_terms<sizeof(T) == 4>(Lt, D1, D2, u, pt, false, counts[p], lnl, g1, g2, ul);
+    if (site && blockIdx.y == 0) site[p] = ul;
   }
-  block_reduce3(lnl, g1, g2, a.partials + (size_t(blockIdx.y) * gridDim.x + blockIdx.x) * 3);
+  block_reduce3(lnl, g1, g2, partials + (size_t(blockIdx.y) * gridDim.x + blockIdx.x) * 3);
 }
 
 template <typename T, int S>
@@ -1089,7 +1223,42 @@ static int blocks_for(const Instance *I) { return I->Ppad / (256 / lanes_for(I->
 // Every CLA kernel can walk a whole topologically ordered op list in one launch:
 // a (pattern, rate) lane reads only what it wrote itself, and the kernels that
 // share data across lanes (LDS-staged P, the codon MFMA tiles) barrier per op.
-bool engine_supports_fused(const Instance *) { return true; }
+// The fused walk gives each thread a whole op list, so the grid holds only one
+// thread per pattern: worthwhile only once that fills the chip several times
+// over (256 CUs x 4 SIMDs x 8 waves x 64 lanes = 512k patterns).  Below that,
+// launching per dependency level exposes far more parallelism.
+bool engine_supports_fused(const Instance *I) { return I->Ppad >= I->fused_min_patterns; }
+
+// Dependency levels of a topologically ordered op list: an op's level is one
+// more than the level of the ops that wrote its children.  Returns false when a
+// slot is written twice or written after being read (then order must be kept).
+static bool level_ops(const DevClaOp *ops, int n, std::vector<DevClaOp> &sorted, std::vector<int> &start) {
+  std::unordered_map<int, int> lvl_of_slot;
+  std::unordered_set<int> read;
+  std::vector<int> lvl(n);
+  int maxl = 0;
+  for (int i = 0; i < n; i++) {
+    const DevClaOp &o = ops[i];
+    if (lvl_of_slot.count(o.dest) || read.count(o.dest)) return false;
+    int l = 0;
+    for (int c : {o.c1, o.c2}) {
+      if (c < 0) continue;
+      read.insert(c);
+      auto it = lvl_of_slot.find(c);
+      if (it != lvl_of_slot.end()) l = std::max(l, it->second + 1);
+    }
+    lvl[i] = l;
+    lvl_of_slot[o.dest] = l;
+    maxl = std::max(maxl, l);
+  }
+  start.assign(maxl + 2, 0);
+  for (int i = 0; i < n; i++) start[lvl[i] + 1]++;
+  for (int l = 0; l <= maxl; l++) start[l + 1] += start[l];
+  sorted.resize(n);
+  std::vector<int> pos(start.begin(), start.end() - 1);
+  for (int i = 0; i < n; i++) sorted[pos[lvl[i]]++] = ops[i];
+  return true;
+}
 
 int engine_update_pmats(Instance *I, const int *idx, const double *t, int n, int derivs) {
   if (I->heavy) {
@@ -1128,74 +1297,54 @@ int engine_update_pmats(Instance *I, const int *idx, const double *t, int n, int
 }
 
 template <typename T, int R>
-static void launch_nuc(Instance *I, const DevClaOp *d_ops, int n, bool fused, int nblk) {
-  NucClaArgs<T> a;
-  a.cla = reinterpret_cast<T *>(I->d_cla);
-  a.u = I->d_u;
-  a.tips = I->d_tips;
-  a.pm = sizeof(T) == 8 ? reinterpret_cast<const T *>(I->d_pm64) : reinterpret_cast<const T *>(I->d_pm32);
-  a.pm_stride = sizeof(T) == 8 ? size_t(3) * R * 16 : size_t(2) * R * 16;
-  a.pm_lo = size_t(R) * 16;
-  a.slot_stride = I->slot_elems();
-  a.Ppad = I->Ppad;
-  a.P = I->P;
-  a.flags = I->d_flags;
-  hipLaunchKernelGGL((k_cla_nuc<T, R>), dim3(nblk, fused ? 1 : n), dim3(256), 0, I->stream, a, d_ops,
-                     fused ? n : 0);
+static void launch_nuc(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
+  hipLaunchKernelGGL((k_cla_nuc<T, R>), dim3(I->Ppad / 256, fused ? 1 : n), dim3(256), 0, I->stream,
+                     reinterpret_cast<T *>(I->d_cla), I->d_u, I->d_tips, I->d_pm64, d_ops, I->d_flags, I->Ppad,
+                     I->P, fused ? n : 0);
 }
 
 template <typename T>
-static int launch_nuc_r(Instance *I, const DevClaOp *d_ops, int n, bool fused, int nblk) {
+static int launch_nuc_r(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
   switch (I->R) {
-    case 1: launch_nuc<T, 1>(I, d_ops, n, fused, nblk); break;
-    case 2: launch_nuc<T, 2>(I, d_ops, n, fused, nblk); break;
-    case 3: launch_nuc<T, 3>(I, d_ops, n, fused, nblk); break;
-    case 4: launch_nuc<T, 4>(I, d_ops, n, fused, nblk); break;
-    case 5: launch_nuc<T, 5>(I, d_ops, n, fused, nblk); break;
-    case 6: launch_nuc<T, 6>(I, d_ops, n, fused, nblk); break;
-    case 7: launch_nuc<T, 7>(I, d_ops, n, fused, nblk); break;
-    case 8: launch_nuc<T, 8>(I, d_ops, n, fused, nblk); break;
+    case 1: launch_nuc<T, 1>(I, d_ops, n, fused); break;
+    case 2: launch_nuc<T, 2>(I, d_ops, n, fused); break;
+    case 3: launch_nuc<T, 3>(I, d_ops, n, fused); break;
+    case 4: launch_nuc<T, 4>(I, d_ops, n, fused); break;
+    case 5: launch_nuc<T, 5>(I, d_ops, n, fused); break;
+    case 6: launch_nuc<T, 6>(I, d_ops, n, fused); break;
+    case 7: launch_nuc<T, 7>(I, d_ops, n, fused); break;
+    case 8: launch_nuc<T, 8>(I, d_ops, n, fused); break;
     default: return set_error(GLE_ERR_UNSUPPORTED, "rate categories must be 1..8");
   }
   return GLE_SUCCESS;
 }
 
-template <typename T, int S>
-static void launch_ns(Instance *I, const DevClaOp *d_ops, int n, bool fused, int nblk) {
+template <typename T>
+static NsClaArgs<T> ns_args(Instance *I) {
   NsClaArgs<T> a;
   a.cla = reinterpret_cast<T *>(I->d_cla);
   a.u = I->d_u;
   a.tips = I->d_tips;
-  a.pm = sizeof(T) == 8 ? reinterpret_cast<const T *>(I->d_pm64) : reinterpret_cast<const T *>(I->d_pm32);
-  a.prow = sizeof(T) == 8 ? S : I->Sp;
-  a.pm_stride = sizeof(T) == 8 ? size_t(3) * I->R * S * S : size_t(2) * I->R * I->Sp * I->Sp;
-  a.pm_lo = size_t(I->R) * I->Sp * I->Sp;
+  a.pm64 = I->d_pm64;
+  a.pm32 = I->d_pm32;
   a.slot_stride = I->slot_elems();
   a.Ppad = I->Ppad;
   a.P = I->P;
   a.R = I->R;
   a.Sp = I->Sp;
   a.flags = I->d_flags;
-  const size_t shmem = S <= 32 ? sizeof(T) * 2 * (sizeof(T) == 4 ? 2 : 1) * I->R * a.prow * a.prow : 0;
-  hipLaunchKernelGGL((k_cla_ns<T, S>), dim3(nblk, fused ? 1 : n), dim3(256), shmem, I->stream, a, d_ops,
-                     fused ? n : 0);
+  return a;
+}
+
+template <typename T, int S>
+static void launch_ns(Instance *I, const DevClaOp *d_ops, int n, bool fused, int nblk) {
+  const size_t shmem = S <= 32 ? sizeof(double) * 2 * I->R * S * S : 0;
+  hipLaunchKernelGGL((k_cla_ns<T, S>), dim3(nblk, fused ? 1 : n), dim3(256), shmem, I->stream, ns_args<T>(I),
+                     d_ops, fused ? n : 0);
 }
 
 static int launch_codon_mfma(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
-  NsClaArgs<float> a;
-  a.cla = reinterpret_cast<float *>(I->d_cla);
-  a.u = I->d_u;
-  a.tips = I->d_tips;
-  a.pm = I->d_pm32;
-  a.prow = 64;
-  a.pm_stride = size_t(2) * I->R * 64 * 64;
-  a.pm_lo = size_t(I->R) * 64 * 64;
-  a.slot_stride = I->slot_elems();
-  a.Ppad = I->Ppad;
-  a.P = I->P;
-  a.R = I->R;
-  a.Sp = I->Sp;
-  a.flags = I->d_flags;
+  const NsClaArgs<float> a = ns_args<float>(I);
   const dim3 grid(I->Ppad / 16, fused ? 1 : n), block(64);
   const int fn = fused ? n : 0;
   switch (I->R) {
@@ -1208,13 +1357,36 @@ static int launch_codon_mfma(Instance *I, const DevClaOp *d_ops, int n, bool fus
   return GLE_SUCCESS;
 }
 
+template <typename T, int R>
+static void launch_aa(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
+  hipLaunchKernelGGL((k_cla_aa<T, R>), dim3(I->Ppad / 64, fused ? 1 : n), dim3(64 * R), 0, I->stream,
+                     reinterpret_cast<T *>(I->d_cla), I->d_u, I->d_tips, I->d_pm64, d_ops, I->d_flags, I->Ppad,
+                     I->P, fused ? n : 0);
+}
+
+template <typename T>
+static int launch_aa_r(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
+  switch (I->R) {
+    case 1: launch_aa<T, 1>(I, d_ops, n, fused); break;
+    case 2: launch_aa<T, 2>(I, d_ops, n, fused); break;
+    case 3: launch_aa<T, 3>(I, d_ops, n, fused); break;
+    case 4: launch_aa<T, 4>(I, d_ops, n, fused); break;
+    case 5: launch_aa<T, 5>(I, d_ops, n, fused); break;
+    case 6: launch_aa<T, 6>(I, d_ops, n, fused); break;
+    case 7: launch_aa<T, 7>(I, d_ops, n, fused); break;
+    case 8: launch_aa<T, 8>(I, d_ops, n, fused); break;
+    default: return set_error(GLE_ERR_UNSUPPORTED, "rate categories must be 1..8");
+  }
+  return GLE_SUCCESS;
+}
+
 template <typename T>
 static int launch_cla(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
   const int nblk = blocks_for(I);
   if (sizeof(T) == 4 && I->S == 61 && I->R <= 4) return launch_codon_mfma(I, d_ops, n, fused);
   switch (I->S) {
-    case 4: return launch_nuc_r<T>(I, d_ops, n, fused, nblk);
-    case 20: launch_ns<T, 20>(I, d_ops, n, fused, nblk); return GLE_SUCCESS;
+    case 4: return launch_nuc_r<T>(I, d_ops, n, fused);
+    case 20: return launch_aa_r<T>(I, d_ops, n, fused);
     case 61: launch_ns<T, 61>(I, d_ops, n, fused, nblk); return GLE_SUCCESS;
     default: return set_error(GLE_ERR_UNSUPPORTED, "state count %d has no kernel", I->S);
   }
@@ -1228,6 +1400,14 @@ int engine_update_partials(Instance *I, const DevClaOp *ops, int n_ops, const in
   }
   std::lock_guard<std::mutex> lk_(I->mu);
   if (n_ops <= 0) return GLE_SUCCESS;
+  fused = fused && engine_supports_fused(I);
+  std::vector<DevClaOp> lv_ops;
+  std::vector<int> lv_start;
+  if (!fused && n_levels <= 0 && n_ops > 1 && level_ops(ops, n_ops, lv_ops, lv_start)) {
+    ops = lv_ops.data();
+    level_start = lv_start.data();
+    n_levels = int(lv_start.size()) - 1;
+  }
   for (int i = 0; i < n_ops; i++) {
     const DevClaOp &o = ops[i];
     auto bad_child = [&](int c) { return c >= I->n_slots || (c < 0 && -c - 1 >= I->N); };
@@ -1244,7 +1424,7 @@ int engine_update_partials(Instance *I, const DevClaOp *ops, int n_ops, const in
     for (int i = b; i < e; i++) t += cla_op_bytes(I, ops[i]);
     return t;
   };
-  if (fused && engine_supports_fused(I)) {
+  if (fused) {
     timed_begin(I, 0, bytes_of(0, n_ops));
     rc = I->prec == GLE_PREC_FP64 ? launch_cla<double>(I, I->d_ops, n_ops, true)
                                   : launch_cla<float>(I, I->d_ops, n_ops, true);
@@ -1290,17 +1470,21 @@ static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk) {
   a.site = I->d_site;
   dim3 grid(nblk, n);
   if (I->S == 4) {
+#define GLE_EDGE_NUC(RR)                                                                                          \
+  hipLaunchKernelGGL((k_edge_nuc<T, RR>), grid, dim3(256), 0, I->stream, a.cla, a.u, a.tips, a.pm64, a.model,    \
+                     a.counts, a.cst, d_ops, a.partials, a.site, a.pinv, a.pinv_in_model, a.Ppad, a.P)
     switch (I->R) {
-      case 1: hipLaunchKernelGGL((k_edge_nuc<T, 1>), grid, dim3(256), 0, I->stream, a, d_ops); break;
-      case 2: hipLaunchKernelGGL((k_edge_nuc<T, 2>), grid, dim3(256), 0, I->stream, a, d_ops); break;
-      case 3: hipLaunchKernelGGL((k_edge_nuc<T, 3>), grid, dim3(256), 0, I->stream, a, d_ops); break;
-      case 4: hipLaunchKernelGGL((k_edge_nuc<T, 4>), grid, dim3(256), 0, I->stream, a, d_ops); break;
-      case 5: hipLaunchKernelGGL((k_edge_nuc<T, 5>), grid, dim3(256), 0, I->stream, a, d_ops); break;
-      case 6: hipLaunchKernelGGL((k_edge_nuc<T, 6>), grid, dim3(256), 0, I->stream, a, d_ops); break;
-      case 7: hipLaunchKernelGGL((k_edge_nuc<T, 7>), grid, dim3(256), 0, I->stream, a, d_ops); break;
-      case 8: hipLaunchKernelGGL((k_edge_nuc<T, 8>), grid, dim3(256), 0, I->stream, a, d_ops); break;
+      case 1: GLE_EDGE_NUC(1); break;
+      case 2: GLE_EDGE_NUC(2); break;
+      case 3: GLE_EDGE_NUC(3); break;
+      case 4: GLE_EDGE_NUC(4); break;
+      case 5: GLE_EDGE_NUC(5); break;
+      case 6: GLE_EDGE_NUC(6); break;
+      case 7: GLE_EDGE_NUC(7); break;
+      case 8: GLE_EDGE_NUC(8); break;
       default: return set_error(GLE_ERR_UNSUPPORTED, "rate categories must be 1..8");
     }
+#undef GLE_EDGE_NUC
   } else if (I->S == 20) {
     const size_t shmem = sizeof(double) * 3 * I->R * 400;
     if (shmem > 65536) return set_error(GLE_ERR_UNSUPPORTED, "20-state edge kernel takes at most 6 categories");
@@ -1660,6 +1844,7 @@ static int apply_patterns(Instance *I) {
     if ((rc = create_internal(&d, I->heavy))) return rc;
     I->heavy->heavy_threshold = 0;
     I->heavy->timing = I->timing;
+    I->heavy->fused_min_patterns = I->fused_min_patterns;
     // P(t) slots already filled for the main part stay valid for the new one
     GLE_HIP(hipStreamSynchronize(I->stream));
     GLE_HIP(hipMemcpy(I->heavy->d_pm64, I->d_pm64, sizeof(double) * size_t(I->n_pm) * 3 * I->R * I->S * I->S,
@@ -1765,6 +1950,14 @@ extern "C" int gle_set_heavy_split(int h, int threshold) {
   if (threshold < 0) return set_error(GLE_ERR_ARG, "threshold must be >= 0");
   I->heavy_threshold = threshold;
   return I->h_patterns ? apply_patterns(I) : GLE_SUCCESS;
+}
+
+extern "C" int gle_set_fused_threshold(int h, int min_patterns) {
+  GET_INSTANCE(h);
+  if (min_patterns < 0) return set_error(GLE_ERR_ARG, "threshold must be >= 0");
+  I->fused_min_patterns = min_patterns;
+  if (I->heavy) I->heavy->fused_min_patterns = min_patterns;
+  return GLE_SUCCESS;
 }
 
 extern "C" int gle_get_heavy_split(int h, int *n_heavy) {

@@ -73,6 +73,7 @@ SIGNATURES = {
     "gle_set_model": ([C.c_int, _D, _D, C.c_double, C.c_int], C.c_int),
     "gle_set_heavy_split": ([C.c_int, C.c_int], C.c_int),
     "gle_get_heavy_split": ([C.c_int, _I], C.c_int),
+    "gle_set_fused_threshold": ([C.c_int, C.c_int], C.c_int),
     "gle_set_eigen": ([C.c_int, C.c_int, _D, _D, _D], C.c_int),
     "gle_set_category_rates": ([C.c_int, _I, _D], C.c_int),
     "gle_update_transition_matrices": ([C.c_int, _I, _D, C.c_int, C.c_int], C.c_int),
@@ -277,6 +278,10 @@ class Instance:
     def set_heavy_split(self, threshold):
         """FP32 instances: patterns with count > threshold run in FP64 (0 = none)."""
         _check(lib().gle_set_heavy_split(self.h, int(threshold)))
+
+    def set_fused_threshold(self, min_patterns):
+        """Fused single-launch walks only for instances with >= min_patterns (padded)."""
+        _check(lib().gle_set_fused_threshold(self.h, int(min_patterns)))
 
     def heavy_patterns(self):
         n = C.c_int(0)

@@ -112,6 +112,11 @@ int gle_set_patterns(int h, const int *counts, int last_const, const int *const_
  * counterpart (the reference has one precision per build, defs.h:86-112). */
 int gle_set_heavy_split(int h, int threshold);
 int gle_get_heavy_split(int h, int *n_heavy);
+/* Trees with schedule 1 walk a whole op list in one launch only when the
+ * instance has at least `min_patterns` padded patterns (default 2^19, enough
+ * threads to fill the GPU); otherwise they launch per dependency level.  No
+ * reference counterpart (a launch-policy knob). */
+int gle_set_fused_threshold(int h, int min_patterns);
 /* state freqs [S], category weights [R] (Model::GetRateProbs, model.h:2051-2064), pinv */
 int gle_set_model(int h, const double *pi, const double *rate_probs, double pinv, int pinv_in_model);
 /* eigensystem e: V, Vinv [S*S], lam [S] (already scaled) */

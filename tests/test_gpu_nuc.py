@@ -109,6 +109,7 @@ def test_fp32_heavy_pattern_split(gpu):
     ds = sim.gtr_gamma(n_tips=24, n_sites=40000, pinv=0.3, mean_blen=0.05, seed=52)
     ref = O.Oracle(ds).score()
     f64 = E.Tree(ds.make_instance(E.PREC_FP64), ds.parent, ds.blen)
+    f64.score()
     site64 = f64.site_likelihoods()
     inst = ds.make_instance(E.PREC_FP32)
     n_heavy = inst.heavy_patterns()
@@ -132,9 +133,11 @@ def test_fp32_heavy_pattern_split(gpu):
     assert t.score() == got
 
 
-def test_fused_schedule_equals_levels(gpu):
+@pytest.mark.parametrize("prec", [E.PREC_FP64, E.PREC_FP32])
+def test_fused_schedule_equals_levels(gpu, prec):
     ds = sim.gtr_gamma(n_tips=30, n_sites=1500, seed=61)
-    inst = ds.make_instance(E.PREC_FP64)
+    inst = ds.make_instance(prec)
+    inst.set_fused_threshold(0)  # force the single-launch walk at this size
     a = E.Tree(inst, ds.parent, ds.blen)
     b = E.Tree(inst, ds.parent, ds.blen)
     b.set_schedule(1)
