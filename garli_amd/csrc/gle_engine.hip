@@ -7,8 +7,7 @@
 //   u      i32 [slots][Ppad]       per-pattern rescale counts (FP64: natural-log units
 //                                   exactly as the reference; FP32: powers of two)
 //   pm64   f64 [pmats][3][R][S][S] P, P', P'' (row = parent state, model.cpp:1271-1358)
-//   pm32   f32 [pmats][2][R][Sp][Sp] P as an FP32 pair hi + lo (lo = P - hi) for the
-//                                   FP32-bulk CLA kernels (codon: zero-padded to 64 x 64)
+//   pmpad  f64 [pmats][3][R][64][64] codons only: P, P', P'' zero-padded for the MFMA kernels
 //   counts i32 [Ppad], cst i32 [Ppad] (constant-pattern state/mask, -1 otherwise)
 // Ppad rounds P up to a multiple of 256 so every 256-thread tile is whole.
 #include <hip/hip_runtime.h>
@@ -77,7 +76,7 @@ struct Instance {
   int *d_counts = nullptr, *d_cst = nullptr, *d_u = nullptr, *d_flags = nullptr, *d_eig_of_rate = nullptr;
   void *d_cla = nullptr;
   double *d_pm64 = nullptr;
-  float *d_pm32 = nullptr;
+  double *d_pmpad = nullptr;  // codons only
   double *d_V = nullptr, *d_Vinv = nullptr, *d_lam = nullptr, *d_model = nullptr;  // model: pi[S] w[R] mult[R]
   // staging
   DevClaOp *d_ops = nullptr;
@@ -354,7 +353,7 @@ __global__ __launch_bounds__(256) void k_pmat(int S, int Sp, int R, const int *_
                                               const double *__restrict__ tt, const int *__restrict__ eig_of_rate,
                                               const double *__restrict__ rate_mult, const double *__restrict__ V,
                                               const double *__restrict__ Vinv, const double *__restrict__ lam,
-                                              double *__restrict__ pm64, float *__restrict__ pm32, int derivs) {
+                                              double *__restrict__ pm64, double *__restrict__ pmpad, int derivs) {
 #pragma clang fp contract(off)
   extern __shared__ double sm[];
   const int SS = S * S, op = blockIdx.x, r = blockIdx.y;
@@ -376,7 +375,7 @@ __global__ __launch_bounds__(256) void k_pmat(int S, int Sp, int R, const int *_
   double *P = pm64 + (size_t(slot) * 3 * R + r) * SS;
   double *D1 = pm64 + (size_t(slot) * 3 * R + R + r) * SS;
   double *D2 = pm64 + (size_t(slot) * 3 * R + 2 * R + r) * SS;
-  float *P32 = pm32 + (size_t(slot) * 2 * R + r) * Sp * Sp, *P32lo = P32 + size_t(R) * Sp * Sp;
+  double *Pp = pmpad ? pmpad + (size_t(slot) * 3 * R + r) * Sp * Sp : nullptr;
   for (int q = threadIdx.x; q < SS; q += blockDim.x) {
     const int i = q / S, j = q - i * S;
     double sp = 0.0, s1 = 0.0, s2 = 0.0;
@@ -390,12 +389,14 @@ __global__ __launch_bounds__(256) void k_pmat(int S, int Sp, int R, const int *_
     }
     const double pv = sp > 0.0 ? sp : 0.0;
     P[q] = pv;
-    const float hi = float(pv);
-    P32[i * Sp + j] = hi;
-    P32lo[i * Sp + j] = float(pv - double(hi));
+    if (Pp) Pp[i * Sp + j] = pv;
     if (derivs) {
       D1[q] = s1;
       D2[q] = s2;
+      if (Pp) {
+        Pp[size_t(R) * Sp * Sp + i * Sp + j] = s1;
+        Pp[size_t(2 * R) * Sp * Sp + i * Sp + j] = s2;
+      }
     }
   }
 }
@@ -557,131 +558,12 @@ __global__ __launch_bounds__(256) void k_cla_nuc(T *__restrict__ cla, int *__res
 }
 
 // =====================================================================
-// General-S CLA update (amino acids, codons): one thread per (pattern, rate),
-// P of the op staged in LDS.  tree.cpp:7843-7910, 8082-8160, 8537-8599.
-// =====================================================================
-template <typename T>
-struct NsClaArgs {
-  T *cla;
-  int *u;
-  const unsigned char *tips;
-  const double *pm64;  // FP64 P slots [slot][3][R][S][S]
-  const float *pm32;   // padded FP32 P slots [slot][2][R][Sp][Sp] (codon MFMA path)
-  size_t slot_stride;
-  int Ppad, P, R, Sp;
-  int *flags;
-};
-
-// FP64 arithmetic in both forms (the FP32 form stores with round_dither); the
-// FP64 form keeps the reference's mul-then-add order.
-template <typename T, int S>
-__global__ __launch_bounds__(256) void k_cla_ns(NsClaArgs<T> a, const DevClaOp *__restrict__ ops, int fused_n) {
-  // P staged in LDS for S <= 32; larger state spaces read P through the caches.
-  constexpr bool kLds = S <= 32;
-  constexpr bool kF32 = sizeof(T) == 4;
-  extern __shared__ unsigned char smem_raw[];
-  double *sP = reinterpret_cast<double *>(smem_raw);  // [2 children][R][S][S]
-  const int R = a.R, RP = R <= 1 ? 1 : R <= 2 ? 2 : R <= 4 ? 4 : 8;
-  const int r = threadIdx.x % RP;
-  const int p = blockIdx.x * (256 / RP) + threadIdx.x / RP;
-  const bool live = (p < a.P) && (r < R);
-  const int rr = r < R ? r : R - 1;
-  const int first = fused_n ? 0 : blockIdx.y, last = fused_n ? fused_n : blockIdx.y + 1;
-  const int RSS = R * S * S;
-  const size_t pm_stride = size_t(3) * RSS;
-  for (int oi = first; oi < last; ++oi) {
-    const DevClaOp op = ops[oi];
-    const double *P1, *P2;
-    if constexpr (kLds) {
-      if (oi != first) __syncthreads();
-      for (int q = threadIdx.x; q < RSS; q += blockDim.x) {
-        sP[q] = a.pm64[op.pm1 * pm_stride + q];
-        sP[RSS + q] = a.pm64[op.pm2 * pm_stride + q];
-      }
-      __syncthreads();
-      P1 = sP + rr * S * S;
-      P2 = sP + RSS + rr * S * S;
-    } else {
-      P1 = a.pm64 + op.pm1 * pm_stride + rr * S * S;
-      P2 = a.pm64 + op.pm2 * pm_stride + rr * S * S;
-    }
-    T x1[S], x2[S];
-    double out[S];
-    int t1 = S, t2 = S, u = 0;
-    const size_t off = (size_t(p) * R + rr) * a.Sp;
-    if (live) {
-      if (op.c1 >= 0) {
-#pragma unroll
-        for (int q = 0; q < S; q++) x1[q] = a.cla[op.c1 * a.slot_stride + off + q];
-        u += a.u[op.c1 * size_t(a.Ppad) + p];
-      } else
-        t1 = a.tips[size_t(-op.c1 - 1) * a.Ppad + p];
-      if (op.c2 >= 0) {
-#pragma unroll
-        for (int q = 0; q < S; q++) x2[q] = a.cla[op.c2 * a.slot_stride + off + q];
-        u += a.u[op.c2 * size_t(a.Ppad) + p];
-      } else
-        t2 = a.tips[size_t(-op.c2 - 1) * a.Ppad + p];
-#pragma unroll 4
-      for (int f = 0; f < S; f++) {
-        double L1, R1;
-        if (op.c1 >= 0) {
-          L1 = 0.0;
-          if constexpr (kF32) {
-#pragma unroll
-            for (int q = 0; q < S; q++) L1 = __builtin_fma(P1[f * S + q], double(x1[q]), L1);
-          } else {
-#pragma unroll
-            for (int q = 0; q < S; q++) L1 = add_nc(L1, mul_nc(P1[f * S + q], x1[q]));
-          }
-        } else
-          L1 = t1 < S ? P1[f * S + t1] : 1.0;
-        if (op.c2 >= 0) {
-          R1 = 0.0;
-          if constexpr (kF32) {
-#pragma unroll
-            for (int q = 0; q < S; q++) R1 = __builtin_fma(P2[f * S + q], double(x2[q]), R1);
-          } else {
-#pragma unroll
-            for (int q = 0; q < S; q++) R1 = add_nc(R1, mul_nc(P2[f * S + q], x2[q]));
-          }
-        } else
-          R1 = t2 < S ? P2[f * S + t2] : 1.0;
-        out[f] = mul_nc(L1, R1);
-      }
-    } else {
-#pragma unroll
-      for (int f = 0; f < S; f++) out[f] = 0.0;
-    }
-    if (op.flags & kOpRescale) {
-      if constexpr (kF32) {
-        double m = out[0];
-#pragma unroll
-        for (int q = 1; q < S; q++) m = fmax(m, out[q]);
-        m = group_max(m, RP);
-        if (p < a.P) u += rescale_pow2<S>(out, m);
-      } else
-        u += rescale_slice<double, S>(out, RP, a.flags, p < a.P);
-    }
-    if (live) {
-#pragma unroll
-      for (int f = 0; f < S; f++) {
-        if constexpr (kF32)
-          a.cla[op.dest * a.slot_stride + off + f] = round_dither(out[f], unsigned(p) * 512u + unsigned(rr * S + f));
-        else
-          a.cla[op.dest * a.slot_stride + off + f] = out[f];
-      }
-      if (r == 0) a.u[op.dest * size_t(a.Ppad) + p] = u;
-    }
-  }
-}
-
-// =====================================================================
 // Amino-acid CLA update (S = 20).  A block is R waves over the same 64
 // patterns and wave r takes category r, so the op's P rows are wave-uniform:
 // scalar loads, SGPR operands, no LDS traffic for P.  The per-pattern rescale
-// takes the max over the R waves through a small LDS table.  Arithmetic as in
-// k_cla_ns (FP64; reference mul-then-add order for the FP64 form).
+// takes the max over the R waves through a small LDS table.  FP64 arithmetic
+// in both forms (reference mul-then-add order for the FP64 form).
+// tree.cpp:7843-7910, 8082-8160, 8537-8599.
 // =====================================================================
 template <typename T, int R>
 __global__ __launch_bounds__(64 * R) void k_cla_aa(T *__restrict__ cla, int *__restrict__ ua,
@@ -794,124 +676,149 @@ __global__ __launch_bounds__(64 * R) void k_cla_aa(T *__restrict__ cla, int *__r
 }
 
 // =====================================================================
-// Codon CLA update, FP32 bulk form, on MFMA (61 states padded to 64).
-// One wave owns 16 patterns of an op and all R categories.  For each
-// (category, child) the 64x64 contraction out[p][i] = sum_j P[i][j] x[p][j]
-// runs as 4 output tiles x 16 steps of v_mfma_f32_16x16x4_f32.  The K index
-// is permuted so that, at step kk, the four K rows of the instruction are
-// j = 16g + kk (g = lane/16): each lane's A and B fragments are then 16
-// contiguous floats of one CLA row / one P row (four 16-byte loads).
-// MFMA D layout: lane l, element v holds row 4*(l/16)+v (pattern), column l%16.
+// Codon CLA update (61 states padded to 64) on FP64 MFMA, both forms.
+// One wave owns 16 patterns of an op.  For each (category, child) the
+// contraction out[p][i] = sum_j P[i][j] x[p][j] runs as 4 output tiles x 16
+// steps of v_mfma_f64_16x16x4_f64.  The K index is permuted so that at step kk
+// the four K rows of the instruction are j = 16g + kk (g = lane/16): each
+// lane's A and B fragments are then 16 contiguous values of one CLA row / one
+// P row.  v_mfma_f64_16x16x4_f64 D layout: lane l, element v holds row
+// l/16 + 4v (pattern), column l%16 (state).  P comes from the zero-padded FP64 copy
+// pmpad [slot][3][R][64][64] (P, P', P'').  The MFMA sums in its own order,
+// so FP64 codon CLAs match the reference's sequential sums to rounding
+// (about 1e-16 relative), not bit for bit.
 // =====================================================================
-typedef float gle_floatx4 __attribute__((ext_vector_type(4)));
+typedef double gle_doublex4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ void codon_child_f32(const NsClaArgs<float> &a, int c, int pm, int r, int R, int lo,
-                                                int g, int p0, gle_floatx4 acc[4], int &u_add) {
-  const float *P = a.pm32 + size_t(pm) * (2 * R * 64 * 64) + size_t(r) * 64 * 64;
+template <typename T>
+__device__ __forceinline__ void load16(const T *src, double *x) {
+#pragma unroll
+  for (int q = 0; q < 16; q += 4) {
+    T v[4];
+    load4(src + q, v);
+    x[q] = double(v[0]);
+    x[q + 1] = double(v[1]);
+    x[q + 2] = double(v[2]);
+    x[q + 3] = double(v[3]);
+  }
+}
+
+// acc[t] = tile t of P . x for the wave's 16 patterns (x a CLA or a tip column).
+template <typename T>
+__device__ __forceinline__ void codon_child(const T *__restrict__ cla, const unsigned char *__restrict__ tips,
+                                            const double *__restrict__ Pr, int c, size_t slot, int Ppad, int P, int R,
+                                            int r, int lo, int g, int p0, gle_doublex4 acc[4]) {
   if (c >= 0) {
-    float x[16];
+    double x[16];
     const int p = p0 + lo;
-    if (p < a.P) {
-      const float4 *src = reinterpret_cast<const float4 *>(a.cla + size_t(c) * a.slot_stride +
-                                                           (size_t(p) * R + r) * 64 + 16 * g);
+    if (p < P)
+      load16(cla + c * slot + (size_t(p) * R + r) * 64 + 16 * g, x);
+    else {
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const float4 v = src[q];
-        x[4 * q] = v.x; x[4 * q + 1] = v.y; x[4 * q + 2] = v.z; x[4 * q + 3] = v.w;
-      }
-    } else {
-#pragma unroll
-      for (int q = 0; q < 16; q++) x[q] = 0.0f;
+      for (int q = 0; q < 16; q++) x[q] = 0.0;
     }
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-      float b[16];
-      const float4 *ps = reinterpret_cast<const float4 *>(P + (16 * t + lo) * 64 + 16 * g);
+      double b[16];
+      load16(Pr + (16 * t + lo) * 64 + 16 * g, b);
+      gle_doublex4 d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const float4 v = ps[q];
-        b[4 * q] = v.x; b[4 * q + 1] = v.y; b[4 * q + 2] = v.z; b[4 * q + 3] = v.w;
-      }
-      gle_floatx4 d = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-      for (int kk = 0; kk < 16; kk++) d = __builtin_amdgcn_mfma_f32_16x16x4f32(x[kk], b[kk], d, 0, 0, 0);
+      for (int kk = 0; kk < 16; kk++) d = __builtin_amdgcn_mfma_f64_16x16x4f64(x[kk], b[kk], d, 0, 0, 0);
       acc[t] = d;
     }
-    if (r == 0) u_add = 1;
   } else {
-    const unsigned char *tip = a.tips + size_t(-c - 1) * a.Ppad;
+    const unsigned char *tip = tips + size_t(-c - 1) * Ppad;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-      const int p = p0 + 4 * g + v;
-      const int st = p < a.P ? tip[p] : 61;
+      const int p = p0 + g + 4 * v;
+      const int st = p < P ? tip[p] : 61;
 #pragma unroll
-      for (int t = 0; t < 4; t++) acc[t][v] = st < 61 ? P[(16 * t + lo) * 64 + st] : 1.0f;
+      for (int t = 0; t < 4; t++) acc[t][v] = st < 61 ? Pr[(16 * t + lo) * 64 + st] : 1.0;
     }
   }
 }
 
-template <int R>
-__global__ __launch_bounds__(64) void k_cla_codon_mfma(NsClaArgs<float> a, const DevClaOp *__restrict__ ops,
-                                                       int fused_n) {
+// grid (Ppad/16, fused ? 1 : n_ops), one wave per block.  Each category's
+// product is stored as soon as it is formed; the rare rescale of a pattern
+// then re-reads and scales what this lane stored.
+template <typename T, int R>
+__global__ __launch_bounds__(64) void k_cla_codon(T *__restrict__ cla, int *__restrict__ ua,
+                                                  const unsigned char *__restrict__ tips,
+                                                  const double *__restrict__ pmpad, const DevClaOp *__restrict__ ops,
+                                                  int *__restrict__ flags, int Ppad, int P, int fused_n) {
+  constexpr bool kF32 = sizeof(T) == 4;
   const int l = threadIdx.x, lo = l & 15, g = l >> 4;
   const int p0 = blockIdx.x * 16;
+  const size_t slot = size_t(Ppad) * R * 64;
   const int first = fused_n ? 0 : blockIdx.y, last = fused_n ? fused_n : blockIdx.y + 1;
   for (int oi = first; oi < last; ++oi) {
-    if (oi != first) __syncthreads();  // make the previous op's stores visible to every lane
+    if (oi != first) __syncthreads();  // the previous op's stores are read by other lanes
     const DevClaOp op = ops[oi];
-    float out[R][4][4];
-    int u1 = 0, u2 = 0;
-#pragma unroll
+    T *dst = cla + op.dest * slot;
+    double mx[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
     for (int r = 0; r < R; r++) {
-      gle_floatx4 dl[4], dr[4];
-      codon_child_f32(a, op.c1, op.pm1, r, R, lo, g, p0, dl, u1);
-      codon_child_f32(a, op.c2, op.pm2, r, R, lo, g, p0, dr, u2);
-#pragma unroll
-      for (int t = 0; t < 4; t++)
-#pragma unroll
-        for (int v = 0; v < 4; v++) out[r][t][v] = (16 * t + lo < 61) ? dl[t][v] * dr[t][v] : 0.0f;
-    }
-    int uv[4];
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const int p = p0 + 4 * g + v;
-      int u = 0;
-      if (p < a.P) {
-        if (op.c1 >= 0) u += a.u[size_t(op.c1) * a.Ppad + p];
-        if (op.c2 >= 0) u += a.u[size_t(op.c2) * a.Ppad + p];
-      }
-      uv[v] = u;
-    }
-    if (op.flags & kOpRescale) {
+      gle_doublex4 a1[4], a2[4];
+      codon_child(cla, tips, pmpad + (size_t(op.pm1) * 3 * R + r) * 4096, op.c1, slot, Ppad, P, R, r, lo, g, p0, a1);
+      codon_child(cla, tips, pmpad + (size_t(op.pm2) * 3 * R + r) * 4096, op.c2, slot, Ppad, P, R, r, lo, g, p0, a2);
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        float m = 0.0f;
+        const int p = p0 + g + 4 * v;
 #pragma unroll
-        for (int r = 0; r < R; r++)
-#pragma unroll
-          for (int t = 0; t < 4; t++) m = fmaxf(m, out[r][t][v]);
-        for (int o = 1; o < 16; o <<= 1) m = fmaxf(m, __shfl_xor(m, o));
-        if (p0 + 4 * g + v < a.P && m < 0x1p-32f && m > 0.0f) {
-          const int e = -ilogbf(m);
-#pragma unroll
-          for (int r = 0; r < R; r++)
-#pragma unroll
-            for (int t = 0; t < 4; t++) out[r][t][v] = ldexpf(out[r][t][v], e);
-          uv[v] += e;
+        for (int t = 0; t < 4; t++) {
+          const int st = 16 * t + lo;
+          const double o = st < 61 ? mul_nc(a1[t][v], a2[t][v]) : 0.0;
+          mx[v] = fmax(mx[v], o);
+          if (p < P) {
+            T ov;
+            if constexpr (kF32)
+              ov = round_dither(o, unsigned(p) * 512u + unsigned(r * 64 + st));
+            else
+              ov = o;
+            dst[(size_t(p) * R + r) * 64 + st] = ov;
+          }
         }
       }
     }
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-      const int p = p0 + 4 * g + v;
-      if (p < a.P) {
-#pragma unroll
-        for (int r = 0; r < R; r++)
-#pragma unroll
-          for (int t = 0; t < 4; t++)
-            a.cla[size_t(op.dest) * a.slot_stride + (size_t(p) * R + r) * 64 + 16 * t + lo] = out[r][t][v];
-        if (lo == 0) a.u[size_t(op.dest) * a.Ppad + p] = uv[v];
+      const int p = p0 + g + 4 * v;
+      int u = 0;
+      if (p < P) {
+        if (op.c1 >= 0) u += ua[size_t(op.c1) * Ppad + p];
+        if (op.c2 >= 0) u += ua[size_t(op.c2) * Ppad + p];
       }
+      if (op.flags & kOpRescale) {
+        double m = mx[v];
+        for (int o = 1; o < 16; o <<= 1) m = fmax(m, __shfl_xor(m, o));
+        if (p < P) {
+          if constexpr (kF32) {
+            if (m < 0x1p-32 && m > 0.0) {
+              const int e = 1 - __builtin_amdgcn_frexp_exp(m);
+              for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                  T *q = dst + (size_t(p) * R + r) * 64 + 16 * t + lo;
+                  *q = ldexpf(*q, e);
+                }
+              u += e;
+            }
+          } else if (m < c_rescale_below) {
+            if (m < c_reduce_below && lo == 0) atomicOr(flags, m < c_bail_below ? 3 : 1);
+            int idx = 0;
+            while (idx + 1 < kRescaleLen && c_thresh[idx + 1] > m) idx++;
+            const double mult = c_mult[idx];
+            for (int r = 0; r < R; r++)
+#pragma unroll
+              for (int t = 0; t < 4; t++) {
+                T *q = dst + (size_t(p) * R + r) * 64 + 16 * t + lo;
+                *q = mul_nc(*q, mult);
+              }
+            u += c_incr[idx];
+          }
+        }
+      }
+      if (p < P && lo == 0) ua[size_t(op.dest) * Ppad + p] = u;
     }
   }
 }
@@ -1091,6 +998,91 @@ __global__ __launch_bounds__(256) void k_edge_nuc(const T *__restrict__ cla, con
     double ul;
     site_terms<sizeof(T) == 4>(Lt, D1, D2, u, pt, false, counts[p], lnl, g1, g2, ul);
     if (site && blockIdx.y == 0) site[p] = ul;
+  }
+  block_reduce3(lnl, g1, g2, partials + (size_t(blockIdx.y) * gridDim.x + blockIdx.x) * 3);
+}
+
+// Codon edge: per category, Y = P.y (and P'.y, P''.y) on FP64 MFMA in the
+// layout of k_cla_codon, then each lane weights its (pattern, state) cells by
+// x * pi, the 16 lanes of a pattern row reduce, and the categories combine in
+// order with their weights (GetScorePartialNState / GetDerivsPartialNState,
+// tree.cpp:6431-6480, optimization.cpp:3230-3290).  grid (Ppad/16, n_ops).
+template <typename T, int R>
+__global__ __launch_bounds__(64) void k_edge_codon(const T *__restrict__ cla, const int *__restrict__ ua,
+                                                   const unsigned char *__restrict__ tips,
+                                                   const double *__restrict__ pmpad, const double *__restrict__ model,
+                                                   const int *__restrict__ counts, const int *__restrict__ cst,
+                                                   const DevEdgeOp *__restrict__ ops, double *__restrict__ partials,
+                                                   double *__restrict__ site, double pinv, int pinv_in_model,
+                                                   int Ppad, int P) {
+  const int l = threadIdx.x, lo = l & 15, g = l >> 4;
+  const int p0 = blockIdx.x * 16;
+  const size_t slot = size_t(Ppad) * R * 64;
+  const DevEdgeOp op = ops[blockIdx.y];
+  const double *pi = model, *w = model + 61;
+  const int nm = op.derivs ? 3 : 1;
+  double sL[4] = {0, 0, 0, 0}, s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0};
+#pragma unroll 1
+  for (int r = 0; r < R; r++) {
+    double acc[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};  // [matrix][v]: sum over this lane's states
+#pragma unroll 1
+    for (int mi = 0; mi < nm; mi++) {
+      gle_doublex4 Y[4];
+      codon_child(cla, tips, pmpad + ((size_t(op.pm) * 3 + mi) * R + r) * 4096, op.child, slot, Ppad, P, R, r, lo,
+                  g, p0, Y);
+      if (op.child < 0 && mi > 0) {  // a missing tip contributes no derivative
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int p = p0 + g + 4 * v;
+          const int st = p < P ? tips[size_t(-op.child - 1) * Ppad + p] : 61;
+          if (st >= 61)
+#pragma unroll
+            for (int t = 0; t < 4; t++) Y[t][v] = 0.0;
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int p = p0 + g + 4 * v;
+        if (p >= P) continue;
+        const T *xp = cla + op.partial * slot + (size_t(p) * R + r) * 64;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          const int st = 16 * t + lo;
+          if (st < 61) acc[mi][v] += Y[t][v] * double(xp[st]) * pi[st];
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      double a0 = acc[0][v], a1 = acc[1][v], a2 = acc[2][v];
+      for (int o = 1; o < 16; o <<= 1) {
+        a0 += __shfl_xor(a0, o);
+        a1 += __shfl_xor(a1, o);
+        a2 += __shfl_xor(a2, o);
+      }
+      sL[v] += a0 * w[r];
+      s1[v] += a1 * w[r];
+      s2[v] += a2 * w[r];
+    }
+  }
+  double lnl = 0, g1 = 0, g2 = 0;
+  if (lo == 0) {
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int p = p0 + g + 4 * v;
+      if (p >= P) continue;
+      int u = ua[size_t(op.partial) * Ppad + p];
+      if (op.child >= 0) u += ua[size_t(op.child) * Ppad + p];
+      double pt = 0.0;
+      const int cs = cst[p];
+      if (pinv_in_model && cs >= 0 && cs < 61) pt = pinv * pi[cs];
+      double l1, d1, d2, ul;
+      site_terms<sizeof(T) == 4>(sL[v], s1[v], s2[v], u, pt, true, counts[p], l1, d1, d2, ul);
+      lnl += l1;
+      g1 += d1;
+      g2 += d2;
+      if (site && blockIdx.y == 0) site[p] = ul;
+    }
   }
   block_reduce3(lnl, g1, g2, partials + (size_t(blockIdx.y) * gridDim.x + blockIdx.x) * 3);
 }
@@ -1286,7 +1278,7 @@ int engine_update_pmats(Instance *I, const int *idx, const double *t, int n, int
   const size_t shmem = sizeof(double) * (2 * S * S + 3 * S);
   timed_begin(I, 2, double(n) * I->R * S * S * 8.0 * (derivs ? 3 : 1));
   hipLaunchKernelGGL(k_pmat, dim3(n, I->R), dim3(256), shmem, I->stream, S, I->Sp, I->R, I->d_pmidx, I->d_pmt,
-                     I->d_eig_of_rate, I->d_model + S + I->R, I->d_V, I->d_Vinv, I->d_lam, I->d_pm64, I->d_pm32,
+                     I->d_eig_of_rate, I->d_model + S + I->R, I->d_V, I->d_Vinv, I->d_lam, I->d_pm64, I->d_pmpad,
                      derivs);
   timed_end(I);
   GLE_HIP(hipGetLastError());
@@ -1319,40 +1311,25 @@ static int launch_nuc_r(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
   return GLE_SUCCESS;
 }
 
+template <typename T, int R>
+static void launch_codon(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
+  hipLaunchKernelGGL((k_cla_codon<T, R>), dim3(I->Ppad / 16, fused ? 1 : n), dim3(64), 0, I->stream,
+                     reinterpret_cast<T *>(I->d_cla), I->d_u, I->d_tips, I->d_pmpad, d_ops, I->d_flags, I->Ppad,
+                     I->P, fused ? n : 0);
+}
+
 template <typename T>
-static NsClaArgs<T> ns_args(Instance *I) {
-  NsClaArgs<T> a;
-  a.cla = reinterpret_cast<T *>(I->d_cla);
-  a.u = I->d_u;
-  a.tips = I->d_tips;
-  a.pm64 = I->d_pm64;
-  a.pm32 = I->d_pm32;
-  a.slot_stride = I->slot_elems();
-  a.Ppad = I->Ppad;
-  a.P = I->P;
-  a.R = I->R;
-  a.Sp = I->Sp;
-  a.flags = I->d_flags;
-  return a;
-}
-
-template <typename T, int S>
-static void launch_ns(Instance *I, const DevClaOp *d_ops, int n, bool fused, int nblk) {
-  const size_t shmem = S <= 32 ? sizeof(double) * 2 * I->R * S * S : 0;
-  hipLaunchKernelGGL((k_cla_ns<T, S>), dim3(nblk, fused ? 1 : n), dim3(256), shmem, I->stream, ns_args<T>(I),
-                     d_ops, fused ? n : 0);
-}
-
-static int launch_codon_mfma(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
-  const NsClaArgs<float> a = ns_args<float>(I);
-  const dim3 grid(I->Ppad / 16, fused ? 1 : n), block(64);
-  const int fn = fused ? n : 0;
+static int launch_codon_r(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
   switch (I->R) {
-    case 1: hipLaunchKernelGGL(k_cla_codon_mfma<1>, grid, block, 0, I->stream, a, d_ops, fn); break;
-    case 2: hipLaunchKernelGGL(k_cla_codon_mfma<2>, grid, block, 0, I->stream, a, d_ops, fn); break;
-    case 3: hipLaunchKernelGGL(k_cla_codon_mfma<3>, grid, block, 0, I->stream, a, d_ops, fn); break;
-    case 4: hipLaunchKernelGGL(k_cla_codon_mfma<4>, grid, block, 0, I->stream, a, d_ops, fn); break;
-    default: return set_error(GLE_ERR_UNSUPPORTED, "codon MFMA path takes 1..4 categories");
+    case 1: launch_codon<T, 1>(I, d_ops, n, fused); break;
+    case 2: launch_codon<T, 2>(I, d_ops, n, fused); break;
+    case 3: launch_codon<T, 3>(I, d_ops, n, fused); break;
+    case 4: launch_codon<T, 4>(I, d_ops, n, fused); break;
+    case 5: launch_codon<T, 5>(I, d_ops, n, fused); break;
+    case 6: launch_codon<T, 6>(I, d_ops, n, fused); break;
+    case 7: launch_codon<T, 7>(I, d_ops, n, fused); break;
+    case 8: launch_codon<T, 8>(I, d_ops, n, fused); break;
+    default: return set_error(GLE_ERR_UNSUPPORTED, "rate categories must be 1..8");
   }
   return GLE_SUCCESS;
 }
@@ -1382,12 +1359,10 @@ static int launch_aa_r(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
 
 template <typename T>
 static int launch_cla(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
-  const int nblk = blocks_for(I);
-  if (sizeof(T) == 4 && I->S == 61 && I->R <= 4) return launch_codon_mfma(I, d_ops, n, fused);
   switch (I->S) {
     case 4: return launch_nuc_r<T>(I, d_ops, n, fused);
     case 20: return launch_aa_r<T>(I, d_ops, n, fused);
-    case 61: launch_ns<T, 61>(I, d_ops, n, fused, nblk); return GLE_SUCCESS;
+    case 61: return launch_codon_r<T>(I, d_ops, n, fused);
     default: return set_error(GLE_ERR_UNSUPPORTED, "state count %d has no kernel", I->S);
   }
 }
@@ -1490,13 +1465,29 @@ static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk) {
     if (shmem > 65536) return set_error(GLE_ERR_UNSUPPORTED, "20-state edge kernel takes at most 6 categories");
     hipLaunchKernelGGL((k_edge_ns<T, 20>), grid, dim3(256), shmem, I->stream, a, d_ops);
   } else if (I->S == 61) {
-    hipLaunchKernelGGL((k_edge_ns<T, 61>), grid, dim3(256), 0, I->stream, a, d_ops);
+#define GLE_EDGE_CODON(RR)                                                                                        \
+  hipLaunchKernelGGL((k_edge_codon<T, RR>), grid, dim3(64), 0, I->stream, a.cla, a.u, a.tips, I->d_pmpad, a.model, \
+                     a.counts, a.cst, d_ops, a.partials, a.site, a.pinv, a.pinv_in_model, a.Ppad, a.P)
+    switch (I->R) {
+      case 1: GLE_EDGE_CODON(1); break;
+      case 2: GLE_EDGE_CODON(2); break;
+      case 3: GLE_EDGE_CODON(3); break;
+      case 4: GLE_EDGE_CODON(4); break;
+      case 5: GLE_EDGE_CODON(5); break;
+      case 6: GLE_EDGE_CODON(6); break;
+      case 7: GLE_EDGE_CODON(7); break;
+      case 8: GLE_EDGE_CODON(8); break;
+      default: return set_error(GLE_ERR_UNSUPPORTED, "rate categories must be 1..8");
+    }
+#undef GLE_EDGE_CODON
   } else
     return set_error(GLE_ERR_UNSUPPORTED, "state count %d has no kernel", I->S);
   return GLE_SUCCESS;
 }
 
-static int edge_blocks(const Instance *I) { return I->S == 4 ? I->Ppad / 256 : blocks_for(I); }
+static int edge_blocks(const Instance *I) {
+  return I->S == 4 ? I->Ppad / 256 : I->S == 61 ? I->Ppad / 16 : blocks_for(I);
+}
 
 static int engine_edges_one(Instance *I, const DevEdgeOp *ops, int n, double *out, double *site_lnl) {
   std::lock_guard<std::mutex> lk_(I->mu);
@@ -1650,7 +1641,7 @@ static int destroy(Instance *I) {
     I->heavy.reset();
   }
   void *ptrs[] = {I->d_tips, I->d_counts, I->d_cst, I->d_u, I->d_flags, I->d_eig_of_rate, I->d_cla,
-                  I->d_pm64, I->d_pm32,  I->d_V,   I->d_Vinv, I->d_lam, I->d_model, I->d_ops,
+                  I->d_pm64, I->d_pmpad,  I->d_V,   I->d_Vinv, I->d_lam, I->d_model, I->d_ops,
                   I->d_eops, I->d_part,  I->d_eres, I->d_site, I->d_pmidx, I->d_pmt};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -1726,7 +1717,7 @@ static int create_internal(const gle_instance_desc *d, std::unique_ptr<Instance>
   GLE_ALLOC(I->d_eig_of_rate, sizeof(int) * R);
   GLE_ALLOC(I->d_cla, I->slot_elems() * I->elem_bytes() * I->n_slots);
   GLE_ALLOC(I->d_pm64, sizeof(double) * size_t(I->n_pm) * 3 * R * S * S);
-  GLE_ALLOC(I->d_pm32, sizeof(float) * size_t(I->n_pm) * 2 * R * I->Sp * I->Sp);
+  if (S == 61) GLE_ALLOC(I->d_pmpad, sizeof(double) * size_t(I->n_pm) * 3 * R * 64 * 64);
   GLE_ALLOC(I->d_V, sizeof(double) * size_t(I->n_eig) * S * S);
   GLE_ALLOC(I->d_Vinv, sizeof(double) * size_t(I->n_eig) * S * S);
   GLE_ALLOC(I->d_lam, sizeof(double) * size_t(I->n_eig) * S);
@@ -1742,7 +1733,7 @@ static int create_internal(const gle_instance_desc *d, std::unique_ptr<Instance>
       hipMemset(I->d_flags, 0, sizeof(int)) != hipSuccess ||
       hipMemset(I->d_cla, 0, I->slot_elems() * I->elem_bytes() * I->n_slots) != hipSuccess ||
       hipMemset(I->d_pm64, 0, sizeof(double) * size_t(I->n_pm) * 3 * R * S * S) != hipSuccess ||
-      hipMemset(I->d_pm32, 0, sizeof(float) * size_t(I->n_pm) * 2 * R * I->Sp * I->Sp) != hipSuccess)
+      (I->d_pmpad && hipMemset(I->d_pmpad, 0, sizeof(double) * size_t(I->n_pm) * 3 * R * 64 * 64) != hipSuccess))
     return fail(set_error(GLE_ERR_DEVICE, "device memset failed"));
   I->h_tips.assign(size_t(I->N) * I->P_orig, missing);
   I->h_pi.assign(S, 1.0 / S);
@@ -1849,6 +1840,9 @@ static int apply_patterns(Instance *I) {
     GLE_HIP(hipStreamSynchronize(I->stream));
     GLE_HIP(hipMemcpy(I->heavy->d_pm64, I->d_pm64, sizeof(double) * size_t(I->n_pm) * 3 * I->R * I->S * I->S,
                       hipMemcpyDeviceToDevice));
+    if (I->d_pmpad)
+      GLE_HIP(hipMemcpy(I->heavy->d_pmpad, I->d_pmpad, sizeof(double) * size_t(I->n_pm) * 3 * I->R * 64 * 64,
+                        hipMemcpyDeviceToDevice));
   }
   if ((rc = push_model(I->heavy.get(), I))) return rc;
   if ((rc = push_patterns(I, I, &I->light_idx))) return rc;

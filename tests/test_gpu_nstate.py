@@ -82,6 +82,28 @@ def test_codon_fp64_matches_oracle(gpu, seed, ntax, npat):
         assert abs(d1 - r1) < 1e-8 * max(1, abs(r1)) and abs(d2 - r2) < 1e-8 * max(1, abs(r2))
 
 
+def test_codon_every_cla_and_counts(gpu):
+    """FP64 MFMA codon CLAs against the oracle's sequential sums, element-wise
+    to 1e-12 of each pattern's largest entry (the MFMA sums in another order,
+    so entries far below the maximum carry that order's rounding); rescale
+    counts equal."""
+    ds = sim.codon_m3(n_tips=20, n_patterns=300, seed=15)
+    ds.tips[2, ::3] = 61
+    t = E.Tree(ds.make_instance(E.PREC_FP64), ds.parent, ds.blen)
+    o = O.Oracle(ds, own_model=False)
+    rescaled = 0
+    for a in (0, ds.n_tips + 1, ds.n_tips + 7, 2 * ds.n_tips - 3):
+        nbrs = ([int(ds.parent[a])] if a else []) + [n for n in range(1, 2 * ds.n_tips - 2) if ds.parent[n] == a]
+        for b in nbrs:
+            g, gu = t.get_cla(a, b)
+            r, ru = o.cla(a, b)
+            np.testing.assert_array_equal(gu, ru)
+            scale = r.max(axis=(1, 2), keepdims=True)
+            assert np.all(np.abs(g - r) <= 1e-12 * scale)
+            rescaled += int((ru > 0).sum())
+    assert rescaled > 0
+
+
 def test_codon_fp32_within_tolerance(gpu):
     ds = sim.codon_m3(n_tips=16, n_patterns=800, seed=14)
     t = E.Tree(ds.make_instance(E.PREC_FP32), ds.parent, ds.blen)
