@@ -283,8 +283,12 @@ def bench_states(args):
         inst.close()
         log("%s sweep %.3f ms, CLA kernels %.1f GB/s" % (tag, ms, res[tag]["cla_kernel_GBs"]))
     if S == 61:
-        flops = 3 * (2 * 2 * 61 * 61) * n_cla * P  # useful codon CLA flops per sweep
-        res["fp32"]["codon_mfma_TFLOPs"] = flops / (res["fp32"]["cla_kernel_ms_per_sweep"] / 1000.0) / 1e12
+        flops = ds.n_rates * (2 * 2 * 61 * 61) * n_cla * P  # useful codon CLA flops per sweep
+        for tag in ("fp64", "fp32"):
+            res[tag]["codon_mfma_TFLOPs"] = flops / (res[tag]["cla_kernel_ms_per_sweep"] / 1000.0) / 1e12
+    if args.quick:
+        print(json.dumps({"workload": name, "sweeps": res}), flush=True)
+        return
     pop = _population(ds, 64, 11)
     e64, lat64 = _eval_rates(ds, E, pop, E.PREC_FP64)
     e32, _ = _eval_rates(ds, E, pop, E.PREC_FP32)

@@ -1252,12 +1252,24 @@ static bool level_ops(const DevClaOp *ops, int n, std::vector<DevClaOp> &sorted,
   return true;
 }
 
-int engine_update_pmats(Instance *I, const int *idx, const double *t, int n, int derivs) {
+// Work on an instance and its heavy-pattern part goes out on both streams
+// before either is waited for, so the two parts overlap on the device.
+static int finish(Instance *I) {
+  GLE_HIP(hipGetLastError());
+  GLE_HIP(hipStreamSynchronize(I->stream));
+  timed_collect(I);
+  return GLE_SUCCESS;
+}
+static int finish_both(Instance *I, int rc) {
   if (I->heavy) {
-    const int rc = engine_update_pmats(I->heavy.get(), idx, t, n, derivs);
-    if (rc) return rc;
+    const int rh = finish(I->heavy.get());
+    if (!rc) rc = rh;
   }
-  std::lock_guard<std::mutex> lk_(I->mu);
+  const int rm = finish(I);
+  return rc ? rc : rm;
+}
+
+static int enqueue_pmats(Instance *I, const int *idx, const double *t, int n, int derivs) {
   if (n <= 0) return GLE_SUCCESS;
   for (int i = 0; i < n; i++)
     if (idx[i] < 0 || idx[i] >= I->n_pm || !(t[i] >= 0.0))
@@ -1282,10 +1294,15 @@ int engine_update_pmats(Instance *I, const int *idx, const double *t, int n, int
                      derivs);
   timed_end(I);
   GLE_HIP(hipGetLastError());
-  // the staging buffers are reused by the next call only after this stream drains
-  GLE_HIP(hipStreamSynchronize(I->stream));
-  timed_collect(I);
   return GLE_SUCCESS;
+}
+
+// The staging buffers are reused by the next call only after the streams drain.
+int engine_update_pmats(Instance *I, const int *idx, const double *t, int n, int derivs) {
+  std::lock_guard<std::mutex> lk_(I->mu);
+  int rc = I->heavy ? enqueue_pmats(I->heavy.get(), idx, t, n, derivs) : GLE_SUCCESS;
+  if (!rc) rc = enqueue_pmats(I, idx, t, n, derivs);
+  return finish_both(I, rc);
 }
 
 template <typename T, int R>
@@ -1367,13 +1384,8 @@ static int launch_cla(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
   }
 }
 
-int engine_update_partials(Instance *I, const DevClaOp *ops, int n_ops, const int *level_start, int n_levels,
-                           bool fused) {
-  if (I->heavy) {
-    const int rc = engine_update_partials(I->heavy.get(), ops, n_ops, level_start, n_levels, fused);
-    if (rc) return rc;
-  }
-  std::lock_guard<std::mutex> lk_(I->mu);
+static int enqueue_partials(Instance *I, const DevClaOp *ops, int n_ops, const int *level_start, int n_levels,
+                            bool fused) {
   if (n_ops <= 0) return GLE_SUCCESS;
   fused = fused && engine_supports_fused(I);
   std::vector<DevClaOp> lv_ops;
@@ -1418,9 +1430,15 @@ int engine_update_partials(Instance *I, const DevClaOp *ops, int n_ops, const in
     }
   }
   GLE_HIP(hipGetLastError());
-  GLE_HIP(hipStreamSynchronize(I->stream));
-  timed_collect(I);
   return GLE_SUCCESS;
+}
+
+int engine_update_partials(Instance *I, const DevClaOp *ops, int n_ops, const int *level_start, int n_levels,
+                           bool fused) {
+  std::lock_guard<std::mutex> lk_(I->mu);
+  int rc = I->heavy ? enqueue_partials(I->heavy.get(), ops, n_ops, level_start, n_levels, fused) : GLE_SUCCESS;
+  if (!rc) rc = enqueue_partials(I, ops, n_ops, level_start, n_levels, fused);
+  return finish_both(I, rc);
 }
 
 template <typename T>
@@ -1489,9 +1507,8 @@ static int edge_blocks(const Instance *I) {
   return I->S == 4 ? I->Ppad / 256 : I->S == 61 ? I->Ppad / 16 : blocks_for(I);
 }
 
-static int engine_edges_one(Instance *I, const DevEdgeOp *ops, int n, double *out, double *site_lnl) {
-  std::lock_guard<std::mutex> lk_(I->mu);
-  if (n <= 0) return GLE_SUCCESS;
+// Edge kernels and the per-op reduction into d_eres; d_site when `site`.
+static int enqueue_edges(Instance *I, const DevEdgeOp *ops, int n, bool site) {
   for (int i = 0; i < n; i++) {
     const DevEdgeOp &o = ops[i];
     if (o.partial < 0 || o.partial >= I->n_slots || o.child >= I->n_slots || (o.child < 0 && -o.child - 1 >= I->N) ||
@@ -1511,7 +1528,7 @@ static int engine_edges_one(Instance *I, const DevEdgeOp *ops, int n, double *ou
   }
   GLE_HIP(hipMemcpyAsync(I->d_eops, ops, sizeof(DevEdgeOp) * n, hipMemcpyHostToDevice, I->stream));
   double *site_save = I->d_site;
-  if (!site_lnl) I->d_site = nullptr;
+  if (!site) I->d_site = nullptr;
   double eb = 0.0;
   for (int i = 0; i < n; i++) eb += edge_op_bytes(I, ops[i]);
   timed_begin(I, 1, eb);
@@ -1522,29 +1539,34 @@ static int engine_edges_one(Instance *I, const DevEdgeOp *ops, int n, double *ou
   if (rc) return rc;
   hipLaunchKernelGGL(k_edge_reduce, dim3(n), dim3(64), 0, I->stream, I->d_part, nblk, I->d_eres);
   GLE_HIP(hipGetLastError());
+  return GLE_SUCCESS;
+}
+
+static int copy_edges_out(Instance *I, int n, double *out, double *site_lnl) {
   GLE_HIP(hipMemcpyAsync(out, I->d_eres, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, I->stream));
   if (site_lnl)
     GLE_HIP(hipMemcpyAsync(site_lnl, I->d_site, sizeof(double) * I->P, hipMemcpyDeviceToHost, I->stream));
-  GLE_HIP(hipStreamSynchronize(I->stream));
-  timed_collect(I);
   return GLE_SUCCESS;
 }
 
 // Edge results of an instance and its heavy-pattern part, in the caller's pattern order.
 int engine_edges(Instance *I, const DevEdgeOp *ops, int n, double *out, double *site_lnl) {
-  if (!I->heavy) {
-    if (I->light_idx.empty() || !site_lnl) return engine_edges_one(I, ops, n, out, site_lnl);
-  }
-  std::vector<double> site_l(site_lnl ? std::max(I->P, 1) : 0), site_h, out_h(3 * size_t(n), 0.0);
-  int rc = engine_edges_one(I, ops, n, out, site_lnl ? site_l.data() : nullptr);
+  std::lock_guard<std::mutex> lk_(I->mu);
+  if (n <= 0) return GLE_SUCCESS;
+  Instance *H = I->heavy.get();
+  const bool scatter = site_lnl && !I->light_idx.empty();
+  std::vector<double> site_l(scatter ? std::max(I->P, 1) : 0), site_h(H && site_lnl ? std::max(H->P, 1) : 0);
+  std::vector<double> out_h(H ? 3 * size_t(n) : 0, 0.0);
+  // kernels of both parts first; the copies to host memory may wait for them
+  int rc = enqueue_edges(I, ops, n, site_lnl != nullptr);
+  if (!rc && H) rc = enqueue_edges(H, ops, n, site_lnl != nullptr);
+  if (!rc) rc = copy_edges_out(I, n, out, site_lnl ? (scatter ? site_l.data() : site_lnl) : nullptr);
+  if (!rc && H) rc = copy_edges_out(H, n, out_h.data(), site_lnl ? site_h.data() : nullptr);
+  rc = finish_both(I, rc);
   if (rc) return rc;
-  if (I->heavy) {
-    site_h.resize(site_lnl ? std::max(I->heavy->P, 1) : 0);
-    rc = engine_edges_one(I->heavy.get(), ops, n, out_h.data(), site_lnl ? site_h.data() : nullptr);
-    if (rc) return rc;
+  if (H)
     for (int i = 0; i < 3 * n; i++) out[i] += out_h[i];
-  }
-  if (site_lnl) {
+  if (scatter) {
     for (size_t k = 0; k < I->light_idx.size(); k++) site_lnl[I->light_idx[k]] = site_l[k];
     for (size_t k = 0; k < I->heavy_idx.size(); k++) site_lnl[I->heavy_idx[k]] = site_h[k];
   }
