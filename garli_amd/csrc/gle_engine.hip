@@ -3,7 +3,10 @@
 // Device data layout (per instance, all in HBM):
 //   tips   u8  [N][Ppad]           nucleotide 4-bit masks (1..15) or state (S = missing)
 //   cla    T   [slots][Ppad][R][Sp] the reference's pattern-major layout (condlike.h:20-43);
-//                                   Sp = S except codon (61 -> 64)
+//                                   Sp = S except codon (61 -> 64).  Nucleotides use a
+//                                   rate-blocked order instead: [slots][Ppad/64][R][64][4],
+//                                   so each (64 patterns, category) is one contiguous 1 KB
+//                                   (FP32) run and a wave's 16-byte loads coalesce fully.
 //   u      i32 [slots][Ppad]       per-pattern rescale counts (FP64: natural-log units
 //                                   exactly as the reference; FP32: powers of two)
 //   pm64   f64 [pmats][3][R][S][S] P, P', P'' (row = parent state, model.cpp:1271-1358)
@@ -401,6 +404,32 @@ __global__ __launch_bounds__(256) void k_pmat(int S, int Sp, int R, const int *_
   }
 }
 
+// 16 wave-uniform doubles (one category's 4x4 P, 128-byte aligned) as two
+// 64-byte blocks, so the compiler issues wide scalar loads.
+struct alignas(64) Dbl8 {
+  double v[8];
+};
+__device__ __forceinline__ void load_uniform16(const double *__restrict__ src, double *v) {
+  const Dbl8 a = reinterpret_cast<const Dbl8 *>(src)[0], b = reinterpret_cast<const Dbl8 *>(src)[1];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    v[q] = a.v[q];
+    v[8 + q] = b.v[q];
+  }
+}
+// Tip factor of mask m for one P row: 1 for total ambiguity, else the
+// ascending-order sum of the row's entries over the mask's states.
+__device__ __forceinline__ double tip_sum(const double *row, const double *bit, int m) {
+  const double s = __builtin_fma(bit[3], row[3], __builtin_fma(bit[2], row[2],
+                   __builtin_fma(bit[1], row[1], bit[0] * row[0])));
+  return m == 15 ? 1.0 : s;
+}
+
+// Offset of (pattern p, category r) within a nucleotide CLA slot.
+__host__ __device__ __forceinline__ size_t nuc_off(int p, int r, int R) {
+  return (size_t(p >> 6) * R + r) * 256 + size_t(p & 63) * 4;
+}
+
 // =====================================================================
 // Nucleotide CLA update (S = 4): one thread per pattern holding every rate
 // category, so the per-pattern rescale needs no cross-lane traffic.  The op's
@@ -475,7 +504,7 @@ __global__ __launch_bounds__(256) void k_cla_nuc(T *__restrict__ cla, int *__res
   constexpr int kPmStride = 3 * R * 16;
   const int p = blockIdx.x * 256 + threadIdx.x;
   if (p >= P) return;  // no cross-lane work below
-  const size_t slot = size_t(Ppad) * R * 4, off = size_t(p) * R * 4;
+  const size_t slot = size_t(Ppad) * R * 4, off = nuc_off(p, 0, R);
   const int first = fused_n ? 0 : blockIdx.y, last = fused_n ? fused_n : blockIdx.y + 1;
   for (int oi = first; oi < last; ++oi) {
     const DevClaOp op = ops[oi];
@@ -484,20 +513,31 @@ __global__ __launch_bounds__(256) void k_cla_nuc(T *__restrict__ cla, int *__res
     int m1 = 15, m2 = 15, u = 0;
     if (op.c1 >= 0) {
 #pragma unroll
-      for (int r = 0; r < R; r++) load4(cla + op.c1 * slot + off + 4 * r, x1[r]);
+      for (int r = 0; r < R; r++) load4(cla + op.c1 * slot + off + 256 * r, x1[r]);
       u += ua[op.c1 * size_t(Ppad) + p];
     } else
       m1 = tips[size_t(-op.c1 - 1) * Ppad + p];
     if (op.c2 >= 0) {
 #pragma unroll
-      for (int r = 0; r < R; r++) load4(cla + op.c2 * slot + off + 4 * r, x2[r]);
+      for (int r = 0; r < R; r++) load4(cla + op.c2 * slot + off + 256 * r, x2[r]);
       u += ua[op.c2 * size_t(Ppad) + p];
     } else
       m2 = tips[size_t(-op.c2 - 1) * Ppad + p];
     const double *pm1 = pm + size_t(op.pm1) * kPmStride, *pm2 = pm + size_t(op.pm2) * kPmStride;
+    // tip masks as 0/1 factors: fma(bit, p, acc) adds p exactly when the state
+    // is in the mask and leaves acc unchanged otherwise, so the ascending-order
+    // sum of the reference comes out bit for bit without per-lane branches
+    double b1[4], b2[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      b1[q] = double((m1 >> q) & 1);
+      b2[q] = double((m2 >> q) & 1);
+    }
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      const double *P1 = pm1 + 16 * r, *P2 = pm2 + 16 * r;
+      double P1[16], P2[16];
+      load_uniform16(pm1 + 16 * r, P1);
+      load_uniform16(pm2 + 16 * r, P2);
       if constexpr (sizeof(T) == 8) {
         // reference summation order: paired for four categories when the other
         // child is a CLA or an unambiguous tip
@@ -506,9 +546,9 @@ __global__ __launch_bounds__(256) void k_cla_nuc(T *__restrict__ cla, int *__res
 #pragma unroll
         for (int i = 0; i < 4; i++) {
           const double v1 = op.c1 >= 0 ? (paired1 ? dot4_paired(P1 + 4 * i, x1[r]) : dot4_seq(P1 + 4 * i, x1[r]))
-                                       : nuc_tip_factor(P1 + 4 * i, m1);
+                                       : tip_sum(P1 + 4 * i, b1, m1);
           const double v2 = op.c2 >= 0 ? (paired2 ? dot4_paired(P2 + 4 * i, x2[r]) : dot4_seq(P2 + 4 * i, x2[r]))
-                                       : nuc_tip_factor(P2 + 4 * i, m2);
+                                       : tip_sum(P2 + 4 * i, b2, m2);
           out[r][i] = mul_nc(v1, v2);
         }
       } else {
@@ -520,13 +560,13 @@ __global__ __launch_bounds__(256) void k_cla_nuc(T *__restrict__ cla, int *__res
             v1 = __builtin_fma(q[0], double(x1[r][0]), __builtin_fma(q[1], double(x1[r][1]),
                  __builtin_fma(q[2], double(x1[r][2]), q[3] * double(x1[r][3]))));
           } else
-            v1 = nuc_tip_factor(P1 + 4 * i, m1);
+            v1 = tip_sum(P1 + 4 * i, b1, m1);
           if (op.c2 >= 0) {
             const double *q = P2 + 4 * i;
             v2 = __builtin_fma(q[0], double(x2[r][0]), __builtin_fma(q[1], double(x2[r][1]),
                  __builtin_fma(q[2], double(x2[r][2]), q[3] * double(x2[r][3]))));
           } else
-            v2 = nuc_tip_factor(P2 + 4 * i, m2);
+            v2 = tip_sum(P2 + 4 * i, b2, m2);
           out[r][i] = v1 * v2;
         }
       }
@@ -551,7 +591,7 @@ __global__ __launch_bounds__(256) void k_cla_nuc(T *__restrict__ cla, int *__res
         else
           o[i] = round_dither(out[r][i], unsigned(p) * 64u + unsigned(4 * r + i));
       }
-      store4(cla + op.dest * slot + off + 4 * r, o);
+      store4(cla + op.dest * slot + off + 256 * r, o);
     }
     ua[op.dest * size_t(Ppad) + p] = u;
   }
@@ -930,56 +970,58 @@ __global__ __launch_bounds__(256) void k_edge_nuc(const T *__restrict__ cla, con
   if (p < P) {
     double L[4] = {0, 0, 0, 0}, G1[4] = {0, 0, 0, 0}, G2[4] = {0, 0, 0, 0};
     int u = ua[op.partial * size_t(Ppad) + p];
-    const T *xp = cla + op.partial * slot + size_t(p) * R * 4;
-    if (op.child >= 0) {
-      const T *yp = cla + op.child * slot + size_t(p) * R * 4;
-      u += ua[op.child * size_t(Ppad) + p];
+    const T *xp = cla + op.partial * slot + nuc_off(p, 0, R);
+    const bool internal = op.child >= 0;
+    int m = 15;
+    T xt[R][4], yt[R][4];
 #pragma unroll
-      for (int r = 0; r < R; r++) {
-        T xt[4], yt[4];
-        load4(xp + 4 * r, xt);
-        load4(yp + 4 * r, yt);
-        const double x[4] = {double(xt[0]), double(xt[1]), double(xt[2]), double(xt[3])};
-        const double y[4] = {double(yt[0]), double(yt[1]), double(yt[2]), double(yt[3])};
-        const double *Pm = Pb + r * 16, *D1m = Pb + (R + r) * 16, *D2m = Pb + (2 * R + r) * 16;
+    for (int r = 0; r < R; r++) load4(xp + 256 * r, xt[r]);
+    if (internal) {
+      const T *yp = cla + op.child * slot + nuc_off(p, 0, R);
+#pragma unroll
+      for (int r = 0; r < R; r++) load4(yp + 256 * r, yt[r]);
+      u += ua[op.child * size_t(Ppad) + p];
+    } else {
+      m = tips[size_t(-op.child - 1) * Ppad + p];
+    }
+    double bit[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) bit[q] = double((m >> q) & 1);
+    // FP64 form: the reference's sequential sums; FP32 form: fused multiply-adds
+    auto dot = [](const double *q, const double *v) {
+      if constexpr (sizeof(T) == 8)
+        return dot4_seq(q, v);
+      else
+        return __builtin_fma(q[3], v[3], __builtin_fma(q[2], v[2], __builtin_fma(q[1], v[1], q[0] * v[0])));
+    };
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const double x[4] = {double(xt[r][0]), double(xt[r][1]), double(xt[r][2]), double(xt[r][3])};
+      double Pm[16], D1m[16], D2m[16];
+      load_uniform16(Pb + r * 16, Pm);
+      if (op.derivs) {
+        load_uniform16(Pb + (R + r) * 16, D1m);
+        load_uniform16(Pb + (2 * R + r) * 16, D2m);
+      }
+      if (internal) {
+        const double y[4] = {double(yt[r][0]), double(yt[r][1]), double(yt[r][2]), double(yt[r][3])};
 #pragma unroll
         for (int s = 0; s < 4; s++) {
-          L[s] += dot4_seq(Pm + 4 * s, y) * x[s] * w[r];
+          L[s] += dot(Pm + 4 * s, y) * x[s] * w[r];
           if (op.derivs) {
-            G1[s] += dot4_seq(D1m + 4 * s, y) * x[s] * w[r];
-            G2[s] += dot4_seq(D2m + 4 * s, y) * x[s] * w[r];
+            G1[s] += dot(D1m + 4 * s, y) * x[s] * w[r];
+            G2[s] += dot(D2m + 4 * s, y) * x[s] * w[r];
           }
         }
-      }
-    } else {
-      const int m = tips[size_t(-op.child - 1) * Ppad + p];
+      } else {
+        // tip: ascending-order sums over the mask's states; a missing tip
+        // contributes x * w to L and nothing to the derivatives
 #pragma unroll
-      for (int r = 0; r < R; r++) {
-        T xt[4];
-        load4(xp + 4 * r, xt);
-        const double x[4] = {double(xt[0]), double(xt[1]), double(xt[2]), double(xt[3])};
-        const double *Pm = Pb + r * 16, *D1m = Pb + (R + r) * 16, *D2m = Pb + (2 * R + r) * 16;
-        if (m == 15) {
-#pragma unroll
-          for (int s = 0; s < 4; s++) L[s] += x[s] * w[r];
-        } else {
-#pragma unroll
-          for (int s = 0; s < 4; s++) {
-            double tl = 0, t1 = 0, t2 = 0;
-#pragma unroll
-            for (int st = 0; st < 4; st++)
-              if (m & (1 << st)) {
-                tl += Pm[4 * s + st];
-                if (op.derivs) {
-                  t1 += D1m[4 * s + st];
-                  t2 += D2m[4 * s + st];
-                }
-              }
-            L[s] += tl * x[s] * w[r];
-            if (op.derivs) {
-              G1[s] += t1 * x[s] * w[r];
-              G2[s] += t2 * x[s] * w[r];
-            }
+        for (int s = 0; s < 4; s++) {
+          L[s] += tip_sum(Pm + 4 * s, bit, m) * x[s] * w[r];
+          if (op.derivs && m != 15) {
+            G1[s] += tip_sum(D1m + 4 * s, bit, m) * x[s] * w[r];
+            G2[s] += tip_sum(D2m + 4 * s, bit, m) * x[s] * w[r];
           }
         }
       }
@@ -1632,7 +1674,7 @@ int engine_get_partials(Instance *I, int slot, double *out, int *u) {
 static int engine_get_partials_one(Instance *I, int slot, double *out, int *u) {
   std::lock_guard<std::mutex> lk_(I->mu);
   if (slot < 0 || slot >= I->n_slots) return set_error(GLE_ERR_ARG, "slot %d out of range", slot);
-  const size_t W = size_t(I->P) * I->R * I->Sp;
+  const size_t W = I->S == 4 ? I->slot_elems() : size_t(I->P) * I->R * I->Sp;
   std::vector<unsigned char> tmp(W * I->elem_bytes());
   const char *base = reinterpret_cast<const char *>(I->d_cla) + slot * I->slot_elems() * I->elem_bytes();
   GLE_HIP(hipMemcpy(tmp.data(), base, tmp.size(), hipMemcpyDeviceToHost));
@@ -1640,7 +1682,8 @@ static int engine_get_partials_one(Instance *I, int slot, double *out, int *u) {
   for (int p = 0; p < I->P; p++)
     for (int r = 0; r < I->R; r++)
       for (int s = 0; s < I->S; s++) {
-        const size_t src = (size_t(p) * I->R + r) * I->Sp + s, dst = (size_t(p) * I->R + r) * I->S + s;
+        const size_t src = I->S == 4 ? nuc_off(p, r, I->R) + s : (size_t(p) * I->R + r) * I->Sp + s;
+        const size_t dst = (size_t(p) * I->R + r) * I->S + s;
         out[dst] = I->prec == GLE_PREC_FP64 ? reinterpret_cast<const double *>(tmp.data())[src]
                                             : double(reinterpret_cast<const float *>(tmp.data())[src]);
       }
