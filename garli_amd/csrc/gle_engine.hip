@@ -7,6 +7,9 @@
 //                                   rate-blocked order instead: [slots][Ppad/64][R][64][4],
 //                                   so each (64 patterns, category) is one contiguous 1 KB
 //                                   (FP32) run and a wave's 16-byte loads coalesce fully.
+//                                   Amino acids: [slots][Ppad/64][R][20][64], state-major
+//                                   inside a 64-pattern block, so every per-state load of a
+//                                   wave is one contiguous 256-byte (FP32) run.
 //   u      i32 [slots][Ppad]       per-pattern rescale counts (FP64: natural-log units
 //                                   exactly as the reference; FP32: powers of two)
 //   pm64   f64 [pmats][3][R][S][S] P, P', P'' (row = parent state, model.cpp:1271-1358)
@@ -248,17 +251,6 @@ static double edge_op_bytes(const Instance *I, const DevEdgeOp &o) {
 // Device helpers
 // =====================================================================
 
-template <int R>
-struct Lanes {  // lanes per pattern: next power of two >= R
-  static constexpr int value = R <= 1 ? 1 : R <= 2 ? 2 : R <= 4 ? 4 : 8;
-};
-
-template <typename T>
-__device__ __forceinline__ T group_max(T v, int RP) {
-  for (int o = 1; o < RP; o <<= 1) v = fmax(v, __shfl_xor(v, o));
-  return v;
-}
-
 // Reference-order dot products for one P row.  FP64 paths run with
 // contraction off so the arithmetic is the reference's mul-then-add.
 __device__ __forceinline__ double dot4_paired(const double *p, const double *x) {
@@ -269,10 +261,6 @@ __device__ __forceinline__ double dot4_seq(const double *p, const double *x) {
 #pragma clang fp contract(off)
   return p[0] * x[0] + p[1] * x[1] + p[2] * x[2] + p[3] * x[3];
 }
-__device__ __forceinline__ float dot4_paired(const float *p, const float *x) {
-  return __builtin_fmaf(p[0], x[0], p[1] * x[1]) + __builtin_fmaf(p[2], x[2], p[3] * x[3]);
-}
-__device__ __forceinline__ float dot4_seq(const float *p, const float *x) { return dot4_paired(p, x); }
 
 __device__ __forceinline__ double mul_nc(double a, double b) {
 #pragma clang fp contract(off)
@@ -301,51 +289,6 @@ __device__ __forceinline__ void store4(T *dst, const T *v) {
     reinterpret_cast<double2 *>(dst)[0] = make_double2(v[0], v[1]);
     reinterpret_cast<double2 *>(dst)[1] = make_double2(v[2], v[3]);
   }
-}
-
-// Tip factor of a nucleotide mask for parent state i: P[i][t] for one state,
-// 1 for total ambiguity, and the ascending-order sum for partial ambiguity
-// (tree.cpp:7941-8047, 8387-8446; the stream lists states in ascending order).
-template <typename T>
-__device__ __forceinline__ T nuc_tip_factor(const T *prow, int m) {
-  if (m == 15) return T(1);
-  T acc = T(0);
-#pragma unroll
-  for (int s = 0; s < 4; s++)
-    if (m & (1 << s)) acc = acc + prow[s];
-  return acc;
-}
-
-// Rescale one pattern slice held by the RP lanes of a pattern group.
-// FP64 (mode 0): RescaleRateHet (tree.cpp:3370-3490) table scheme, natural-log counts.
-// FP32 (mode 1): exact power-of-two scaling when the slice max drops below 2^-32.
-template <typename T, int W>
-__device__ __forceinline__ int rescale_slice(T *v, int RP, int *flags, bool group_live) {
-  T m = v[0];
-#pragma unroll
-  for (int q = 1; q < W; q++) m = fmax(m, v[q]);
-  m = group_max(m, RP);  // every lane of the group takes part in the shuffle
-  if (!group_live) return 0;  // padding patterns beyond P
-  int add = 0;
-  if constexpr (sizeof(T) == 8) {
-    if (m < c_rescale_below) {
-      if (m < c_reduce_below) atomicOr(flags, m < c_bail_below ? 3 : 1);
-      int idx = 0;
-      while (idx + 1 < kRescaleLen && c_thresh[idx + 1] > m) idx++;
-      const double mult = c_mult[idx];
-#pragma unroll
-      for (int q = 0; q < W; q++) v[q] = mul_nc(v[q], mult);
-      add = c_incr[idx];
-    }
-  } else {
-    if (m < 0x1p-32f && m > 0.0f) {
-      const int e = -ilogbf(m);  // m * 2^e in [1, 2)
-#pragma unroll
-      for (int q = 0; q < W; q++) v[q] = ldexpf(v[q], e);
-      add = e;
-    }
-  }
-  return add;
 }
 
 // =====================================================================
@@ -418,11 +361,19 @@ __device__ __forceinline__ void load_uniform16(const double *__restrict__ src, d
   }
 }
 // Tip factor of mask m for one P row: 1 for total ambiguity, else the
-// ascending-order sum of the row's entries over the mask's states.
+// ascending-order sum of the row's entries over the mask's states (one entry
+// for an unambiguous tip).  tree.cpp:7941-8047, 8387-8446: the ambiguity
+// stream lists a tip's states in ascending order.
 __device__ __forceinline__ double tip_sum(const double *row, const double *bit, int m) {
   const double s = __builtin_fma(bit[3], row[3], __builtin_fma(bit[2], row[2],
                    __builtin_fma(bit[1], row[1], bit[0] * row[0])));
   return m == 15 ? 1.0 : s;
+}
+
+// Offset of (pattern p, category r, state 0) within an amino-acid CLA slot;
+// state s is 64 elements further on.
+__host__ __device__ __forceinline__ size_t aa_off(int p, int r, int R) {
+  return (size_t(p >> 6) * R + r) * 1280 + size_t(p & 63);
 }
 
 // Offset of (pattern p, category r) within a nucleotide CLA slot.
@@ -617,7 +568,7 @@ __global__ __launch_bounds__(64 * R) void k_cla_aa(T *__restrict__ cla, int *__r
   const int r = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
   const int p = blockIdx.x * 64 + lane;
   const bool live = p < P;
-  const size_t slot = size_t(Ppad) * R * S, off = (size_t(p) * R + r) * S;
+  const size_t slot = size_t(Ppad) * R * S, off = aa_off(p, r, R);
   const int first = fused_n ? 0 : blockIdx.y, last = fused_n ? fused_n : blockIdx.y + 1;
   for (int oi = first; oi < last; ++oi) {
     if (oi != first) __syncthreads();  // smax of the previous op has been read
@@ -630,13 +581,13 @@ __global__ __launch_bounds__(64 * R) void k_cla_aa(T *__restrict__ cla, int *__r
     if (live) {
       if (op.c1 >= 0) {
 #pragma unroll
-        for (int q = 0; q < S; q += 4) load4(cla + op.c1 * slot + off + q, x1 + q);
+        for (int q = 0; q < S; q++) x1[q] = cla[op.c1 * slot + off + 64 * q];
         u += ua[op.c1 * size_t(Ppad) + p];
       } else
         t1 = tips[size_t(-op.c1 - 1) * Ppad + p];
       if (op.c2 >= 0) {
 #pragma unroll
-        for (int q = 0; q < S; q += 4) load4(cla + op.c2 * slot + off + q, x2 + q);
+        for (int q = 0; q < S; q++) x2[q] = cla[op.c2 * slot + off + 64 * q];
         u += ua[op.c2 * size_t(Ppad) + p];
       } else
         t2 = tips[size_t(-op.c2 - 1) * Ppad + p];
@@ -709,7 +660,7 @@ __global__ __launch_bounds__(64 * R) void k_cla_aa(T *__restrict__ cla, int *__r
           o[f] = out[f];
       }
 #pragma unroll
-      for (int q = 0; q < S; q += 4) store4(cla + op.dest * slot + off + q, o + q);
+      for (int q = 0; q < S; q++) cla[op.dest * slot + off + 64 * q] = o[q];
       if (r == 0) ua[op.dest * size_t(Ppad) + p] = u;
     }
   }
@@ -1129,95 +1080,102 @@ __global__ __launch_bounds__(64) void k_edge_codon(const T *__restrict__ cla, co
   block_reduce3(lnl, g1, g2, partials + (size_t(blockIdx.y) * gridDim.x + blockIdx.x) * 3);
 }
 
-template <typename T, int S>
-__global__ __launch_bounds__(256) void k_edge_ns(EdgeArgs<T> a, const DevEdgeOp *__restrict__ ops) {
+// Amino-acid edge: the block is R waves over 64 patterns, wave r takes
+// category r (P, P', P'' rows wave-uniform: scalar loads), and the per-category
+// sums meet in LDS, where wave 0 weights them in category order and finishes
+// the site (GetScorePartialNState / GetDerivsPartialNState, tree.cpp:6431-6480,
+// optimization.cpp:3230-3290).  grid (Ppad/64, n_ops).
+template <typename T, int R>
+__global__ __launch_bounds__(64 * R) void k_edge_aa(const T *__restrict__ cla, const int *__restrict__ ua,
+                                                    const unsigned char *__restrict__ tips,
+                                                    const double *__restrict__ pm64, const double *__restrict__ model,
+                                                    const int *__restrict__ counts, const int *__restrict__ cst,
+                                                    const DevEdgeOp *__restrict__ ops, double *__restrict__ partials,
+                                                    double *__restrict__ site, double pinv, int pinv_in_model,
+                                                    int Ppad, int P) {
 #pragma clang fp contract(off)
-  const int R = a.R, RP = R <= 1 ? 1 : R <= 2 ? 2 : R <= 4 ? 4 : 8;
-  const int r = threadIdx.x % RP;
-  const int p = blockIdx.x * (256 / RP) + threadIdx.x / RP;
-  const bool live = (p < a.P) && (r < R);
-  const int rr = r < R ? r : R - 1;
+  constexpr int S = 20;
+  __shared__ double sred[3][R][64];
+  const int lane = threadIdx.x & 63;
+  const int r = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+  const int p = blockIdx.x * 64 + lane;
   const DevEdgeOp op = ops[blockIdx.y];
-  const double *pi = a.model, *w = a.model + S;
-  const double *Pm, *D1m, *D2m;
-  if constexpr (S <= 32) {  // P, P', P'' of every category staged in LDS
-    extern __shared__ double sPm[];
-    const double *src = a.pm64 + size_t(op.pm) * 3 * R * S * S;
-    const int n = (op.derivs ? 3 : 1) * R * S * S;
-    for (int q = threadIdx.x; q < n; q += blockDim.x) sPm[q] = src[q];
-    __syncthreads();
-    Pm = sPm + size_t(rr) * S * S;
-  } else {
-    Pm = a.pm64 + (size_t(op.pm) * 3 * R + rr) * S * S;
-  }
-  D1m = Pm + size_t(R) * S * S;
-  D2m = Pm + size_t(2 * R) * S * S;
+  const size_t slot = size_t(Ppad) * R * S, off = aa_off(p, r, R);
+  const double *Pm = pm64 + (size_t(op.pm) * 3 * R + r) * S * S;
+  const double *D1m = Pm + size_t(R) * S * S, *D2m = Pm + size_t(2 * R) * S * S;
+  const double *pi = model, *w = model + S;
   double rL = 0, r1 = 0, r2 = 0;
-  int u = 0;
-  const size_t off = (size_t(p) * R + rr) * a.Sp;
-  if (live) {
-    const T *x = a.cla + op.partial * a.slot_stride + off;
-    u = a.u[op.partial * size_t(a.Ppad) + p];
-    if (op.child >= 0) {
-      const T *y = a.cla + op.child * a.slot_stride + off;
-      u += a.u[op.child * size_t(a.Ppad) + p];
-      double yv[S];
+  if (p < P) {
+    double x[S];
 #pragma unroll
-      for (int q = 0; q < S; q++) yv[q] = double(y[q]);
+    for (int f = 0; f < S; f++) x[f] = double(cla[op.partial * slot + off + 64 * f]);
+    if (op.child >= 0) {
+      double y[S];
+#pragma unroll
+      for (int q = 0; q < S; q++) y[q] = double(cla[op.child * slot + off + 64 * q]);
+#pragma unroll 4
       for (int f = 0; f < S; f++) {
         double tL = 0, t1 = 0, t2 = 0;
 #pragma unroll
         for (int q = 0; q < S; q++) {
-          tL += Pm[f * S + q] * yv[q];
-          if (op.derivs) {
-            t1 += D1m[f * S + q] * yv[q];
-            t2 += D2m[f * S + q] * yv[q];
+          if constexpr (sizeof(T) == 8) {
+            tL += Pm[f * S + q] * y[q];
+            if (op.derivs) {
+              t1 += D1m[f * S + q] * y[q];
+              t2 += D2m[f * S + q] * y[q];
+            }
+          } else {
+            tL = __builtin_fma(Pm[f * S + q], y[q], tL);
+            if (op.derivs) {
+              t1 = __builtin_fma(D1m[f * S + q], y[q], t1);
+              t2 = __builtin_fma(D2m[f * S + q], y[q], t2);
+            }
           }
         }
-        const double xf = double(x[f]);
-        rL += tL * xf * pi[f];
-        r1 += t1 * xf * pi[f];
-        r2 += t2 * xf * pi[f];
+        rL += tL * x[f] * pi[f];
+        r1 += t1 * x[f] * pi[f];
+        r2 += t2 * x[f] * pi[f];
       }
     } else {
-      const int t = a.tips[size_t(-op.child - 1) * a.Ppad + p];
+      const int t = tips[size_t(-op.child - 1) * Ppad + p];
+#pragma unroll 4
       for (int f = 0; f < S; f++) {
-        const double xf = double(x[f]);
         if (t < S) {
-          rL += Pm[f * S + t] * xf * pi[f];
+          rL += Pm[f * S + t] * x[f] * pi[f];
           if (op.derivs) {
-            r1 += D1m[f * S + t] * xf * pi[f];
-            r2 += D2m[f * S + t] * xf * pi[f];
+            r1 += D1m[f * S + t] * x[f] * pi[f];
+            r2 += D2m[f * S + t] * x[f] * pi[f];
           }
         } else
-          rL += xf * pi[f];
+          rL += x[f] * pi[f];
       }
     }
   }
-  double sL = 0, s1 = 0, s2 = 0;
-  for (int k = 0; k < RP; k++) {
-    const double l = __shfl(rL, (threadIdx.x & ~(RP - 1)) + k);
-    const double g1 = __shfl(r1, (threadIdx.x & ~(RP - 1)) + k);
-    const double g2 = __shfl(r2, (threadIdx.x & ~(RP - 1)) + k);
-    if (k < R) {
-      sL += l * w[k];
-      s1 += g1 * w[k];
-      s2 += g2 * w[k];
-    }
-  }
+  sred[0][r][lane] = rL;
+  sred[1][r][lane] = r1;
+  sred[2][r][lane] = r2;
+  __syncthreads();
   double lnl = 0, g1 = 0, g2 = 0;
-  if (live && r == 0) {
+  if (r == 0 && p < P) {
+    double sL = 0, s1 = 0, s2 = 0;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      sL += sred[0][k][lane] * w[k];
+      s1 += sred[1][k][lane] * w[k];
+      s2 += sred[2][k][lane] * w[k];
+    }
+    int u = ua[op.partial * size_t(Ppad) + p];
+    if (op.child >= 0) u += ua[op.child * size_t(Ppad) + p];
     double pt = 0.0;
-    const int cs = a.cst[p];
-    if (a.pinv_in_model && cs >= 0 && cs < S) pt = a.pinv * pi[cs];
+    const int cs = cst[p];
+    if (pinv_in_model && cs >= 0 && cs < S) pt = pinv * pi[cs];
     double ul;
-    site_terms<sizeof(T) == 4>(sL, s1, s2, u, pt, true, a.counts[p], lnl, g1, g2, ul);
-    if (a.site && blockIdx.y == 0) a.site[p] = ul;
+    site_terms<sizeof(T) == 4>(sL, s1, s2, u, pt, true, counts[p], lnl, g1, g2, ul);
+    if (site && blockIdx.y == 0) site[p] = ul;
   }
-  block_reduce3(lnl, g1, g2, a.partials + (size_t(blockIdx.y) * gridDim.x + blockIdx.x) * 3);
+  block_reduce3(lnl, g1, g2, partials + (size_t(blockIdx.y) * gridDim.x + blockIdx.x) * 3);
 }
 
-// Deterministic per-op reduction of block partials: one 64-lane wave per op.
 __global__ __launch_bounds__(64) void k_edge_reduce(const double *__restrict__ part, int nblk,
                                                     double *__restrict__ out) {
   const int op = blockIdx.x, l = threadIdx.x;
@@ -1251,8 +1209,6 @@ __global__ __launch_bounds__(256) void k_copy_calibrate(const float4 *__restrict
 // =====================================================================
 // Host-side launchers
 // =====================================================================
-static int lanes_for(int R) { return R <= 1 ? 1 : R <= 2 ? 2 : R <= 4 ? 4 : 8; }
-static int blocks_for(const Instance *I) { return I->Ppad / (256 / lanes_for(I->R)); }
 
 // Every CLA kernel can walk a whole topologically ordered op list in one launch:
 // a (pattern, rate) lane reads only what it wrote itself, and the kernels that
@@ -1521,9 +1477,21 @@ static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk) {
     }
 #undef GLE_EDGE_NUC
   } else if (I->S == 20) {
-    const size_t shmem = sizeof(double) * 3 * I->R * 400;
-    if (shmem > 65536) return set_error(GLE_ERR_UNSUPPORTED, "20-state edge kernel takes at most 6 categories");
-    hipLaunchKernelGGL((k_edge_ns<T, 20>), grid, dim3(256), shmem, I->stream, a, d_ops);
+#define GLE_EDGE_AA(RR)                                                                                           \
+  hipLaunchKernelGGL((k_edge_aa<T, RR>), grid, dim3(64 * RR), 0, I->stream, a.cla, a.u, a.tips, a.pm64, a.model,  \
+                     a.counts, a.cst, d_ops, a.partials, a.site, a.pinv, a.pinv_in_model, a.Ppad, a.P)
+    switch (I->R) {
+      case 1: GLE_EDGE_AA(1); break;
+      case 2: GLE_EDGE_AA(2); break;
+      case 3: GLE_EDGE_AA(3); break;
+      case 4: GLE_EDGE_AA(4); break;
+      case 5: GLE_EDGE_AA(5); break;
+      case 6: GLE_EDGE_AA(6); break;
+      case 7: GLE_EDGE_AA(7); break;
+      case 8: GLE_EDGE_AA(8); break;
+      default: return set_error(GLE_ERR_UNSUPPORTED, "rate categories must be 1..8");
+    }
+#undef GLE_EDGE_AA
   } else if (I->S == 61) {
 #define GLE_EDGE_CODON(RR)                                                                                        \
   hipLaunchKernelGGL((k_edge_codon<T, RR>), grid, dim3(64), 0, I->stream, a.cla, a.u, a.tips, I->d_pmpad, a.model, \
@@ -1546,7 +1514,7 @@ static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk) {
 }
 
 static int edge_blocks(const Instance *I) {
-  return I->S == 4 ? I->Ppad / 256 : I->S == 61 ? I->Ppad / 16 : blocks_for(I);
+  return I->S == 4 ? I->Ppad / 256 : I->S == 61 ? I->Ppad / 16 : I->Ppad / 64;
 }
 
 // Edge kernels and the per-op reduction into d_eres; d_site when `site`.
@@ -1674,7 +1642,7 @@ int engine_get_partials(Instance *I, int slot, double *out, int *u) {
 static int engine_get_partials_one(Instance *I, int slot, double *out, int *u) {
   std::lock_guard<std::mutex> lk_(I->mu);
   if (slot < 0 || slot >= I->n_slots) return set_error(GLE_ERR_ARG, "slot %d out of range", slot);
-  const size_t W = I->S == 4 ? I->slot_elems() : size_t(I->P) * I->R * I->Sp;
+  const size_t W = I->S == 61 ? size_t(I->P) * I->R * I->Sp : I->slot_elems();
   std::vector<unsigned char> tmp(W * I->elem_bytes());
   const char *base = reinterpret_cast<const char *>(I->d_cla) + slot * I->slot_elems() * I->elem_bytes();
   GLE_HIP(hipMemcpy(tmp.data(), base, tmp.size(), hipMemcpyDeviceToHost));
@@ -1682,7 +1650,9 @@ static int engine_get_partials_one(Instance *I, int slot, double *out, int *u) {
   for (int p = 0; p < I->P; p++)
     for (int r = 0; r < I->R; r++)
       for (int s = 0; s < I->S; s++) {
-        const size_t src = I->S == 4 ? nuc_off(p, r, I->R) + s : (size_t(p) * I->R + r) * I->Sp + s;
+        const size_t src = I->S == 4    ? nuc_off(p, r, I->R) + s
+                           : I->S == 20 ? aa_off(p, r, I->R) + 64 * size_t(s)
+                                        : (size_t(p) * I->R + r) * I->Sp + s;
         const size_t dst = (size_t(p) * I->R + r) * I->S + s;
         out[dst] = I->prec == GLE_PREC_FP64 ? reinterpret_cast<const double *>(tmp.data())[src]
                                             : double(reinterpret_cast<const float *>(tmp.data())[src]);

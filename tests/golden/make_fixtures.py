@@ -44,9 +44,19 @@ def read_start(path):
     score = re.search(r"GarliScore\s+(-?[0-9.]+)", t)
     model = re.search(r"GarliModel\s+([^\]]*)\]", t)
     newick = re.sub(r"\[.*?\]", "", t).strip()
+    # a "begin garli;" block after the trees holds parameters the GarliModel
+    # comment does not print (e.g. the 190 fixed amino-acid exchangeabilities)
+    garli = {}
+    gb = re.search(r"begin\s+garli\s*;(.*?)end\s*;", txt, re.S | re.I)
+    if gb:
+        for stmt in re.split(r"[;\n]", gb.group(1)):
+            toks = stmt.split()
+            if len(toks) > 1 and re.match(r"^[a-z]$", toks[0]):
+                garli[toks[0]] = [float(x) for x in toks[1:]]
     return {"translate": translate, "newick": newick,
             "score": float(score.group(1)) if score else None,
-            "model": model.group(1).strip() if model else None}
+            "model": model.group(1).strip() if model else None,
+            "garli_block": garli}
 
 
 def main():
