@@ -425,26 +425,6 @@ __device__ __forceinline__ int rescale_pow2(double *v, double m) {
   return e;
 }
 
-// FP64 rescale of a pattern slice held by one thread (RescaleRateHet table scheme).
-template <typename T, int W>
-__device__ __forceinline__ int rescale_thread(T *v, int *flags) {
-  static_assert(sizeof(T) == 8, "FP64 scheme");
-  T m = v[0];
-#pragma unroll
-  for (int q = 1; q < W; q++) m = fmax(m, v[q]);
-  int add = 0;
-  if (m < c_rescale_below) {
-    if (m < c_reduce_below) atomicOr(flags, m < c_bail_below ? 3 : 1);
-    int idx = 0;
-    while (idx + 1 < kRescaleLen && c_thresh[idx + 1] > m) idx++;
-    const double mult = c_mult[idx];
-#pragma unroll
-    for (int q = 0; q < W; q++) v[q] = mul_nc(v[q], mult);
-    add = c_incr[idx];
-  }
-  return add;
-}
-
 // grid (Ppad/256, fused ? 1 : n_ops); fused_n > 0 walks ops[0..fused_n) in order.
 // pm: FP64 P slots [slot][3][R][4][4] (P, P', P'').
 template <typename T, int R>
@@ -460,7 +440,6 @@ __global__ __launch_bounds__(256) void k_cla_nuc(T *__restrict__ cla, int *__res
   for (int oi = first; oi < last; ++oi) {
     const DevClaOp op = ops[oi];
     T x1[R][4], x2[R][4];
-    double out[R][4];
     int m1 = 15, m2 = 15, u = 0;
     if (op.c1 >= 0) {
 #pragma unroll
@@ -484,9 +463,13 @@ __global__ __launch_bounds__(256) void k_cla_nuc(T *__restrict__ cla, int *__res
       b1[q] = double((m1 >> q) & 1);
       b2[q] = double((m2 >> q) & 1);
     }
+    T *dst = cla + op.dest * slot + off;
+    double mx = 0.0;
+    // each category is stored as soon as it is formed (few live registers);
+    // the rare rescale re-reads and scales what this thread stored
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      double P1[16], P2[16];
+      double P1[16], P2[16], out[4];
       load_uniform16(pm1 + 16 * r, P1);
       load_uniform16(pm2 + 16 * r, P2);
       if constexpr (sizeof(T) == 8) {
@@ -500,7 +483,7 @@ __global__ __launch_bounds__(256) void k_cla_nuc(T *__restrict__ cla, int *__res
                                        : tip_sum(P1 + 4 * i, b1, m1);
           const double v2 = op.c2 >= 0 ? (paired2 ? dot4_paired(P2 + 4 * i, x2[r]) : dot4_seq(P2 + 4 * i, x2[r]))
                                        : tip_sum(P2 + 4 * i, b2, m2);
-          out[r][i] = mul_nc(v1, v2);
+          out[i] = mul_nc(v1, v2);
         }
       } else {
 #pragma unroll
@@ -518,31 +501,48 @@ __global__ __launch_bounds__(256) void k_cla_nuc(T *__restrict__ cla, int *__res
                  __builtin_fma(q[2], double(x2[r][2]), q[3] * double(x2[r][3]))));
           } else
             v2 = tip_sum(P2 + 4 * i, b2, m2);
-          out[r][i] = v1 * v2;
+          out[i] = v1 * v2;
         }
       }
-    }
-    if (op.flags & kOpRescale) {
-      if constexpr (sizeof(T) == 8) {
-        u += rescale_thread<double, 4 * R>(&out[0][0], flags);
-      } else {
-        double m = out[0][0];
-#pragma unroll
-        for (int q = 1; q < 4 * R; q++) m = fmax(m, (&out[0][0])[q]);
-        u += rescale_pow2<4 * R>(&out[0][0], m);
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < R; r++) {
       T o[4];
 #pragma unroll
       for (int i = 0; i < 4; i++) {
+        mx = fmax(mx, out[i]);
         if constexpr (sizeof(T) == 8)
-          o[i] = out[r][i];
+          o[i] = out[i];
         else
-          o[i] = round_dither(out[r][i], unsigned(p) * 64u + unsigned(4 * r + i));
+          o[i] = round_dither(out[i], unsigned(p) * 64u + unsigned(4 * r + i));
       }
-      store4(cla + op.dest * slot + off + 256 * r, o);
+      store4(dst + 256 * r, o);
+    }
+    if (op.flags & kOpRescale) {
+      if constexpr (sizeof(T) == 8) {
+        // RescaleRateHet (tree.cpp:3423-3471) on the stored slice
+        if (mx < c_rescale_below) {
+          if (mx < c_reduce_below) atomicOr(flags, mx < c_bail_below ? 3 : 1);
+          int idx = 0;
+          while (idx + 1 < kRescaleLen && c_thresh[idx + 1] > mx) idx++;
+          const double mult = c_mult[idx];
+          for (int r = 0; r < R; r++) {
+            T v[4];
+            load4(dst + 256 * r, v);
+#pragma unroll
+            for (int i = 0; i < 4; i++) v[i] = mul_nc(v[i], mult);
+            store4(dst + 256 * r, v);
+          }
+          u += c_incr[idx];
+        }
+      } else if (mx < 0x1p-32 && mx > 0.0) {
+        const int e = 1 - __builtin_amdgcn_frexp_exp(mx);  // max * 2^e in [1, 2)
+        for (int r = 0; r < R; r++) {
+          T v[4];
+          load4(dst + 256 * r, v);
+#pragma unroll
+          for (int i = 0; i < 4; i++) v[i] = ldexpf(v[i], e);
+          store4(dst + 256 * r, v);
+        }
+        u += e;
+      }
     }
     ua[op.dest * size_t(Ppad) + p] = u;
   }
