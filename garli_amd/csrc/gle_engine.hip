@@ -903,7 +903,7 @@ __device__ __forceinline__ void site_terms(double L, double D1, double D2, int u
 // argument is __restrict__).  Per state the categories accumulate in order, as
 // GetScorePartial*RateHet / GetDerivsPartial* do (tree.cpp:6208-6230,
 // optimization.cpp:3123-3158).  grid (Ppad/256, n_ops).
-template <typename T, int R>
+template <typename T, int R, bool D>
 __global__ __launch_bounds__(256) void k_edge_nuc(const T *__restrict__ cla, const int *__restrict__ ua,
                                                   const unsigned char *__restrict__ tips,
                                                   const double *__restrict__ pm64, const double *__restrict__ model,
@@ -945,35 +945,34 @@ __global__ __launch_bounds__(256) void k_edge_nuc(const T *__restrict__ cla, con
       else
         return __builtin_fma(q[3], v[3], __builtin_fma(q[2], v[2], __builtin_fma(q[1], v[1], q[0] * v[0])));
     };
+    // P, then P' and P'' when the launch computes derivatives (D)
+    constexpr int nm = D ? 3 : 1;
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const double x[4] = {double(xt[r][0]), double(xt[r][1]), double(xt[r][2]), double(xt[r][3])};
-      double Pm[16], D1m[16], D2m[16];
-      load_uniform16(Pb + r * 16, Pm);
-      if (op.derivs) {
-        load_uniform16(Pb + (R + r) * 16, D1m);
-        load_uniform16(Pb + (2 * R + r) * 16, D2m);
-      }
-      if (internal) {
-        const double y[4] = {double(yt[r][0]), double(yt[r][1]), double(yt[r][2]), double(yt[r][3])};
+      const double y[4] = {double(yt[r][0]), double(yt[r][1]), double(yt[r][2]), double(yt[r][3])};
 #pragma unroll
-        for (int s = 0; s < 4; s++) {
-          L[s] += dot(Pm + 4 * s, y) * x[s] * w[r];
-          if (op.derivs) {
-            G1[s] += dot(D1m + 4 * s, y) * x[s] * w[r];
-            G2[s] += dot(D2m + 4 * s, y) * x[s] * w[r];
-          }
+      for (int mi = 0; mi < nm; mi++) {
+        double M[16], t[4];
+        load_uniform16(Pb + (mi * R + r) * 16, M);
+        if (internal) {
+#pragma unroll
+          for (int s = 0; s < 4; s++) t[s] = dot(M + 4 * s, y) * x[s] * w[r];
+        } else {
+          // tip: ascending-order sums over the mask's states; a missing tip
+          // contributes x * w to L and nothing to the derivatives
+#pragma unroll
+          for (int s = 0; s < 4; s++) t[s] = (mi > 0 && m == 15) ? 0.0 : tip_sum(M + 4 * s, bit, m) * x[s] * w[r];
         }
-      } else {
-        // tip: ascending-order sums over the mask's states; a missing tip
-        // contributes x * w to L and nothing to the derivatives
+        // constant indices keep L, G1, G2 in registers
 #pragma unroll
         for (int s = 0; s < 4; s++) {
-          L[s] += tip_sum(Pm + 4 * s, bit, m) * x[s] * w[r];
-          if (op.derivs && m != 15) {
-            G1[s] += tip_sum(D1m + 4 * s, bit, m) * x[s] * w[r];
-            G2[s] += tip_sum(D2m + 4 * s, bit, m) * x[s] * w[r];
-          }
+          if (mi == 0)
+            L[s] += t[s];
+          else if (mi == 1)
+            G1[s] += t[s];
+          else
+            G2[s] += t[s];
         }
       }
     }
@@ -1439,8 +1438,10 @@ int engine_update_partials(Instance *I, const DevClaOp *ops, int n_ops, const in
   return finish_both(I, rc);
 }
 
+// derivs: some op of the launch wants (d1, d2); the nucleotide kernel then
+// computes them for every op (the others ignore them).
 template <typename T>
-static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk) {
+static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk, bool derivs) {
   EdgeArgs<T> a;
   a.cla = reinterpret_cast<const T *>(I->d_cla);
   a.u = I->d_u;
@@ -1462,8 +1463,12 @@ static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk) {
   dim3 grid(nblk, n);
   if (I->S == 4) {
 #define GLE_EDGE_NUC(RR)                                                                                          \
-  hipLaunchKernelGGL((k_edge_nuc<T, RR>), grid, dim3(256), 0, I->stream, a.cla, a.u, a.tips, a.pm64, a.model,    \
-                     a.counts, a.cst, d_ops, a.partials, a.site, a.pinv, a.pinv_in_model, a.Ppad, a.P)
+  if (derivs)                                                                                                     \
+    hipLaunchKernelGGL((k_edge_nuc<T, RR, true>), grid, dim3(256), 0, I->stream, a.cla, a.u, a.tips, a.pm64,     \
+                       a.model, a.counts, a.cst, d_ops, a.partials, a.site, a.pinv, a.pinv_in_model, a.Ppad, a.P); \
+  else                                                                                                            \
+    hipLaunchKernelGGL((k_edge_nuc<T, RR, false>), grid, dim3(256), 0, I->stream, a.cla, a.u, a.tips, a.pm64,    \
+                       a.model, a.counts, a.cst, d_ops, a.partials, a.site, a.pinv, a.pinv_in_model, a.Ppad, a.P)
     switch (I->R) {
       case 1: GLE_EDGE_NUC(1); break;
       case 2: GLE_EDGE_NUC(2); break;
@@ -1542,8 +1547,10 @@ static int enqueue_edges(Instance *I, const DevEdgeOp *ops, int n, bool site) {
   double eb = 0.0;
   for (int i = 0; i < n; i++) eb += edge_op_bytes(I, ops[i]);
   timed_begin(I, 1, eb);
-  rc = I->prec == GLE_PREC_FP64 ? launch_edges<double>(I, I->d_eops, n, nblk)
-                                : launch_edges<float>(I, I->d_eops, n, nblk);
+  bool derivs = false;
+  for (int i = 0; i < n; i++) derivs = derivs || ops[i].derivs;
+  rc = I->prec == GLE_PREC_FP64 ? launch_edges<double>(I, I->d_eops, n, nblk, derivs)
+                                : launch_edges<float>(I, I->d_eops, n, nblk, derivs);
   timed_end(I);
   I->d_site = site_save;
   if (rc) return rc;
