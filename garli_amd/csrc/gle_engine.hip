@@ -1297,8 +1297,8 @@ static int enqueue_pmats(Instance *I, const int *idx, const double *t, int n, in
 // The staging buffers are reused by the next call only after the streams drain.
 int engine_update_pmats(Instance *I, const int *idx, const double *t, int n, int derivs) {
   std::lock_guard<std::mutex> lk_(I->mu);
-  int rc = I->heavy ? enqueue_pmats(I->heavy.get(), idx, t, n, derivs) : GLE_SUCCESS;
-  if (!rc) rc = enqueue_pmats(I, idx, t, n, derivs);
+  int rc = enqueue_pmats(I, idx, t, n, derivs);
+  if (!rc && I->heavy) rc = enqueue_pmats(I->heavy.get(), idx, t, n, derivs);
   return finish_both(I, rc);
 }
 
@@ -1433,8 +1433,9 @@ static int enqueue_partials(Instance *I, const DevClaOp *ops, int n_ops, const i
 int engine_update_partials(Instance *I, const DevClaOp *ops, int n_ops, const int *level_start, int n_levels,
                            bool fused) {
   std::lock_guard<std::mutex> lk_(I->mu);
-  int rc = I->heavy ? enqueue_partials(I->heavy.get(), ops, n_ops, level_start, n_levels, fused) : GLE_SUCCESS;
-  if (!rc) rc = enqueue_partials(I, ops, n_ops, level_start, n_levels, fused);
+  // the bulk first: its stream starts working while the small part is issued
+  int rc = enqueue_partials(I, ops, n_ops, level_start, n_levels, fused);
+  if (!rc && I->heavy) rc = enqueue_partials(I->heavy.get(), ops, n_ops, level_start, n_levels, fused);
   return finish_both(I, rc);
 }
 
@@ -1590,30 +1591,22 @@ int engine_edges(Instance *I, const DevEdgeOp *ops, int n, double *out, double *
   return GLE_SUCCESS;
 }
 
+// Rescale flags of an instance and its heavy part.  flags may be null (reset
+// only, no wait).  The reset is queued on the streams, ahead of later work.
 int engine_flags(Instance *I, int *flags, int reset) {
-  if (I->heavy) {
-    int hf = 0;
-    const int rc = engine_flags(I->heavy.get(), &hf, reset);
-    if (rc) return rc;
-    std::lock_guard<std::mutex> lk_(I->mu);
-    int f = 0;
-    GLE_HIP(hipMemcpyAsync(&f, I->d_flags, sizeof(int), hipMemcpyDeviceToHost, I->stream));
-    GLE_HIP(hipStreamSynchronize(I->stream));
-    if (reset) {
-      const int z = 0;
-      GLE_HIP(hipMemcpyAsync(I->d_flags, &z, sizeof(int), hipMemcpyHostToDevice, I->stream));
-      GLE_HIP(hipStreamSynchronize(I->stream));
-    }
-    *flags = f | hf;
-    return GLE_SUCCESS;
-  }
   std::lock_guard<std::mutex> lk_(I->mu);
-  GLE_HIP(hipMemcpyAsync(flags, I->d_flags, sizeof(int), hipMemcpyDeviceToHost, I->stream));
-  GLE_HIP(hipStreamSynchronize(I->stream));
-  if (reset) {
-    const int z = 0;
-    GLE_HIP(hipMemcpyAsync(I->d_flags, &z, sizeof(int), hipMemcpyHostToDevice, I->stream));
+  Instance *H = I->heavy.get();
+  if (flags) {
+    int f = 0, hf = 0;
+    GLE_HIP(hipMemcpyAsync(&f, I->d_flags, sizeof(int), hipMemcpyDeviceToHost, I->stream));
+    if (H) GLE_HIP(hipMemcpyAsync(&hf, H->d_flags, sizeof(int), hipMemcpyDeviceToHost, H->stream));
     GLE_HIP(hipStreamSynchronize(I->stream));
+    if (H) GLE_HIP(hipStreamSynchronize(H->stream));
+    *flags = f | hf;
+  }
+  if (reset) {
+    GLE_HIP(hipMemsetAsync(I->d_flags, 0, sizeof(int), I->stream));
+    if (H) GLE_HIP(hipMemsetAsync(H->d_flags, 0, sizeof(int), H->stream));
   }
   return GLE_SUCCESS;
 }

@@ -377,7 +377,7 @@ static int run_tick(Server &sv, ServeInstance &si, std::vector<Req *> &batch) {
       for (size_t i = 0; i < tb.pm_idx.size(); i++) si.pm[dbits(tb.pm_len[i])] = {tb.pm_idx[i], 0, sv.tick};
     }
     int flags = 0;
-    int rc = engine_flags(si.I, &flags, 1);
+    int rc = engine_flags(si.I, nullptr, 1);
     if (rc) return rc;
     rc = engine_update_partials(si.I, ops.data(), int(ops.size()), level_start.data(),
                                 int(level_start.size()) - 1, false);

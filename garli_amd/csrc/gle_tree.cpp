@@ -262,7 +262,7 @@ static int edge_attempt(Tree &t, int pa, int pb, bool derivs, double *res, doubl
   std::vector<std::array<int, 3>> want;
   collect(t, pa, pb, want);
   collect(t, pb, pa, want);
-  int flags = 0, rc = engine_flags(t.I, &flags, 1);
+  int flags = 0, rc = engine_flags(t.I, nullptr, 1);
   if (rc) return rc;
   rc = run_ops(t, want);
   if (rc) return rc;
@@ -382,7 +382,7 @@ static int sweep(Tree &t, double *out) {
     for (int a = 0; a < t.nn; a++)
       if (!is_tip(t, a))
         for (int k = 0; k < t.nnb[a]; k++) collect(t, a, t.nb[a][k], want);
-    int flags = 0, rc = engine_flags(t.I, &flags, 1);
+    int flags = 0, rc = engine_flags(t.I, nullptr, 1);
     if (rc) return rc;
     rc = run_ops(t, want);
     if (rc) return rc;
