@@ -1,0 +1,94 @@
+"""Edge cases on the GPU against the oracle: ragged and tiny pattern counts
+(padding inside the last tile), the smallest tree, 1..8 rate categories,
+zero and very long branches, a taxon with no data, and single-class codon
+models.  FP64: lnL and every edge's (lnL, d1, d2) to 1e-9 relative; FP32
+form: |dlnL| < 1e-3."""
+import numpy as np
+import pytest
+
+from garli_amd import engine as E
+from garli_amd import sim
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+PRECS = [E.PREC_FP64, E.PREC_FP32]
+
+
+def _check(ds, prec, sweep=True, blen=None):
+    blen = ds.blen if blen is None else blen
+    t = E.Tree(ds.make_instance(prec), ds.parent, blen)
+    o = O.Oracle(ds)
+    ref = o.score(blen=blen)
+    got = t.score()
+    if prec == E.PREC_FP64:
+        assert abs(got - ref) <= 1e-9 * abs(ref), (got, ref)
+    else:
+        assert abs(got - ref) < 1e-3, (got, ref)
+    if sweep:
+        sw = t.sweep()
+        osw = o.sweep(blen=blen)
+        if prec == E.PREC_FP64:
+            np.testing.assert_allclose(sw[:, 0], osw[:, 0], rtol=1e-9)
+            np.testing.assert_allclose(sw[:, 1:], osw[:, 1:], rtol=1e-7, atol=1e-7 * np.abs(osw[:, 1:]).max())
+        else:
+            assert np.max(np.abs(sw[:, 0] - osw[:, 0])) < 1e-3
+    t.close()
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("npat", [1, 2, 255, 257])
+def test_ragged_nucleotide_patterns(gpu, prec, npat):
+    ds = sim.gtr_gamma(n_tips=7, n_sites=None, n_patterns=npat, pinv=0.1, seed=300 + npat)
+    print("nuc patterns", ds.n_patterns, flush=True)
+    _check(ds, prec)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_three_taxa(gpu, prec):
+    """The smallest unrooted tree: the root and three tips, every CLA a
+    tip-tip op."""
+    ds = sim.gtr_gamma(n_tips=3, n_sites=400, pinv=0.2, seed=310)
+    _check(ds, prec)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("ncat", [1, 2, 5, 8])
+def test_nucleotide_category_counts(gpu, prec, ncat):
+    ds = sim.gtr_gamma(n_tips=9, n_sites=700, ncat=ncat, seed=320 + ncat)
+    _check(ds, prec)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_branch_length_extremes(gpu, prec):
+    """Zero-length branches (P = I) next to very long ones (P rows -> pi)."""
+    ds = sim.gtr_gamma(n_tips=10, n_sites=600, seed=330)
+    blen = ds.blen.copy()
+    blen[1] = 0.0
+    blen[4] = 0.0
+    blen[6] = 50.0
+    blen[ds.n_tips + 2] = 30.0
+    _check(ds, prec, blen=blen)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_taxon_without_data(gpu, prec):
+    ds = sim.gtr_gamma(n_tips=8, n_sites=500, seed=340)
+    ds.tips[3, :] = 15
+    _check(ds, prec)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("npat,ncat", [(1, 1), (65, 3), (130, 8)])
+def test_amino_acid_shapes(gpu, prec, npat, ncat):
+    ds = sim.aa_gamma(n_tips=5, n_patterns=npat, ncat=ncat, seed=350 + npat)
+    print("aa patterns", ds.n_patterns, "categories", ncat, flush=True)
+    _check(ds, prec)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("omegas,npat", [((0.4,), 17), ((0.05, 0.5, 1.0, 2.0, 4.0), 40)])
+def test_codon_shapes(gpu, prec, omegas, npat):
+    ds = sim.codon_m3(n_tips=5, n_patterns=npat, omegas=omegas, seed=360 + npat)
+    print("codon patterns", ds.n_patterns, "classes", len(omegas), flush=True)
+    _check(ds, prec)
