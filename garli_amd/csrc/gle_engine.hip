@@ -16,6 +16,7 @@
 //   pmpad  f64 [pmats][3][R][64][64] codons only: P, P', P'' zero-padded for the MFMA kernels
 //   counts i32 [Ppad], cst i32 [Ppad] (constant-pattern state/mask, -1 otherwise)
 // Ppad rounds P up to a multiple of 256 so every 256-thread tile is whole.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -121,6 +122,7 @@ struct Instance {
   // rounding is never multiplied by a large pattern weight.
   int heavy_threshold = 4;  // 0 keeps every pattern in FP32
   int fused_min_patterns = 1 << 19;  // smallest padded pattern count that runs fused walks
+  hipEvent_t tev_a = nullptr, tev_b = nullptr;  // events for the next timed launch
   std::unique_ptr<Instance> heavy;
   int P_orig = 0;
   std::vector<int> light_idx, heavy_idx;  // original pattern indices of each part
@@ -210,16 +212,26 @@ static hipEvent_t take_event(Instance *I) {
   (void)hipEventCreate(&e);
   return e;
 }
+// timed_begin arms a pair of events that the next GLE_LAUNCH hands to
+// hipExtLaunchKernelGGL, which stamps them at that kernel's own start and end
+// (no queue latency in the interval); timed_end is a no-op kept for symmetry.
 static void timed_begin(Instance *I, int kind, double bytes) {
   if (!I->timing) return;
   Instance::Pending p{take_event(I), take_event(I), kind, bytes};
-  (void)hipEventRecord(p.a, I->stream);
   I->pending.push_back(p);
+  I->tev_a = p.a;
+  I->tev_b = p.b;
 }
-static void timed_end(Instance *I) {
-  if (!I->timing || I->pending.empty()) return;
-  (void)hipEventRecord(I->pending.back().b, I->stream);
-}
+static void timed_end(Instance *I) { I->tev_a = I->tev_b = nullptr; }
+#define GLE_LAUNCH(I, K, grid, block, shmem, ...)                                                           \
+  do {                                                                                                      \
+    if ((I)->tev_a) {                                                                                       \
+      hipExtLaunchKernelGGL(K, grid, block, shmem, (I)->stream, (I)->tev_a, (I)->tev_b, 0, __VA_ARGS__);    \
+      (I)->tev_a = (I)->tev_b = nullptr;                                                                    \
+    } else {                                                                                                \
+      hipLaunchKernelGGL(K, grid, block, shmem, (I)->stream, __VA_ARGS__);                                  \
+    }                                                                                                       \
+  } while (0)
 static void timed_collect(Instance *I) {  // call after the stream drained
   for (auto &p : I->pending) {
     float ms = 0.0f;
@@ -1286,7 +1298,7 @@ static int enqueue_pmats(Instance *I, const int *idx, const double *t, int n, in
   const int S = I->S;
   const size_t shmem = sizeof(double) * (2 * S * S + 3 * S);
   timed_begin(I, 2, double(n) * I->R * S * S * 8.0 * (derivs ? 3 : 1));
-  hipLaunchKernelGGL(k_pmat, dim3(n, I->R), dim3(256), shmem, I->stream, S, I->Sp, I->R, I->d_pmidx, I->d_pmt,
+  GLE_LAUNCH(I, k_pmat, dim3(n, I->R), dim3(256), shmem, S, I->Sp, I->R, I->d_pmidx, I->d_pmt,
                      I->d_eig_of_rate, I->d_model + S + I->R, I->d_V, I->d_Vinv, I->d_lam, I->d_pm64, I->d_pmpad,
                      derivs);
   timed_end(I);
@@ -1304,7 +1316,7 @@ int engine_update_pmats(Instance *I, const int *idx, const double *t, int n, int
 
 template <typename T, int R>
 static void launch_nuc(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
-  hipLaunchKernelGGL((k_cla_nuc<T, R>), dim3(I->Ppad / 256, fused ? 1 : n), dim3(256), 0, I->stream,
+  GLE_LAUNCH(I, (k_cla_nuc<T, R>), dim3(I->Ppad / 256, fused ? 1 : n), dim3(256), 0,
                      reinterpret_cast<T *>(I->d_cla), I->d_u, I->d_tips, I->d_pm64, d_ops, I->d_flags, I->Ppad,
                      I->P, fused ? n : 0);
 }
@@ -1327,7 +1339,7 @@ static int launch_nuc_r(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
 
 template <typename T, int R>
 static void launch_codon(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
-  hipLaunchKernelGGL((k_cla_codon<T, R>), dim3(I->Ppad / 16, fused ? 1 : n), dim3(64), 0, I->stream,
+  GLE_LAUNCH(I, (k_cla_codon<T, R>), dim3(I->Ppad / 16, fused ? 1 : n), dim3(64), 0,
                      reinterpret_cast<T *>(I->d_cla), I->d_u, I->d_tips, I->d_pmpad, d_ops, I->d_flags, I->Ppad,
                      I->P, fused ? n : 0);
 }
@@ -1350,7 +1362,7 @@ static int launch_codon_r(Instance *I, const DevClaOp *d_ops, int n, bool fused)
 
 template <typename T, int R>
 static void launch_aa(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
-  hipLaunchKernelGGL((k_cla_aa<T, R>), dim3(I->Ppad / 64, fused ? 1 : n), dim3(64 * R), 0, I->stream,
+  GLE_LAUNCH(I, (k_cla_aa<T, R>), dim3(I->Ppad / 64, fused ? 1 : n), dim3(64 * R), 0,
                      reinterpret_cast<T *>(I->d_cla), I->d_u, I->d_tips, I->d_pm64, d_ops, I->d_flags, I->Ppad,
                      I->P, fused ? n : 0);
 }
@@ -1465,10 +1477,10 @@ static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk, bo
   if (I->S == 4) {
 #define GLE_EDGE_NUC(RR)                                                                                          \
   if (derivs)                                                                                                     \
-    hipLaunchKernelGGL((k_edge_nuc<T, RR, true>), grid, dim3(256), 0, I->stream, a.cla, a.u, a.tips, a.pm64,     \
+    GLE_LAUNCH(I, (k_edge_nuc<T, RR, true>), grid, dim3(256), 0, a.cla, a.u, a.tips, a.pm64,                     \
                        a.model, a.counts, a.cst, d_ops, a.partials, a.site, a.pinv, a.pinv_in_model, a.Ppad, a.P); \
   else                                                                                                            \
-    hipLaunchKernelGGL((k_edge_nuc<T, RR, false>), grid, dim3(256), 0, I->stream, a.cla, a.u, a.tips, a.pm64,    \
+    GLE_LAUNCH(I, (k_edge_nuc<T, RR, false>), grid, dim3(256), 0, a.cla, a.u, a.tips, a.pm64,                    \
                        a.model, a.counts, a.cst, d_ops, a.partials, a.site, a.pinv, a.pinv_in_model, a.Ppad, a.P)
     switch (I->R) {
       case 1: GLE_EDGE_NUC(1); break;
@@ -1484,7 +1496,7 @@ static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk, bo
 #undef GLE_EDGE_NUC
   } else if (I->S == 20) {
 #define GLE_EDGE_AA(RR)                                                                                           \
-  hipLaunchKernelGGL((k_edge_aa<T, RR>), grid, dim3(64 * RR), 0, I->stream, a.cla, a.u, a.tips, a.pm64, a.model,  \
+  GLE_LAUNCH(I, (k_edge_aa<T, RR>), grid, dim3(64 * RR), 0, a.cla, a.u, a.tips, a.pm64, a.model,                 \
                      a.counts, a.cst, d_ops, a.partials, a.site, a.pinv, a.pinv_in_model, a.Ppad, a.P)
     switch (I->R) {
       case 1: GLE_EDGE_AA(1); break;
@@ -1500,7 +1512,7 @@ static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk, bo
 #undef GLE_EDGE_AA
   } else if (I->S == 61) {
 #define GLE_EDGE_CODON(RR)                                                                                        \
-  hipLaunchKernelGGL((k_edge_codon<T, RR>), grid, dim3(64), 0, I->stream, a.cla, a.u, a.tips, I->d_pmpad, a.model, \
+  GLE_LAUNCH(I, (k_edge_codon<T, RR>), grid, dim3(64), 0, a.cla, a.u, a.tips, I->d_pmpad, a.model,              \
                      a.counts, a.cst, d_ops, a.partials, a.site, a.pinv, a.pinv_in_model, a.Ppad, a.P)
     switch (I->R) {
       case 1: GLE_EDGE_CODON(1); break;
