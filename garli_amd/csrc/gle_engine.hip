@@ -13,7 +13,9 @@
 //   u      i32 [slots][Ppad]       per-pattern rescale counts (FP64: natural-log units
 //                                   exactly as the reference; FP32: powers of two)
 //   pm64   f64 [pmats][3][R][S][S] P, P', P'' (row = parent state, model.cpp:1271-1358)
-//   pmpad  f64 [pmats][3][R][64][64] codons only: P, P', P'' zero-padded for the MFMA kernels
+//   pmpad  f64 [pmats][3][R][Sp][Sp] a second copy of P, P', P'' in the layout a kernel family
+//                                   wants: codons zero-padded to 64 x 64 (MFMA tiles); amino
+//                                   acids transposed (column j of P contiguous)
 //   counts i32 [Ppad], cst i32 [Ppad] (constant-pattern state/mask, -1 otherwise)
 // Ppad rounds P up to a multiple of 256 so every 256-thread tile is whole.
 #include <hip/hip_ext.h>
@@ -83,7 +85,7 @@ struct Instance {
   int *d_counts = nullptr, *d_cst = nullptr, *d_u = nullptr, *d_flags = nullptr, *d_eig_of_rate = nullptr;
   void *d_cla = nullptr;
   double *d_pm64 = nullptr;
-  double *d_pmpad = nullptr;  // codons only
+  double *d_pmpad = nullptr;  // codons (padded) and amino acids (transposed) only
   double *d_V = nullptr, *d_Vinv = nullptr, *d_lam = nullptr, *d_model = nullptr;  // model: pi[S] w[R] mult[R]
   // staging
   DevClaOp *d_ops = nullptr;
@@ -311,7 +313,8 @@ __global__ __launch_bounds__(256) void k_pmat(int S, int Sp, int R, const int *_
                                               const double *__restrict__ tt, const int *__restrict__ eig_of_rate,
                                               const double *__restrict__ rate_mult, const double *__restrict__ V,
                                               const double *__restrict__ Vinv, const double *__restrict__ lam,
-                                              double *__restrict__ pm64, double *__restrict__ pmpad, int derivs) {
+                                              double *__restrict__ pm64, double *__restrict__ pmpad, int derivs,
+                                              int transpose) {
 #pragma clang fp contract(off)
   extern __shared__ double sm[];
   const int SS = S * S, op = blockIdx.x, r = blockIdx.y;
@@ -347,13 +350,14 @@ __global__ __launch_bounds__(256) void k_pmat(int S, int Sp, int R, const int *_
     }
     const double pv = sp > 0.0 ? sp : 0.0;
     P[q] = pv;
-    if (Pp) Pp[i * Sp + j] = pv;
+    const int pq = transpose ? j * Sp + i : i * Sp + j;
+    if (Pp) Pp[pq] = pv;
     if (derivs) {
       D1[q] = s1;
       D2[q] = s2;
       if (Pp) {
-        Pp[size_t(R) * Sp * Sp + i * Sp + j] = s1;
-        Pp[size_t(2 * R) * Sp * Sp + i * Sp + j] = s2;
+        Pp[size_t(R) * Sp * Sp + pq] = s1;
+        Pp[size_t(2 * R) * Sp * Sp + pq] = s2;
       }
     }
   }
@@ -562,16 +566,20 @@ __global__ __launch_bounds__(256) void k_cla_nuc(T *__restrict__ cla, int *__res
 
 // =====================================================================
 // Amino-acid CLA update (S = 20).  A block is R waves over the same 64
-// patterns and wave r takes category r, so the op's P rows are wave-uniform:
-// scalar loads, SGPR operands, no LDS traffic for P.  The per-pattern rescale
-// takes the max over the R waves through a small LDS table.  FP64 arithmetic
-// in both forms (reference mul-then-add order for the FP64 form).
-// tree.cpp:7843-7910, 8082-8160, 8537-8599.
+// patterns and wave r takes category r, so the op's P is wave-uniform.  The
+// products accumulate input state by input state: for q = 0..19 every lane
+// adds P[f][q] * x[q] to its 20 sums, with column q of P (contiguous in the
+// transposed copy pmT) arriving by scalar loads.  One child at a time, the
+// live registers are its 20 sums plus the other child's; the sums run in the
+// reference's q order, so the FP64 form keeps its mul-then-add sequence
+// (tree.cpp:7843-7910, 8082-8160, 8537-8599).  Each output is stored as
+// formed; a rescale (per pattern, over the R waves through a small LDS table)
+// re-reads and scales what the lane stored.
 // =====================================================================
 template <typename T, int R>
 __global__ __launch_bounds__(64 * R) void k_cla_aa(T *__restrict__ cla, int *__restrict__ ua,
                                                     const unsigned char *__restrict__ tips,
-                                                    const double *__restrict__ pm, const DevClaOp *__restrict__ ops,
+                                                    const double *__restrict__ pmT, const DevClaOp *__restrict__ ops,
                                                     int *__restrict__ flags, int Ppad, int P, int fused_n) {
   constexpr int S = 20;
   constexpr bool kF32 = sizeof(T) == 4;
@@ -582,99 +590,91 @@ __global__ __launch_bounds__(64 * R) void k_cla_aa(T *__restrict__ cla, int *__r
   const bool live = p < P;
   const size_t slot = size_t(Ppad) * R * S, off = aa_off(p, r, R);
   const int first = fused_n ? 0 : blockIdx.y, last = fused_n ? fused_n : blockIdx.y + 1;
+  // acc[f] = (P x)[f] for one child: a CLA (input state by input state) or a tip
+  auto child = [&](int c, int pmslot, double(&acc)[S]) {
+    const double *PT = pmT + (size_t(pmslot) * 3 * R + r) * S * S;  // PT[q * S + f] = P[f][q]
+    if (c >= 0) {
+      T x[S];
+#pragma unroll
+      for (int q = 0; q < S; q++) x[q] = live ? cla[c * slot + off + 64 * q] : T(0);
+#pragma unroll
+      for (int f = 0; f < S; f++) acc[f] = 0.0;
+#pragma unroll
+      for (int q = 0; q < S; q++) {
+        const double xq = double(x[q]);
+        double col[S];
+#pragma unroll
+        for (int f = 0; f < S; f++) col[f] = PT[q * S + f];
+#pragma unroll
+        for (int f = 0; f < S; f++) {
+          if constexpr (kF32)
+            acc[f] = __builtin_fma(col[f], xq, acc[f]);
+          else
+            acc[f] = add_nc(acc[f], mul_nc(col[f], xq));
+        }
+      }
+    } else {
+      const int t = live ? tips[size_t(-c - 1) * Ppad + p] : S;
+      if (t < S) {
+        const double *row = PT + t * S;  // P[f][t] for f = 0..19
+#pragma unroll
+        for (int f = 0; f < S; f++) acc[f] = row[f];
+      } else {
+#pragma unroll
+        for (int f = 0; f < S; f++) acc[f] = 1.0;
+      }
+    }
+  };
   for (int oi = first; oi < last; ++oi) {
     if (oi != first) __syncthreads();  // smax of the previous op has been read
     const DevClaOp op = ops[oi];
-    const double *P1 = pm + size_t(op.pm1) * 3 * R * S * S + r * S * S;
-    const double *P2 = pm + size_t(op.pm2) * 3 * R * S * S + r * S * S;
-    T x1[S], x2[S];
-    double out[S];
-    int t1 = S, t2 = S, u = 0;
+    int u = 0;
     if (live) {
-      if (op.c1 >= 0) {
-#pragma unroll
-        for (int q = 0; q < S; q++) x1[q] = cla[op.c1 * slot + off + 64 * q];
-        u += ua[op.c1 * size_t(Ppad) + p];
-      } else
-        t1 = tips[size_t(-op.c1 - 1) * Ppad + p];
-      if (op.c2 >= 0) {
-#pragma unroll
-        for (int q = 0; q < S; q++) x2[q] = cla[op.c2 * slot + off + 64 * q];
-        u += ua[op.c2 * size_t(Ppad) + p];
-      } else
-        t2 = tips[size_t(-op.c2 - 1) * Ppad + p];
-    } else {
-#pragma unroll
-      for (int q = 0; q < S; q++) x1[q] = x2[q] = T(0);
+      if (op.c1 >= 0) u += ua[op.c1 * size_t(Ppad) + p];
+      if (op.c2 >= 0) u += ua[op.c2 * size_t(Ppad) + p];
     }
-    // one child at a time keeps the live registers to x1, x2 and out
+    double a1[S], a2[S];
+    child(op.c1, op.pm1, a1);
+    child(op.c2, op.pm2, a2);
+    T *dst = cla + op.dest * slot + off;
+    double m = 0.0;
 #pragma unroll
     for (int f = 0; f < S; f++) {
-      double L1;
-      if (op.c1 >= 0) {
-        L1 = 0.0;
-#pragma unroll
-        for (int q = 0; q < S; q++) {
-          if constexpr (kF32)
-            L1 = __builtin_fma(P1[f * S + q], double(x1[q]), L1);
-          else
-            L1 = add_nc(L1, mul_nc(P1[f * S + q], x1[q]));
-        }
-      } else
-        L1 = t1 < S ? P1[f * S + t1] : 1.0;
-      out[f] = L1;
-    }
-#pragma unroll
-    for (int f = 0; f < S; f++) {
-      double R1;
-      if (op.c2 >= 0) {
-        R1 = 0.0;
-#pragma unroll
-        for (int q = 0; q < S; q++) {
-          if constexpr (kF32)
-            R1 = __builtin_fma(P2[f * S + q], double(x2[q]), R1);
-          else
-            R1 = add_nc(R1, mul_nc(P2[f * S + q], x2[q]));
-        }
-      } else
-        R1 = t2 < S ? P2[f * S + t2] : 1.0;
-      out[f] = mul_nc(out[f], R1);
+      const double o = mul_nc(a1[f], a2[f]);
+      m = fmax(m, o);
+      if (live) {
+        if constexpr (kF32)
+          dst[64 * f] = round_dither(o, unsigned(p) * 512u + unsigned(r * S + f));
+        else
+          dst[64 * f] = o;
+      }
     }
     if (op.flags & kOpRescale) {
-      double m = out[0];
-#pragma unroll
-      for (int f = 1; f < S; f++) m = fmax(m, out[f]);
       smax[r][lane] = m;
       __syncthreads();
 #pragma unroll
       for (int k = 0; k < R; k++) m = fmax(m, smax[k][lane]);
       if (live) {
         if constexpr (kF32) {
-          u += rescale_pow2<S>(out, m);
+          if (m < 0x1p-32 && m > 0.0) {
+            const int e = 1 - __builtin_amdgcn_frexp_exp(m);  // max * 2^e in [1, 2)
+#pragma unroll
+            for (int f = 0; f < S; f++) dst[64 * f] = ldexpf(dst[64 * f], e);
+            u += e;
+          }
         } else if (m < c_rescale_below) {
+          // RescaleRateHet / RescaleRateHetNState (tree.cpp:3423-3471, 3492-3598)
           if (m < c_reduce_below && r == 0) atomicOr(flags, m < c_bail_below ? 3 : 1);
           int idx = 0;
           while (idx + 1 < kRescaleLen && c_thresh[idx + 1] > m) idx++;
           const double mult = c_mult[idx];
 #pragma unroll
-          for (int f = 0; f < S; f++) out[f] = mul_nc(out[f], mult);
+          for (int f = 0; f < S; f++) dst[64 * f] = mul_nc(dst[64 * f], mult);
           u += c_incr[idx];
         }
       }
     }
-    if (live) {
-      T o[S];
-#pragma unroll
-      for (int f = 0; f < S; f++) {
-        if constexpr (kF32)
-          o[f] = round_dither(out[f], unsigned(p) * 512u + unsigned(r * S + f));
-        else
-          o[f] = out[f];
-      }
-#pragma unroll
-      for (int q = 0; q < S; q++) cla[op.dest * slot + off + 64 * q] = o[q];
-      if (r == 0) ua[op.dest * size_t(Ppad) + p] = u;
-    }
+    if (live && r == 0) ua[op.dest * size_t(Ppad) + p] = u;
   }
 }
 
@@ -1300,7 +1300,7 @@ static int enqueue_pmats(Instance *I, const int *idx, const double *t, int n, in
   timed_begin(I, 2, double(n) * I->R * S * S * 8.0 * (derivs ? 3 : 1));
   GLE_LAUNCH(I, k_pmat, dim3(n, I->R), dim3(256), shmem, S, I->Sp, I->R, I->d_pmidx, I->d_pmt,
                      I->d_eig_of_rate, I->d_model + S + I->R, I->d_V, I->d_Vinv, I->d_lam, I->d_pm64, I->d_pmpad,
-                     derivs);
+                     derivs, S == 20 ? 1 : 0);
   timed_end(I);
   GLE_HIP(hipGetLastError());
   return GLE_SUCCESS;
@@ -1363,7 +1363,7 @@ static int launch_codon_r(Instance *I, const DevClaOp *d_ops, int n, bool fused)
 template <typename T, int R>
 static void launch_aa(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
   GLE_LAUNCH(I, (k_cla_aa<T, R>), dim3(I->Ppad / 64, fused ? 1 : n), dim3(64 * R), 0,
-                     reinterpret_cast<T *>(I->d_cla), I->d_u, I->d_tips, I->d_pm64, d_ops, I->d_flags, I->Ppad,
+                     reinterpret_cast<T *>(I->d_cla), I->d_u, I->d_tips, I->d_pmpad, d_ops, I->d_flags, I->Ppad,
                      I->P, fused ? n : 0);
 }
 
@@ -1764,7 +1764,7 @@ static int create_internal(const gle_instance_desc *d, std::unique_ptr<Instance>
   GLE_ALLOC(I->d_eig_of_rate, sizeof(int) * R);
   GLE_ALLOC(I->d_cla, I->slot_elems() * I->elem_bytes() * I->n_slots);
   GLE_ALLOC(I->d_pm64, sizeof(double) * size_t(I->n_pm) * 3 * R * S * S);
-  if (S == 61) GLE_ALLOC(I->d_pmpad, sizeof(double) * size_t(I->n_pm) * 3 * R * 64 * 64);
+  if (S == 61 || S == 20) GLE_ALLOC(I->d_pmpad, sizeof(double) * size_t(I->n_pm) * 3 * R * I->Sp * I->Sp);
   GLE_ALLOC(I->d_V, sizeof(double) * size_t(I->n_eig) * S * S);
   GLE_ALLOC(I->d_Vinv, sizeof(double) * size_t(I->n_eig) * S * S);
   GLE_ALLOC(I->d_lam, sizeof(double) * size_t(I->n_eig) * S);
@@ -1780,7 +1780,8 @@ static int create_internal(const gle_instance_desc *d, std::unique_ptr<Instance>
       hipMemset(I->d_flags, 0, sizeof(int)) != hipSuccess ||
       hipMemset(I->d_cla, 0, I->slot_elems() * I->elem_bytes() * I->n_slots) != hipSuccess ||
       hipMemset(I->d_pm64, 0, sizeof(double) * size_t(I->n_pm) * 3 * R * S * S) != hipSuccess ||
-      (I->d_pmpad && hipMemset(I->d_pmpad, 0, sizeof(double) * size_t(I->n_pm) * 3 * R * 64 * 64) != hipSuccess))
+      (I->d_pmpad && hipMemset(I->d_pmpad, 0, sizeof(double) * size_t(I->n_pm) * 3 * R * I->Sp * I->Sp) !=
+                         hipSuccess))
     return fail(set_error(GLE_ERR_DEVICE, "device memset failed"));
   I->h_tips.assign(size_t(I->N) * I->P_orig, missing);
   I->h_pi.assign(S, 1.0 / S);
@@ -1888,7 +1889,7 @@ static int apply_patterns(Instance *I) {
     GLE_HIP(hipMemcpy(I->heavy->d_pm64, I->d_pm64, sizeof(double) * size_t(I->n_pm) * 3 * I->R * I->S * I->S,
                       hipMemcpyDeviceToDevice));
     if (I->d_pmpad)
-      GLE_HIP(hipMemcpy(I->heavy->d_pmpad, I->d_pmpad, sizeof(double) * size_t(I->n_pm) * 3 * I->R * 64 * 64,
+      GLE_HIP(hipMemcpy(I->heavy->d_pmpad, I->d_pmpad, sizeof(double) * size_t(I->n_pm) * 3 * I->R * I->Sp * I->Sp,
                         hipMemcpyDeviceToDevice));
   }
   if ((rc = push_model(I->heavy.get(), I))) return rc;
