@@ -1099,7 +1099,7 @@ __global__ __launch_bounds__(64) void k_edge_codon(const T *__restrict__ cla, co
 template <typename T, int R>
 __global__ __launch_bounds__(64 * R) void k_edge_aa(const T *__restrict__ cla, const int *__restrict__ ua,
                                                     const unsigned char *__restrict__ tips,
-                                                    const double *__restrict__ pm64, const double *__restrict__ model,
+                                                    const double *__restrict__ pmT, const double *__restrict__ model,
                                                     const int *__restrict__ counts, const int *__restrict__ cst,
                                                     const DevEdgeOp *__restrict__ ops, double *__restrict__ partials,
                                                     double *__restrict__ site, double pinv, int pinv_in_model,
@@ -1112,56 +1112,74 @@ __global__ __launch_bounds__(64 * R) void k_edge_aa(const T *__restrict__ cla, c
   const int p = blockIdx.x * 64 + lane;
   const DevEdgeOp op = ops[blockIdx.y];
   const size_t slot = size_t(Ppad) * R * S, off = aa_off(p, r, R);
-  const double *Pm = pm64 + (size_t(op.pm) * 3 * R + r) * S * S;
-  const double *D1m = Pm + size_t(R) * S * S, *D2m = Pm + size_t(2 * R) * S * S;
+  // transposed P, P', P'' of category r: PT[(mi * R) * S * S + q * S + f] = M_mi[f][q]
+  const double *PT = pmT + (size_t(op.pm) * 3 * R + r) * S * S;
   const double *pi = model, *w = model + S;
-  double rL = 0, r1 = 0, r2 = 0;
+  double rr[3] = {0, 0, 0};  // this category's L, D1, D2 for the pattern
   if (p < P) {
-    double x[S];
+    T x[S];
 #pragma unroll
-    for (int f = 0; f < S; f++) x[f] = double(cla[op.partial * slot + off + 64 * f]);
+    for (int f = 0; f < S; f++) x[f] = cla[op.partial * slot + off + 64 * f];
+    const int nm = op.derivs ? 3 : 1;
     if (op.child >= 0) {
-      double y[S];
+      T y[S];
 #pragma unroll
-      for (int q = 0; q < S; q++) y[q] = double(cla[op.child * slot + off + 64 * q]);
-#pragma unroll 4
-      for (int f = 0; f < S; f++) {
-        double tL = 0, t1 = 0, t2 = 0;
+      for (int q = 0; q < S; q++) y[q] = cla[op.child * slot + off + 64 * q];
+      // (M y)[f] input state by input state (the reference's q order), one
+      // matrix at a time, then the x * pi weighting in f order
+#pragma unroll 1
+      for (int mi = 0; mi < nm; mi++) {
+        const double *MT = PT + size_t(mi) * R * S * S;
+        double acc[S];
+#pragma unroll
+        for (int f = 0; f < S; f++) acc[f] = 0.0;
 #pragma unroll
         for (int q = 0; q < S; q++) {
-          if constexpr (sizeof(T) == 8) {
-            tL += Pm[f * S + q] * y[q];
-            if (op.derivs) {
-              t1 += D1m[f * S + q] * y[q];
-              t2 += D2m[f * S + q] * y[q];
-            }
-          } else {
-            tL = __builtin_fma(Pm[f * S + q], y[q], tL);
-            if (op.derivs) {
-              t1 = __builtin_fma(D1m[f * S + q], y[q], t1);
-              t2 = __builtin_fma(D2m[f * S + q], y[q], t2);
-            }
+          const double yq = double(y[q]);
+          double col[S];
+#pragma unroll
+          for (int f = 0; f < S; f++) col[f] = MT[q * S + f];
+#pragma unroll
+          for (int f = 0; f < S; f++) {
+            if constexpr (sizeof(T) == 8)
+              acc[f] += col[f] * yq;
+            else
+              acc[f] = __builtin_fma(col[f], yq, acc[f]);
           }
         }
-        rL += tL * x[f] * pi[f];
-        r1 += t1 * x[f] * pi[f];
-        r2 += t2 * x[f] * pi[f];
+        double v = 0.0;
+#pragma unroll
+        for (int f = 0; f < S; f++) v += acc[f] * double(x[f]) * pi[f];
+        if (mi == 0)
+          rr[0] = v;
+        else if (mi == 1)
+          rr[1] = v;
+        else
+          rr[2] = v;
       }
     } else {
       const int t = tips[size_t(-op.child - 1) * Ppad + p];
-#pragma unroll 4
-      for (int f = 0; f < S; f++) {
-        if (t < S) {
-          rL += Pm[f * S + t] * x[f] * pi[f];
-          if (op.derivs) {
-            r1 += D1m[f * S + t] * x[f] * pi[f];
-            r2 += D2m[f * S + t] * x[f] * pi[f];
-          }
-        } else
-          rL += x[f] * pi[f];
+      if (t < S) {
+#pragma unroll 1
+        for (int mi = 0; mi < nm; mi++) {
+          const double *row = PT + size_t(mi) * R * S * S + t * S;  // M[f][t], f = 0..19
+          double v = 0.0;
+#pragma unroll
+          for (int f = 0; f < S; f++) v += row[f] * double(x[f]) * pi[f];
+          if (mi == 0)
+            rr[0] = v;
+          else if (mi == 1)
+            rr[1] = v;
+          else
+            rr[2] = v;
+        }
+      } else {
+#pragma unroll
+        for (int f = 0; f < S; f++) rr[0] += double(x[f]) * pi[f];
       }
     }
   }
+  const double rL = rr[0], r1 = rr[1], r2 = rr[2];
   sred[0][r][lane] = rL;
   sred[1][r][lane] = r1;
   sred[2][r][lane] = r2;
@@ -1496,7 +1514,7 @@ static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk, bo
 #undef GLE_EDGE_NUC
   } else if (I->S == 20) {
 #define GLE_EDGE_AA(RR)                                                                                           \
-  GLE_LAUNCH(I, (k_edge_aa<T, RR>), grid, dim3(64 * RR), 0, a.cla, a.u, a.tips, a.pm64, a.model,                 \
+  GLE_LAUNCH(I, (k_edge_aa<T, RR>), grid, dim3(64 * RR), 0, a.cla, a.u, a.tips, I->d_pmpad, a.model,            \
                      a.counts, a.cst, d_ops, a.partials, a.site, a.pinv, a.pinv_in_model, a.Ppad, a.P)
     switch (I->R) {
       case 1: GLE_EDGE_AA(1); break;
