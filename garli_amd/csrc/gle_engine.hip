@@ -124,6 +124,9 @@ struct Instance {
   // rounding is never multiplied by a large pattern weight.
   int heavy_threshold = 4;  // 0 keeps every pattern in FP32
   int fused_min_patterns = 1 << 19;  // smallest padded pattern count that runs fused walks
+  // bumped by every change of data or model: cached CLAs and P(t) of trees and
+  // servers built before it are stale (MakeAllNodesDirty, tree.cpp)
+  unsigned long long generation = 0;
   hipEvent_t tev_a = nullptr, tev_b = nullptr;  // events for the next timed launch
   std::unique_ptr<Instance> heavy;
   int P_orig = 0;
@@ -155,6 +158,7 @@ int instance_rates(const Instance *I) { return I->R; }
 int instance_patterns(const Instance *I) { return I->P_orig; }
 int instance_tips(const Instance *I) { return I->N; }
 int instance_precision(const Instance *I) { return I->prec; }
+unsigned long long instance_generation(const Instance *I) { return I->generation; }
 int instance_cla_slots(const Instance *I) { return I->n_slots; }
 int instance_pmat_slots(const Instance *I) { return I->n_pm; }
 
@@ -1969,6 +1973,7 @@ extern "C" int gle_destroy_instance(int h) {
 
 extern "C" int gle_set_tip_ambig_stream(int h, int tip, const signed char *stream, int len) {
   GET_INSTANCE(h);
+  I->generation++;
   if (I->S != 4) return set_error(GLE_ERR_ARG, "ambiguity streams are nucleotide data");
   if (tip < 0 || tip >= I->N || !stream) return set_error(GLE_ERR_ARG, "bad tip");
   std::vector<unsigned char> m(I->P_orig, 15);
@@ -1980,6 +1985,7 @@ extern "C" int gle_set_tip_ambig_stream(int h, int tip, const signed char *strea
 
 extern "C" int gle_set_tip_states(int h, int tip, const unsigned char *states) {
   GET_INSTANCE(h);
+  I->generation++;
   if (tip < 0 || tip >= I->N || !states) return set_error(GLE_ERR_ARG, "bad tip");
   for (int q = 0; q < I->P_orig; q++) {
     const int v = states[q];
@@ -1992,6 +1998,7 @@ extern "C" int gle_set_tip_states(int h, int tip, const unsigned char *states) {
 
 extern "C" int gle_set_patterns(int h, const int *counts, int last_const, const int *cst) {
   GET_INSTANCE(h);
+  I->generation++;
   if (!counts || last_const < -1 || last_const >= I->P_orig || (last_const >= 0 && !cst))
     return set_error(GLE_ERR_ARG, "bad pattern info");
   for (int q = 0; q < I->P_orig; q++)
@@ -2007,6 +2014,7 @@ extern "C" int gle_set_patterns(int h, const int *counts, int last_const, const 
 
 extern "C" int gle_set_heavy_split(int h, int threshold) {
   GET_INSTANCE(h);
+  I->generation++;
   if (threshold < 0) return set_error(GLE_ERR_ARG, "threshold must be >= 0");
   I->heavy_threshold = threshold;
   return I->h_patterns ? apply_patterns(I) : GLE_SUCCESS;
@@ -2029,6 +2037,7 @@ extern "C" int gle_get_heavy_split(int h, int *n_heavy) {
 
 extern "C" int gle_set_model(int h, const double *pi, const double *w, double pinv, int pinv_in_model) {
   GET_INSTANCE(h);
+  I->generation++;
   if (!pi || !w || pinv < 0.0 || pinv >= 1.0) return set_error(GLE_ERR_ARG, "bad model");
   I->h_pi.assign(pi, pi + I->S);
   I->h_w.assign(w, w + I->R);
@@ -2040,6 +2049,7 @@ extern "C" int gle_set_model(int h, const double *pi, const double *w, double pi
 
 extern "C" int gle_set_eigen(int h, int e, const double *V, const double *Vinv, const double *lam) {
   GET_INSTANCE(h);
+  I->generation++;
   if (e < 0 || e >= I->n_eig || !V || !Vinv || !lam) return set_error(GLE_ERR_ARG, "bad eigensystem");
   const size_t SS = size_t(I->S) * I->S;
   std::copy(V, V + SS, I->h_V.begin() + e * SS);
@@ -2051,6 +2061,7 @@ extern "C" int gle_set_eigen(int h, int e, const double *V, const double *Vinv, 
 
 extern "C" int gle_set_category_rates(int h, const int *eig_of_rate, const double *mult) {
   GET_INSTANCE(h);
+  I->generation++;
   if (!eig_of_rate || !mult) return set_error(GLE_ERR_ARG, "null argument");
   for (int r = 0; r < I->R; r++)
     if (eig_of_rate[r] < 0 || eig_of_rate[r] >= I->n_eig || !(mult[r] >= 0.0))

@@ -50,6 +50,7 @@ int instance_rates(const Instance *);
 int instance_patterns(const Instance *);
 int instance_tips(const Instance *);
 int instance_precision(const Instance *);
+unsigned long long instance_generation(const Instance *);
 int instance_cla_slots(const Instance *);
 int instance_pmat_slots(const Instance *);
 int engine_update_pmats(Instance *, const int *idx, const double *t, int n, int derivs);

@@ -93,6 +93,7 @@ struct ServeInstance {
   std::unordered_map<Sig, CacheEntry, SigHash> cla;
   std::unordered_map<uint64_t, CacheEntry> pm;  // blen bits -> P, P', P'' slot
   int reserved = 0;                              // working-set slots of admitted requests
+  unsigned long long gen = 0;                    // instance generation the caches belong to
 };
 
 enum { kReqScore = 0, kReqOptimize = 1 };
@@ -344,7 +345,16 @@ static int run_tick(Server &sv, ServeInstance &si, std::vector<Req *> &batch) {
     std::vector<std::pair<Sig, TickNew>> fresh;
     {
       std::lock_guard<std::mutex> pk(pool_mutex());
-      TickBuilder tb{&si, pool_for(si.I), sv.tick, {}, {}, {}, {}, {}, GLE_SUCCESS};
+      ClaPool *pool = pool_for(si.I);
+      if (si.gen != instance_generation(si.I)) {  // data or model changed: drop both caches
+        for (auto &c : si.cla)
+          if (--pool->refs[c.second.slot] == 0) pool->free_slots.push_back(c.second.slot);
+        si.cla.clear();
+        for (auto &c : si.pm) pool->pm_free.push_back(c.second.slot);
+        si.pm.clear();
+        si.gen = instance_generation(si.I);
+      }
+      TickBuilder tb{&si, pool, sv.tick, {}, {}, {}, {}, {}, GLE_SUCCESS};
       for (Req *r : batch) {
         r->sig.assign(r->nn, {Sig(), Sig(), Sig()});
         next_edge(*r);
@@ -549,6 +559,7 @@ extern "C" int gle_serve_submit(int h, int inst, const int *parent, const double
       si.h = inst;
       si.I = I;
       si.rescale_every = instance_precision(I) == GLE_PREC_FP64 ? 16 : 1;
+      si.gen = instance_generation(I);
     }
     if (r->need > instance_cla_slots(I))
       return set_error(GLE_ERR_NOMEM, "request needs %d CLA slots, instance has %d", r->need,

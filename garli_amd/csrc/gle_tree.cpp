@@ -39,6 +39,7 @@ struct Tree {
   std::vector<char> pm_has_derivs;
   int rescale_every = 16;
   int schedule = 0;
+  unsigned long long gen = 0;  // instance generation the cached CLAs and P(t) belong to
   double lnl = 0.0;
   std::vector<double> site;
   // scratch
@@ -135,6 +136,7 @@ static int build_tree(Tree &t, int inst_h, Instance *I, const int *parent, const
   t.pm_len.assign(nn, std::nan(""));
   t.pm_has_derivs.assign(nn, 0);
   t.rescale_every = instance_precision(I) == GLE_PREC_FP64 ? 16 : 1;
+  t.gen = instance_generation(I);
   t.level.assign(size_t(nn) * 3, -1);
   return GLE_SUCCESS;
 }
@@ -418,13 +420,25 @@ static void destroy_tree(Tree &t) {
 
 using namespace gle;
 
+// A change of the instance's data or model since the tree last computed makes
+// every cached CLA and P(t) of the tree stale.
+static void sync_generation(Tree &t) {
+  const unsigned long long g = instance_generation(t.I);
+  if (g == t.gen) return;
+  dirty_all(t);
+  std::fill(t.pm_len.begin(), t.pm_len.end(), std::nan(""));
+  std::fill(t.pm_has_derivs.begin(), t.pm_has_derivs.end(), 0);
+  t.gen = g;
+}
+
 #define GET_TREE(h)                                                        \
   std::lock_guard<std::recursive_mutex> lk_(g_tree_rmu);                   \
   std::lock_guard<std::mutex> pk_(pool_mutex());                           \
   Tree *T = get_tree(h);                                                   \
   if (!T) return set_error(GLE_ERR_ARG, "bad tree handle %d", (h));        \
   if (get_instance(T->inst_h) != T->I)                                     \
-    return set_error(GLE_ERR_STATE, "the tree's instance was destroyed");
+    return set_error(GLE_ERR_STATE, "the tree's instance was destroyed");  \
+  sync_generation(*T);
 
 extern "C" int gle_tree_create(int inst, const int *parent, const double *blen, int *tree) {
   if (!parent || !blen || !tree) return set_error(GLE_ERR_ARG, "null argument");

@@ -92,3 +92,32 @@ def test_codon_shapes(gpu, prec, omegas, npat):
     ds = sim.codon_m3(n_tips=5, n_patterns=npat, omegas=omegas, seed=360 + npat)
     print("codon patterns", ds.n_patterns, "classes", len(omegas), flush=True)
     _check(ds, prec)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_model_change_refreshes_caches(gpu, prec):
+    """A new Gamma shape or new frequencies after a score make the tree's and
+    the server's cached CLAs and P(t) stale (GARLI's MakeAllNodesDirty); the
+    next score uses the new model without an explicit invalidate."""
+    import dataclasses
+    ds = sim.gtr_gamma(n_tips=12, n_sites=900, pinv=0.1, seed=370)
+    inst = ds.make_instance(prec, n_cla_slots=400, n_pmat_slots=400)
+    t = E.Tree(inst, ds.parent, ds.blen)
+    srv = E.Server()
+    before = t.score()
+    srv.wait(srv.submit(inst, ds.parent, ds.blen))
+    rates = E.discrete_gamma(ds.n_rates, 1.7)
+    ds2 = dataclasses.replace(ds, rate_mult=rates / (1.0 - ds.pinv), meta=dict(ds.meta, alpha=1.7))
+    inst.set_category_rates(ds2.eig_of_rate, ds2.rate_mult)
+    pi = ds.pi[::-1].copy()
+    V, W, lam = E.model_gtr(pi, ds.meta["rel"])
+    ds2 = dataclasses.replace(ds2, pi=pi, eig=[(V, W, lam)])
+    inst.set_model(pi, ds.rate_probs, ds.pinv, ds.pinv_in_model)
+    inst.set_eigen(0, V, W, lam)
+    ref = O.Oracle(ds2).score()
+    after = t.score()
+    served = srv.wait(srv.submit(inst, ds.parent, ds.blen))[0]
+    srv.close()
+    tol = 1e-9 * abs(ref) if prec == E.PREC_FP64 else 1e-3
+    assert abs(after - before) > 1.0
+    assert abs(after - ref) < tol and abs(served - ref) < tol
