@@ -894,16 +894,15 @@ __device__ __forceinline__ void site_terms(double L, double D1, double D2, int u
     r1 = D1 / sL;
     r2 = (D2 / sL) - r1 * r1;
   } else {
-    if (pinv_term > 0.0) {
-      sL = ldexp(L, -u) + pinv_term;
-      ul = log(sL);
-      r1 = ldexp(D1, -u) / sL;
-      r2 = (ldexp(D2, -u) / sL) - r1 * r1;
-    } else {
-      ul = log(L) - double(u) * 0.69314718055994530942;
-      r1 = D1 / L;
-      r2 = (D2 / L) - r1 * r1;
-    }
+    // invariant-site patterns are summed at true scale (2^-u); others keep
+    // the scale and subtract u ln 2.  Branch-free, one log, one division.
+    const bool inv_site = pinv_term > 0.0;
+    const double sc = inv_site ? ldexp(1.0, -u) : 1.0;
+    sL = inv_site ? L * sc + pinv_term : L;
+    const double rs = 1.0 / sL;
+    ul = log(sL) - (inv_site ? 0.0 : double(u) * 0.69314718055994530942);
+    r1 = D1 * sc * rs;
+    r2 = D2 * sc * rs - r1 * r1;
   }
   if (nstate && !(ul < 0.0)) {  // NState scoring skips non-negative site lnL (tree.cpp:6473)
     lnl = g1 = g2 = 0.0;
