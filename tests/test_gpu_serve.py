@@ -84,3 +84,26 @@ def test_admission_under_small_budget(gpu):
     for (p, b), g in zip(pop, got):
         assert abs(g - o.score(parent=p, blen=b)) < 1e-3
     srv.close()
+
+
+@pytest.mark.parametrize("kind", ["aa", "codon"])
+def test_serving_state_spaces(gpu, kind):
+    """20- and 61-state instances through the server: alone equals in the mix,
+    and both equal the oracle's answer for the request's tree."""
+    if kind == "aa":
+        ds = sim.aa_gamma(n_tips=10, n_patterns=300, seed=104)
+    else:
+        ds = sim.codon_m3(n_tips=8, n_patterns=120, seed=105)
+    srv = E.Server()
+    pop = _population(ds, 10, 9)
+    for prec in (E.PREC_FP64, E.PREC_FP32):
+        inst = ds.make_instance(prec, n_cla_slots=300, n_pmat_slots=400)
+        alone = [srv.wait(srv.submit(inst, p, b))[0] for p, b in pop[:3]]
+        ids = [srv.submit(inst, p, b) for p, b in pop]
+        mix = [srv.wait(i)[0] for i in ids]
+        assert alone == mix[:3]
+        o = O.Oracle(ds)
+        for (p, b), got in zip(pop[::3], mix[::3]):
+            ref = o.score(parent=p, blen=b)
+            assert abs(got - ref) < (1e-9 * abs(ref) if prec == E.PREC_FP64 else 1e-3)
+    srv.close()
