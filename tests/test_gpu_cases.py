@@ -121,3 +121,35 @@ def test_model_change_refreshes_caches(gpu, prec):
     tol = 1e-9 * abs(ref) if prec == E.PREC_FP64 else 1e-3
     assert abs(after - before) > 1.0
     assert abs(after - ref) < tol and abs(served - ref) < tol
+
+
+def _caterpillar(n_tips, rng, mean_blen):
+    """The deepest unrooted tree: every internal node has a tip child."""
+    nn = 2 * n_tips - 2
+    parent = np.full(nn, -1, np.int32)
+    # root 0 holds tips 1 and 2 and the first internal node; each internal
+    # node below holds one tip and the next internal node, the last two tips
+    parent[1] = parent[2] = 0
+    prev, tip = 0, 3
+    for k in range(n_tips + 1, nn):
+        parent[k] = prev
+        parent[tip] = k
+        prev, tip = k, tip + 1
+    parent[tip] = prev
+    blen = rng.exponential(mean_blen, size=nn)
+    blen[0] = 0.0
+    return parent, blen
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_large_and_deep_trees(gpu, prec):
+    """500 taxa (a Yule tree) and a 160-taxon caterpillar, whose dependency
+    chain is as long as the tree: many levels, many rescales."""
+    import dataclasses
+    ds = sim.gtr_gamma(n_tips=500, n_sites=1500, mean_blen=0.05, seed=380)
+    _check(ds, prec, sweep=False)
+    rng = np.random.default_rng(381)
+    cat = sim.gtr_gamma(n_tips=160, n_sites=800, mean_blen=0.3, seed=382)
+    parent, blen = _caterpillar(160, rng, 0.3)
+    cat = dataclasses.replace(cat, parent=parent, blen=blen)
+    _check(cat, prec)
