@@ -435,6 +435,19 @@ __device__ __forceinline__ float round_dither(double x, unsigned key) {
   const double ulp = __builtin_amdgcn_ldexp(1.0, e - 24 > -149 ? e - 24 : -149);
   return float(x + double(int(h)) * 0x1p-32 * ulp);
 }
+// The same dithered rounding done on the bit pattern, for kernels that store
+// many values per pattern: adding an integer k to the bits of a positive normal
+// double moves it by k units of 2^-29 FP32 ulps (within its binade), so with
+// k = (r >> 3) - 2^28 uniform in [-2^28, 2^28) the round to nearest that
+// follows is the dithered one.  Zero stays zero.  r comes from dither_stream.
+__device__ __forceinline__ float round_dither_bits(double x, unsigned r) {
+  const long long b = __double_as_longlong(x) + (long long)(int(r >> 3) - (1 << 28));
+  return x == 0.0 ? 0.0f : float(__longlong_as_double(b));
+}
+// Element k's random word from a per-pattern seed h (one mix32 per pattern).
+__device__ __forceinline__ unsigned dither_stream(unsigned h, unsigned k) {
+  return (h ^ (k * 0x9e3779b9u)) * 0x85ebca6bu;
+}
 // FP32-form rescale on FP64 values: exact power of two once the max drops below 2^-32.
 template <int W>
 __device__ __forceinline__ int rescale_pow2(double *v, double m) {
@@ -485,6 +498,7 @@ __global__ __launch_bounds__(256) void k_cla_nuc(T *__restrict__ cla, int *__res
     }
     T *dst = cla + op.dest * slot + off;
     double mx = 0.0;
+    unsigned seed = 0;  // FP32 form: dither seed of this pattern and op
     // each category is stored as soon as it is formed (few live registers);
     // the rare rescale re-reads and scales what this thread stored
 #pragma unroll
@@ -525,13 +539,19 @@ __global__ __launch_bounds__(256) void k_cla_nuc(T *__restrict__ cla, int *__res
         }
       }
       T o[4];
+      if constexpr (sizeof(T) == 4) {
+        // one seed per pattern and op: the pattern index and the first value
+        if (r == 0)
+          seed = mix32(unsigned(p) * 0x9e3779b1u ^
+                       unsigned(static_cast<unsigned long long>(__double_as_longlong(out[0])) >> 29));
+      }
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         mx = fmax(mx, out[i]);
         if constexpr (sizeof(T) == 8)
           o[i] = out[i];
         else
-          o[i] = round_dither(out[i], unsigned(p) * 64u + unsigned(4 * r + i));
+          o[i] = round_dither_bits(out[i], dither_stream(seed, unsigned(4 * r + i)));
       }
       store4(dst + 256 * r, o);
     }
@@ -642,13 +662,19 @@ __global__ __launch_bounds__(64 * R) void k_cla_aa(T *__restrict__ cla, int *__r
     child(op.c2, op.pm2, a2);
     T *dst = cla + op.dest * slot + off;
     double m = 0.0;
+    unsigned seed = 0;  // FP32 form: dither seed of this (pattern, category) and op
 #pragma unroll
     for (int f = 0; f < S; f++) {
       const double o = mul_nc(a1[f], a2[f]);
       m = fmax(m, o);
+      if constexpr (kF32) {
+        if (f == 0)
+          seed = mix32((unsigned(p) * R + unsigned(r)) * 0x9e3779b1u ^
+                       unsigned(static_cast<unsigned long long>(__double_as_longlong(o)) >> 29));
+      }
       if (live) {
         if constexpr (kF32)
-          dst[64 * f] = round_dither(o, unsigned(p) * 512u + unsigned(r * S + f));
+          dst[64 * f] = round_dither_bits(o, dither_stream(seed, unsigned(f)));
         else
           dst[64 * f] = o;
       }
