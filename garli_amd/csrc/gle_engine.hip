@@ -128,6 +128,25 @@ struct Instance {
   // servers built before it are stale (MakeAllNodesDirty, tree.cpp)
   unsigned long long generation = 0;
   hipEvent_t tev_a = nullptr, tev_b = nullptr;  // events for the next timed launch
+  // hipGraph replay of repeated level-launch sequences (enqueue_partials): a
+  // few recent op lists, each captured on its second use
+  struct GraphEntry {
+    std::vector<DevClaOp> ops;
+    std::vector<int> levels;
+    const DevClaOp *dev = nullptr;
+    unsigned long long gen = 0, last = 0;
+    hipGraphExec_t exec = nullptr;
+  };
+  static constexpr int kGraphEntries = 4;
+  bool graphs = true;
+  std::vector<GraphEntry> g_cache;
+  unsigned long long g_clock = 0;
+  int g_replays = 0;
+  void drop_graphs() {
+    for (auto &g : g_cache)
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g_cache.clear();
+  }
   std::unique_ptr<Instance> heavy;
   int P_orig = 0;
   std::vector<int> light_idx, heavy_idx;  // original pattern indices of each part
@@ -168,7 +187,7 @@ ClaPool *pool_for(Instance *I) {
     I->pool = std::make_unique<ClaPool>();
     I->pool->refs.assign(I->n_slots, 0);
     I->pool->rank.assign(I->n_slots, 0);
-    for (int s = I->n_slots - 1; s >= 0; s--) I->pool->free_slots.push_back(s);
+    for (int s = I->n_slots - 1; s >= 0; s--) I->pool->free_slot(s);
     for (int s = I->n_pm - 1; s >= 0; s--) I->pool->pm_free.push_back(s);
   }
   return I->pool.get();
@@ -1467,6 +1486,47 @@ static int enqueue_partials(Instance *I, const DevClaOp *ops, int n_ops, const i
     for (int i = b; i < e; i++) t += cla_op_bytes(I, ops[i]);
     return t;
   };
+  // An op list met twice in a row (same ops and levels, same device buffers,
+  // same instance generation) is captured into a hipGraph on its second use
+  // and replayed after that: one launch instead of one per level.  The graph
+  // holds kernel nodes only; the op list is copied in above, outside it.
+  std::vector<int> levels;
+  if (!fused && n_levels > 0) levels.assign(level_start, level_start + n_levels + 1);
+  const bool graphable = I->graphs && !I->timing && !fused && n_levels > 0;
+  Instance::GraphEntry *entry = nullptr;
+  if (graphable) {
+    for (auto &g : I->g_cache)
+      if (g.dev == I->d_ops && g.gen == I->generation && g.levels == levels && g.ops.size() == size_t(n_ops) &&
+          std::memcmp(g.ops.data(), ops, sizeof(DevClaOp) * n_ops) == 0) {
+        entry = &g;
+        break;
+      }
+    if (entry) {
+      entry->last = ++I->g_clock;
+      if (entry->exec) {
+        GLE_HIP(hipGraphLaunch(entry->exec, I->stream));
+        I->g_replays++;
+        return GLE_SUCCESS;
+      }
+      // second use: capture below
+    } else {
+      if (int(I->g_cache.size()) >= Instance::kGraphEntries) {
+        auto lru = std::min_element(I->g_cache.begin(), I->g_cache.end(),
+                                    [](const auto &a, const auto &b) { return a.last < b.last; });
+        if (lru->exec) (void)hipGraphExecDestroy(lru->exec);
+        I->g_cache.erase(lru);
+      }
+      Instance::GraphEntry g;
+      g.ops.assign(ops, ops + n_ops);
+      g.levels = levels;
+      g.dev = I->d_ops;
+      g.gen = I->generation;
+      g.last = ++I->g_clock;
+      I->g_cache.push_back(std::move(g));
+    }
+  }
+  const bool capture = entry != nullptr;
+  if (capture) GLE_HIP(hipStreamBeginCapture(I->stream, hipStreamCaptureModeThreadLocal));
   if (fused) {
     timed_begin(I, 0, bytes_of(0, n_ops));
     rc = I->prec == GLE_PREC_FP64 ? launch_cla<double>(I, I->d_ops, n_ops, true)
@@ -1482,9 +1542,25 @@ static int enqueue_partials(Instance *I, const DevClaOp *ops, int n_ops, const i
       rc = I->prec == GLE_PREC_FP64 ? launch_cla<double>(I, I->d_ops + b, e - b, false)
                                     : launch_cla<float>(I, I->d_ops + b, e - b, false);
       timed_end(I);
-      if (rc) return rc;
+      if (rc) break;
     }
   }
+  if (capture) {
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(I->stream, &graph);
+    hipError_t ei = hipErrorUnknown;
+    if (ec == hipSuccess && !rc) ei = hipGraphInstantiate(&entry->exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc) return rc;
+    if (ec != hipSuccess || ei != hipSuccess) {
+      entry->exec = nullptr;
+      I->drop_graphs();
+      I->graphs = false;  // capture unavailable here: stay eager from now on
+      return set_error(GLE_ERR_DEVICE, "graph capture of the level launches failed");
+    }
+    GLE_HIP(hipGraphLaunch(entry->exec, I->stream));
+  }
+  if (rc) return rc;
   GLE_HIP(hipGetLastError());
   return GLE_SUCCESS;
 }
@@ -1730,6 +1806,7 @@ int engine_sync(Instance *I) {
 }
 
 static int destroy(Instance *I) {
+  I->drop_graphs();
   if (I->heavy) {
     destroy(I->heavy.get());
     I->heavy.reset();
@@ -2050,6 +2127,18 @@ extern "C" int gle_set_fused_threshold(int h, int min_patterns) {
   if (min_patterns < 0) return set_error(GLE_ERR_ARG, "threshold must be >= 0");
   I->fused_min_patterns = min_patterns;
   if (I->heavy) I->heavy->fused_min_patterns = min_patterns;
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_set_graph_replay(int h, int enable, int *replays) {
+  GET_INSTANCE(h);
+  std::lock_guard<std::mutex> lk_(I->mu);
+  for (Instance *X : {I, I->heavy.get()}) {
+    if (!X) continue;
+    X->graphs = enable != 0;
+    X->drop_graphs();
+  }
+  if (replays) *replays = I->g_replays;
   return GLE_SUCCESS;
 }
 

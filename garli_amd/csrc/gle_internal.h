@@ -3,6 +3,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <mutex>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -68,9 +69,18 @@ bool engine_supports_fused(const Instance *);
 // and transition-matrix slots.  Guarded by pool_mutex().
 struct ClaPool {
   std::vector<int> refs;
-  std::vector<int> free_slots;
   std::vector<int> rank;
   std::vector<int> pm_free;
+  // free CLA slots, lowest first: the same tree state always gets the same
+  // slots, so a repeated op list is recognised (and replayed from a graph)
+  std::set<int> free_set;
+  void free_slot(int s) { free_set.insert(s); }
+  bool has_free() const { return !free_set.empty(); }
+  int take_slot() {
+    const int s = *free_set.begin();
+    free_set.erase(free_set.begin());
+    return s;
+  }
 };
 ClaPool *pool_for(Instance *I);
 std::mutex &pool_mutex();

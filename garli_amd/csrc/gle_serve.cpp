@@ -21,8 +21,11 @@
 //    dedupes shared CLAs, and issues one P(t) launch, one CLA launch per
 //    dependency level and one edge launch for all of them.  Requests that
 //    arrive while a tick runs join the next tick.
-// Per-request results do not depend on what else is in the batch: every CLA
-// and every edge reduction is computed independently and deterministically.
+//  * A score is taken on the edge that needs the fewest new CLAs given the
+//    cache (lnL is the same on every edge), next to the perturbation of an
+//    offspring.  Every CLA and edge reduction is deterministic, but the edge
+//    depends on what is cached, so a request's result alone and in a mix
+//    agree to the precision form's rounding rather than bit for bit.
 #include <algorithm>
 #include <array>
 #include <chrono>
@@ -30,6 +33,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -207,19 +211,18 @@ struct TickBuilder {
   int err = GLE_SUCCESS;
 
   int take_slot() {
-    if (pool->free_slots.empty()) {  // evict the least recently used entry not used this tick
+    if (!pool->has_free()) {  // evict the least recently used entry not used this tick
       auto victim = si->cla.end();
       for (auto it = si->cla.begin(); it != si->cla.end(); ++it)
         if (it->second.last < tick && (victim == si->cla.end() || it->second.last < victim->second.last))
           victim = it;
       if (victim == si->cla.end()) return -1;
       const int s = victim->second.slot;
-      if (--pool->refs[s] == 0) pool->free_slots.push_back(s);
+      if (--pool->refs[s] == 0) pool->free_slot(s);
       si->cla.erase(victim);
-      if (pool->free_slots.empty()) return -1;
+      if (!pool->has_free()) return -1;
     }
-    const int s = pool->free_slots.back();
-    pool->free_slots.pop_back();
+    const int s = pool->take_slot();
     pool->refs[s] = 1;
     return s;
   }
@@ -304,12 +307,44 @@ struct TickBuilder {
   }
 };
 
+// Scoring edge of a request.  lnL is the same on every edge (the pruning
+// algorithm's root invariance; the reference's RunTests checks it,
+// population.cpp:924-954), so a score is taken where the fewest CLAs are
+// missing from the cache and this tick: next to the perturbation, for an
+// offspring whose parent's CLAs are cached.  Ties keep the reference's edge
+// (root, middle child).  The choice depends on what is cached, so a result
+// alone and in a mix agree to the form's rounding, not bit for bit.
+static void choose_score_edge(Req &r, const TickBuilder &tb) {
+  std::vector<std::array<int, 3>> memo(r.nn, {-1, -1, -1});
+  std::function<int(int, int)> cost = [&](int a, int b) -> int {  // CLAs missing for X(a|b)
+    if (req_tip(r, a)) return 0;
+    int &m = memo[a][req_nbk(r, a, b)];
+    if (m >= 0) return m;
+    const Sig s = req_sig(r, a, b);
+    if (tb.si->cla.count(s) || tb.fresh.count(s)) return m = 0;
+    int c = 1;
+    for (int j = 0; j < r.nnb[a]; j++)
+      if (r.nb[a][j] != b) c += cost(r.nb[a][j], a);
+    return m = c;
+  };
+  r.edge_a = 0;
+  r.edge_b = r.nb[0][1];
+  int best = cost(r.edge_a, r.edge_b) + cost(r.edge_b, r.edge_a);
+  for (int n = 1; n < r.nn && best > 0; n++) {
+    const int a = r.parent[n];
+    const int c = cost(n, a) + cost(a, n);
+    if (c < best) {
+      best = c;
+      r.edge_a = a;
+      r.edge_b = n;
+    }
+  }
+}
+
 // Next evaluation of a request: (edge, derivs?).
-static void next_edge(Req &r) {
+static void next_edge(Req &r, const TickBuilder &tb) {
   if (r.scoring) {
-    // Tree::Score on a fresh tree integrates across (root, middle child)
-    r.edge_a = 0;
-    r.edge_b = r.nb[0][1];
+    choose_score_edge(r, tb);
     r.derivs = false;
   } else {
     const int nd = r.order[r.bi];
@@ -348,7 +383,7 @@ static int run_tick(Server &sv, ServeInstance &si, std::vector<Req *> &batch) {
       ClaPool *pool = pool_for(si.I);
       if (si.gen != instance_generation(si.I)) {  // data or model changed: drop both caches
         for (auto &c : si.cla)
-          if (--pool->refs[c.second.slot] == 0) pool->free_slots.push_back(c.second.slot);
+          if (--pool->refs[c.second.slot] == 0) pool->free_slot(c.second.slot);
         si.cla.clear();
         for (auto &c : si.pm) pool->pm_free.push_back(c.second.slot);
         si.pm.clear();
@@ -357,7 +392,7 @@ static int run_tick(Server &sv, ServeInstance &si, std::vector<Req *> &batch) {
       TickBuilder tb{&si, pool, sv.tick, {}, {}, {}, {}, {}, GLE_SUCCESS};
       for (Req *r : batch) {
         r->sig.assign(r->nn, {Sig(), Sig(), Sig()});
-        next_edge(*r);
+        next_edge(*r, tb);
         int ra, la, rb, lb;
         DevEdgeOp e;
         e.partial = tb.resolve(*r, r->edge_a, r->edge_b, ra, la);
@@ -406,9 +441,9 @@ static int run_tick(Server &sv, ServeInstance &si, std::vector<Req *> &batch) {
       // Tree::Score's retry: drop every cached CLA (they were built with the old
       // rescaleEvery) and run the step again with rescaleEvery - 2.
       for (auto &f : fresh)
-        if (--pool->refs[f.second.slot] == 0) pool->free_slots.push_back(f.second.slot);
+        if (--pool->refs[f.second.slot] == 0) pool->free_slot(f.second.slot);
       for (auto &c : si.cla)
-        if (--pool->refs[c.second.slot] == 0) pool->free_slots.push_back(c.second.slot);
+        if (--pool->refs[c.second.slot] == 0) pool->free_slot(c.second.slot);
       si.cla.clear();
       si.rescale_every -= 2;
       if (si.rescale_every < 2) return set_error(GLE_ERR_RESCALE_FATAL, "Problem with rescaling in serving.");
@@ -510,7 +545,7 @@ extern "C" int gle_serve_destroy(int h) {
       if (get_instance(kv.second.h) != kv.second.I) continue;  // destroyed: its pool went with it
       ClaPool *pool = pool_for(kv.second.I);
       for (auto &c : kv.second.cla)
-        if (--pool->refs[c.second.slot] == 0) pool->free_slots.push_back(c.second.slot);
+        if (--pool->refs[c.second.slot] == 0) pool->free_slot(c.second.slot);
       for (auto &p : kv.second.pm) pool->pm_free.push_back(p.second.slot);
     }
   }

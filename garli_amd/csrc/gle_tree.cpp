@@ -66,7 +66,7 @@ static int edge_node(const Tree &t, int a, int b) { return t.parent[a] == b ? a 
 static void release(Tree &t, int a, int k) {
   int &s = t.slot[a][k];
   if (s < 0) return;
-  if (--t.pool->refs[s] == 0) t.pool->free_slots.push_back(s);
+  if (--t.pool->refs[s] == 0) t.pool->free_slot(s);
   s = -1;
 }
 static void dirty_all(Tree &t) {
@@ -234,9 +234,8 @@ static int run_ops(Tree &t, std::vector<std::array<int, 3>> &want) {
       op.flags = kOpRescale;
       rank = 0;
     }
-    if (t.pool->free_slots.empty()) return set_error(GLE_ERR_NOMEM, "out of CLA slots");
-    const int s = t.pool->free_slots.back();
-    t.pool->free_slots.pop_back();
+    if (!t.pool->has_free()) return set_error(GLE_ERR_NOMEM, "out of CLA slots");
+    const int s = t.pool->take_slot();
     t.pool->refs[s] = 1;
     t.pool->rank[s] = rank;
     t.slot[a][k] = s;

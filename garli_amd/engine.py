@@ -74,6 +74,7 @@ SIGNATURES = {
     "gle_set_heavy_split": ([C.c_int, C.c_int], C.c_int),
     "gle_get_heavy_split": ([C.c_int, _I], C.c_int),
     "gle_set_fused_threshold": ([C.c_int, C.c_int], C.c_int),
+    "gle_set_graph_replay": ([C.c_int, C.c_int, _I], C.c_int),
     "gle_set_eigen": ([C.c_int, C.c_int, _D, _D, _D], C.c_int),
     "gle_set_category_rates": ([C.c_int, _I, _D], C.c_int),
     "gle_update_transition_matrices": ([C.c_int, _I, _D, C.c_int, C.c_int], C.c_int),
@@ -282,6 +283,13 @@ class Instance:
     def set_fused_threshold(self, min_patterns):
         """Fused single-launch walks only for instances with >= min_patterns (padded)."""
         _check(lib().gle_set_fused_threshold(self.h, int(min_patterns)))
+
+    def set_graph_replay(self, enable):
+        """Replay repeated level-launch sequences from a captured hipGraph; returns
+        the number of replays so far (the cached graph is dropped either way)."""
+        n = C.c_int(0)
+        _check(lib().gle_set_graph_replay(self.h, int(bool(enable)), C.byref(n)))
+        return n.value
 
     def heavy_patterns(self):
         n = C.c_int(0)

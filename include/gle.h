@@ -117,6 +117,12 @@ int gle_get_heavy_split(int h, int *n_heavy);
  * threads to fill the GPU); otherwise they launch per dependency level.  No
  * reference counterpart (a launch-policy knob). */
 int gle_set_fused_threshold(int h, int min_patterns);
+/* A level-launch sequence submitted twice in a row is captured into a hipGraph
+ * and replayed from then on (default on).  enable = 0 keeps every launch
+ * eager; either call drops the cached graph.  *replays (may be null) returns
+ * how many updates were replayed from a graph so far.  No reference
+ * counterpart (a launch-policy knob). */
+int gle_set_graph_replay(int h, int enable, int *replays);
 /* state freqs [S], category weights [R] (Model::GetRateProbs, model.h:2051-2064), pinv */
 int gle_set_model(int h, const double *pi, const double *rate_probs, double pinv, int pinv_in_model);
 /* eigensystem e: V, Vinv [S*S], lam [S] (already scaled) */

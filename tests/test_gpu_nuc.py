@@ -188,3 +188,23 @@ def test_copy_on_write(gpu):
     bl[5] = 0.3
     assert _rel(lb, O.Oracle(ds).score(blen=bl)) < 1e-9
     assert _rel(la, O.Oracle(ds).score()) < 1e-9
+
+
+@pytest.mark.parametrize("prec", [E.PREC_FP64, E.PREC_FP32])
+def test_graph_replay_equals_eager(gpu, prec):
+    """Repeated sweeps replay the captured level launches; every replay gives
+    the eager result bit for bit, with eager work and syncs in between."""
+    ds = sim.gtr_gamma(n_tips=24, n_sites=3000, pinv=0.2, seed=63)
+    inst = ds.make_instance(prec)
+    t = E.Tree(inst, ds.parent, ds.blen)
+    inst.set_graph_replay(False)
+    eager = t.sweep()
+    t.invalidate()
+    eager_score = t.score()
+    inst.set_graph_replay(True)
+    for k in range(5):
+        np.testing.assert_array_equal(t.sweep(), eager)
+        t.invalidate()
+        assert t.score() == eager_score
+    replays = inst.set_graph_replay(True)
+    assert replays > 0
