@@ -14,7 +14,8 @@
 //                                   exactly as the reference; FP32: powers of two)
 //   pm64   f64 [pmats][3][R][S][S] P, P', P'' (row = parent state, model.cpp:1271-1358)
 //   pmpad  f64 [pmats][3][R][Sp][Sp] a second copy of P, P', P'' in the layout a kernel family
-//                                   wants: codons zero-padded to 64 x 64 (MFMA tiles); amino
+//                                   wants: codons zero-padded to 64 x 64 in MFMA fragment
+//                                   order (codon_frag); amino
 //                                   acids transposed (column j of P contiguous)
 //   counts i32 [Ppad], cst i32 [Ppad] (constant-pattern state/mask, -1 otherwise)
 // Ppad rounds P up to a multiple of 256 so every 256-thread tile is whole.
@@ -328,6 +329,16 @@ __device__ __forceinline__ void store4(T *dst, const T *v) {
   }
 }
 
+// Layouts of the padded FP64 copy of P, P', P'' (pmpad).
+enum { kPadRowMajor = 0, kPadTransposed = 1, kPadCodonFrag = 2 };
+// Codon fragment order: entry P[i][j] of a 64x64 block sits where lane
+// l = 16 (j / 16) + i % 16 finds it at step kk = j % 16 of output tile
+// t = i / 16 of the codon MFMA kernels, so that every B-fragment load of a
+// wave reads 64 consecutive doubles.
+__host__ __device__ __forceinline__ int codon_frag(int i, int j) {
+  return ((i >> 4) * 16 + (j & 15)) * 64 + (j >> 4) * 16 + (i & 15);
+}
+
 // =====================================================================
 // P(t), P', P'' (model.cpp:1271-1358, 1469-1597) -- FP64
 // grid (n, R); V/Vinv of the category's eigensystem staged in LDS.
@@ -337,7 +348,7 @@ __global__ __launch_bounds__(256) void k_pmat(int S, int Sp, int R, const int *_
                                               const double *__restrict__ rate_mult, const double *__restrict__ V,
                                               const double *__restrict__ Vinv, const double *__restrict__ lam,
                                               double *__restrict__ pm64, double *__restrict__ pmpad, int derivs,
-                                              int transpose) {
+                                              int layout) {
 #pragma clang fp contract(off)
   extern __shared__ double sm[];
   const int SS = S * S, op = blockIdx.x, r = blockIdx.y;
@@ -373,7 +384,7 @@ __global__ __launch_bounds__(256) void k_pmat(int S, int Sp, int R, const int *_
     }
     const double pv = sp > 0.0 ? sp : 0.0;
     P[q] = pv;
-    const int pq = transpose ? j * Sp + i : i * Sp + j;
+    const int pq = layout == kPadTransposed ? j * Sp + i : layout == kPadCodonFrag ? codon_frag(i, j) : i * Sp + j;
     if (Pp) Pp[pq] = pv;
     if (derivs) {
       D1[q] = s1;
@@ -736,7 +747,8 @@ __global__ __launch_bounds__(64 * R) void k_cla_aa(T *__restrict__ cla, int *__r
 // lane's A and B fragments are then 16 contiguous values of one CLA row / one
 // P row.  v_mfma_f64_16x16x4_f64 D layout: lane l, element v holds row
 // l/16 + 4v (pattern), column l%16 (state).  P comes from the zero-padded FP64 copy
-// pmpad [slot][3][R][64][64] (P, P', P'').  The MFMA sums in its own order,
+// pmpad [slot][3][R][64*64] (P, P', P'') in codon fragment order, so the B
+// fragment of step kk is one coalesced 512-byte load.  The MFMA sums in its own order,
 // so FP64 codon CLAs match the reference's sequential sums to rounding
 // (about 1e-16 relative), not bit for bit.
 // =====================================================================
@@ -755,38 +767,75 @@ __device__ __forceinline__ void load16(const T *src, double *x) {
   }
 }
 
-// acc[t] = tile t of P . x for the wave's 16 patterns (x a CLA or a tip column).
+// x = this lane's 16 CLA values (pattern p0 + lane%16, states 16 (lane/16)
+// .. +15) of internal child c, zero beyond the last pattern.
 template <typename T>
-__device__ __forceinline__ void codon_child(const T *__restrict__ cla, const unsigned char *__restrict__ tips,
-                                            const double *__restrict__ Pr, int c, size_t slot, int Ppad, int P, int R,
-                                            int r, int lo, int g, int p0, gle_doublex4 acc[4]) {
+__device__ __forceinline__ void codon_load_x(const T *__restrict__ cla, int c, size_t slot, int R, int r, int lo,
+                                             int g, int p0, int P, double x[16]) {
+  const int p = p0 + lo;
+  if (p < P)
+    load16(cla + c * slot + (size_t(p) * R + r) * 64 + 16 * g, x);
+  else
+#pragma unroll
+    for (int q = 0; q < 16; q++) x[q] = 0.0;
+}
+// Tip states of the patterns p0 + g + 4v this lane's D rows hold (61 =
+// missing, also beyond the last pattern).
+__device__ __forceinline__ void codon_tip_states(const unsigned char *__restrict__ tips, int c, int Ppad, int P,
+                                                 int g, int p0, int st[4]) {
+  const unsigned char *tip = tips + size_t(-c - 1) * Ppad;
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int p = p0 + g + 4 * v;
+    st[v] = p < P ? tip[p] : 61;
+  }
+}
+// Output tile t (states 16t .. 16t+15) of Pr . x for the wave's 16 patterns.
+__device__ __forceinline__ gle_doublex4 codon_mfma_tile(const double *__restrict__ Pr, const double x[16], int t,
+                                                        int l) {
+  const double *B = Pr + t * 1024 + l;
+  double b[16];
+#pragma unroll
+  for (int kk = 0; kk < 16; kk++) b[kk] = B[kk * 64];
+  // all 16 loads in flight before the first MFMA waits on one
+  __builtin_amdgcn_sched_barrier(0);
+  gle_doublex4 d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int kk = 0; kk < 16; kk++) d = __builtin_amdgcn_mfma_f64_16x16x4f64(x[kk], b[kk], d, 0, 0, 0);
+  return d;
+}
+// The same tile for a tip child: column st of Pr, `missing` for a missing state.
+__device__ __forceinline__ gle_doublex4 codon_tip_tile(const double *__restrict__ Pr, const int st[4], int t, int lo,
+                                                       double missing) {
+  gle_doublex4 d;
+#pragma unroll
+  for (int v = 0; v < 4; v++) d[v] = st[v] < 61 ? Pr[codon_frag(16 * t + lo, st[v])] : missing;
+  return d;
+}
+
+// Tiles 0..3 of Pr . x (internal child c) or of the tip column (tip child c).
+template <typename T>
+__device__ __forceinline__ void codon_child(const T *__restrict__ cla, const double *__restrict__ Pr, int c,
+                                            const int ts[4], size_t slot, int R, int r, int l, int p0, int P,
+                                            gle_doublex4 acc[4]) {
+  const int lo = l & 15, g = l >> 4;
   if (c >= 0) {
     double x[16];
-    const int p = p0 + lo;
-    if (p < P)
-      load16(cla + c * slot + (size_t(p) * R + r) * 64 + 16 * g, x);
-    else {
-#pragma unroll
-      for (int q = 0; q < 16; q++) x[q] = 0.0;
-    }
+    codon_load_x(cla, c, slot, R, r, lo, g, p0, P, x);
 #pragma unroll
     for (int t = 0; t < 4; t++) {
+      const double *B = Pr + t * 1024 + l;
       double b[16];
-      load16(Pr + (16 * t + lo) * 64 + 16 * g, b);
+#pragma unroll
+      for (int kk = 0; kk < 16; kk++) b[kk] = B[kk * 64];
       gle_doublex4 d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int kk = 0; kk < 16; kk++) d = __builtin_amdgcn_mfma_f64_16x16x4f64(x[kk], b[kk], d, 0, 0, 0);
       acc[t] = d;
     }
   } else {
-    const unsigned char *tip = tips + size_t(-c - 1) * Ppad;
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const int p = p0 + g + 4 * v;
-      const int st = p < P ? tip[p] : 61;
-#pragma unroll
-      for (int t = 0; t < 4; t++) acc[t][v] = st < 61 ? Pr[(16 * t + lo) * 64 + st] : 1.0;
-    }
+    for (int t = 0; t < 4; t++) acc[t] = codon_tip_tile(Pr, ts, t, lo, 1.0);
   }
 }
 
@@ -794,7 +843,7 @@ __device__ __forceinline__ void codon_child(const T *__restrict__ cla, const uns
 // product is stored as soon as it is formed; the rare rescale of a pattern
 // then re-reads and scales what this lane stored.
 template <typename T, int R>
-__global__ __launch_bounds__(64) void k_cla_codon(T *__restrict__ cla, int *__restrict__ ua,
+__global__ __launch_bounds__(64, 2) void k_cla_codon(T *__restrict__ cla, int *__restrict__ ua,
                                                   const unsigned char *__restrict__ tips,
                                                   const double *__restrict__ pmpad, const DevClaOp *__restrict__ ops,
                                                   int *__restrict__ flags, int Ppad, int P, int fused_n) {
@@ -807,12 +856,15 @@ __global__ __launch_bounds__(64) void k_cla_codon(T *__restrict__ cla, int *__re
     if (oi != first) __syncthreads();  // the previous op's stores are read by other lanes
     const DevClaOp op = ops[oi];
     T *dst = cla + op.dest * slot;
+    int ts1[4], ts2[4];
+    if (op.c1 < 0) codon_tip_states(tips, op.c1, Ppad, P, g, p0, ts1);
+    if (op.c2 < 0) codon_tip_states(tips, op.c2, Ppad, P, g, p0, ts2);
     double mx[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 1
     for (int r = 0; r < R; r++) {
       gle_doublex4 a1[4], a2[4];
-      codon_child(cla, tips, pmpad + (size_t(op.pm1) * 3 * R + r) * 4096, op.c1, slot, Ppad, P, R, r, lo, g, p0, a1);
-      codon_child(cla, tips, pmpad + (size_t(op.pm2) * 3 * R + r) * 4096, op.c2, slot, Ppad, P, R, r, lo, g, p0, a2);
+      codon_child(cla, pmpad + (size_t(op.pm1) * 3 * R + r) * 4096, op.c1, ts1, slot, R, r, l, p0, P, a1);
+      codon_child(cla, pmpad + (size_t(op.pm2) * 3 * R + r) * 4096, op.c2, ts2, slot, R, r, l, p0, P, a2);
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const int p = p0 + g + 4 * v;
@@ -1074,34 +1126,31 @@ __global__ __launch_bounds__(64) void k_edge_codon(const T *__restrict__ cla, co
   const double *pi = model, *w = model + 61;
   const int nm = op.derivs ? 3 : 1;
   double sL[4] = {0, 0, 0, 0}, s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0};
+  int ts[4];
+  if (op.child < 0) codon_tip_states(tips, op.child, Ppad, P, g, p0, ts);
 #pragma unroll 1
   for (int r = 0; r < R; r++) {
     double acc[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};  // [matrix][v]: sum over this lane's states
+    double y[16];
+    if (op.child >= 0) codon_load_x(cla, op.child, slot, R, r, lo, g, p0, P, y);
 #pragma unroll 1
-    for (int mi = 0; mi < nm; mi++) {
-      gle_doublex4 Y[4];
-      codon_child(cla, tips, pmpad + ((size_t(op.pm) * 3 + mi) * R + r) * 4096, op.child, slot, Ppad, P, R, r, lo,
-                  g, p0, Y);
-      if (op.child < 0 && mi > 0) {  // a missing tip contributes no derivative
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-          const int p = p0 + g + 4 * v;
-          const int st = p < P ? tips[size_t(-op.child - 1) * Ppad + p] : 61;
-          if (st >= 61)
-#pragma unroll
-            for (int t = 0; t < 4; t++) Y[t][v] = 0.0;
-        }
-      }
+    for (int t = 0; t < 4; t++) {
+      const int st = 16 * t + lo;
+      double xw[4];  // x * pi of the partial side for this lane's cells
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const int p = p0 + g + 4 * v;
-        if (p >= P) continue;
-        const T *xp = cla + op.partial * slot + (size_t(p) * R + r) * 64;
+        xw[v] = p < P && st < 61 ? double(cla[op.partial * slot + (size_t(p) * R + r) * 64 + st]) * pi[st] : 0.0;
+      }
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-          const int st = 16 * t + lo;
-          if (st < 61) acc[mi][v] += Y[t][v] * double(xp[st]) * pi[st];
-        }
+      for (int mi = 0; mi < 3; mi++) {
+        if (mi >= nm) break;
+        const double *Pm = pmpad + ((size_t(op.pm) * 3 + mi) * R + r) * 4096;
+        // a missing tip contributes no derivative
+        const gle_doublex4 Y = op.child >= 0 ? codon_mfma_tile(Pm, y, t, l)
+                                             : codon_tip_tile(Pm, ts, t, lo, mi == 0 ? 1.0 : 0.0);
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc[mi][v] += Y[v] * xw[v];
       }
     }
 #pragma unroll
@@ -1366,7 +1415,7 @@ static int enqueue_pmats(Instance *I, const int *idx, const double *t, int n, in
   timed_begin(I, 2, double(n) * I->R * S * S * 8.0 * (derivs ? 3 : 1));
   GLE_LAUNCH(I, k_pmat, dim3(n, I->R), dim3(256), shmem, S, I->Sp, I->R, I->d_pmidx, I->d_pmt,
                      I->d_eig_of_rate, I->d_model + S + I->R, I->d_V, I->d_Vinv, I->d_lam, I->d_pm64, I->d_pmpad,
-                     derivs, S == 20 ? 1 : 0);
+                     derivs, S == 20 ? kPadTransposed : S == 61 ? kPadCodonFrag : kPadRowMajor);
   timed_end(I);
   GLE_HIP(hipGetLastError());
   return GLE_SUCCESS;

@@ -455,7 +455,11 @@ static int run_tick(Server &sv, ServeInstance &si, std::vector<Req *> &batch) {
   }
 }
 
-static int working_set(const Req &r) { return 2 * (r.N - 2) + 4; }
+// CLA slots a request can hold in one tick: a step evaluates one edge and
+// touches at most the N - 2 directional CLAs that point at it (cached or
+// new).  A branch-length pass keeps room for both directions so that its
+// CLAs survive between Newton steps.
+static int working_set(const Req &r) { return r.kind == kReqScore ? r.N - 2 : 2 * (r.N - 2) + 4; }
 
 static void worker_loop(Server *sv) {
   for (;;) {

@@ -88,6 +88,24 @@ def test_admission_under_small_budget(gpu):
     srv.close()
 
 
+def test_admission_at_minimum_budget(gpu):
+    """A score touches at most the N - 2 CLAs that point at its edge, so an
+    instance with exactly N - 2 slots serves scores one at a time."""
+    ds = sim.gtr_gamma(n_tips=16, n_sites=500, seed=106)
+    inst = ds.make_instance(E.PREC_FP64, n_cla_slots=ds.n_tips - 2, n_pmat_slots=400)
+    srv = E.Server()
+    pop = _population(ds, 6, 8)
+    ids = [srv.submit(inst, p, b) for p, b in pop]
+    got = [srv.wait(i)[0] for i in ids]
+    st = srv.stats()
+    srv.close()
+    o = O.Oracle(ds)
+    for (p, b), g in zip(pop, got):
+        ref = o.score(parent=p, blen=b)
+        assert abs(g - ref) < 1e-9 * abs(ref)
+    assert st["mean_batch"] == 1.0
+
+
 @pytest.mark.parametrize("kind", ["aa", "codon"])
 def test_serving_state_spaces(gpu, kind):
     """20- and 61-state instances through the server: alone equals in the mix,
