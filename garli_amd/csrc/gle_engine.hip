@@ -839,16 +839,22 @@ __device__ __forceinline__ void codon_child(const T *__restrict__ cla, const dou
   }
 }
 
-// grid (Ppad/16, fused ? 1 : n_ops), one wave per block.  Each category's
-// product is stored as soon as it is formed; the rare rescale of a pattern
-// then re-reads and scales what this lane stored.
+// grid (Ppad/16, fused ? 1 : n_ops); a block is R waves over the same 16
+// patterns and wave r takes category r, so that a wave's serial MFMA chain is
+// one category long (small dependency levels fill the GPU R times faster).
+// Each product is stored as soon as it is formed; a rescale takes each
+// pattern's maximum over its 16 lanes and over the R waves (through LDS) and
+// each wave re-reads and scales what it stored.
 template <typename T, int R>
-__global__ __launch_bounds__(64, 2) void k_cla_codon(T *__restrict__ cla, int *__restrict__ ua,
-                                                  const unsigned char *__restrict__ tips,
-                                                  const double *__restrict__ pmpad, const DevClaOp *__restrict__ ops,
-                                                  int *__restrict__ flags, int Ppad, int P, int fused_n) {
+__global__ __launch_bounds__(64 * R, 3) void k_cla_codon(T *__restrict__ cla, int *__restrict__ ua,
+                                                         const unsigned char *__restrict__ tips,
+                                                         const double *__restrict__ pmpad,
+                                                         const DevClaOp *__restrict__ ops, int *__restrict__ flags,
+                                                         int Ppad, int P, int fused_n) {
   constexpr bool kF32 = sizeof(T) == 4;
-  const int l = threadIdx.x, lo = l & 15, g = l >> 4;
+  __shared__ double smax[R][16];
+  const int l = threadIdx.x & 63, lo = l & 15, g = l >> 4;
+  const int r = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int p0 = blockIdx.x * 16;
   const size_t slot = size_t(Ppad) * R * 64;
   const int first = fused_n ? 0 : blockIdx.y, last = fused_n ? fused_n : blockIdx.y + 1;
@@ -860,8 +866,7 @@ __global__ __launch_bounds__(64, 2) void k_cla_codon(T *__restrict__ cla, int *_
     if (op.c1 < 0) codon_tip_states(tips, op.c1, Ppad, P, g, p0, ts1);
     if (op.c2 < 0) codon_tip_states(tips, op.c2, Ppad, P, g, p0, ts2);
     double mx[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 1
-    for (int r = 0; r < R; r++) {
+    {
       gle_doublex4 a1[4], a2[4];
       codon_child(cla, pmpad + (size_t(op.pm1) * 3 * R + r) * 4096, op.c1, ts1, slot, R, r, l, p0, P, a1);
       codon_child(cla, pmpad + (size_t(op.pm2) * 3 * R + r) * 4096, op.c2, ts2, slot, R, r, l, p0, P, a2);
@@ -884,45 +889,58 @@ __global__ __launch_bounds__(64, 2) void k_cla_codon(T *__restrict__ cla, int *_
         }
       }
     }
+    if (op.flags & kOpRescale) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        double m = mx[v];
+        for (int o = 1; o < 16; o <<= 1) m = fmax(m, __shfl_xor(m, o));
+        mx[v] = m;
+        if (lo == 0) smax[r][g + 4 * v] = m;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        double m = smax[0][g + 4 * v];
+        for (int q = 1; q < R; q++) m = fmax(m, smax[q][g + 4 * v]);
+        mx[v] = m;
+      }
+    }
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int p = p0 + g + 4 * v;
       int u = 0;
-      if (p < P) {
+      if (p < P && r == 0) {
         if (op.c1 >= 0) u += ua[size_t(op.c1) * Ppad + p];
         if (op.c2 >= 0) u += ua[size_t(op.c2) * Ppad + p];
       }
       if (op.flags & kOpRescale) {
-        double m = mx[v];
-        for (int o = 1; o < 16; o <<= 1) m = fmax(m, __shfl_xor(m, o));
+        const double m = mx[v];
         if (p < P) {
           if constexpr (kF32) {
             if (m < 0x1p-32 && m > 0.0) {
               const int e = 1 - __builtin_amdgcn_frexp_exp(m);
-              for (int r = 0; r < R; r++)
-#pragma unroll
-                for (int t = 0; t < 4; t++) {
-                  T *q = dst + (size_t(p) * R + r) * 64 + 16 * t + lo;
-                  *q = ldexpf(*q, e);
-                }
-              u += e;
-            }
-          } else if (m < c_rescale_below) {
-            if (m < c_reduce_below && lo == 0) atomicOr(flags, m < c_bail_below ? 3 : 1);
-            int idx = 0;
-            while (idx + 1 < kRescaleLen && c_thresh[idx + 1] > m) idx++;
-            const double mult = c_mult[idx];
-            for (int r = 0; r < R; r++)
 #pragma unroll
               for (int t = 0; t < 4; t++) {
                 T *q = dst + (size_t(p) * R + r) * 64 + 16 * t + lo;
-                *q = mul_nc(*q, mult);
+                *q = ldexpf(*q, e);
               }
+              u += e;
+            }
+          } else if (m < c_rescale_below) {
+            if (m < c_reduce_below && lo == 0 && r == 0) atomicOr(flags, m < c_bail_below ? 3 : 1);
+            int idx = 0;
+            while (idx + 1 < kRescaleLen && c_thresh[idx + 1] > m) idx++;
+            const double mult = c_mult[idx];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+              T *q = dst + (size_t(p) * R + r) * 64 + 16 * t + lo;
+              *q = mul_nc(*q, mult);
+            }
             u += c_incr[idx];
           }
         }
       }
-      if (p < P && lo == 0) ua[size_t(op.dest) * Ppad + p] = u;
+      if (p < P && lo == 0 && r == 0) ua[size_t(op.dest) * Ppad + p] = u;
     }
   }
 }
@@ -1454,7 +1472,7 @@ static int launch_nuc_r(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
 
 template <typename T, int R>
 static void launch_codon(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
-  GLE_LAUNCH(I, (k_cla_codon<T, R>), dim3(I->Ppad / 16, fused ? 1 : n), dim3(64), 0,
+  GLE_LAUNCH(I, (k_cla_codon<T, R>), dim3(I->Ppad / 16, fused ? 1 : n), dim3(64 * R), 0,
                      reinterpret_cast<T *>(I->d_cla), I->d_u, I->d_tips, I->d_pmpad, d_ops, I->d_flags, I->Ppad,
                      I->P, fused ? n : 0);
 }
