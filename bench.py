@@ -28,6 +28,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "lnL evals/s and CLA site-updates/s at |ΔlnL|<1e-3 vs FP64; p99 latency"
 HBM_PEAK_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6  # MI355X published FP64 (vector and matrix)
 
 
 def log(*a):
@@ -229,6 +230,23 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def _cla_kinds(parent, n_tips):
+    """(internal-internal, internal-tip, tip-tip) counts over all directional
+    CLAs of the tree (tips are nodes 1..n_tips)."""
+    nb = [[] for _ in range(len(parent))]
+    for n, a in enumerate(parent):
+        if a >= 0:
+            nb[n].append(a)
+            nb[a].append(n)
+    kinds = [0, 0, 0]
+    for a in range(len(parent)):
+        if 1 <= a <= n_tips:
+            continue
+        for b in nb[a]:
+            kinds[sum(1 for c in nb[a] if c != b and 1 <= c <= n_tips)] += 1
+    return kinds
+
+
 def _population(ds, n, seed):
     from garli_amd import sim
     rng = np.random.default_rng(seed)
@@ -302,11 +320,25 @@ def bench_states(args):
         inst.close()
         log("%s sweep %.3f ms, CLA kernels %.1f GB/s" % (tag, ms, res[tag]["cla_kernel_GBs"]))
     if S == 61:
-        flops = ds.n_rates * (2 * 2 * 61 * 61) * n_cla * P  # useful codon CLA flops per sweep
+        # useful codon CLA flops per sweep: R(4S^2+S) per pattern for an
+        # internal-internal update, R(2S^2+S) with one tip child, R S with two
+        n_ii, n_it, n_tt = _cla_kinds(ds.parent, N)
+        flops = ds.n_rates * P * (n_ii * (4 * S * S + S) + n_it * (2 * S * S + S) + n_tt * S)
         for tag in ("fp64", "fp32"):
             res[tag]["codon_mfma_TFLOPs"] = flops / (res[tag]["cla_kernel_ms_per_sweep"] / 1000.0) / 1e12
+        ceiling = E.mfma_calibrate()
+        log("codon CLA MFMA: %.1f TFLOP/s (fp32 form) of %.1f published, %.1f measured register-only ceiling"
+            % (res["fp32"]["codon_mfma_TFLOPs"], PEAK_FP64_TFLOPS, ceiling))
+        roof = {"bound": "mfma", "achieved": res["fp32"]["codon_mfma_TFLOPs"], "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": res["fp32"]["codon_mfma_TFLOPs"] / PEAK_FP64_TFLOPS, "traffic": None,
+                "measured_ceiling": ceiling, "frac_of_measured_ceiling": res["fp32"]["codon_mfma_TFLOPs"] / ceiling,
+                "kernel": "k_cla_codon<float,3>", "useful_flops_per_sweep": flops}
+    else:
+        gbs = res["fp32"]["cla_kernel_GBs"]
+        roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                "traffic": None, "kernel": "k_cla_aa<float,4>"}
     if args.quick:
-        print(json.dumps({"workload": name, "sweeps": res}), flush=True)
+        print(json.dumps({"workload": name, "sweeps": res, "roofline": roof}), flush=True)
         return
     pop = _population(ds, 64, 11)
     e64, lat64 = _eval_rates(ds, E, pop, E.PREC_FP64)
@@ -325,7 +357,7 @@ def bench_states(args):
            "config": {"workload": name, "taxa": N, "patterns": P, "states": S, "rate_cats": ds.n_rates,
                       "requests": len(pop)},
            "fp64_plain_evals_per_s": e64, "fp32_plain_evals_per_s": e32, "fp32_served_evals_per_s": s32,
-           "served_vs_fp64_plain": s32 / e64, "serve_stats": st, "sweeps": res, "accuracy": acc}
+           "served_vs_fp64_plain": s32 / e64, "serve_stats": st, "sweeps": res, "accuracy": acc, "roofline": roof}
     print(json.dumps(out), flush=True)
 
 

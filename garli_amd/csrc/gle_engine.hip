@@ -1350,6 +1350,21 @@ __global__ __launch_bounds__(256) void k_copy_calibrate(const float4 *__restrict
     dst[i] = src[i];
 }
 
+// FP64 MFMA with register operands only (four independent accumulator
+// chains per wave): the achievable v_mfma_f64_16x16x4_f64 rate, against which
+// the codon kernels' MFMA utilisation is read beside the published peak.
+__global__ __launch_bounds__(256) void k_mfma_calibrate(double *__restrict__ out, int iters) {
+  gle_doublex4 d0 = {0.0, 0.0, 0.0, 0.0}, d1 = d0, d2 = d0, d3 = d0;
+  const double a = threadIdx.x * 1e-3, b = blockIdx.x * 1e-3 + 1.0;
+  for (int i = 0; i < iters; i++) {
+    d0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, d0, 0, 0, 0);
+    d1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, d1, 0, 0, 0);
+    d2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, d2, 0, 0, 0);
+    d3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, d3, 0, 0, 0);
+  }
+  out[size_t(blockIdx.x) * blockDim.x + threadIdx.x] = (d0[0] + d1[1]) + (d2[2] + d3[3]);
+}
+
 // =====================================================================
 // Host-side launchers
 // =====================================================================
@@ -2353,6 +2368,35 @@ extern "C" int gle_copy_calibrate(size_t bytes, int reps, double *ms_per_copy) {
   (void)hipFree(a);
   (void)hipFree(b);
   if (err != hipSuccess) return set_error(GLE_ERR_DEVICE, "calibration copy failed: %s", hipGetErrorString(err));
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_mfma_calibrate(double *tflops) {
+  if (!tflops) return set_error(GLE_ERR_ARG, "bad calibration request");
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc = ensure_device();
+    if (rc) return rc;
+  }
+  constexpr int kBlocks = 2048, kThreads = 256, kIters = 500;  // 8 waves per SIMD on 256 CUs
+  double *out = nullptr;
+  GLE_HIP(hipMalloc(&out, sizeof(double) * kBlocks * kThreads));
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  hipLaunchKernelGGL(k_mfma_calibrate, dim3(kBlocks), dim3(kThreads), 0, nullptr, out, kIters);
+  (void)hipEventRecord(e0, nullptr);
+  hipLaunchKernelGGL(k_mfma_calibrate, dim3(kBlocks), dim3(kThreads), 0, nullptr, out, kIters);
+  (void)hipEventRecord(e1, nullptr);
+  const hipError_t err = hipEventSynchronize(e1);
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  const double flops = double(kBlocks) * (kThreads / 64) * kIters * 4 * 2048.0;
+  *tflops = flops / (double(ms) * 1e-3) / 1e12;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(out);
+  if (err != hipSuccess) return set_error(GLE_ERR_DEVICE, "MFMA calibration failed: %s", hipGetErrorString(err));
   return GLE_SUCCESS;
 }
 

@@ -86,6 +86,7 @@ SIGNATURES = {
     "gle_synchronize": ([C.c_int], C.c_int),
     "gle_set_timing": ([C.c_int, C.c_int], C.c_int),
     "gle_copy_calibrate": ([C.c_size_t, C.c_int, _D], C.c_int),
+    "gle_mfma_calibrate": ([_D], C.c_int),
     "gle_get_kernel_stats": ([C.c_int, C.c_int, C.POINTER(C.c_longlong), _D, _D, C.c_int], C.c_int),
     "gle_tree_create": ([C.c_int, _I, _D, _I], C.c_int),
     "gle_tree_copy": ([C.c_int, _I], C.c_int),
@@ -147,6 +148,13 @@ def init(device=0):
 
 def device_count():
     return lib().gle_device_count()
+
+
+def mfma_calibrate():
+    """Achievable FP64 MFMA TFLOP/s (register operands, 8 waves per SIMD)."""
+    tf = C.c_double(0)
+    _check(lib().gle_mfma_calibrate(C.byref(tf)))
+    return tf.value
 
 
 def copy_calibrate(nbytes, reps=10):

@@ -172,6 +172,10 @@ int gle_set_timing(int h, int enable);
  * the mean ms per copy.  No reference counterpart: it calibrates hardware
  * byte counters and the achievable HBM bandwidth for the roofline. */
 int gle_copy_calibrate(size_t bytes, int reps, double *ms_per_copy);
+/* Achievable FP64 MFMA rate (TFLOP/s) of v_mfma_f64_16x16x4_f64 with
+ * register operands.  No reference counterpart: it calibrates the codon
+ * kernels' MFMA utilisation. */
+int gle_mfma_calibrate(double *tflops);
 int gle_get_kernel_stats(int h, int kind, long long *launches, double *ms, double *bytes, int reset);
 
 /* ---------------------------------------------------------------------------
