@@ -623,21 +623,26 @@ __global__ __launch_bounds__(256) void k_cla_nuc(T *__restrict__ cla, int *__res
 // patterns and wave r takes category r, so the op's P is wave-uniform.  The
 // products accumulate input state by input state: for q = 0..19 every lane
 // adds P[f][q] * x[q] to its 20 sums, with column q of P (contiguous in the
-// transposed copy pmT) arriving by scalar loads.  One child at a time, the
-// live registers are its 20 sums plus the other child's; the sums run in the
+// transposed copy pmT) arriving by scalar loads in the FP32 form and from an
+// LDS copy of the wave's two matrices in the FP64 form (whose longer
+// mul-then-add chains hide the LDS latency better than scalar-cache misses).
+// One child at a time, the live registers are its 20 sums plus the other
+// child's; the sums run in the
 // reference's q order, so the FP64 form keeps its mul-then-add sequence
 // (tree.cpp:7843-7910, 8082-8160, 8537-8599).  Each output is stored as
 // formed; a rescale (per pattern, over the R waves through a small LDS table)
 // re-reads and scales what the lane stored.
 // =====================================================================
 template <typename T, int R>
-__global__ __launch_bounds__(64 * R) void k_cla_aa(T *__restrict__ cla, int *__restrict__ ua,
+__global__ __launch_bounds__(64 * R, sizeof(T) == 4 ? 1 : 3) void k_cla_aa(T *__restrict__ cla, int *__restrict__ ua,
                                                     const unsigned char *__restrict__ tips,
                                                     const double *__restrict__ pmT, const DevClaOp *__restrict__ ops,
                                                     int *__restrict__ flags, int Ppad, int P, int fused_n) {
   constexpr int S = 20;
   constexpr bool kF32 = sizeof(T) == 4;
+  constexpr bool kLdsP = !kF32;
   __shared__ double smax[R][64];
+  __shared__ double sPT[kLdsP ? R : 1][2][kLdsP ? S * S : 2];  // FP64: the wave's P^T of both children
   const int lane = threadIdx.x & 63;
   const int r = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
   const int p = blockIdx.x * 64 + lane;
@@ -645,12 +650,15 @@ __global__ __launch_bounds__(64 * R) void k_cla_aa(T *__restrict__ cla, int *__r
   const size_t slot = size_t(Ppad) * R * S, off = aa_off(p, r, R);
   const int first = fused_n ? 0 : blockIdx.y, last = fused_n ? fused_n : blockIdx.y + 1;
   // acc[f] = (P x)[f] for one child: a CLA (input state by input state) or a tip
-  auto child = [&](int c, int pmslot, double(&acc)[S]) {
-    const double *PT = pmT + (size_t(pmslot) * 3 * R + r) * S * S;  // PT[q * S + f] = P[f][q]
+  auto child = [&](int c, int pmslot, int ci, double(&acc)[S]) {
+    // PT[q * S + f] = P[f][q]
+    const double *PT = kLdsP ? sPT[kLdsP ? r : 0][ci] : pmT + (size_t(pmslot) * 3 * R + r) * S * S;
     if (c >= 0) {
       T x[S];
 #pragma unroll
-      for (int q = 0; q < S; q++) x[q] = live ? cla[c * slot + off + 64 * q] : T(0);
+      for (int q = 0; q < S; q++) x[q] = cla[c * slot + off + 64 * q];  // p < Ppad: inside the slot
+#pragma unroll
+      for (int q = 0; q < S; q++) x[q] = live ? x[q] : T(0);
 #pragma unroll
       for (int f = 0; f < S; f++) acc[f] = 0.0;
 #pragma unroll
@@ -668,7 +676,8 @@ __global__ __launch_bounds__(64 * R) void k_cla_aa(T *__restrict__ cla, int *__r
         }
       }
     } else {
-      const int t = live ? tips[size_t(-c - 1) * Ppad + p] : S;
+      const int tp = tips[size_t(-c - 1) * Ppad + p];
+      const int t = live ? tp : S;
       if (t < S) {
         const double *row = PT + t * S;  // P[f][t] for f = 0..19
 #pragma unroll
@@ -682,14 +691,24 @@ __global__ __launch_bounds__(64 * R) void k_cla_aa(T *__restrict__ cla, int *__r
   for (int oi = first; oi < last; ++oi) {
     if (oi != first) __syncthreads();  // smax of the previous op has been read
     const DevClaOp op = ops[oi];
+    if constexpr (kLdsP) {
+      const double2 *g1 = reinterpret_cast<const double2 *>(pmT + (size_t(op.pm1) * 3 * R + r) * S * S);
+      const double2 *g2 = reinterpret_cast<const double2 *>(pmT + (size_t(op.pm2) * 3 * R + r) * S * S);
+      double2 *l1 = reinterpret_cast<double2 *>(sPT[r][0]), *l2 = reinterpret_cast<double2 *>(sPT[r][1]);
+      for (int k = lane; k < S * S / 2; k += 64) {
+        l1[k] = g1[k];
+        l2[k] = g2[k];
+      }
+      __syncthreads();
+    }
     int u = 0;
     if (live) {
       if (op.c1 >= 0) u += ua[op.c1 * size_t(Ppad) + p];
       if (op.c2 >= 0) u += ua[op.c2 * size_t(Ppad) + p];
     }
     double a1[S], a2[S];
-    child(op.c1, op.pm1, a1);
-    child(op.c2, op.pm2, a2);
+    child(op.c1, op.pm1, 0, a1);
+    child(op.c2, op.pm2, 1, a2);
     T *dst = cla + op.dest * slot + off;
     double m = 0.0;
     unsigned seed = 0;  // FP32 form: dither seed of this (pattern, category) and op
@@ -787,7 +806,8 @@ __device__ __forceinline__ void codon_tip_states(const unsigned char *__restrict
 #pragma unroll
   for (int v = 0; v < 4; v++) {
     const int p = p0 + g + 4 * v;
-    st[v] = p < P ? tip[p] : 61;
+    const int s = tip[p];  // p < Ppad: inside the tip row
+    st[v] = p < P ? s : 61;
   }
 }
 // Output tile t (states 16t .. 16t+15) of Pr . x for the wave's 16 patterns.
@@ -809,7 +829,10 @@ __device__ __forceinline__ gle_doublex4 codon_tip_tile(const double *__restrict_
                                                        double missing) {
   gle_doublex4 d;
 #pragma unroll
-  for (int v = 0; v < 4; v++) d[v] = st[v] < 61 ? Pr[codon_frag(16 * t + lo, st[v])] : missing;
+  for (int v = 0; v < 4; v++) {
+    const double e = Pr[codon_frag(16 * t + lo, st[v] < 61 ? st[v] : 0)];
+    d[v] = st[v] < 61 ? e : missing;
+  }
   return d;
 }
 
