@@ -1151,16 +1151,21 @@ __global__ __launch_bounds__(256) void k_edge_nuc(const T *__restrict__ cla, con
 // layout of k_cla_codon, then each lane weights its (pattern, state) cells by
 // x * pi, the 16 lanes of a pattern row reduce, and the categories combine in
 // order with their weights (GetScorePartialNState / GetDerivsPartialNState,
-// tree.cpp:6431-6480, optimization.cpp:3230-3290).  grid (Ppad/16, n_ops).
+// tree.cpp:6431-6480, optimization.cpp:3230-3290).  grid (Ppad/16, n_ops); a
+// block is R waves over the same 16 patterns and wave r takes category r; the
+// per-category row sums meet in LDS, where wave 0 weights them in category
+// order and finishes the sites.
 template <typename T, int R>
-__global__ __launch_bounds__(64) void k_edge_codon(const T *__restrict__ cla, const int *__restrict__ ua,
+__global__ __launch_bounds__(64 * R) void k_edge_codon(const T *__restrict__ cla, const int *__restrict__ ua,
                                                    const unsigned char *__restrict__ tips,
                                                    const double *__restrict__ pmpad, const double *__restrict__ model,
                                                    const int *__restrict__ counts, const int *__restrict__ cst,
                                                    const DevEdgeOp *__restrict__ ops, double *__restrict__ partials,
                                                    double *__restrict__ site, double pinv, int pinv_in_model,
                                                    int Ppad, int P) {
-  const int l = threadIdx.x, lo = l & 15, g = l >> 4;
+  __shared__ double srow[3][R][16];
+  const int l = threadIdx.x & 63, lo = l & 15, g = l >> 4;
+  const int r = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
   const int p0 = blockIdx.x * 16;
   const size_t slot = size_t(Ppad) * R * 64;
   const DevEdgeOp op = ops[blockIdx.y];
@@ -1169,8 +1174,7 @@ __global__ __launch_bounds__(64) void k_edge_codon(const T *__restrict__ cla, co
   double sL[4] = {0, 0, 0, 0}, s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0};
   int ts[4];
   if (op.child < 0) codon_tip_states(tips, op.child, Ppad, P, g, p0, ts);
-#pragma unroll 1
-  for (int r = 0; r < R; r++) {
+  {
     double acc[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};  // [matrix][v]: sum over this lane's states
     double y[16];
     if (op.child >= 0) codon_load_x(cla, op.child, slot, R, r, lo, g, p0, P, y);
@@ -1202,13 +1206,24 @@ __global__ __launch_bounds__(64) void k_edge_codon(const T *__restrict__ cla, co
         a1 += __shfl_xor(a1, o);
         a2 += __shfl_xor(a2, o);
       }
-      sL[v] += a0 * w[r];
-      s1[v] += a1 * w[r];
-      s2[v] += a2 * w[r];
+      if (lo == 0) {
+        srow[0][r][g + 4 * v] = a0;
+        srow[1][r][g + 4 * v] = a1;
+        srow[2][r][g + 4 * v] = a2;
+      }
     }
   }
+  __syncthreads();
+  if (r == 0 && lo == 0)
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+      for (int q = 0; q < R; q++) {
+        sL[v] += srow[0][q][g + 4 * v] * w[q];
+        s1[v] += srow[1][q][g + 4 * v] * w[q];
+        s2[v] += srow[2][q][g + 4 * v] * w[q];
+      }
   double lnl = 0, g1 = 0, g2 = 0;
-  if (lo == 0) {
+  if (r == 0 && lo == 0) {
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int p = p0 + g + 4 * v;
@@ -1740,7 +1755,7 @@ static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk, bo
 #undef GLE_EDGE_AA
   } else if (I->S == 61) {
 #define GLE_EDGE_CODON(RR)                                                                                        \
-  GLE_LAUNCH(I, (k_edge_codon<T, RR>), grid, dim3(64), 0, a.cla, a.u, a.tips, I->d_pmpad, a.model,              \
+  GLE_LAUNCH(I, (k_edge_codon<T, RR>), grid, dim3(64 * RR), 0, a.cla, a.u, a.tips, I->d_pmpad, a.model,              \
                      a.counts, a.cst, d_ops, a.partials, a.site, a.pinv, a.pinv_in_model, a.Ppad, a.P)
     switch (I->R) {
       case 1: GLE_EDGE_CODON(1); break;
