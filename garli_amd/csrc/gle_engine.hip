@@ -176,6 +176,14 @@ Instance *get_instance(int h) {
 int instance_states(const Instance *I) { return I->S; }
 int instance_rates(const Instance *I) { return I->R; }
 int instance_patterns(const Instance *I) { return I->P_orig; }
+double instance_constant_fraction(const Instance *I) {
+  double all = 0.0, cst = 0.0;
+  for (int q = 0; q < I->P_orig; q++) {
+    all += I->h_counts[q];
+    if (I->h_cst[q] >= 0) cst += I->h_counts[q];
+  }
+  return all > 0.0 ? cst / all : 0.0;
+}
 int instance_tips(const Instance *I) { return I->N; }
 int instance_precision(const Instance *I) { return I->prec; }
 unsigned long long instance_generation(const Instance *I) { return I->generation; }

@@ -49,6 +49,7 @@ Instance *get_instance(int h);
 int instance_states(const Instance *);
 int instance_rates(const Instance *);
 int instance_patterns(const Instance *);
+double instance_constant_fraction(const Instance *);  // count-weighted share of constant patterns
 int instance_tips(const Instance *);
 int instance_precision(const Instance *);
 unsigned long long instance_generation(const Instance *);

@@ -1,0 +1,529 @@
+// gle_io.cpp -- alignment ingestion and site-likelihood output.
+//
+// Reads the character data GARLI reads through NCL (GarliReader,
+// garlireader.cpp; SequenceData::CreateMatrixFromNCL, sequencedata.cpp):
+// NEXUS DATA / CHARACTERS blocks (interleaved or not, MISSING / GAP /
+// MATCHCHAR symbols, {..} and (..) state sets, quoted and underscore names,
+// a TAXA block's labels), the default (starred) EXSET and WTSET of an
+// ASSUMPTIONS block, relaxed PHYLIP (sequential or interleaved) and FASTA.
+// Rows keep the file's symbols (upper-cased, matchchar resolved, a state set
+// turned into its IUPAC code for nucleotides); each character carries a
+// weight: the WTSET weight, or 0 when the EXSET excludes it.
+//
+// Writes <ofprefix>.sitelikes.log as Tree::OutputSiteLikelihoods and the
+// score-only loop of Population write it (tree.cpp:5501-5565,
+// population.cpp:3459-3478, 3702-3727): a header, one line per character
+// (site number and -lnL, or "-" for a character that is excluded or carries
+// no data), then the tree's line with its -lnL.
+#include <algorithm>
+#include <cctype>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iterator>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "gle_internal.h"
+
+namespace gle {
+
+struct Alignment {
+  std::vector<std::string> names, rows;
+  std::vector<int> weights;
+  int datatype = GLE_DATA_DNA;
+};
+
+static std::mutex g_aln_mu;
+static std::vector<std::unique_ptr<Alignment>> g_alns;
+
+static std::string upper(std::string s) {
+  for (char &c : s) c = char(std::toupper(static_cast<unsigned char>(c)));
+  return s;
+}
+
+// IUPAC code of a set of nucleotide symbols ("AG" -> 'R'); '?' if not DNA.
+static char iupac_of_set(const std::string &set) {
+  int mask = 0;
+  for (char c : set) {
+    const int d = nuc_datum(c);
+    if (d < 0) return '?';
+    mask |= d;
+  }
+  static const char kCode[16] = {'-', 'A', 'C', 'M', 'G', 'R', 'S', 'V', 'T', 'W', 'Y', 'H', 'K', 'D', 'B', 'N'};
+  return kCode[mask & 15];
+}
+
+// ------------------------------------------------------------------ NEXUS
+class NexusText {
+ public:
+  explicit NexusText(std::string t) : s_(std::move(t)) {}
+  bool eof() {
+    skip();
+    return p_ >= s_.size();
+  }
+  // next token: a quoted word, a run of word characters, or one punctuation
+  // character; comments are skipped
+  std::string token() {
+    skip();
+    if (p_ >= s_.size()) return "";
+    const char c = s_[p_];
+    if (c == '\'') return quoted();
+    if (std::strchr(";=,(){}:*\"", c)) {
+      ++p_;
+      return std::string(1, c);
+    }
+    size_t b = p_;
+    while (p_ < s_.size() && !std::isspace(static_cast<unsigned char>(s_[p_])) && !std::strchr(";=,(){}:*\"[", s_[p_]))
+      ++p_;
+    return s_.substr(b, p_ - b);
+  }
+  std::string peek() {
+    const size_t save = p_;
+    std::string t = token();
+    p_ = save;
+    return t;
+  }
+  // a taxon name: quoted as written, else underscores become blanks
+  std::string name() {
+    skip();
+    if (p_ < s_.size() && s_[p_] == '\'') return quoted();
+    std::string t = token();
+    std::replace(t.begin(), t.end(), '_', ' ');
+    return t;
+  }
+  // the rest of the statement up to ';' (consumed), as tokens
+  std::vector<std::string> statement() {
+    std::vector<std::string> out;
+    for (std::string t = token(); !t.empty() && t != ";"; t = token()) out.push_back(t);
+    return out;
+  }
+  // sequence symbols up to the end of the line (interleaved) or until n
+  // symbols were read (n > 0); comments and blanks skipped
+  std::string symbols(size_t n, bool to_eol) {
+    std::string out;
+    while (p_ < s_.size()) {
+      const char c = s_[p_];
+      if (c == '\n' || c == '\r') {
+        if (to_eol) {
+          if (!out.empty()) break;
+          ++p_;
+          continue;
+        }
+        ++p_;
+        continue;
+      }
+      if (std::isspace(static_cast<unsigned char>(c))) {
+        ++p_;
+        continue;
+      }
+      if (c == '[') {
+        comment();
+        continue;
+      }
+      if (c == ';') break;
+      if (n && out.size() >= n) break;
+      if (c == '{' || c == '(') {
+        const char close = c == '{' ? '}' : ')';
+        std::string set;
+        ++p_;
+        while (p_ < s_.size() && s_[p_] != close) {
+          if (!std::isspace(static_cast<unsigned char>(s_[p_])) && s_[p_] != ',') set += s_[p_];
+          ++p_;
+        }
+        ++p_;
+        out += iupac_of_set(upper(set));
+        continue;
+      }
+      out += c;
+      ++p_;
+    }
+    return out;
+  }
+ private:
+  void comment() {
+    int depth = 0;
+    do {
+      if (s_[p_] == '[') depth++;
+      else if (s_[p_] == ']') depth--;
+      ++p_;
+    } while (p_ < s_.size() && depth > 0);
+  }
+  void skip() {
+    while (p_ < s_.size()) {
+      if (std::isspace(static_cast<unsigned char>(s_[p_]))) ++p_;
+      else if (s_[p_] == '[') comment();
+      else break;
+    }
+  }
+  std::string quoted() {
+    std::string out;
+    ++p_;
+    while (p_ < s_.size()) {
+      if (s_[p_] == '\'') {
+        if (p_ + 1 < s_.size() && s_[p_ + 1] == '\'') {
+          out += '\'';
+          p_ += 2;
+          continue;
+        }
+        ++p_;
+        break;
+      }
+      out += s_[p_++];
+    }
+    return out;
+  }
+  std::string s_;
+  size_t p_ = 0;
+};
+
+// "key = value" pairs of a DIMENSIONS / FORMAT statement (upper-cased keys)
+static std::map<std::string, std::string> settings(const std::vector<std::string> &toks) {
+  std::map<std::string, std::string> out;
+  for (size_t i = 0; i < toks.size(); i++) {
+    const std::string k = upper(toks[i]);
+    if (i + 2 < toks.size() && toks[i + 1] == "=") {
+      out[k] = toks[i + 2];
+      i += 2;
+    } else {
+      out[k] = "";
+    }
+  }
+  return out;
+}
+
+// Character list "1-749 751-1008 1930-. 4" (1-based, '.' = last) -> indices.
+static int char_list(const std::vector<std::string> &toks, size_t b, size_t e, int nchar, std::vector<int> &out) {
+  auto num = [&](const std::string &t, int &v) {
+    if (t == ".") {
+      v = nchar;
+      return true;
+    }
+    char *end = nullptr;
+    const long x = std::strtol(t.c_str(), &end, 10);
+    if (!end || *end || x < 1 || x > nchar) return false;
+    v = int(x);
+    return true;
+  };
+  for (size_t i = b; i < e; i++) {
+    const std::string &t = toks[i];
+    const size_t dash = t.find('-', 1);
+    int a, z;
+    if (dash != std::string::npos) {
+      if (!num(t.substr(0, dash), a) || !num(t.substr(dash + 1), z)) return -1;
+    } else if (i + 2 < e && toks[i + 1] == "-") {  // "1 - 5" written apart
+      if (!num(t, a) || !num(toks[i + 2], z)) return -1;
+      i += 2;
+    } else {
+      if (!num(t, a)) return -1;
+      z = a;
+    }
+    for (int k = a; k <= z; k++) out.push_back(k - 1);
+  }
+  return 0;
+}
+
+static int read_nexus(const std::string &text, Alignment &A) {
+  NexusText tx(text);
+  if (upper(tx.token()) != "#NEXUS") return set_error(GLE_ERR_ARG, "not a NEXUS file");
+  std::vector<std::string> taxa;
+  std::vector<std::string> excluded_sets, weight_sets;  // starred statements, applied after the matrix
+  while (!tx.eof()) {
+    if (upper(tx.token()) != "BEGIN") continue;
+    const std::string block = upper(tx.token());
+    tx.statement();
+    if (block == "TAXA") {
+      for (;;) {
+        const std::string cmd = upper(tx.token());
+        if (cmd.empty() || cmd == "END" || cmd == "ENDBLOCK") break;
+        if (cmd == "TAXLABELS") {
+          while (tx.peek() != ";" && !tx.peek().empty()) taxa.push_back(tx.name());
+          tx.token();
+        } else {
+          tx.statement();
+        }
+      }
+      if (upper(tx.peek()) == ";") tx.token();
+    } else if (block == "DATA" || block == "CHARACTERS") {
+      int ntax = int(taxa.size()), nchar = 0;
+      char missing = '?', gap = '-', match = 0;
+      bool interleave = false;
+      for (;;) {
+        const std::string cmd = upper(tx.token());
+        if (cmd.empty()) return set_error(GLE_ERR_ARG, "NEXUS: unterminated %s block", block.c_str());
+        if (cmd == "END" || cmd == "ENDBLOCK") break;
+        if (cmd == "DIMENSIONS") {
+          auto st = settings(tx.statement());
+          if (st.count("NTAX")) ntax = std::atoi(st["NTAX"].c_str());
+          if (st.count("NCHAR")) nchar = std::atoi(st["NCHAR"].c_str());
+        } else if (cmd == "FORMAT") {
+          auto st = settings(tx.statement());
+          if (st.count("DATATYPE")) {
+            const std::string d = upper(st["DATATYPE"]);
+            A.datatype = (d == "PROTEIN") ? GLE_DATA_PROTEIN
+                         : (d == "DNA" || d == "RNA" || d == "NUCLEOTIDE") ? GLE_DATA_DNA
+                                                                           : GLE_DATA_OTHER;
+          }
+          if (st.count("MISSING") && !st["MISSING"].empty()) missing = st["MISSING"][0];
+          if (st.count("GAP") && !st["GAP"].empty()) gap = st["GAP"][0];
+          if (st.count("MATCHCHAR") && !st["MATCHCHAR"].empty()) match = st["MATCHCHAR"][0];
+          if (st.count("INTERLEAVE")) interleave = upper(st["INTERLEAVE"]) != "NO";
+        } else if (cmd == "MATRIX") {
+          if (ntax <= 0 || nchar <= 0) return set_error(GLE_ERR_ARG, "NEXUS: MATRIX before DIMENSIONS");
+          A.names.assign(ntax, "");
+          A.rows.assign(ntax, "");
+          std::map<std::string, int> index;
+          for (int i = 0; i < int(taxa.size()) && i < ntax; i++) index[taxa[i]] = i;
+          int next = 0;
+          while (upper(tx.peek()) != ";") {
+            if (tx.eof()) return set_error(GLE_ERR_ARG, "NEXUS: unterminated MATRIX");
+            const std::string nm = tx.name();
+            int i;
+            auto it = index.find(nm);
+            if (it != index.end()) {
+              i = it->second;
+            } else {
+              if (next >= ntax) return set_error(GLE_ERR_ARG, "NEXUS: more taxa than NTAX (%s)", nm.c_str());
+              i = next++;
+              index[nm] = i;
+            }
+            A.names[i] = nm;
+            std::string &row = A.rows[i];
+            row += interleave ? tx.symbols(0, true) : tx.symbols(size_t(nchar) - row.size(), false);
+          }
+          tx.token();  // ';'
+          for (int i = 0; i < ntax; i++) {
+            std::string &row = A.rows[i];
+            if (int(row.size()) != nchar)
+              return set_error(GLE_ERR_ARG, "NEXUS: taxon %s has %d characters, NCHAR is %d", A.names[i].c_str(),
+                               int(row.size()), nchar);
+            for (int k = 0; k < nchar; k++) {
+              char c = row[k];
+              if (match && c == match) {
+                if (i == 0) return set_error(GLE_ERR_ARG, "NEXUS: matchchar in the first taxon");
+                c = A.rows[0][k];
+              }
+              if (c == missing) c = '?';
+              else if (c == gap) c = '-';
+              row[k] = char(std::toupper(static_cast<unsigned char>(c)));
+            }
+          }
+        } else {
+          tx.statement();
+        }
+      }
+      if (upper(tx.peek()) == ";") tx.token();
+    } else if (block == "ASSUMPTIONS" || block == "SETS") {
+      for (;;) {
+        const std::string cmd = upper(tx.token());
+        if (cmd.empty() || cmd == "END" || cmd == "ENDBLOCK") break;
+        std::vector<std::string> st = tx.statement();
+        if ((cmd == "EXSET" || cmd == "WTSET") && !st.empty() && st[0] == "*") {
+          std::string joined;
+          for (auto &t : st) joined += t + "\x1f";
+          (cmd == "EXSET" ? excluded_sets : weight_sets).push_back(joined);
+        }
+      }
+      if (upper(tx.peek()) == ";") tx.token();
+    } else {
+      for (std::string t = upper(tx.token()); !t.empty() && t != "END" && t != "ENDBLOCK"; t = upper(tx.token())) {
+      }
+      if (upper(tx.peek()) == ";") tx.token();
+    }
+  }
+  if (A.rows.empty()) return set_error(GLE_ERR_ARG, "NEXUS: no character matrix");
+  const int nchar = int(A.rows[0].size());
+  A.weights.assign(nchar, 1);
+  auto split = [](const std::string &j) {
+    std::vector<std::string> v;
+    size_t b = 0;
+    for (size_t e; (e = j.find('\x1f', b)) != std::string::npos; b = e + 1) v.push_back(j.substr(b, e - b));
+    return v;
+  };
+  for (const std::string &j : weight_sets) {  // wtset * name [(vector)] = w: list, w: list ...
+    std::vector<std::string> t = split(j);
+    size_t eq = std::find(t.begin(), t.end(), "=") - t.begin();
+    if (eq >= t.size()) return set_error(GLE_ERR_ARG, "NEXUS: bad WTSET");
+    bool vector = false;
+    for (size_t k = 0; k < eq; k++) vector |= upper(t[k]) == "VECTOR";
+    if (vector) {
+      if (t.size() - eq - 1 != size_t(nchar)) return set_error(GLE_ERR_ARG, "NEXUS: WTSET vector length");
+      for (int k = 0; k < nchar; k++) A.weights[k] = std::atoi(t[eq + 1 + k].c_str());
+      continue;
+    }
+    size_t i = eq + 1;
+    while (i < t.size()) {
+      if (i + 1 >= t.size() || t[i + 1] != ":") return set_error(GLE_ERR_ARG, "NEXUS: bad WTSET near %s", t[i].c_str());
+      const int w = std::atoi(t[i].c_str());
+      size_t e = i + 2;
+      while (e < t.size() && t[e] != ",") e++;
+      std::vector<int> sites;
+      if (char_list(t, i + 2, e, nchar, sites)) return set_error(GLE_ERR_ARG, "NEXUS: bad WTSET character list");
+      for (int s : sites) A.weights[s] = w;
+      i = e + 1;
+    }
+  }
+  for (const std::string &j : excluded_sets) {  // exset * name = list
+    std::vector<std::string> t = split(j);
+    size_t eq = std::find(t.begin(), t.end(), "=") - t.begin();
+    std::vector<int> sites;
+    if (eq >= t.size() || char_list(t, eq + 1, t.size(), nchar, sites))
+      return set_error(GLE_ERR_ARG, "NEXUS: bad EXSET");
+    for (int s : sites) A.weights[s] = 0;
+  }
+  return GLE_SUCCESS;
+}
+
+// ------------------------------------------------------------- PHYLIP, FASTA
+static int guess_type(Alignment &A) {
+  bool dna = true;
+  for (auto &r : A.rows)
+    for (char c : r)
+      if (c != '?' && c != '-' && nuc_datum(c) < 0) dna = false;
+  A.datatype = dna ? GLE_DATA_DNA : GLE_DATA_PROTEIN;
+  return GLE_SUCCESS;
+}
+
+static std::string clean(const std::string &s) {
+  std::string out;
+  for (char c : s)
+    if (!std::isspace(static_cast<unsigned char>(c))) out += char(std::toupper(static_cast<unsigned char>(c)));
+  return out;
+}
+
+static int read_phylip(const std::string &text, Alignment &A) {
+  std::istringstream in(text);
+  int ntax = 0, nchar = 0;
+  if (!(in >> ntax >> nchar) || ntax < 1 || nchar < 1) return set_error(GLE_ERR_ARG, "PHYLIP: bad header");
+  std::string line;
+  std::getline(in, line);
+  std::vector<std::string> lines;
+  while (std::getline(in, line)) {
+    if (clean(line).empty()) continue;
+    lines.push_back(line);
+  }
+  if (int(lines.size()) < ntax) return set_error(GLE_ERR_ARG, "PHYLIP: %d taxa expected", ntax);
+  A.names.resize(ntax);
+  A.rows.resize(ntax);
+  size_t li = 0;
+  for (int i = 0; i < ntax; i++, li++) {  // relaxed: the name is the first word
+    std::istringstream ls(lines[li]);
+    ls >> A.names[i];
+    std::string rest;
+    std::getline(ls, rest);
+    A.rows[i] = clean(rest);
+  }
+  for (int i = 0; li < lines.size(); li++, i = (i + 1) % ntax) A.rows[i] += clean(lines[li]);  // interleaved
+  for (int i = 0; i < ntax; i++)
+    if (int(A.rows[i].size()) != nchar)
+      return set_error(GLE_ERR_ARG, "PHYLIP: taxon %s has %d characters, not %d", A.names[i].c_str(),
+                       int(A.rows[i].size()), nchar);
+  A.weights.assign(nchar, 1);
+  return guess_type(A);
+}
+
+static int read_fasta(const std::string &text, Alignment &A) {
+  std::istringstream in(text);
+  std::string line;
+  while (std::getline(in, line)) {
+    if (!line.empty() && line[0] == '>') {
+      std::string nm = line.substr(1);
+      nm.erase(0, nm.find_first_not_of(" \t"));
+      nm.erase(nm.find_last_not_of(" \t\r") + 1);
+      A.names.push_back(nm);
+      A.rows.emplace_back();
+    } else if (!A.rows.empty()) {
+      A.rows.back() += clean(line);
+    }
+  }
+  if (A.rows.empty()) return set_error(GLE_ERR_ARG, "FASTA: no sequences");
+  for (auto &r : A.rows)
+    if (r.size() != A.rows[0].size()) return set_error(GLE_ERR_ARG, "FASTA: sequences differ in length");
+  A.weights.assign(A.rows[0].size(), 1);
+  return guess_type(A);
+}
+
+static Alignment *get_aln(int h) {
+  std::lock_guard<std::mutex> lk(g_aln_mu);
+  if (h < 0 || h >= int(g_alns.size())) return nullptr;
+  return g_alns[h].get();
+}
+
+}  // namespace gle
+
+using namespace gle;
+
+extern "C" int gle_alignment_read(const char *path, int *handle) {
+  if (!path || !handle) return set_error(GLE_ERR_ARG, "null argument");
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return set_error(GLE_ERR_ARG, "cannot open %s", path);
+  std::string text((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+  auto A = std::make_unique<Alignment>();
+  size_t b = text.find_first_not_of(" \t\r\n");
+  if (b == std::string::npos) return set_error(GLE_ERR_ARG, "%s is empty", path);
+  int rc;
+  if (text[b] == '>') rc = read_fasta(text, *A);
+  else if (upper(text.substr(b, 6)) == "#NEXUS") rc = read_nexus(text.substr(b), *A);
+  else rc = read_phylip(text, *A);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_aln_mu);
+  g_alns.push_back(std::move(A));
+  *handle = int(g_alns.size()) - 1;
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_alignment_dims(int h, int *ntax, int *nchar, int *datatype) {
+  Alignment *A = get_aln(h);
+  if (!A || !ntax || !nchar || !datatype) return set_error(GLE_ERR_ARG, "bad alignment handle or argument");
+  *ntax = int(A->rows.size());
+  *nchar = A->rows.empty() ? 0 : int(A->rows[0].size());
+  *datatype = A->datatype;
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_alignment_taxon(int h, int i, char *name, int cap) {
+  Alignment *A = get_aln(h);
+  if (!A || !name || cap < 1 || i < 0 || i >= int(A->names.size())) return set_error(GLE_ERR_ARG, "bad argument");
+  std::snprintf(name, size_t(cap), "%s", A->names[i].c_str());
+  return int(A->names[i].size());
+}
+
+extern "C" int gle_alignment_row(int h, int i, char *row) {
+  Alignment *A = get_aln(h);
+  if (!A || !row || i < 0 || i >= int(A->rows.size())) return set_error(GLE_ERR_ARG, "bad argument");
+  std::memcpy(row, A->rows[i].data(), A->rows[i].size());
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_alignment_weights(int h, int *weights) {
+  Alignment *A = get_aln(h);
+  if (!A || !weights) return set_error(GLE_ERR_ARG, "bad argument");
+  std::copy(A->weights.begin(), A->weights.end(), weights);
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_alignment_free(int h) {
+  std::lock_guard<std::mutex> lk(g_aln_mu);
+  if (h < 0 || h >= int(g_alns.size()) || !g_alns[h]) return set_error(GLE_ERR_ARG, "bad alignment handle");
+  g_alns[h].reset();
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_write_sitelikes(const char *path, int append, int tree_num, double lnl, int nchar,
+                                   const double *site_lnl, const int *included) {
+  if (!path || nchar < 0 || (nchar > 0 && (!site_lnl || !included))) return set_error(GLE_ERR_ARG, "bad argument");
+  std::FILE *f = std::fopen(path, append ? "a" : "w");
+  if (!f) return set_error(GLE_ERR_ARG, "cannot write %s", path);
+  if (!append) std::fputs("Tree\t-lnL\tSite\t-lnL\n", f);
+  for (int s = 0; s < nchar; s++) {
+    if (included[s]) std::fprintf(f, "\t\t%d\t%.8f\n", s + 1, -site_lnl[s]);
+    else std::fprintf(f, "\t\t%d\t-\n", s + 1);
+  }
+  std::fprintf(f, "%d\t%.12g\n", tree_num, -lnl);
+  const bool ok = std::fclose(f) == 0;
+  return ok ? GLE_SUCCESS : set_error(GLE_ERR_ARG, "write to %s failed", path);
+}

@@ -574,3 +574,182 @@ extern "C" int gle_tree_set_schedule(int h, int mode) {
   T->schedule = mode;
   return GLE_SUCCESS;
 }
+
+// ------------------------------------------------------------------ GTR+I+G
+// Model parameters on a fixed tree: the nst=6 nucleotide model of the
+// reference (Model::UpdateQMat + CalcEigenStuff, model.cpp:233-294, 827-856;
+// DiscreteGamma, :2170-2201; pinv in the model, model.h:1733-1740) and its
+// optimisation on a fixed topology (Population::OptimizeInputOnly ->
+// BetterFinalOptimization, population.cpp:2566; OptimizeRelativeNucRates,
+// tree.cpp:7307; OptimizeBoundedParameter, optimization.cpp:332): rounds of
+// one-dimensional maximisations, one parameter at a time, with a Newton pass
+// over all branches, until a round gains less than the precision.  Each
+// trial value is a full model push and rescore on the device.
+
+static int apply_gtr(Tree &t, const gle_gtr_params &m) {
+  const int R = instance_rates(t.I);
+  if (instance_states(t.I) != 4) return set_error(GLE_ERR_UNSUPPORTED, "GTR parameters need a nucleotide instance");
+  double V[16], W[16], lam[4];
+  int rc = gle_model_gtr(m.pi, m.rel, V, W, lam);
+  if (rc) return rc;
+  std::vector<double> rates(R, 1.0), mult(R), w(R);
+  if (R > 1 && (rc = discrete_gamma(R, m.alpha, rates.data()))) return rc;
+  const double pinv = m.pinv_in_model ? m.pinv : 0.0;
+  for (int r = 0; r < R; r++) {
+    mult[r] = m.pinv_in_model ? rates[r] / (1.0 - pinv) : rates[r];
+    w[r] = (1.0 - pinv) / R;
+  }
+  std::vector<int> eor(R, 0);
+  if ((rc = gle_set_eigen(t.inst_h, 0, V, W, lam)) || (rc = gle_set_category_rates(t.inst_h, eor.data(), mult.data())) ||
+      (rc = gle_set_model(t.inst_h, m.pi, w.data(), pinv, m.pinv_in_model)))
+    return rc;
+  sync_generation(t);
+  return GLE_SUCCESS;
+}
+
+// Brent's one-dimensional minimiser (golden section with parabolic steps,
+// Brent 1973, ch. 5) applied to -f on [a, b] from x; returns the best point
+// and its value in *xbest, *fbest (f maximised).
+template <typename F>
+static int brent_max(F &&f, double a, double b, double x, double xtol, int max_eval, double *xbest, double *fbest) {
+  const double cgold = 0.3819660112501051;
+  double fx;
+  int rc = f(x, &fx);
+  if (rc) return rc;
+  fx = -fx;
+  double v = x, w = x, fv = fx, fw = fx, d = 0.0, e = 0.0;
+  for (int it = 0; it < max_eval; it++) {
+    const double xm = 0.5 * (a + b), tol1 = xtol * std::fabs(x) + 1e-10, tol2 = 2.0 * tol1;
+    if (std::fabs(x - xm) <= tol2 - 0.5 * (b - a)) break;
+    bool golden = true;
+    if (std::fabs(e) > tol1) {
+      double r = (x - w) * (fx - fv), q = (x - v) * (fx - fw), pp = (x - v) * q - (x - w) * r;
+      q = 2.0 * (q - r);
+      if (q > 0.0) pp = -pp;
+      q = std::fabs(q);
+      const double etemp = e;
+      e = d;
+      if (!(std::fabs(pp) >= std::fabs(0.5 * q * etemp) || pp <= q * (a - x) || pp >= q * (b - x))) {
+        d = pp / q;
+        const double u = x + d;
+        if (u - a < tol2 || b - u < tol2) d = xm >= x ? tol1 : -tol1;
+        golden = false;
+      }
+    }
+    if (golden) {
+      e = (x >= xm ? a : b) - x;
+      d = cgold * e;
+    }
+    const double u = std::fabs(d) >= tol1 ? x + d : x + (d > 0 ? tol1 : -tol1);
+    double fu;
+    if ((rc = f(u, &fu))) return rc;
+    fu = -fu;
+    if (fu <= fx) {
+      (u >= x ? a : b) = x;
+      v = w;
+      fv = fw;
+      w = x;
+      fw = fx;
+      x = u;
+      fx = fu;
+    } else {
+      (u < x ? a : b) = u;
+      if (fu <= fw || w == x) {
+        v = w;
+        fv = fw;
+        w = u;
+        fw = fu;
+      } else if (fu <= fv || v == x || v == w) {
+        v = u;
+        fv = fu;
+      }
+    }
+  }
+  *xbest = x;
+  *fbest = -fx;
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_tree_set_gtr(int h, const gle_gtr_params *m) {
+  if (!m) return set_error(GLE_ERR_ARG, "null model");
+  GET_TREE(h);
+  return apply_gtr(*T, *m);
+}
+
+extern "C" int gle_tree_optimize_gtr(int h, gle_gtr_params *m, int which, double prec, double *lnl) {
+  if (!m || !lnl || !(prec > 0.0)) return set_error(GLE_ERR_ARG, "bad argument");
+  GET_TREE(h);
+  Tree &t = *T;
+  const int R = instance_rates(t.I);
+  gle_gtr_params cur = *m;
+  int rc = apply_gtr(t, cur);
+  if (rc) return rc;
+  double best;
+  if ((rc = tree_score(t, &best))) return rc;
+  // one coordinate: value through `set`, searched on a transformed axis
+  // (log for rates and the shape, log-ratio to T for frequencies, direct for
+  // pinv), at most a factor 5 from the current value per round as the
+  // reference limits a pass (maxChangeProportion, tree.cpp:7330)
+  auto line = [&](double x0, double lo, double hi, auto &&set) -> int {
+    auto f = [&](double x, double *val) -> int {
+      gle_gtr_params trial = cur;
+      set(trial, x);
+      int r2 = apply_gtr(t, trial);
+      if (r2) return r2;
+      return tree_score(t, val);
+    };
+    double xb, fb;
+    int r2 = brent_max(f, lo, hi, x0, 1e-5, 60, &xb, &fb);
+    if (r2) return r2;
+    if (fb > best) {
+      set(cur, xb);
+      best = fb;
+    }
+    return apply_gtr(t, cur);  // leave the instance at the accepted point
+  };
+  const double kStep = std::log(5.0);
+  for (int round = 0; round < 200; round++) {
+    const double start = best;
+    if (which & GLE_OPT_BRANCHES) {
+      double imp, l;
+      if ((rc = optimize_all(t, prec, &imp, &l))) return rc;
+      best = l;
+    }
+    if (which & GLE_OPT_RATES)
+      for (int i = 0; i < 5; i++) {
+        const double x0 = std::log(cur.rel[i]);
+        rc = line(x0, std::max(x0 - kStep, std::log(1e-5)), std::min(x0 + kStep, std::log(999.0)),
+                  [i](gle_gtr_params &q, double x) { q.rel[i] = std::exp(x); });
+        if (rc) return rc;
+      }
+    if (which & GLE_OPT_FREQS)
+      for (int i = 0; i < 3; i++) {
+        const double x0 = std::log(cur.pi[i] / cur.pi[3]);
+        rc = line(x0, x0 - kStep, x0 + kStep, [i](gle_gtr_params &q, double x) {
+          double u[4];
+          for (int k = 0; k < 3; k++) u[k] = q.pi[k] / q.pi[3];
+          u[i] = std::exp(x);
+          u[3] = 1.0;
+          const double s = u[0] + u[1] + u[2] + u[3];
+          for (int k = 0; k < 4; k++) q.pi[k] = u[k] / s;
+        });
+        if (rc) return rc;
+      }
+    if ((which & GLE_OPT_ALPHA) && R > 1) {
+      const double x0 = std::log(cur.alpha);
+      rc = line(x0, std::max(x0 - kStep, std::log(0.05)), std::min(x0 + kStep, std::log(999.0)),
+                [](gle_gtr_params &q, double x) { q.alpha = std::exp(x); });
+      if (rc) return rc;
+    }
+    if ((which & GLE_OPT_PINV) && cur.pinv_in_model) {
+      const double hi = 0.999 * instance_constant_fraction(t.I);
+      if (hi > 0.0) {
+        rc = line(std::min(cur.pinv, hi), 0.0, hi, [](gle_gtr_params &q, double x) { q.pinv = x; });
+        if (rc) return rc;
+      }
+    }
+    if (!(best - start >= prec)) break;
+  }
+  *m = cur;
+  return tree_score(t, lnl);
+}

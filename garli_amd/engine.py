@@ -65,6 +65,13 @@ SIGNATURES = {
     "gle_make_ambig_string": ([_U8, C.c_int, _S8], C.c_int),
     "gle_ambig_string_to_masks": ([_S8, C.c_int, C.c_int, _U8], C.c_int),
     "gle_compress_patterns": ([C.c_int, C.c_int, C.c_int, _U8, _U8, _I, _I, _I, _I], C.c_int),
+    "gle_alignment_read": ([C.c_char_p, C.POINTER(C.c_int)], C.c_int),
+    "gle_alignment_dims": ([C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+    "gle_alignment_taxon": ([C.c_int, C.c_int, C.c_char_p, C.c_int], C.c_int),
+    "gle_alignment_row": ([C.c_int, C.c_int, C.c_char_p], C.c_int),
+    "gle_alignment_weights": ([C.c_int, C.POINTER(C.c_int)], C.c_int),
+    "gle_alignment_free": ([C.c_int], C.c_int),
+    "gle_write_sitelikes": ([C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_int, _D, C.POINTER(C.c_int)], C.c_int),
     "gle_create_instance": ([C.POINTER(InstanceDesc), _I], C.c_int),
     "gle_destroy_instance": ([C.c_int], C.c_int),
     "gle_set_tip_ambig_stream": ([C.c_int, C.c_int, _S8, C.c_int], C.c_int),
@@ -103,6 +110,8 @@ SIGNATURES = {
     "gle_tree_rescale_every": ([C.c_int], C.c_int),
     "gle_tree_set_rescale_every": ([C.c_int, C.c_int], C.c_int),
     "gle_tree_set_schedule": ([C.c_int, C.c_int], C.c_int),
+    "gle_tree_set_gtr": ([C.c_int, C.c_void_p], C.c_int),
+    "gle_tree_optimize_gtr": ([C.c_int, C.c_void_p, C.c_int, C.c_double, _D], C.c_int),
     "gle_serve_create": ([_I], C.c_int),
     "gle_serve_destroy": ([C.c_int], C.c_int),
     "gle_serve_submit": ([C.c_int, C.c_int, _I, _D, C.c_int, C.c_double, _I], C.c_int),
@@ -223,6 +232,63 @@ def ambig_string_to_masks(stream, n):
     out = np.zeros(n, dtype=np.uint8)
     _check(lib().gle_ambig_string_to_masks(_p(s, C.c_int8), len(s), n, _p(out, C.c_uint8)), allow=range(1 << 30))
     return out
+
+
+DATA_DNA, DATA_PROTEIN, DATA_OTHER = 0, 1, 2
+
+
+class Alignment:
+    """Character data read by gle_alignment_read (NEXUS, PHYLIP or FASTA):
+    taxon names, rows of symbols and per-character weights (the WTSET weight,
+    0 where the EXSET excludes the character)."""
+
+    def __init__(self, path):
+        h = C.c_int(-1)
+        _check(lib().gle_alignment_read(os.fsencode(path), C.byref(h)))
+        try:
+            nt, nc, dt = C.c_int(), C.c_int(), C.c_int()
+            _check(lib().gle_alignment_dims(h, C.byref(nt), C.byref(nc), C.byref(dt)))
+            self.datatype = dt.value
+            self.names, self.rows = [], []
+            buf = C.create_string_buffer(4096)
+            row = C.create_string_buffer(nc.value + 1)
+            for i in range(nt.value):
+                _check(lib().gle_alignment_taxon(h, i, buf, len(buf)), allow=range(1 << 30))
+                self.names.append(buf.value.decode())
+                _check(lib().gle_alignment_row(h, i, row))
+                self.rows.append(row.raw[:nc.value].decode())
+            self.weights = np.zeros(nc.value, dtype=np.int32)
+            _check(lib().gle_alignment_weights(h, _p(self.weights, C.c_int)))
+        finally:
+            lib().gle_alignment_free(h)
+
+    @property
+    def n_chars(self):
+        return len(self.weights)
+
+    def nucleotide_patterns(self):
+        """Compressed nucleotide patterns with the weights applied:
+        (patterns [ntax][P], counts, const_states, last_const, pattern of each
+        character or -1 where it is excluded or holds no data)."""
+        keep = np.nonzero(self.weights > 0)[0]
+        codes = np.array([[nuc_char_to_datum(r[k]) for k in keep] for r in self.rows], dtype=np.int32)
+        if (codes < 0).any():
+            raise GleError(-1, "symbol outside the nucleotide alphabet")
+        pats, counts, cs, lc, s2p = compress_patterns(codes.astype(np.uint8), 4)
+        w = np.zeros(pats.shape[1], dtype=np.int64)
+        np.add.at(w, s2p[s2p >= 0], self.weights[keep][s2p >= 0])
+        site = np.full(self.n_chars, -1, dtype=np.int32)
+        site[keep] = s2p
+        return pats, w.astype(np.int32), cs, lc, site
+
+
+def write_sitelikes(path, lnl, site_lnl, included, tree_num=1, append=False):
+    """<ofprefix>.sitelikes.log in GARLI's format (per character -lnL or "-",
+    then the tree's -lnL)."""
+    s = np.ascontiguousarray(site_lnl, dtype=np.float64)
+    inc = np.ascontiguousarray(included, dtype=np.int32)
+    _check(lib().gle_write_sitelikes(os.fsencode(path), int(append), int(tree_num), float(lnl), len(s),
+                                     _p(s, C.c_double), _p(inc, C.c_int)))
 
 
 def compress_patterns(matrix, max_states):
@@ -446,6 +512,57 @@ class Tree:
 
     def set_schedule(self, mode):
         _check(lib().gle_tree_set_schedule(self.h, int(mode)))
+
+    def set_gtr(self, params):
+        """Set the instance to GTR+I+G parameters (a GtrParams)."""
+        c = params.to_c()
+        _check(lib().gle_tree_set_gtr(self.h, C.byref(c)))
+
+    def optimize_gtr(self, params, which=None, precision=0.01):
+        """Maximise lnL over the selected model parameters (and all branch
+        lengths with OPT_BRANCHES) on the fixed topology; returns
+        (lnL, optimised GtrParams)."""
+        if which is None:
+            which = OPT_RATES | OPT_FREQS | OPT_ALPHA | OPT_PINV | OPT_BRANCHES
+        c = params.to_c()
+        out = C.c_double(0)
+        _check(lib().gle_tree_optimize_gtr(self.h, C.byref(c), int(which), float(precision), C.byref(out)))
+        return out.value, GtrParams.from_c(c)
+
+
+OPT_RATES, OPT_FREQS, OPT_ALPHA, OPT_PINV, OPT_BRANCHES = 1, 2, 4, 8, 16
+
+
+class _GtrC(C.Structure):
+    _fields_ = [("rel", C.c_double * 5), ("pi", C.c_double * 4), ("alpha", C.c_double), ("pinv", C.c_double),
+                ("pinv_in_model", C.c_int)]
+
+
+class GtrParams:
+    """GARLI's nst=6 nucleotide model: rel = (AC, AG, AT, CG, CT) with GT = 1,
+    base frequencies, gamma shape and proportion of invariant sites."""
+
+    def __init__(self, rel, pi, alpha=1.0, pinv=0.0, pinv_in_model=False):
+        self.rel = np.asarray(rel, dtype=np.float64).copy()
+        self.pi = np.asarray(pi, dtype=np.float64).copy()
+        self.alpha = float(alpha)
+        self.pinv = float(pinv)
+        self.pinv_in_model = bool(pinv_in_model)
+
+    def to_c(self):
+        c = _GtrC()
+        c.rel[:] = [float(x) for x in self.rel]
+        c.pi[:] = [float(x) for x in self.pi]
+        c.alpha, c.pinv, c.pinv_in_model = self.alpha, self.pinv, int(self.pinv_in_model)
+        return c
+
+    @classmethod
+    def from_c(cls, c):
+        return cls(list(c.rel), list(c.pi), c.alpha, c.pinv, bool(c.pinv_in_model))
+
+    def __repr__(self):
+        return "GtrParams(rel=%s, pi=%s, alpha=%.5f, pinv=%.5f)" % (np.round(self.rel, 5), np.round(self.pi, 5),
+                                                                     self.alpha, self.pinv)
 
 
 REQ_SCORE = 0

@@ -81,6 +81,31 @@ int gle_compress_patterns(int ntax, int nsite, int max_states, const unsigned ch
                           int *site_to_pattern);
 
 /* ---------------------------------------------------------------------------
+ * Ingestion and output (host-side).  They replace the character-data part of
+ * GarliReader / SequenceData::CreateMatrixFromNCL (garlireader.cpp,
+ * sequencedata.cpp) and Tree::OutputSiteLikelihoods (tree.cpp:5501-5565).
+ * ------------------------------------------------------------------------- */
+enum { GLE_DATA_DNA = 0, GLE_DATA_PROTEIN = 1, GLE_DATA_OTHER = 2 };
+/* NEXUS (DATA/CHARACTERS, TAXA labels, the starred EXSET and WTSET of an
+ * ASSUMPTIONS block), relaxed PHYLIP or FASTA, told apart by content. */
+int gle_alignment_read(const char *path, int *handle);
+int gle_alignment_dims(int h, int *ntax, int *nchar, int *datatype);
+/* name of taxon i into name[cap] (NUL-terminated); returns its length */
+int gle_alignment_taxon(int h, int i, char *name, int cap);
+/* the nchar symbols of taxon i (upper case; '?' missing, '-' gap; matchchar
+ * resolved; a NEXUS state set as its IUPAC code) */
+int gle_alignment_row(int h, int i, char *row);
+/* per character: the WTSET weight (1 without one), 0 when the EXSET excludes it */
+int gle_alignment_weights(int h, int *weights);
+int gle_alignment_free(int h);
+/* <ofprefix>.sitelikes.log as GARLI writes it: the header unless append, one
+ * line per character (-lnL, or "-" where included[s] == 0), then
+ * "tree_num <tab> -lnl" (Tree::OutputSiteLikelihoods, tree.cpp:5501-5565;
+ * population.cpp:3459-3478). */
+int gle_write_sitelikes(const char *path, int append, int tree_num, double lnl, int nchar, const double *site_lnl,
+                        const int *included);
+
+/* ---------------------------------------------------------------------------
  * Instance layer.
  * ------------------------------------------------------------------------- */
 typedef struct gle_instance_desc {
@@ -215,6 +240,32 @@ int gle_tree_set_rescale_every(int tree, int every);
 /* Launch schedule for sweeps: 0 = one launch per dependency level,
  * 1 = one fused launch for the whole op list (pattern-local kernels only). */
 int gle_tree_set_schedule(int tree, int mode);
+
+/* GARLI's nst=6 nucleotide model with gamma rates and invariant sites.
+ * rel = {AC, AG, AT, CG, CT} relative to GT = 1; alpha is used when the
+ * instance has more than one rate category, pinv when pinv_in_model. */
+typedef struct gle_gtr_params {
+  double rel[5];
+  double pi[4];
+  double alpha;
+  double pinv;
+  int pinv_in_model;
+} gle_gtr_params;
+enum { GLE_OPT_RATES = 1, GLE_OPT_FREQS = 2, GLE_OPT_ALPHA = 4, GLE_OPT_PINV = 8, GLE_OPT_BRANCHES = 16 };
+/* Sets the tree's instance to the model *m: Q and its eigensystem, discrete
+ * gamma category rates, rate weights (1 - pinv) / R and pinv; replaces
+ * Model::UpdateQMat + CalcEigenStuff + DiscreteGamma for nst=6
+ * (model.cpp:233-294, 827-856, 2170-2201). */
+int gle_tree_set_gtr(int tree, const gle_gtr_params *m);
+/* Maximises lnL on the tree's fixed topology over the parameters selected by
+ * `which` (GLE_OPT_*; GLE_OPT_BRANCHES adds a Newton pass over all branches
+ * each round): rounds of one-dimensional Brent searches, one parameter at a
+ * time, until a round gains less than `precision`.  *m is updated to the
+ * optimum and *lnl is its score.  Replaces the parameter loop of
+ * Population::OptimizeInputOnly / BetterFinalOptimization (population.cpp:2566)
+ * with OptimizeRelativeNucRates (tree.cpp:7307) and OptimizeBoundedParameter
+ * (optimization.cpp:332). */
+int gle_tree_optimize_gtr(int tree, gle_gtr_params *m, int which, double precision, double *lnl);
 
 /* ---------------------------------------------------------------------------
  * Serving layer: continuous batching of scoring / branch-optimisation requests

@@ -3,7 +3,8 @@
 Reads (as data, never executed) garli/trunk/tests/data/ from the reference
 checkout: the 11-taxon alignments z.11x2178.nex / z.11x2178.AA.nex, the
 starting trees *.start with their [!GarliScore] and [!GarliModel] comments,
-and tests/data/expected.scr.  Writes tests/golden/garli_testdata.json.
+and tests/data/expected.scr.  Writes tests/golden/garli_testdata.json and
+copies the alignment files the ingestion tests read into tests/golden/data/.
 
     python tests/golden/make_fixtures.py [/path/to/garli/trunk/tests/data]
 """
@@ -14,6 +15,9 @@ import sys
 
 DEFAULT = "/root/reference/garli/trunk/tests/data"
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "garli_testdata.json")
+DATA_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+DATA_FILES = ["z.11x2178.nex", "z.11x2178.wtset.nex", "z.11x2178.AA.nex", "z.11x30.phy", "z.11x30.AA.fas",
+              "z.11x30.wackyNames.nex", "z.11x30.stop.nex"]
 
 
 def read_matrix(path):
@@ -80,6 +84,11 @@ def main():
            "starts": starts, "expected_scr": expected}
     with open(OUT, "w") as fh:
         json.dump(out, fh, indent=1)
+    # alignment files the ingestion tests read, copied as data
+    os.makedirs(DATA_OUT, exist_ok=True)
+    for f in DATA_FILES:
+        with open(os.path.join(src, f), "rb") as fi, open(os.path.join(DATA_OUT, f), "wb") as fo:
+            fo.write(fi.read())
     print("wrote", OUT, len(nuc), "taxa x", len(nuc[0]), "nt;", len(starts), "start trees")
 
 

@@ -1,0 +1,123 @@
+"""Model-parameter optimisation on a fixed tree, checked end to end against
+the reference's own scoring tests.
+
+trunk/tests/scoring/n.G4.conf and n.G5.conf run GARLI with
+optimizeinputonly = 1 on data/z.11x2178.wtset.nex (GTR+I+G with 4 or 5 gamma
+categories; relative rates, base frequencies, shape and pinv estimated;
+branch lengths optimised) from the trees and parameters of data/n.G4.start /
+n.G5.start.  runtests.sh then sums column 2 of the run's .sitelikes.log (the
+tree's -lnL plus each listed character's -lnL) and halves it; the result must
+be within 0.01 of data/expected.scr.  Here the same file is ingested
+(its EXSET and WTSET included), optimised on the GPU, written by
+gle_write_sitelikes and summed the same way."""
+import dataclasses
+import os
+
+import numpy as np
+import pytest
+
+from garli_amd import engine as E
+from oracle import oracle as O
+import refdata
+
+pytestmark = pytest.mark.gpu
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "data")
+
+
+def _start(ds):
+    return E.GtrParams(ds.meta["rel"], ds.pi, ds.meta["alpha"], ds.pinv, ds.pinv_in_model)
+
+
+def _dataset_at(ds, p, blen):
+    """The same data under model p and branch lengths blen; the oracle builds
+    its own Q, eigensystem and gamma rates from these parameters."""
+    R = ds.n_rates
+    pinv = p.pinv if p.pinv_in_model else 0.0
+    return dataclasses.replace(ds, pi=np.array(p.pi), rate_probs=np.full(R, (1.0 - pinv) / R), pinv=pinv,
+                               pinv_in_model=p.pinv_in_model, blen=np.asarray(blen),
+                               meta=dict(ds.meta, rel=np.array(p.rel), alpha=p.alpha))
+
+
+def _runtests_score(path):
+    """runtests.sh: awk '{sum+=$2}' over the sitelikes file, divided by 2."""
+    total = 0.0
+    for ln in open(path):
+        f = ln.split()
+        try:
+            total += float(f[1])
+        except (IndexError, ValueError):
+            pass
+    return total / 2.0
+
+
+@pytest.mark.parametrize("name", ["n.G4", "n.G5"])
+def test_optimize_input_only_matches_expected_scr(gpu, name, tmp_path):
+    plain = refdata.nucleotide_dataset(name)  # start tree and model of <name>.start
+    expect = refdata.load()["expected_scr"][name + ".conf"]
+    aln = E.Alignment(os.path.join(DATA, "z.11x2178.wtset.nex"))
+    pats, counts, cs, lc, site_pat = aln.nucleotide_patterns()
+    ds = dataclasses.replace(plain, tips=pats, counts=counts, const_states=cs, last_const=lc)
+    t = E.Tree(ds.make_instance(E.PREC_FP64), ds.parent, ds.blen)
+    start = _start(ds)
+    t.set_gtr(start)
+    assert abs(t.score() - ds.meta["score"]) < 2e-6  # the start file's own [!GarliScore]
+    lnl, opt = t.optimize_gtr(start, precision=1e-4)
+    assert lnl >= ds.meta["score"] - 1e-9
+    per_pattern = t.site_likelihoods()
+    per_char = np.where(site_pat >= 0, per_pattern[np.maximum(site_pat, 0)], 0.0)
+    out = tmp_path / ("scr.%s.sitelikes.log" % name)
+    E.write_sitelikes(str(out), lnl, per_char, site_pat >= 0)
+    score = _runtests_score(out)
+    print(name, "lnL %.6f, runtests score %.5f, expected.scr %.5f" % (lnl, score, expect), opt, flush=True)
+    assert abs(score - expect) < 0.01
+    # the optimum's score is the oracle's score of the same model and tree
+    ref = O.Oracle(_dataset_at(ds, opt, t.branch_lengths())).score()
+    assert abs(lnl - ref) < 1e-9 * abs(ref)
+    t.close()
+
+
+def test_optimizer_climbs_from_a_perturbed_start(gpu):
+    """From rates, frequencies, shape, pinv and branch lengths moved away from
+    the n.G4 optimum the optimiser returns to it (the start file's parameters
+    are that optimum: its [!GarliScore] is already the maximum)."""
+    ds = refdata.nucleotide_dataset("n.G4")
+    rng = np.random.default_rng(3)
+    far = E.GtrParams(ds.meta["rel"] * rng.uniform(0.5, 2.0, 5), [0.3, 0.2, 0.2, 0.3], 0.6, 0.1, True)
+    t = E.Tree(ds.make_instance(E.PREC_FP64), ds.parent, ds.blen * rng.uniform(0.5, 2.0, len(ds.blen)))
+    t.set_gtr(far)
+    low = t.score()
+    lnl, opt = t.optimize_gtr(far, precision=1e-4)
+    print("perturbed start %.4f -> %.4f (reference optimum %.4f)" % (low, lnl, ds.meta["score"]), opt, flush=True)
+    assert low < ds.meta["score"] - 50.0
+    assert abs(lnl - ds.meta["score"]) < 0.01
+    np.testing.assert_allclose(opt.rel, ds.meta["rel"], rtol=0.02)
+    np.testing.assert_allclose(opt.pi, ds.pi, atol=2e-3)
+    t.close()
+
+
+def test_each_parameter_search_improves(gpu):
+    """One parameter group at a time never lowers lnL, and a second full pass
+    from the optimum gains less than its precision."""
+    ds = refdata.nucleotide_dataset("n.G4")
+    t = E.Tree(ds.make_instance(E.PREC_FP64), ds.parent, ds.blen)
+    p = _start(ds)
+    t.set_gtr(p)
+    prev = t.score()
+    for which in (E.OPT_BRANCHES, E.OPT_RATES, E.OPT_FREQS, E.OPT_ALPHA, E.OPT_PINV):
+        lnl, p = t.optimize_gtr(p, which=which, precision=1e-3)
+        assert lnl >= prev - 1e-9, (which, lnl, prev)
+        prev = lnl
+    lnl, p = t.optimize_gtr(p, precision=1e-3)
+    again, _ = t.optimize_gtr(p, precision=1e-3)
+    assert again - lnl < 0.01
+    assert abs(p.pi.sum() - 1.0) < 1e-12 and np.all(p.rel > 0) and 0.0 <= p.pinv < 1.0
+    t.close()
+
+
+def test_fp32_form_optimum(gpu):
+    """The FP32 bulk form reaches the same optimum within its lnL tolerance."""
+    ds = refdata.nucleotide_dataset("n.G4")
+    t = E.Tree(ds.make_instance(E.PREC_FP32), ds.parent, ds.blen)
+    lnl, _ = t.optimize_gtr(_start(ds), precision=1e-3)
+    assert abs(lnl - ds.meta["score"]) < 0.01
+    t.close()
