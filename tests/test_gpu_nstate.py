@@ -25,10 +25,18 @@ def test_reference_garliscore_nucleotide_engine(gpu, name):
 
 @pytest.mark.parametrize("name", ["c.M3x2", "c"])
 def test_reference_garliscore_codon_engine(gpu, name):
+    """Also the reference's scoring tests trunk/tests/scoring/c.conf and
+    c.M3x2.conf: they score z.11x2178.nex (no weights, so runtests.sh's
+    halved sitelikes sum is -lnL) from these start files, whose parameters
+    are already optimal; expected.scr holds the same numbers, accepted within
+    0.01."""
     ds = refdata.codon_dataset(name)
+    expect = refdata.load()["expected_scr"][name + ".conf"]
     for prec, tol in ((E.PREC_FP64, 2e-5), (E.PREC_FP32, 1e-3)):
         t = E.Tree(ds.make_instance(prec), ds.parent, ds.blen)
-        assert abs(t.score() - ds.meta["score"]) < tol
+        lnl = t.score()
+        assert abs(lnl - ds.meta["score"]) < tol
+        assert abs(-lnl - expect) < 0.01
 
 
 @pytest.mark.parametrize("seed,ntax,npat,ncat", [(5, 6, 300, 4), (6, 17, 1500, 4), (7, 9, 500, 1)])
