@@ -676,6 +676,33 @@ extern "C" int gle_tree_set_gtr(int h, const gle_gtr_params *m) {
   return apply_gtr(*T, *m);
 }
 
+// One coordinate of a model: value through `set`, searched by brent_max on a
+// transformed axis in [lo, hi] from x0; the best point is kept when it beats
+// `best`, and the instance is left at the accepted model.
+template <typename Params, typename Apply, typename Set>
+static int line_search(Tree &t, Params &cur, double &best, Apply &&apply, double x0, double lo, double hi,
+                       Set &&set) {
+  auto f = [&](double x, double *val) -> int {
+    Params trial = cur;
+    set(trial, x);
+    int rc = apply(t, trial);
+    if (rc) return rc;
+    return tree_score(t, val);
+  };
+  double xb, fb;
+  int rc = brent_max(f, lo, hi, x0, 1e-5, 60, &xb, &fb);
+  if (rc) return rc;
+  if (fb > best) {
+    set(cur, xb);
+    best = fb;
+  }
+  return apply(t, cur);
+}
+
+// at most a factor 5 from the current value per round, as the reference
+// limits one pass (maxChangeProportion, tree.cpp:7330)
+static const double kStep = std::log(5.0);
+
 extern "C" int gle_tree_optimize_gtr(int h, gle_gtr_params *m, int which, double prec, double *lnl) {
   if (!m || !lnl || !(prec > 0.0)) return set_error(GLE_ERR_ARG, "bad argument");
   GET_TREE(h);
@@ -686,28 +713,8 @@ extern "C" int gle_tree_optimize_gtr(int h, gle_gtr_params *m, int which, double
   if (rc) return rc;
   double best;
   if ((rc = tree_score(t, &best))) return rc;
-  // one coordinate: value through `set`, searched on a transformed axis
-  // (log for rates and the shape, log-ratio to T for frequencies, direct for
-  // pinv), at most a factor 5 from the current value per round as the
-  // reference limits a pass (maxChangeProportion, tree.cpp:7330)
-  auto line = [&](double x0, double lo, double hi, auto &&set) -> int {
-    auto f = [&](double x, double *val) -> int {
-      gle_gtr_params trial = cur;
-      set(trial, x);
-      int r2 = apply_gtr(t, trial);
-      if (r2) return r2;
-      return tree_score(t, val);
-    };
-    double xb, fb;
-    int r2 = brent_max(f, lo, hi, x0, 1e-5, 60, &xb, &fb);
-    if (r2) return r2;
-    if (fb > best) {
-      set(cur, xb);
-      best = fb;
-    }
-    return apply_gtr(t, cur);  // leave the instance at the accepted point
-  };
-  const double kStep = std::log(5.0);
+  // axes: log for the rates and the shape, log-ratio to T for the
+  // frequencies, direct for pinv
   for (int round = 0; round < 200; round++) {
     const double start = best;
     if (which & GLE_OPT_BRANCHES) {
@@ -718,36 +725,128 @@ extern "C" int gle_tree_optimize_gtr(int h, gle_gtr_params *m, int which, double
     if (which & GLE_OPT_RATES)
       for (int i = 0; i < 5; i++) {
         const double x0 = std::log(cur.rel[i]);
-        rc = line(x0, std::max(x0 - kStep, std::log(1e-5)), std::min(x0 + kStep, std::log(999.0)),
-                  [i](gle_gtr_params &q, double x) { q.rel[i] = std::exp(x); });
+        rc = line_search(t, cur, best, apply_gtr, x0, std::max(x0 - kStep, std::log(1e-5)),
+                         std::min(x0 + kStep, std::log(999.0)),
+                         [i](gle_gtr_params &q, double x) { q.rel[i] = std::exp(x); });
         if (rc) return rc;
       }
     if (which & GLE_OPT_FREQS)
       for (int i = 0; i < 3; i++) {
         const double x0 = std::log(cur.pi[i] / cur.pi[3]);
-        rc = line(x0, x0 - kStep, x0 + kStep, [i](gle_gtr_params &q, double x) {
+        rc = line_search(t, cur, best, apply_gtr, x0, x0 - kStep, x0 + kStep, [i](gle_gtr_params &q, double x) {
           double u[4];
           for (int k = 0; k < 3; k++) u[k] = q.pi[k] / q.pi[3];
           u[i] = std::exp(x);
           u[3] = 1.0;
-          const double s = u[0] + u[1] + u[2] + u[3];
-          for (int k = 0; k < 4; k++) q.pi[k] = u[k] / s;
+          const double sum = u[0] + u[1] + u[2] + u[3];
+          for (int k = 0; k < 4; k++) q.pi[k] = u[k] / sum;
         });
         if (rc) return rc;
       }
     if ((which & GLE_OPT_ALPHA) && R > 1) {
       const double x0 = std::log(cur.alpha);
-      rc = line(x0, std::max(x0 - kStep, std::log(0.05)), std::min(x0 + kStep, std::log(999.0)),
-                [](gle_gtr_params &q, double x) { q.alpha = std::exp(x); });
+      rc = line_search(t, cur, best, apply_gtr, x0, std::max(x0 - kStep, std::log(0.05)),
+                       std::min(x0 + kStep, std::log(999.0)),
+                       [](gle_gtr_params &q, double x) { q.alpha = std::exp(x); });
       if (rc) return rc;
     }
     if ((which & GLE_OPT_PINV) && cur.pinv_in_model) {
       const double hi = 0.999 * instance_constant_fraction(t.I);
       if (hi > 0.0) {
-        rc = line(std::min(cur.pinv, hi), 0.0, hi, [](gle_gtr_params &q, double x) { q.pinv = x; });
+        rc = line_search(t, cur, best, apply_gtr, std::min(cur.pinv, hi), 0.0, hi,
+                         [](gle_gtr_params &q, double x) { q.pinv = x; });
         if (rc) return rc;
       }
     }
+    if (!(best - start >= prec)) break;
+  }
+  *m = cur;
+  return tree_score(t, lnl);
+}
+
+// ------------------------------------------------------------------- codons
+// GARLI's codon models on a fixed tree: nst=6 exchangeabilities between
+// codons one nucleotide apart, an omega class per rate category with its
+// proportion (M0 .. M3, model.cpp:442-687, 1514-1530) and fixed codon
+// frequencies; optimisation of the exchangeabilities (OptimizeRelativeNucRates,
+// tree.cpp:7307) and of the omegas and their proportions
+// (OptimizeOmegaParameters, tree.cpp:7164) with Newton branch passes.
+
+static int apply_codon(Tree &t, const gle_codon_params &m) {
+  const int R = instance_rates(t.I);
+  if (instance_states(t.I) != 61) return set_error(GLE_ERR_UNSUPPORTED, "codon parameters need a 61-state instance");
+  if (m.n_omega != R) return set_error(GLE_ERR_ARG, "%d omega classes for %d rate categories", m.n_omega, R);
+  std::vector<double> V(size_t(R) * 3721), W(size_t(R) * 3721), lam(size_t(R) * 61);
+  int rc = gle_model_codon(m.pi, m.rel, m.omega, m.omega_prob, R, V.data(), W.data(), lam.data());
+  if (rc) return rc;
+  std::vector<int> eor(R);
+  std::vector<double> mult(R, 1.0);
+  for (int k = 0; k < R; k++) {
+    eor[k] = k;
+    if ((rc = gle_set_eigen(t.inst_h, k, V.data() + size_t(k) * 3721, W.data() + size_t(k) * 3721,
+                            lam.data() + size_t(k) * 61)))
+      return rc;
+  }
+  if ((rc = gle_set_category_rates(t.inst_h, eor.data(), mult.data())) ||
+      (rc = gle_set_model(t.inst_h, m.pi, m.omega_prob, 0.0, 0)))
+    return rc;
+  sync_generation(t);
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_tree_set_codon(int h, const gle_codon_params *m) {
+  if (!m) return set_error(GLE_ERR_ARG, "null model");
+  GET_TREE(h);
+  return apply_codon(*T, *m);
+}
+
+extern "C" int gle_tree_optimize_codon(int h, gle_codon_params *m, int which, double prec, double *lnl) {
+  if (!m || !lnl || !(prec > 0.0)) return set_error(GLE_ERR_ARG, "bad argument");
+  GET_TREE(h);
+  Tree &t = *T;
+  const int K = m->n_omega;
+  gle_codon_params cur = *m;
+  int rc = apply_codon(t, cur);
+  if (rc) return rc;
+  double best;
+  if ((rc = tree_score(t, &best))) return rc;
+  for (int round = 0; round < 200; round++) {
+    const double start = best;
+    if (which & GLE_OPT_BRANCHES) {
+      double imp, l;
+      if ((rc = optimize_all(t, prec, &imp, &l))) return rc;
+      best = l;
+    }
+    if (which & GLE_OPT_RATES)  // AC .. CT relative to GT
+      for (int i = 0; i < 5; i++) {
+        const double x0 = std::log(cur.rel[i] / cur.rel[5]);
+        rc = line_search(t, cur, best, apply_codon, x0, std::max(x0 - kStep, std::log(1e-3)),
+                         std::min(x0 + kStep, std::log(999.0)),
+                         [i](gle_codon_params &q, double x) { q.rel[i] = std::exp(x) * q.rel[5]; });
+        if (rc) return rc;
+      }
+    if (which & GLE_OPT_OMEGA)
+      for (int k = 0; k < K; k++) {
+        const double x0 = std::log(cur.omega[k]);
+        rc = line_search(t, cur, best, apply_codon, x0, std::max(x0 - kStep, std::log(1e-4)),
+                         std::min(x0 + kStep, std::log(999.0)),
+                         [k](gle_codon_params &q, double x) { q.omega[k] = std::exp(x); });
+        if (rc) return rc;
+      }
+    if ((which & GLE_OPT_OMEGA_PROBS) && K > 1)  // log-ratio of each proportion to the last class's
+      for (int k = 0; k < K - 1; k++) {
+        const double x0 = std::log(cur.omega_prob[k] / cur.omega_prob[K - 1]);
+        rc = line_search(t, cur, best, apply_codon, x0, std::max(x0 - kStep, std::log(1e-4)),
+                         std::min(x0 + kStep, std::log(1e4)), [k, K](gle_codon_params &q, double x) {
+                           double u[8];
+                           for (int j = 0; j < K; j++) u[j] = q.omega_prob[j] / q.omega_prob[K - 1];
+                           u[k] = std::exp(x);
+                           double sum = 0.0;
+                           for (int j = 0; j < K; j++) sum += u[j];
+                           for (int j = 0; j < K; j++) q.omega_prob[j] = u[j] / sum;
+                         });
+        if (rc) return rc;
+      }
     if (!(best - start >= prec)) break;
   }
   *m = cur;

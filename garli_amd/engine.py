@@ -112,6 +112,8 @@ SIGNATURES = {
     "gle_tree_set_schedule": ([C.c_int, C.c_int], C.c_int),
     "gle_tree_set_gtr": ([C.c_int, C.c_void_p], C.c_int),
     "gle_tree_optimize_gtr": ([C.c_int, C.c_void_p, C.c_int, C.c_double, _D], C.c_int),
+    "gle_tree_set_codon": ([C.c_int, C.c_void_p], C.c_int),
+    "gle_tree_optimize_codon": ([C.c_int, C.c_void_p, C.c_int, C.c_double, _D], C.c_int),
     "gle_serve_create": ([_I], C.c_int),
     "gle_serve_destroy": ([C.c_int], C.c_int),
     "gle_serve_submit": ([C.c_int, C.c_int, _I, _D, C.c_int, C.c_double, _I], C.c_int),
@@ -518,6 +520,21 @@ class Tree:
         c = params.to_c()
         _check(lib().gle_tree_set_gtr(self.h, C.byref(c)))
 
+    def set_codon(self, params):
+        """Set the instance to codon-model parameters (a CodonParams)."""
+        c = params.to_c()
+        _check(lib().gle_tree_set_codon(self.h, C.byref(c)))
+
+    def optimize_codon(self, params, which=None, precision=0.01):
+        """Maximise lnL over exchangeabilities, omegas and omega proportions
+        (and branch lengths); returns (lnL, optimised CodonParams)."""
+        if which is None:
+            which = OPT_RATES | OPT_OMEGA | OPT_OMEGA_PROBS | OPT_BRANCHES
+        c = params.to_c()
+        out = C.c_double(0)
+        _check(lib().gle_tree_optimize_codon(self.h, C.byref(c), int(which), float(precision), C.byref(out)))
+        return out.value, CodonParams.from_c(c)
+
     def optimize_gtr(self, params, which=None, precision=0.01):
         """Maximise lnL over the selected model parameters (and all branch
         lengths with OPT_BRANCHES) on the fixed topology; returns
@@ -530,7 +547,7 @@ class Tree:
         return out.value, GtrParams.from_c(c)
 
 
-OPT_RATES, OPT_FREQS, OPT_ALPHA, OPT_PINV, OPT_BRANCHES = 1, 2, 4, 8, 16
+OPT_RATES, OPT_FREQS, OPT_ALPHA, OPT_PINV, OPT_BRANCHES, OPT_OMEGA, OPT_OMEGA_PROBS = 1, 2, 4, 8, 16, 32, 64
 
 
 class _GtrC(C.Structure):
@@ -563,6 +580,40 @@ class GtrParams:
     def __repr__(self):
         return "GtrParams(rel=%s, pi=%s, alpha=%.5f, pinv=%.5f)" % (np.round(self.rel, 5), np.round(self.pi, 5),
                                                                      self.alpha, self.pinv)
+
+
+class _CodonC(C.Structure):
+    _fields_ = [("rel", C.c_double * 6), ("n_omega", C.c_int), ("omega", C.c_double * 8),
+                ("omega_prob", C.c_double * 8), ("pi", C.c_double * 61)]
+
+
+class CodonParams:
+    """GARLI's codon models: exchangeabilities rel = (AC, AG, AT, CG, CT, GT),
+    omega classes with their proportions, fixed codon frequencies pi[61]."""
+
+    def __init__(self, rel, omega, omega_prob, pi):
+        self.rel = np.asarray(rel, dtype=np.float64).copy()
+        self.omega = np.asarray(omega, dtype=np.float64).copy()
+        self.omega_prob = np.asarray(omega_prob, dtype=np.float64).copy()
+        self.pi = np.asarray(pi, dtype=np.float64).copy()
+
+    def to_c(self):
+        c = _CodonC()
+        c.rel[:] = [float(x) for x in self.rel]
+        c.n_omega = len(self.omega)
+        for k, (w, p) in enumerate(zip(self.omega, self.omega_prob)):
+            c.omega[k], c.omega_prob[k] = float(w), float(p)
+        c.pi[:] = [float(x) for x in self.pi]
+        return c
+
+    @classmethod
+    def from_c(cls, c):
+        k = c.n_omega
+        return cls(list(c.rel), list(c.omega)[:k], list(c.omega_prob)[:k], list(c.pi))
+
+    def __repr__(self):
+        return "CodonParams(rel=%s, omega=%s, omega_prob=%s)" % (np.round(self.rel, 5), np.round(self.omega, 5),
+                                                                 np.round(self.omega_prob, 5))
 
 
 REQ_SCORE = 0

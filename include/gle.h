@@ -251,7 +251,15 @@ typedef struct gle_gtr_params {
   double pinv;
   int pinv_in_model;
 } gle_gtr_params;
-enum { GLE_OPT_RATES = 1, GLE_OPT_FREQS = 2, GLE_OPT_ALPHA = 4, GLE_OPT_PINV = 8, GLE_OPT_BRANCHES = 16 };
+enum {
+  GLE_OPT_RATES = 1,
+  GLE_OPT_FREQS = 2,
+  GLE_OPT_ALPHA = 4,
+  GLE_OPT_PINV = 8,
+  GLE_OPT_BRANCHES = 16,
+  GLE_OPT_OMEGA = 32,
+  GLE_OPT_OMEGA_PROBS = 64
+};
 /* Sets the tree's instance to the model *m: Q and its eigensystem, discrete
  * gamma category rates, rate weights (1 - pinv) / R and pinv; replaces
  * Model::UpdateQMat + CalcEigenStuff + DiscreteGamma for nst=6
@@ -266,6 +274,28 @@ int gle_tree_set_gtr(int tree, const gle_gtr_params *m);
  * with OptimizeRelativeNucRates (tree.cpp:7307) and OptimizeBoundedParameter
  * (optimization.cpp:332). */
 int gle_tree_optimize_gtr(int tree, gle_gtr_params *m, int which, double precision, double *lnl);
+
+/* GARLI's codon models: exchangeabilities rel = {AC, AG, AT, CG, CT, GT}
+ * between codons one nucleotide apart (GY94 with kappa: {1, k, 1, 1, k, 1}),
+ * one omega class per rate category of the instance with its proportion
+ * (n_omega = the instance's categories, at most 8), and 61 codon frequencies
+ * (held fixed). */
+typedef struct gle_codon_params {
+  double rel[6];
+  int n_omega;
+  double omega[8];
+  double omega_prob[8];
+  double pi[61];
+} gle_codon_params;
+/* Sets the tree's instance to the codon model *m (one eigensystem per omega
+ * class; replaces Model::UpdateQMatCodon + CalcEigenStuff, model.cpp:442-687,
+ * 827-856). */
+int gle_tree_set_codon(int tree, const gle_codon_params *m);
+/* As gle_tree_optimize_gtr for codon models: GLE_OPT_RATES (AC..CT relative
+ * to GT), GLE_OPT_OMEGA, GLE_OPT_OMEGA_PROBS, GLE_OPT_BRANCHES.  Replaces
+ * OptimizeRelativeNucRates (tree.cpp:7307) and OptimizeOmegaParameters
+ * (tree.cpp:7164) in the OptimizeInputOnly loop (population.cpp:2566). */
+int gle_tree_optimize_codon(int tree, gle_codon_params *m, int which, double precision, double *lnl);
 
 /* ---------------------------------------------------------------------------
  * Serving layer: continuous batching of scoring / branch-optimisation requests

@@ -121,3 +121,26 @@ def test_fp32_form_optimum(gpu):
     lnl, _ = t.optimize_gtr(_start(ds), precision=1e-3)
     assert abs(lnl - ds.meta["score"]) < 0.01
     t.close()
+
+
+@pytest.mark.parametrize("name", ["c", "c.M3x2"])
+def test_codon_optimizer_returns_to_the_reference_optimum(gpu, name):
+    """trunk/tests/scoring/c.conf and c.M3x2.conf (optimizeinputonly = 1):
+    their start files hold the optimum, which expected.scr repeats.  From
+    exchangeabilities, omegas, omega proportions and branch lengths moved away
+    from it, the optimiser climbs back to that score."""
+    ds = refdata.codon_dataset(name)
+    expect = refdata.load()["expected_scr"][name + ".conf"]
+    rng = np.random.default_rng(11)
+    rel = ds.meta["rel6"] * np.r_[rng.uniform(0.6, 1.6, 5), 1.0]
+    omega = ds.meta["omegas"] * rng.uniform(0.5, 2.0, len(ds.meta["omegas"]))
+    prob = np.full(len(omega), 1.0 / len(omega))
+    far = E.CodonParams(rel, omega, prob, ds.pi)
+    t = E.Tree(ds.make_instance(E.PREC_FP64), ds.parent, ds.blen * rng.uniform(0.7, 1.4, len(ds.blen)))
+    t.set_codon(far)
+    low = t.score()
+    lnl, opt = t.optimize_codon(far, precision=1e-4)
+    print(name, "perturbed start %.4f -> %.4f (reference %.4f)" % (low, lnl, -expect), opt, flush=True)
+    assert low < -expect - 5.0
+    assert abs(-lnl - expect) < 0.01  # runtests.sh's tolerance
+    t.close()
