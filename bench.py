@@ -268,9 +268,13 @@ def _eval_rates(ds, E, pop, prec):
     return 1000.0 / lat.mean(), lat
 
 
+SERVE_WINDOW_US = 250  # an idle server gathers a burst of offspring (ends after a 50 us lull)
+
+
 def _serve_rate(ds, E, pop, prec, slots):
     inst = ds.make_instance(prec, n_cla_slots=slots, n_pmat_slots=4 * slots)
     srv = E.Server()
+    srv.set_batch_window(SERVE_WINDOW_US)
     a = time.perf_counter()
     ids = [srv.submit(inst, p, b) for p, b in pop]
     res = [srv.wait(i)[0] for i in ids]
@@ -343,7 +347,10 @@ def bench_states(args):
     pop = _population(ds, 64, 11)
     e64, lat64 = _eval_rates(ds, E, pop, E.PREC_FP64)
     e32, _ = _eval_rates(ds, E, pop, E.PREC_FP32)
-    s32, st, served = _serve_rate(ds, E, pop, E.PREC_FP32, 3 * (N - 2) * 16)
+    # HBM budget: CLA slots for the whole population's working sets in flight
+    # (N - 2 per score request), 3 (N - 2) x 24 slots
+    slots = 3 * (N - 2) * 24
+    s32, st, served = _serve_rate(ds, E, pop, E.PREC_FP32, slots)
     o = O.Oracle(ds)
     ref = [o.score(parent=p, blen=b) for p, b in pop[:8]]
     acc = {"serve_fp32_max_dlnl": float(max(abs(x - y) for x, y in zip(served[:8], ref))),
@@ -357,7 +364,9 @@ def bench_states(args):
            "config": {"workload": name, "taxa": N, "patterns": P, "states": S, "rate_cats": ds.n_rates,
                       "requests": len(pop)},
            "fp64_plain_evals_per_s": e64, "fp32_plain_evals_per_s": e32, "fp32_served_evals_per_s": s32,
-           "served_vs_fp64_plain": s32 / e64, "serve_stats": st, "sweeps": res, "accuracy": acc, "roofline": roof}
+           "served_vs_fp64_plain": s32 / e64, "serve_stats": st, "sweeps": res, "accuracy": acc, "roofline": roof,
+           "serve_config": {"cla_slots": slots, "hbm_budget_GB": slots * ds.n_rates * ((P + 255) // 256 * 256) * (64 if S == 61 else S) * 4 / 1e9,
+                            "batch_window_us": SERVE_WINDOW_US}}
     print(json.dumps(out), flush=True)
 
 

@@ -137,6 +137,7 @@ struct Server {
   uint64_t tick = 0;
   std::vector<double> latency_ms;
   long long batched_requests = 0, ticks = 0;
+  int batch_window_us = 0;  // an idle server waits this long for a burst to gather
 };
 
 static std::mutex g_servers_mu;
@@ -468,6 +469,22 @@ static void worker_loop(Server *sv) {
       std::unique_lock<std::mutex> lk(sv->mu);
       sv->cv_work.wait(lk, [&] { return sv->stop || !sv->pending.empty() || !sv->active.empty(); });
       if (sv->stop) return;
+      if (sv->active.empty() && sv->batch_window_us > 0) {
+        // idle until now: let a burst of submissions gather into the first
+        // tick, until no request has arrived for a fifth of the window (at
+        // least 20 us) or the window has passed
+        using clock = std::chrono::steady_clock;
+        const auto deadline = clock::now() + std::chrono::microseconds(sv->batch_window_us);
+        const auto gap = std::chrono::microseconds(std::max(20, sv->batch_window_us / 5));
+        size_t seen = sv->pending.size();
+        for (;;) {
+          const auto until = std::min(deadline, clock::now() + gap);
+          sv->cv_work.wait_until(lk, until, [&] { return sv->stop || sv->pending.size() != seen; });
+          if (sv->stop) return;
+          if (sv->pending.size() == seen || clock::now() >= deadline) break;
+          seen = sv->pending.size();
+        }
+      }
       // admission by HBM budget: FIFO while the working sets fit the instance's CLA slots
       while (!sv->pending.empty()) {
         Req *r = sv->pending.front();
@@ -531,6 +548,14 @@ extern "C" int gle_serve_create(int *server) {
   std::lock_guard<std::mutex> lk(g_servers_mu);
   g_servers.push_back(std::move(sv));
   *server = int(g_servers.size()) - 1;
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_serve_set_batch_window(int h, int microseconds) {
+  Server *sv = get_server(h);
+  if (!sv || microseconds < 0) return set_error(GLE_ERR_ARG, "bad server handle or window");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  sv->batch_window_us = microseconds;
   return GLE_SUCCESS;
 }
 

@@ -120,6 +120,7 @@ SIGNATURES = {
     "gle_serve_poll": ([C.c_int, C.c_int, _I], C.c_int),
     "gle_serve_wait": ([C.c_int, C.c_int, _D, _D, _D], C.c_int),
     "gle_serve_stats": ([C.c_int, C.POINTER(C.c_longlong), _D, _D, _D, C.c_int], C.c_int),
+    "gle_serve_set_batch_window": ([C.c_int, C.c_int], C.c_int),
 }
 
 
@@ -658,6 +659,10 @@ class Server:
         _check(lib().gle_serve_wait(self.h, rid, C.byref(lnl), _p(bl, C.c_double) if n_nodes else None,
                                     C.byref(lat)))
         return (lnl.value, lat.value, bl) if n_nodes else (lnl.value, lat.value)
+
+    def set_batch_window(self, microseconds):
+        """An idle server waits this long for a burst of submissions to gather."""
+        _check(lib().gle_serve_set_batch_window(self.h, int(microseconds)))
 
     def stats(self, reset=False):
         n = C.c_longlong(0)

@@ -317,6 +317,12 @@ int gle_serve_poll(int server, int req, int *done);
 int gle_serve_wait(int server, int req, double *lnl, double *blen_out, double *latency_ms);
 /* latency percentiles over completed requests; mean requests per batched step */
 int gle_serve_stats(int server, long long *completed, double *p50_ms, double *p99_ms, double *mean_batch, int reset);
+/* An idle server (nothing in flight) lets a burst of requests (a GA
+ * generation's offspring) gather into one tick: after the first submission it
+ * waits until no request has arrived for a fifth of `microseconds` (at least
+ * 20 us) or `microseconds` have passed; 0 (the default) starts at once.  No
+ * reference counterpart (GARLI scores one individual at a time). */
+int gle_serve_set_batch_window(int server, int microseconds);
 
 #ifdef __cplusplus
 }

@@ -106,6 +106,26 @@ def test_admission_at_minimum_budget(gpu):
     assert st["mean_batch"] == 1.0
 
 
+def test_batch_window_gathers_a_burst(gpu):
+    """With a batching window an idle server lets a burst of submissions
+    gather into one tick; the results are those served without it."""
+    ds = sim.gtr_gamma(n_tips=14, n_sites=700, seed=107)
+    pop = _population(ds, 12, 10)
+    out = []
+    for window in (0, 200000):
+        inst = ds.make_instance(E.PREC_FP64, n_cla_slots=12 * 12 + 4, n_pmat_slots=600)
+        srv = E.Server()
+        srv.set_batch_window(window)
+        ids = [srv.submit(inst, p, b) for p, b in pop]
+        out.append([srv.wait(i)[0] for i in ids])
+        st = srv.stats()
+        srv.close()
+        inst.close()
+        if window:
+            assert st["mean_batch"] == len(pop), st
+    np.testing.assert_allclose(out[0], out[1], rtol=1e-12)
+
+
 @pytest.mark.parametrize("kind", ["aa", "codon"])
 def test_serving_state_spaces(gpu, kind):
     """20- and 61-state instances through the server: alone equals in the mix,
