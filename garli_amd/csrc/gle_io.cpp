@@ -5,7 +5,8 @@
 // NEXUS DATA / CHARACTERS blocks (interleaved or not, MISSING / GAP /
 // MATCHCHAR symbols, {..} and (..) state sets, quoted and underscore names,
 // a TAXA block's labels), the default (starred) EXSET and WTSET of an
-// ASSUMPTIONS block, relaxed PHYLIP (sequential or interleaved) and FASTA.
+// ASSUMPTIONS block, relaxed PHYLIP (a first line per taxon with its name,
+// further lines continuing the taxa in turn: interleaved blocks) and FASTA.
 // Rows keep the file's symbols (upper-cased, matchchar resolved, a state set
 // turned into its IUPAC code for nucleotides); each character carries a
 // weight: the WTSET weight, or 0 when the EXSET excludes it.
