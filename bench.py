@@ -280,6 +280,7 @@ def _serve_rate(ds, E, pop, prec, slots):
     res = [srv.wait(i)[0] for i in ids]
     dt = time.perf_counter() - a
     st = srv.stats()
+    st.update(srv.timing())
     srv.close()
     inst.close()
     return len(pop) / dt, st, res

@@ -138,6 +138,9 @@ struct Server {
   std::vector<double> latency_ms;
   long long batched_requests = 0, ticks = 0;
   int batch_window_us = 0;  // an idle server waits this long for a burst to gather
+  // where tick time goes: building op lists on the host, and from the first
+  // launch until the results are back
+  double host_ms = 0.0, device_ms = 0.0;
 };
 
 static std::mutex g_servers_mu;
@@ -373,8 +376,12 @@ static void advance(Req &r, const double *res) {
 }
 
 // Runs one batched step of every request in `batch` (all on instance si).
-static int run_tick(Server &sv, ServeInstance &si, std::vector<Req *> &batch) {
+static int run_tick(Server &sv, ServeInstance &si, std::vector<Req *> &batch, double &host_ms, double &device_ms) {
+  using clock = std::chrono::steady_clock;
+  auto ms_since = [](clock::time_point a) { return std::chrono::duration<double, std::milli>(clock::now() - a).count(); };
   for (;;) {
+    const auto t_build = clock::now();
+    auto t_launch = t_build;
     std::vector<DevClaOp> ops;
     std::vector<int> level_start;
     std::vector<DevEdgeOp> eops;
@@ -417,6 +424,8 @@ static int run_tick(Server &sv, ServeInstance &si, std::vector<Req *> &batch) {
       }
       level_start.push_back(int(ops.size()));
       for (auto &f : tb.fresh) fresh.push_back(f);
+      host_ms += ms_since(t_build);
+      t_launch = clock::now();
       // P, P', P'' for new branch lengths (one launch)
       int rc = engine_update_pmats(si.I, tb.pm_idx.data(), tb.pm_len.data(), int(tb.pm_idx.size()), 1);
       if (rc) return rc;
@@ -433,6 +442,7 @@ static int run_tick(Server &sv, ServeInstance &si, std::vector<Req *> &batch) {
     if (rc) return rc;
     rc = engine_flags(si.I, &flags, 1);
     if (rc) return rc;
+    device_ms += ms_since(t_launch);
     const bool thrown = instance_precision(si.I) == GLE_PREC_FP64 &&
                         (((flags & GLE_FLAG_REDUCE_RESCALE) && si.rescale_every > 2) ||
                          ((flags & GLE_FLAG_BAIL_OUT) && si.rescale_every <= 2));
@@ -500,9 +510,10 @@ static void worker_loop(Server *sv) {
     }
     std::map<int, std::vector<Req *>> by_inst;
     for (Req *r : work) by_inst[r->inst].push_back(r);
+    double host_ms = 0.0, device_ms = 0.0;
     for (auto &kv : by_inst) {
       ServeInstance &si = sv->insts[kv.first];
-      const int rc = run_tick(*sv, si, kv.second);
+      const int rc = run_tick(*sv, si, kv.second, host_ms, device_ms);
       if (rc)
         for (Req *r : kv.second) {
           r->rc = rc;
@@ -512,6 +523,8 @@ static void worker_loop(Server *sv) {
     {
       std::lock_guard<std::mutex> lk(sv->mu);
       sv->ticks++;
+      sv->host_ms += host_ms;
+      sv->device_ms += device_ms;
       sv->batched_requests += long(work.size());
       std::vector<Req *> still;
       const auto now = std::chrono::steady_clock::now();
@@ -548,6 +561,16 @@ extern "C" int gle_serve_create(int *server) {
   std::lock_guard<std::mutex> lk(g_servers_mu);
   g_servers.push_back(std::move(sv));
   *server = int(g_servers.size()) - 1;
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_serve_timing(int h, double *host_ms, double *device_ms, int reset) {
+  Server *sv = get_server(h);
+  if (!sv || !host_ms || !device_ms) return set_error(GLE_ERR_ARG, "bad argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  *host_ms = sv->host_ms;
+  *device_ms = sv->device_ms;
+  if (reset) sv->host_ms = sv->device_ms = 0.0;
   return GLE_SUCCESS;
 }
 

@@ -121,6 +121,7 @@ SIGNATURES = {
     "gle_serve_wait": ([C.c_int, C.c_int, _D, _D, _D], C.c_int),
     "gle_serve_stats": ([C.c_int, C.POINTER(C.c_longlong), _D, _D, _D, C.c_int], C.c_int),
     "gle_serve_set_batch_window": ([C.c_int, C.c_int], C.c_int),
+    "gle_serve_timing": ([C.c_int, _D, _D, C.c_int], C.c_int),
 }
 
 
@@ -659,6 +660,12 @@ class Server:
         _check(lib().gle_serve_wait(self.h, rid, C.byref(lnl), _p(bl, C.c_double) if n_nodes else None,
                                     C.byref(lat)))
         return (lnl.value, lat.value, bl) if n_nodes else (lnl.value, lat.value)
+
+    def timing(self, reset=False):
+        """(host ms building ticks, ms from first launch to results) so far."""
+        a, b = C.c_double(0), C.c_double(0)
+        _check(lib().gle_serve_timing(self.h, C.byref(a), C.byref(b), int(reset)))
+        return {"host_ms": a.value, "device_ms": b.value}
 
     def set_batch_window(self, microseconds):
         """An idle server waits this long for a burst of submissions to gather."""
