@@ -1,4 +1,4 @@
-"""The benchmark configurations at their full sizes.
+"""The benchmark configurations at their full sizes, and the CPU-runnable one.
 
 Nucleotide GTR+I+G4 on 128 taxa x 100,000 patterns, amino acids G4 on 64 x
 20,000 and codon M3 on 32 x 5,000 (the workloads bench.py times): the FP64
@@ -53,3 +53,19 @@ def test_full_size_parity_and_invariants(gpu, kind):
     d1err = np.abs(sw32[:, 1] - sw64[:, 1]) / np.maximum(1.0, np.abs(sw64[:, 1]))
     assert np.max(d1err) < 1e-3
     t32.close()
+
+
+def test_cpu_configuration_score_and_newton_pass(gpu):
+    """The CPU-runnable configuration: 24 taxa, 2,000 sites under GTR+G4 with
+    5% ambiguity codes; full lnL, then one Newton pass over all 45 edges,
+    against the oracle's own pass."""
+    ds = sim.gtr_gamma()  # 24 taxa, 2000 sites, alpha 0.5, 5% IUPAC codes
+    o = O.Oracle(ds)
+    t = E.Tree(ds.make_instance(E.PREC_FP64), ds.parent, ds.blen)
+    ref = o.score()
+    assert abs(t.score() - ref) < 1e-9 * abs(ref)
+    imp, lnl = t.optimize_all_branches(0.01)
+    oimp, olnl, obl = o.optimize_all(0.01)
+    np.testing.assert_allclose(t.branch_lengths()[1:], obl[1:], rtol=1e-7, atol=1e-10)
+    assert abs(lnl - olnl) < 1e-9 * abs(olnl) and lnl > ref
+    t.close()
