@@ -129,6 +129,10 @@ struct Instance {
   // servers built before it are stale (MakeAllNodesDirty, tree.cpp)
   unsigned long long generation = 0;
   hipEvent_t tev_a = nullptr, tev_b = nullptr;  // events for the next timed launch
+  // what d_ops / d_eops hold, so that a repeated list is not uploaded again
+  std::vector<DevClaOp> ops_held;
+  std::vector<DevEdgeOp> eops_held;
+  const void *ops_held_buf = nullptr, *eops_held_buf = nullptr;
   // hipGraph replay of repeated level-launch sequences (enqueue_partials): a
   // few recent op lists, each captured on its second use
   struct GraphEntry {
@@ -1620,9 +1624,16 @@ static int enqueue_partials(Instance *I, const DevClaOp *ops, int n_ops, const i
       return set_error(GLE_ERR_ARG, "cla op %d out of range (dest %d c1 %d c2 %d pm %d %d)", i, o.dest, o.c1,
                        o.c2, o.pm1, o.pm2);
   }
+  const int ops_cap0 = I->ops_cap;
   int rc = grow(I->d_ops, I->ops_cap, n_ops);
   if (rc) return rc;
-  GLE_HIP(hipMemcpyAsync(I->d_ops, ops, sizeof(DevClaOp) * n_ops, hipMemcpyHostToDevice, I->stream));
+  if (I->ops_cap != ops_cap0) I->ops_held_buf = nullptr;  // a new buffer holds nothing yet
+  if (!(I->ops_held_buf == I->d_ops && I->ops_held.size() == size_t(n_ops) &&
+        std::memcmp(I->ops_held.data(), ops, sizeof(DevClaOp) * n_ops) == 0)) {
+    GLE_HIP(hipMemcpyAsync(I->d_ops, ops, sizeof(DevClaOp) * n_ops, hipMemcpyHostToDevice, I->stream));
+    I->ops_held.assign(ops, ops + n_ops);
+    I->ops_held_buf = I->d_ops;
+  }
   auto bytes_of = [&](int b, int e) {
     double t = 0.0;
     for (int i = b; i < e; i++) t += cla_op_bytes(I, ops[i]);
@@ -1809,8 +1820,10 @@ static int enqueue_edges(Instance *I, const DevEdgeOp *ops, int n, bool site) {
       return set_error(GLE_ERR_ARG, "edge op %d out of range", i);
   }
   const int nblk = edge_blocks(I);
+  const int eops_cap0 = I->eops_cap;
   int rc = grow(I->d_eops, I->eops_cap, n);
   if (rc) return rc;
+  if (I->eops_cap != eops_cap0) I->eops_held_buf = nullptr;  // a new buffer holds nothing yet
   rc = grow(I->d_eres, I->eres_cap, 3 * n);
   if (rc) return rc;
   const size_t need = size_t(n) * nblk * 3;
@@ -1819,7 +1832,12 @@ static int enqueue_edges(Instance *I, const DevEdgeOp *ops, int n, bool site) {
     GLE_HIP(hipMalloc(&I->d_part, sizeof(double) * need));
     I->part_cap = need;
   }
-  GLE_HIP(hipMemcpyAsync(I->d_eops, ops, sizeof(DevEdgeOp) * n, hipMemcpyHostToDevice, I->stream));
+  if (!(I->eops_held_buf == I->d_eops && I->eops_held.size() == size_t(n) &&
+        std::memcmp(I->eops_held.data(), ops, sizeof(DevEdgeOp) * n) == 0)) {
+    GLE_HIP(hipMemcpyAsync(I->d_eops, ops, sizeof(DevEdgeOp) * n, hipMemcpyHostToDevice, I->stream));
+    I->eops_held.assign(ops, ops + n);
+    I->eops_held_buf = I->d_eops;
+  }
   double *site_save = I->d_site;
   if (!site) I->d_site = nullptr;
   double eb = 0.0;
