@@ -67,6 +67,8 @@ static int codon_aa(int c) {
   return (a == '*' || !p) ? 20 : int(p - order);
 }
 
+int standard_code_aa(int codon64) { return codon_aa(codon64); }
+
 // model.cpp:296-687: one-position changes only; the changed nucleotide pair's
 // exchangeability (rel6 = AC AG AT CG CT GT), omega_w on nonsynonymous changes;
 // normaliser over the omega mixture.
@@ -492,4 +494,116 @@ extern "C" int gle_compress_patterns(int ntax, int nsite, int ms, const unsigned
   if (!matrix || !out || !counts || !cs || !lc || !s2p) return set_error(GLE_ERR_ARG, "null argument");
   int rc = compress_patterns(ntax, nsite, ms, matrix, out, counts, cs, lc, s2p);
   return rc < 0 ? set_error(rc, "bad matrix for pattern compression") : rc;
+}
+
+// ------------------------------------------------------ state frequencies
+
+// AminoacidData / CodonData::CalcEmpiricalFreqs (sequencedata.cpp:192-275):
+// every non-missing state of every taxon counted with its pattern's count; one
+// pseudo-count per state when a state is never observed.  Amino-acid data
+// whose A, C, G and T make up more than 90% is refused as nucleotides
+// (sequencedata.cpp:210).
+extern "C" int gle_empirical_freqs(int ntax, int npat, int ns, const unsigned char *st, const int *counts,
+                                   double *pi) {
+  if (ntax < 1 || npat < 0 || ns < 2 || ns > 255 || !st || !counts || !pi) return set_error(GLE_ERR_ARG, "bad argument");
+  std::vector<double> f(size_t(ns), 0.0);
+  double total = 0.0;
+  for (int t = 0; t < ntax; t++)
+    for (int p = 0; p < npat; p++) {
+      const int v = st[size_t(t) * npat + p];
+      if (v == ns) continue;
+      if (v > ns) return set_error(GLE_ERR_ARG, "state %d outside 0..%d", v, ns);
+      f[v] += counts[p];
+      total += counts[p];
+    }
+  if (!(total > 0.0)) return set_error(GLE_ERR_ARG, "no observed states");
+  if (ns == 20 && (f[0] + f[1] + f[5] + f[16]) / total > 0.90)
+    return set_error(GLE_ERR_ARG, "amino-acid model on what looks like nucleotide data");
+  if (std::find(f.begin(), f.end(), 0.0) != f.end()) {
+    for (double &x : f) x += 1.0;
+    total += ns;
+  }
+  for (int j = 0; j < ns; j++) pi[j] = f[j] / total;
+  return GLE_SUCCESS;
+}
+
+// NucleotideData::CalcEmpiricalFreqs (sequencedata.cpp:61-190).  Cells with
+// one base count for it; partly ambiguous cells (2 or 3 bases) are shared
+// among their bases, first equally and then in proportion to the current
+// frequencies, repeated until no frequency moves by more than 1e-8 (the
+// fixed point of that EM iteration); fully ambiguous cells are skipped.  One
+// pseudo-count per base when a base is never seen.  The reference loops per
+// cell; here the ambiguous cells are first summed per bit set, which changes
+// only the rounding.
+extern "C" int gle_empirical_nuc_freqs(int ntax, int npat, const unsigned char *masks, const int *counts, double *pi) {
+  if (ntax < 1 || npat < 0 || !masks || !counts || !pi) return set_error(GLE_ERR_ARG, "bad argument");
+  double single[4] = {0, 0, 0, 0}, amb[4] = {0, 0, 0, 0}, by_mask[16] = {0}, n_single = 0.0, n_amb = 0.0;
+  for (int t = 0; t < ntax; t++)
+    for (int p = 0; p < npat; p++) {
+      const int m = masks[size_t(t) * npat + p] & 15, bits = __builtin_popcount(unsigned(m));
+      const double c = counts[p];
+      if (bits == 1) {
+        single[__builtin_ctz(unsigned(m))] += c;
+        n_single += c;
+      } else if (bits == 2 || bits == 3) {
+        for (int s = 0; s < 4; s++)
+          if (m >> s & 1) amb[s] += c / bits;
+        n_amb += c;
+        by_mask[m] += c;
+      }
+    }
+  for (int j = 0; j < 4; j++)
+    if (single[j] + amb[j] == 0.0) {
+      for (double &x : single) x += 1.0;
+      n_single += 4.0;
+      break;
+    }
+  const double total = n_single + n_amb;
+  if (!(total > 0.0)) return set_error(GLE_ERR_ARG, "no observed bases");
+  for (int j = 0; j < 4; j++) pi[j] = (single[j] + amb[j]) / total;
+  if (n_amb > 0.0)
+    for (int it = 0; it < 100000; it++) {
+      double a[4] = {0, 0, 0, 0};
+      for (int m = 1; m < 16; m++) {
+        if (by_mask[m] == 0.0) continue;
+        double frac = 0.0;
+        for (int s = 0; s < 4; s++)
+          if (m >> s & 1) frac += pi[s];
+        for (int s = 0; s < 4; s++)
+          if (m >> s & 1) a[s] += by_mask[m] * (pi[s] / frac);
+      }
+      bool moved = false;
+      for (int j = 0; j < 4; j++) {
+        const double v = (single[j] + a[j]) / total;
+        if (std::fabs(v - pi[j]) > 1e-8) moved = true;
+        pi[j] = v;
+      }
+      if (!moved) break;
+    }
+  return GLE_SUCCESS;
+}
+
+// CodonData::CalcF1x4Freqs / CalcF3x4Freqs (sequencedata.cpp:605-660):
+// codon frequencies as products of base frequencies, pooled over the three
+// positions (F1x4) or per position (F3x4), renormalised over the 61 sense
+// codons.  base_freqs holds positions 1, 2, 3 (A, C, G, T each).
+extern "C" int gle_codon_freqs_fx4(const double *bf, int f3x4, double *pi61) {
+  if (!bf || !pi61) return set_error(GLE_ERR_ARG, "null argument");
+  double pos[3][4];
+  for (int k = 0; k < 3; k++)
+    for (int b = 0; b < 4; b++) pos[k][b] = f3x4 ? bf[4 * k + b] : (bf[b] + bf[4 + b] + bf[8 + b]) / 3.0;
+  if (f3x4)
+    for (int k = 0; k < 3; k++)
+      for (int b = 0; b < 4; b++)
+        if (pos[k][b] == 0.0) return set_error(GLE_ERR_ARG, "F3x4: base %d never seen at codon position %d", b, k + 1);
+  double total = 0.0;
+  int n = 0;
+  for (int c = 0; c < 64; c++) {
+    if (codon_aa(c) == 20) continue;
+    pi61[n] = pos[0][c >> 4] * pos[1][(c >> 2) & 3] * pos[2][c & 3];
+    total += pi61[n++];
+  }
+  if (!(total > 0.0)) return set_error(GLE_ERR_ARG, "F1x4/F3x4: no base frequencies");
+  for (int j = 0; j < 61; j++) pi61[j] /= total;
+  return GLE_SUCCESS;
 }

@@ -18,6 +18,9 @@ int reversible_eigen(int n, const double *Q, const double *pi, double *V, double
 int discrete_gamma(int K, double alpha, double *rates);
 int nuc_datum(char c);
 int aa_datum(char c);
+// amino acid of codon 16*n1 + 4*n2 + n3 (A, C, G, T = 0..3) under the
+// standard code: ACDEFGHIKLMNPQRSTVWY = 0..19, stop = 20
+int standard_code_aa(int codon64);
 int make_ambig_string(const unsigned char *masks, int n, signed char *out);
 int ambig_string_to_masks(const signed char *st, int len, int n, unsigned char *masks);
 int compress_patterns(int ntax, int nsite, int ms, const unsigned char *matrix, unsigned char *out, int *counts,

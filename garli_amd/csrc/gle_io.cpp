@@ -514,6 +514,140 @@ extern "C" int gle_alignment_free(int h) {
   return GLE_SUCCESS;
 }
 
+// ------------------------------------------------------ state matrices
+
+extern "C" int gle_alignment_states(int h, unsigned char *out) {
+  Alignment *A = get_aln(h);
+  if (!A || !out) return set_error(GLE_ERR_ARG, "bad argument");
+  const bool dna = A->datatype == GLE_DATA_DNA;
+  if (!dna && A->datatype != GLE_DATA_PROTEIN) return set_error(GLE_ERR_UNSUPPORTED, "neither DNA nor protein data");
+  const size_t nc = A->rows.empty() ? 0 : A->rows[0].size();
+  for (size_t t = 0; t < A->rows.size(); t++)
+    for (size_t j = 0; j < nc; j++) {
+      const char c = A->rows[t][j];
+      const int d = dna ? nuc_datum(c) : aa_datum(c);
+      if (d < 0) return set_error(GLE_ERR_ARG, "symbol '%c' of taxon %zu, character %zu", c, t + 1, j + 1);
+      out[t * nc + j] = (unsigned char)d;
+    }
+  return GLE_SUCCESS;
+}
+
+// The three nucleotide bit sets of codon k of taxon t.
+static int codon_masks(const Alignment &A, size_t t, size_t k, int m[3]) {
+  for (int i = 0; i < 3; i++) {
+    const char c = A.rows[t][3 * k + i];
+    m[i] = nuc_datum(c);
+    if (m[i] < 0) return set_error(GLE_ERR_ARG, "symbol '%c' of taxon %zu, character %zu", c, t + 1, 3 * k + i + 1);
+  }
+  return GLE_SUCCESS;
+}
+
+static int codon_dims(Alignment *A, size_t *ncod) {
+  if (!A) return set_error(GLE_ERR_ARG, "bad alignment handle");
+  if (A->datatype != GLE_DATA_DNA) return set_error(GLE_ERR_UNSUPPORTED, "codons need DNA data");
+  const size_t nc = A->rows.empty() ? 0 : A->rows[0].size();
+  if (nc % 3) return set_error(GLE_ERR_ARG, "%zu nucleotides is not a whole number of codons", nc);
+  *ncod = nc / 3;
+  return GLE_SUCCESS;
+}
+
+static const char kBase[4] = {'A', 'C', 'G', 'T'};
+
+// CodonData::FillCodonMatrixFromDNA (sequencedata.cpp:364-470), standard
+// code.  Only single-base sets name a base (the reference's bit-set to index
+// table); a codon with any other set is missing.  Base counts at each codon
+// position are taken over the unambiguous codons, stops included, before a
+// stop is refused or turned into missing data.
+extern "C" int gle_alignment_codons(int h, int ignore_stops, unsigned char *out, double *base_freqs) {
+  Alignment *A = get_aln(h);
+  size_t ncod;
+  int rc = codon_dims(A, &ncod);
+  if (rc) return rc;
+  if (!out) return set_error(GLE_ERR_ARG, "null output");
+  int sense[64], n = 0;
+  for (int c = 0; c < 64; c++) sense[c] = standard_code_aa(c) == 20 ? -1 : n++;
+  double pos[3][4] = {{0}}, total = 0.0;
+  for (size_t t = 0; t < A->rows.size(); t++)
+    for (size_t k = 0; k < ncod; k++) {
+      int m[3], b[3];
+      if ((rc = codon_masks(*A, t, k, m))) return rc;
+      bool whole = true;
+      for (int i = 0; i < 3; i++) {
+        whole = whole && __builtin_popcount(unsigned(m[i])) == 1;
+        b[i] = __builtin_ctz(unsigned(m[i]) | 16u);
+      }
+      unsigned char v = 61;
+      if (whole) {
+        for (int i = 0; i < 3; i++) pos[i][b[i]] += 1.0;
+        total += 1.0;
+        const int c = 16 * b[0] + 4 * b[1] + b[2];
+        if (sense[c] >= 0) v = (unsigned char)sense[c];
+        else if (!ignore_stops)
+          return set_error(GLE_ERR_ARG, "stop codon %c%c%c at codon %zu (nucleotide %zu) of taxon %zu", kBase[b[0]],
+                           kBase[b[1]], kBase[b[2]], k + 1, 3 * k + 1, t + 1);
+      }
+      out[t * ncod + k] = v;
+    }
+  if (base_freqs)
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 4; j++) base_freqs[4 * i + j] = total > 0.0 ? pos[i][j] / total : 0.0;
+  return GLE_SUCCESS;
+}
+
+// AminoacidData::FillAminoacidMatrixFromDNA (sequencedata.cpp:473-590),
+// standard code.  A codon whose first two positions are both fully missing
+// is missing (the reference tests the sum of positions 1, 2 and 1 against
+// three missing codes).  Otherwise every resolution of its bit sets is
+// translated in A, C, G, T order: resolutions to two different amino acids
+// make it missing; a stop among resolutions that also give an amino acid
+// makes it missing; resolutions that are all stops are an error, and with
+// ignore_stops any stop makes the codon missing.
+extern "C" int gle_alignment_translate(int h, int ignore_stops, unsigned char *out) {
+  Alignment *A = get_aln(h);
+  size_t ncod;
+  int rc = codon_dims(A, &ncod);
+  if (rc) return rc;
+  if (!out) return set_error(GLE_ERR_ARG, "null output");
+  for (size_t t = 0; t < A->rows.size(); t++)
+    for (size_t k = 0; k < ncod; k++) {
+      int m[3];
+      if ((rc = codon_masks(*A, t, k, m))) return rc;
+      int prot = -1, first_stop = -1;
+      if (2 * m[0] + m[1] == 45) {
+        prot = 20;
+      } else {
+        bool done = false;
+        for (int x = 0; x < 4 && !done; x++)
+          for (int y = 0; y < 4 && !done; y++)
+            for (int z = 0; z < 4 && !done; z++) {
+              if (!(m[0] >> x & 1) || !(m[1] >> y & 1) || !(m[2] >> z & 1)) continue;
+              const int c = 16 * x + 4 * y + z, a = standard_code_aa(c);
+              if (a == 20) {
+                if (first_stop < 0) first_stop = c;
+                if (ignore_stops) {
+                  prot = 20;
+                  done = true;
+                }
+              } else if (prot < 0) {
+                prot = a;
+              } else if (a != prot) {
+                prot = 20;
+                done = true;
+              }
+            }
+        if (first_stop >= 0 && !done) {
+          if (prot < 0)
+            return set_error(GLE_ERR_ARG, "stop codon %c%c%c at codon %zu (nucleotide %zu) of taxon %zu",
+                             kBase[first_stop >> 4], kBase[(first_stop >> 2) & 3], kBase[first_stop & 3], k + 1,
+                             3 * k + 1, t + 1);
+          prot = 20;
+        }
+      }
+      out[t * ncod + k] = (unsigned char)prot;
+    }
+  return GLE_SUCCESS;
+}
+
 extern "C" int gle_write_sitelikes(const char *path, int append, int tree_num, double lnl, int nchar,
                                    const double *site_lnl, const int *included) {
   if (!path || nchar < 0 || (nchar > 0 && (!site_lnl || !included))) return set_error(GLE_ERR_ARG, "bad argument");

@@ -586,25 +586,44 @@ extern "C" int gle_tree_set_schedule(int h, int mode) {
 // over all branches, until a round gains less than the precision.  Each
 // trial value is a full model push and rescore on the device.
 
-static int apply_gtr(Tree &t, const gle_gtr_params &m) {
+// Discrete-gamma categories with pinv (model.cpp:1478-1486, 2170-2201;
+// model.h:1733-1740, 2051-2064) and one eigensystem for every category.
+static int set_rate_het(Tree &t, const double *V, const double *W, const double *lam, const double *pi, double alpha,
+                        double pinv_value, int pinv_in_model) {
   const int R = instance_rates(t.I);
-  if (instance_states(t.I) != 4) return set_error(GLE_ERR_UNSUPPORTED, "GTR parameters need a nucleotide instance");
-  double V[16], W[16], lam[4];
-  int rc = gle_model_gtr(m.pi, m.rel, V, W, lam);
-  if (rc) return rc;
   std::vector<double> rates(R, 1.0), mult(R), w(R);
-  if (R > 1 && (rc = discrete_gamma(R, m.alpha, rates.data()))) return rc;
-  const double pinv = m.pinv_in_model ? m.pinv : 0.0;
+  int rc;
+  if (R > 1 && (rc = discrete_gamma(R, alpha, rates.data()))) return rc;
+  const double pinv = pinv_in_model ? pinv_value : 0.0;
   for (int r = 0; r < R; r++) {
-    mult[r] = m.pinv_in_model ? rates[r] / (1.0 - pinv) : rates[r];
+    mult[r] = pinv_in_model ? rates[r] / (1.0 - pinv) : rates[r];
     w[r] = (1.0 - pinv) / R;
   }
   std::vector<int> eor(R, 0);
   if ((rc = gle_set_eigen(t.inst_h, 0, V, W, lam)) || (rc = gle_set_category_rates(t.inst_h, eor.data(), mult.data())) ||
-      (rc = gle_set_model(t.inst_h, m.pi, w.data(), pinv, m.pinv_in_model)))
+      (rc = gle_set_model(t.inst_h, pi, w.data(), pinv, pinv_in_model)))
     return rc;
   sync_generation(t);
   return GLE_SUCCESS;
+}
+
+static int apply_gtr(Tree &t, const gle_gtr_params &m) {
+  if (instance_states(t.I) != 4) return set_error(GLE_ERR_UNSUPPORTED, "GTR parameters need a nucleotide instance");
+  double V[16], W[16], lam[4];
+  int rc = gle_model_gtr(m.pi, m.rel, V, W, lam);
+  if (rc) return rc;
+  return set_rate_het(t, V, W, lam, m.pi, m.alpha, m.pinv, m.pinv_in_model);
+}
+
+// UpdateQMatAminoAcid (model.cpp:735-780): 190 exchangeabilities (a named
+// matrix or user values) times the target frequencies; rate heterogeneity as
+// for nucleotides.
+static int apply_aa(Tree &t, const gle_aa_params &m) {
+  if (instance_states(t.I) != 20) return set_error(GLE_ERR_UNSUPPORTED, "amino-acid parameters need a 20-state instance");
+  double V[400], W[400], lam[20];
+  int rc = gle_model_aa(m.pi, m.rel, V, W, lam);
+  if (rc) return rc;
+  return set_rate_het(t, V, W, lam, m.pi, m.alpha, m.pinv, m.pinv_in_model);
 }
 
 // Brent's one-dimensional minimiser (golden section with parabolic steps,
@@ -703,18 +722,20 @@ static int line_search(Tree &t, Params &cur, double &best, Apply &&apply, double
 // limits one pass (maxChangeProportion, tree.cpp:7330)
 static const double kStep = std::log(5.0);
 
-extern "C" int gle_tree_optimize_gtr(int h, gle_gtr_params *m, int which, double prec, double *lnl) {
-  if (!m || !lnl || !(prec > 0.0)) return set_error(GLE_ERR_ARG, "bad argument");
-  GET_TREE(h);
-  Tree &t = *T;
+// The parameter loop shared by the nucleotide and amino-acid models: rounds
+// of a Newton pass over the branches, then one Brent search per free
+// exchangeability (log axis, within [rel_lo, rel_hi]; the last n_rel_fixed
+// entries stay put), per frequency (log-ratio to the last state), for the
+// gamma shape (log axis, [0.05, 999]) and for pinv ([0, 0.999 x the constant
+// fraction]), until a round gains less than prec.
+template <typename Params, typename Apply>
+static int optimize_rate_het_model(Tree &t, Params &cur, int n_rel, int n_states, double rel_lo, double rel_hi,
+                                   int which, double prec, Apply &&apply) {
   const int R = instance_rates(t.I);
-  gle_gtr_params cur = *m;
-  int rc = apply_gtr(t, cur);
+  int rc = apply(t, cur);
   if (rc) return rc;
   double best;
   if ((rc = tree_score(t, &best))) return rc;
-  // axes: log for the rates and the shape, log-ratio to T for the
-  // frequencies, direct for pinv
   for (int round = 0; round < 200; round++) {
     const double start = best;
     if (which & GLE_OPT_BRANCHES) {
@@ -723,45 +744,75 @@ extern "C" int gle_tree_optimize_gtr(int h, gle_gtr_params *m, int which, double
       best = l;
     }
     if (which & GLE_OPT_RATES)
-      for (int i = 0; i < 5; i++) {
+      for (int i = 0; i < n_rel; i++) {
         const double x0 = std::log(cur.rel[i]);
-        rc = line_search(t, cur, best, apply_gtr, x0, std::max(x0 - kStep, std::log(1e-5)),
-                         std::min(x0 + kStep, std::log(999.0)),
-                         [i](gle_gtr_params &q, double x) { q.rel[i] = std::exp(x); });
+        rc = line_search(t, cur, best, apply, x0, std::max(x0 - kStep, std::log(rel_lo)),
+                         std::min(x0 + kStep, std::log(rel_hi)), [i](Params &q, double x) { q.rel[i] = std::exp(x); });
         if (rc) return rc;
       }
     if (which & GLE_OPT_FREQS)
-      for (int i = 0; i < 3; i++) {
-        const double x0 = std::log(cur.pi[i] / cur.pi[3]);
-        rc = line_search(t, cur, best, apply_gtr, x0, x0 - kStep, x0 + kStep, [i](gle_gtr_params &q, double x) {
-          double u[4];
-          for (int k = 0; k < 3; k++) u[k] = q.pi[k] / q.pi[3];
+      for (int i = 0; i < n_states - 1; i++) {
+        const double x0 = std::log(cur.pi[i] / cur.pi[n_states - 1]);
+        rc = line_search(t, cur, best, apply, x0, x0 - kStep, x0 + kStep, [i, n_states](Params &q, double x) {
+          double u[64], sum = 0.0;
+          for (int k = 0; k < n_states - 1; k++) u[k] = q.pi[k] / q.pi[n_states - 1];
           u[i] = std::exp(x);
-          u[3] = 1.0;
-          const double sum = u[0] + u[1] + u[2] + u[3];
-          for (int k = 0; k < 4; k++) q.pi[k] = u[k] / sum;
+          u[n_states - 1] = 1.0;
+          for (int k = 0; k < n_states; k++) sum += u[k];
+          for (int k = 0; k < n_states; k++) q.pi[k] = u[k] / sum;
         });
         if (rc) return rc;
       }
     if ((which & GLE_OPT_ALPHA) && R > 1) {
       const double x0 = std::log(cur.alpha);
-      rc = line_search(t, cur, best, apply_gtr, x0, std::max(x0 - kStep, std::log(0.05)),
-                       std::min(x0 + kStep, std::log(999.0)),
-                       [](gle_gtr_params &q, double x) { q.alpha = std::exp(x); });
+      rc = line_search(t, cur, best, apply, x0, std::max(x0 - kStep, std::log(0.05)),
+                       std::min(x0 + kStep, std::log(999.0)), [](Params &q, double x) { q.alpha = std::exp(x); });
       if (rc) return rc;
     }
     if ((which & GLE_OPT_PINV) && cur.pinv_in_model) {
       const double hi = 0.999 * instance_constant_fraction(t.I);
       if (hi > 0.0) {
-        rc = line_search(t, cur, best, apply_gtr, std::min(cur.pinv, hi), 0.0, hi,
-                         [](gle_gtr_params &q, double x) { q.pinv = x; });
+        rc = line_search(t, cur, best, apply, std::min(cur.pinv, hi), 0.0, hi,
+                         [](Params &q, double x) { q.pinv = x; });
         if (rc) return rc;
       }
     }
     if (!(best - start >= prec)) break;
   }
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_tree_optimize_gtr(int h, gle_gtr_params *m, int which, double prec, double *lnl) {
+  if (!m || !lnl || !(prec > 0.0)) return set_error(GLE_ERR_ARG, "bad argument");
+  GET_TREE(h);
+  gle_gtr_params cur = *m;
+  // five free rates with GT = 1, bounds of the reference's rate search
+  int rc = optimize_rate_het_model(*T, cur, 5, 4, 1e-5, 999.0, which, prec, apply_gtr);
+  if (rc) return rc;
   *m = cur;
-  return tree_score(t, lnl);
+  return tree_score(*T, lnl);
+}
+
+extern "C" int gle_tree_set_aa(int h, const gle_aa_params *m) {
+  if (!m) return set_error(GLE_ERR_ARG, "null model");
+  GET_TREE(h);
+  return apply_aa(*T, *m);
+}
+
+extern "C" int gle_tree_optimize_aa(int h, gle_aa_params *m, int which, double prec, double *lnl) {
+  if (!m || !lnl || !(prec > 0.0)) return set_error(GLE_ERR_ARG, "bad argument");
+  if (which & (GLE_OPT_OMEGA | GLE_OPT_OMEGA_PROBS)) return set_error(GLE_ERR_ARG, "no omegas in an amino-acid model");
+  GET_TREE(h);
+  gle_aa_params cur = *m;
+  if (which & GLE_OPT_RATES)
+    for (double r : cur.rel)
+      if (!(r > 0.0)) return set_error(GLE_ERR_ARG, "estimating exchangeabilities needs them all positive");
+  // 189 free exchangeabilities relative to the last (Y-V), within 1e-5 and 1e5 of it
+  const double ref = cur.rel[189];
+  int rc = optimize_rate_het_model(*T, cur, 189, 20, 1e-5 * ref, 1e5 * ref, which, prec, apply_aa);
+  if (rc) return rc;
+  *m = cur;
+  return tree_score(*T, lnl);
 }
 
 // ------------------------------------------------------------------- codons

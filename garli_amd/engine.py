@@ -71,6 +71,12 @@ SIGNATURES = {
     "gle_alignment_row": ([C.c_int, C.c_int, C.c_char_p], C.c_int),
     "gle_alignment_weights": ([C.c_int, C.POINTER(C.c_int)], C.c_int),
     "gle_alignment_free": ([C.c_int], C.c_int),
+    "gle_alignment_states": ([C.c_int, C.POINTER(C.c_uint8)], C.c_int),
+    "gle_alignment_codons": ([C.c_int, C.c_int, C.POINTER(C.c_uint8), _D], C.c_int),
+    "gle_alignment_translate": ([C.c_int, C.c_int, C.POINTER(C.c_uint8)], C.c_int),
+    "gle_empirical_freqs": ([C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint8), C.POINTER(C.c_int), _D], C.c_int),
+    "gle_empirical_nuc_freqs": ([C.c_int, C.c_int, C.POINTER(C.c_uint8), C.POINTER(C.c_int), _D], C.c_int),
+    "gle_codon_freqs_fx4": ([_D, C.c_int, _D], C.c_int),
     "gle_write_sitelikes": ([C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_int, _D, C.POINTER(C.c_int)], C.c_int),
     "gle_create_instance": ([C.POINTER(InstanceDesc), _I], C.c_int),
     "gle_destroy_instance": ([C.c_int], C.c_int),
@@ -113,6 +119,8 @@ SIGNATURES = {
     "gle_tree_set_gtr": ([C.c_int, C.c_void_p], C.c_int),
     "gle_tree_optimize_gtr": ([C.c_int, C.c_void_p, C.c_int, C.c_double, _D], C.c_int),
     "gle_tree_set_codon": ([C.c_int, C.c_void_p], C.c_int),
+    "gle_tree_set_aa": ([C.c_int, C.c_void_p], C.c_int),
+    "gle_tree_optimize_aa": ([C.c_int, C.c_void_p, C.c_int, C.c_double, _D], C.c_int),
     "gle_tree_optimize_codon": ([C.c_int, C.c_void_p, C.c_int, C.c_double, _D], C.c_int),
     "gle_serve_create": ([_I], C.c_int),
     "gle_serve_destroy": ([C.c_int], C.c_int),
@@ -263,27 +271,113 @@ class Alignment:
                 self.rows.append(row.raw[:nc.value].decode())
             self.weights = np.zeros(nc.value, dtype=np.int32)
             _check(lib().gle_alignment_weights(h, _p(self.weights, C.c_int)))
+            self._h = h.value
+            h = None
         finally:
-            lib().gle_alignment_free(h)
+            if h is not None:
+                lib().gle_alignment_free(h)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and _lib is not None:
+            lib().gle_alignment_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     @property
     def n_chars(self):
         return len(self.weights)
 
+    def states(self):
+        """Per-character states [ntax][nchar]: nucleotide bit sets (A 1, C 2,
+        G 4, T 8, missing 15) or amino-acid indices (missing 20)."""
+        out = np.zeros((len(self.rows), self.n_chars), dtype=np.uint8)
+        _check(lib().gle_alignment_states(self._h, _p(out, C.c_uint8)))
+        return out
+
+    def _codon_dims(self):
+        if self.datatype != DATA_DNA or self.n_chars % 3:
+            raise GleError(-1, "codons need DNA data with a whole number of codons")
+        return len(self.rows), self.n_chars // 3
+
+    def codon_states(self, ignore_stops=False):
+        """Sense-codon states [ntax][nchar/3] (0..60, missing 61) and the base
+        frequencies at the three codon positions [3][4] (datatype = codon)."""
+        out = np.zeros(self._codon_dims(), dtype=np.uint8)
+        bf = np.zeros(12)
+        _check(lib().gle_alignment_codons(self._h, int(bool(ignore_stops)), _p(out, C.c_uint8), _p(bf, C.c_double)))
+        return out, bf.reshape(3, 4)
+
+    def translate(self, ignore_stops=False):
+        """Amino acids of the codons [ntax][nchar/3] (missing 20; datatype =
+        codon-aminoacid)."""
+        out = np.zeros(self._codon_dims(), dtype=np.uint8)
+        _check(lib().gle_alignment_translate(self._h, int(bool(ignore_stops)), _p(out, C.c_uint8)))
+        return out
+
     def nucleotide_patterns(self):
         """Compressed nucleotide patterns with the weights applied:
         (patterns [ntax][P], counts, const_states, last_const, pattern of each
         character or -1 where it is excluded or holds no data)."""
-        keep = np.nonzero(self.weights > 0)[0]
-        codes = np.array([[nuc_char_to_datum(r[k]) for k in keep] for r in self.rows], dtype=np.int32)
-        if (codes < 0).any():
-            raise GleError(-1, "symbol outside the nucleotide alphabet")
-        pats, counts, cs, lc, s2p = compress_patterns(codes.astype(np.uint8), 4)
-        w = np.zeros(pats.shape[1], dtype=np.int64)
-        np.add.at(w, s2p[s2p >= 0], self.weights[keep][s2p >= 0])
-        site = np.full(self.n_chars, -1, dtype=np.int32)
-        site[keep] = s2p
-        return pats, w.astype(np.int32), cs, lc, site
+        if self.datatype != DATA_DNA:
+            raise GleError(-1, "not nucleotide data")
+        return _weighted_patterns(self.states(), self.weights, 4)
+
+    def aa_patterns(self):
+        """As nucleotide_patterns for a protein alignment (20 states)."""
+        if self.datatype != DATA_PROTEIN:
+            raise GleError(-1, "not protein data")
+        return _weighted_patterns(self.states(), self.weights, 20)
+
+    def codon_patterns(self, ignore_stops=False):
+        """Codon patterns (61 states); a pattern of each codon.  Weight sets
+        do not apply to codons (nor to the translation, as in the
+        reference's CodonData / AminoacidData, sequencedata.cpp:369-371)."""
+        cod, _ = self.codon_states(ignore_stops)
+        return _weighted_patterns(cod, np.ones(cod.shape[1], np.int32), 61)
+
+    def translated_patterns(self, ignore_stops=False):
+        """Amino-acid patterns of the translated codons (20 states)."""
+        aa = self.translate(ignore_stops)
+        return _weighted_patterns(aa, np.ones(aa.shape[1], np.int32), 20)
+
+
+def _weighted_patterns(states, weights, n_states):
+    keep = np.nonzero(weights > 0)[0]
+    pats, counts, cs, lc, s2p = compress_patterns(np.ascontiguousarray(states[:, keep]), n_states)
+    w = np.zeros(pats.shape[1], dtype=np.int64)
+    np.add.at(w, s2p[s2p >= 0], weights[keep][s2p >= 0])
+    site = np.full(len(weights), -1, dtype=np.int32)
+    site[keep] = s2p
+    return pats, w.astype(np.int32), cs, lc, site
+
+
+def empirical_freqs(patterns, counts, n_states):
+    """Empirical frequencies of amino-acid (20) or codon (61) patterns; for
+    nucleotide bit-set patterns (n_states = 4) ambiguity is shared out as the
+    reference does."""
+    st = np.ascontiguousarray(patterns, dtype=np.uint8)
+    c = np.ascontiguousarray(counts, dtype=np.int32)
+    ntax, npat = st.shape
+    pi = np.zeros(n_states)
+    if n_states == 4:
+        _check(lib().gle_empirical_nuc_freqs(ntax, npat, _p(st, C.c_uint8), _p(c, C.c_int), _p(pi, C.c_double)))
+    else:
+        _check(lib().gle_empirical_freqs(ntax, npat, n_states, _p(st, C.c_uint8), _p(c, C.c_int),
+                                         _p(pi, C.c_double)))
+    return pi
+
+
+def codon_freqs_fx4(base_freqs, f3x4=False):
+    """F1x4 or F3x4 codon frequencies from codon-position base frequencies."""
+    bf = np.ascontiguousarray(base_freqs, dtype=np.float64).reshape(12)
+    pi = np.zeros(61)
+    _check(lib().gle_codon_freqs_fx4(_p(bf, C.c_double), int(bool(f3x4)), _p(pi, C.c_double)))
+    return pi
 
 
 def write_sitelikes(path, lnl, site_lnl, included, tree_num=1, append=False):
@@ -537,6 +631,22 @@ class Tree:
         _check(lib().gle_tree_optimize_codon(self.h, C.byref(c), int(which), float(precision), C.byref(out)))
         return out.value, CodonParams.from_c(c)
 
+    def set_aa(self, params):
+        """Set the instance to amino-acid model parameters (an AaParams)."""
+        c = params.to_c()
+        _check(lib().gle_tree_set_aa(self.h, C.byref(c)))
+
+    def optimize_aa(self, params, which=None, precision=0.01):
+        """Maximise lnL over the selected amino-acid model parameters (by
+        default the gamma shape, pinv and branch lengths: a named or fixed
+        matrix with fixed frequencies); returns (lnL, optimised AaParams)."""
+        if which is None:
+            which = OPT_ALPHA | OPT_PINV | OPT_BRANCHES
+        c = params.to_c()
+        out = C.c_double(0)
+        _check(lib().gle_tree_optimize_aa(self.h, C.byref(c), int(which), float(precision), C.byref(out)))
+        return out.value, AaParams.from_c(c)
+
     def optimize_gtr(self, params, which=None, precision=0.01):
         """Maximise lnL over the selected model parameters (and all branch
         lengths with OPT_BRANCHES) on the fixed topology; returns
@@ -582,6 +692,39 @@ class GtrParams:
     def __repr__(self):
         return "GtrParams(rel=%s, pi=%s, alpha=%.5f, pinv=%.5f)" % (np.round(self.rel, 5), np.round(self.pi, 5),
                                                                      self.alpha, self.pinv)
+
+
+class _AaC(C.Structure):
+    _fields_ = [("rel", C.c_double * 190), ("pi", C.c_double * 20), ("alpha", C.c_double), ("pinv", C.c_double),
+                ("pinv_in_model", C.c_int)]
+
+
+class AaParams:
+    """GARLI's amino-acid models: 190 exchangeabilities in upper-triangle row
+    order over ACDEFGHIKLMNPQRSTVWY, frequencies, gamma shape and pinv."""
+
+    def __init__(self, rel, pi, alpha=1.0, pinv=0.0, pinv_in_model=False):
+        self.rel = np.asarray(rel, dtype=np.float64).copy()
+        self.pi = np.asarray(pi, dtype=np.float64).copy()
+        if self.rel.shape != (190,) or self.pi.shape != (20,):
+            raise ValueError("AaParams needs 190 exchangeabilities and 20 frequencies")
+        self.alpha = float(alpha)
+        self.pinv = float(pinv)
+        self.pinv_in_model = bool(pinv_in_model)
+
+    def to_c(self):
+        c = _AaC()
+        c.rel[:] = [float(x) for x in self.rel]
+        c.pi[:] = [float(x) for x in self.pi]
+        c.alpha, c.pinv, c.pinv_in_model = self.alpha, self.pinv, int(self.pinv_in_model)
+        return c
+
+    @classmethod
+    def from_c(cls, c):
+        return cls(list(c.rel), list(c.pi), c.alpha, c.pinv, bool(c.pinv_in_model))
+
+    def __repr__(self):
+        return "AaParams(alpha=%.5f, pinv=%.5f)" % (self.alpha, self.pinv)
 
 
 class _CodonC(C.Structure):

@@ -98,6 +98,35 @@ int gle_alignment_row(int h, int i, char *row);
 /* per character: the WTSET weight (1 without one), 0 when the EXSET excludes it */
 int gle_alignment_weights(int h, int *weights);
 int gle_alignment_free(int h);
+/* Per-character states of a DNA or protein alignment, out[ntax][nchar]: the
+ * nucleotide bit set (A 1, C 2, G 4, T 8, missing 15; CharToDatum,
+ * sequencedata.cpp:672-713) or the amino-acid index (ACDEFGHIKLMNPQRSTVWY =
+ * 0..19, missing 20). */
+int gle_alignment_states(int h, unsigned char *out);
+/* Codon states of a DNA alignment (CodonData::FillCodonMatrixFromDNA,
+ * sequencedata.cpp:364-470; standard code): out[ntax][nchar / 3] = sense
+ * codon 0..60 (AAA, AAC, .., TTT without the stops), 61 for a codon with a
+ * gap or ambiguity.  A stop codon fails with GLE_ERR_ARG, or is missing with
+ * ignore_stops.  base_freqs (NULL allowed) receives the base frequencies at
+ * codon positions 1, 2 and 3 (A, C, G, T each) for gle_codon_freqs_fx4. */
+int gle_alignment_codons(int h, int ignore_stops, unsigned char *out, double *base_freqs);
+/* Amino acids translated from a DNA alignment's codons
+ * (AminoacidData::FillAminoacidMatrixFromDNA, sequencedata.cpp:473-590;
+ * datatype = codon-aminoacid): out[ntax][nchar / 3], 20 = missing, an
+ * ambiguous codon keeps its amino acid when every resolution agrees. */
+int gle_alignment_translate(int h, int ignore_stops, unsigned char *out);
+/* Empirical state frequencies of amino-acid or codon patterns
+ * (CalcEmpiricalFreqs, sequencedata.cpp:192-275): states[ntax][n_patterns]
+ * with n_states = missing, weighted by counts. */
+int gle_empirical_freqs(int ntax, int n_patterns, int n_states, const unsigned char *states, const int *counts,
+                        double *pi);
+/* Empirical base frequencies of nucleotide bit-set patterns, ambiguity
+ * shared out iteratively (NucleotideData::CalcEmpiricalFreqs,
+ * sequencedata.cpp:61-190). */
+int gle_empirical_nuc_freqs(int ntax, int n_patterns, const unsigned char *masks, const int *counts, double *pi);
+/* F1x4 (f3x4 = 0) or F3x4 codon frequencies from gle_alignment_codons'
+ * base_freqs (CalcF1x4Freqs / CalcF3x4Freqs, sequencedata.cpp:605-660). */
+int gle_codon_freqs_fx4(const double *base_freqs, int f3x4, double *pi61);
 /* <ofprefix>.sitelikes.log as GARLI writes it: the header unless append, one
  * line per character (-lnL, or "-" where included[s] == 0), then
  * "tree_num <tab> -lnl" (Tree::OutputSiteLikelihoods, tree.cpp:5501-5565;
@@ -274,6 +303,26 @@ int gle_tree_set_gtr(int tree, const gle_gtr_params *m);
  * with OptimizeRelativeNucRates (tree.cpp:7307) and OptimizeBoundedParameter
  * (optimization.cpp:332). */
 int gle_tree_optimize_gtr(int tree, gle_gtr_params *m, int which, double precision, double *lnl);
+
+/* GARLI's amino-acid models (UpdateQMatAminoAcid, model.cpp:735-780):
+ * exchangeabilities rel in upper-triangle row order over
+ * ACDEFGHIKLMNPQRSTVWY (a named matrix such as WAG, or user / estimated
+ * values; only ratios matter), frequencies pi, and rate heterogeneity as in
+ * gle_gtr_params. */
+typedef struct gle_aa_params {
+  double rel[190];
+  double pi[20];
+  double alpha;
+  double pinv;
+  int pinv_in_model;
+} gle_aa_params;
+/* As gle_tree_set_gtr for a 20-state instance. */
+int gle_tree_set_aa(int tree, const gle_aa_params *m);
+/* As gle_tree_optimize_gtr for a 20-state instance: GLE_OPT_RATES (the first
+ * 189 exchangeabilities relative to the last), GLE_OPT_FREQS, GLE_OPT_ALPHA,
+ * GLE_OPT_PINV, GLE_OPT_BRANCHES.  Replaces the OptimizeInputOnly parameter
+ * loop (population.cpp:2566) for ratematrix = fixed / wag / estimate. */
+int gle_tree_optimize_aa(int tree, gle_aa_params *m, int which, double precision, double *lnl);
 
 /* GARLI's codon models: exchangeabilities rel = {AC, AG, AT, CG, CT, GT}
  * between codons one nucleotide apart (GY94 with kappa: {1, k, 1, 1, k, 1}),

@@ -3,8 +3,12 @@
 Reads (as data, never executed) garli/trunk/tests/data/ from the reference
 checkout: the 11-taxon alignments z.11x2178.nex / z.11x2178.AA.nex, the
 starting trees *.start with their [!GarliScore] and [!GarliModel] comments,
-and tests/data/expected.scr.  Writes tests/golden/garli_testdata.json and
-copies the alignment files the ingestion tests read into tests/golden/data/.
+and tests/data/expected.scr; and, from garli/trunk/src/model.cpp, the numbers
+of the published WAG amino-acid exchangeability matrix (Whelan & Goldman
+2001) that ratematrix = wag multiplies into Q (Model::MultiplyByWAGAAMatrix),
+as 190 values in upper-triangle row order over ACDEFGHIKLMNPQRSTVWY.  Writes
+tests/golden/garli_testdata.json and copies the alignment files the ingestion
+tests read into tests/golden/data/.
 
     python tests/golden/make_fixtures.py [/path/to/garli/trunk/tests/data]
 """
@@ -63,6 +67,20 @@ def read_start(path):
             "garli_block": garli}
 
 
+def read_wag(model_cpp):
+    """The WAG exchangeabilities as data: every `qmatOffset[i][j] *= v;` of
+    Model::MultiplyByWAGAAMatrix, checked to fill a symmetric 20 x 20 matrix."""
+    txt = open(model_cpp).read()
+    body = txt[txt.index("void Model::MultiplyByWAGAAMatrix"):]
+    body = body[:body.index("\n}")]
+    r = [[0.0] * 20 for _ in range(20)]
+    for i, j, v in re.findall(r"qmatOffset\[(\d+)\]\[(\d+)\]\s*\*=\s*([0-9.eE+-]+)\s*;", body):
+        r[int(i)][int(j)] = float(v)
+    upper = [r[i][j] for i in range(20) for j in range(i + 1, 20)]
+    assert all(r[i][j] == r[j][i] for i in range(20) for j in range(20)) and all(x > 0 for x in upper)
+    return upper
+
+
 def main():
     src = sys.argv[1] if len(sys.argv) > 1 else DEFAULT
     names, nuc = read_matrix(os.path.join(src, "z.11x2178.nex"))
@@ -79,9 +97,10 @@ def main():
         p = line.split()
         if len(p) == 2:
             expected[p[0]] = float(p[1])
+    wag = read_wag(os.path.join(src, "..", "..", "src", "model.cpp"))
     out = {"source": "garli/trunk/tests/data (reference test fixtures)",
            "taxa": names, "nuc_matrix": nuc, "aa_taxa": anames, "aa_matrix": aa,
-           "starts": starts, "expected_scr": expected}
+           "starts": starts, "expected_scr": expected, "wag_rel190": wag}
     with open(OUT, "w") as fh:
         json.dump(out, fh, indent=1)
     # alignment files the ingestion tests read, copied as data

@@ -187,3 +187,33 @@ def codon_dataset(start_name):
     return sim.Dataset(start_name, ntax, 61, parent, blen, pats, counts, cs, lc, pi, probs, 0.0, False, eig,
                        np.arange(nw, dtype=np.int32), np.ones(nw),
                        dict(rel6=rel6, omegas=omegas, score=st["score"], printed_freqs=np.array(mf["e"])))
+
+
+def aa_dataset(start_name):
+    """trunk/tests/scoring/a.G3.conf, a.G4.conf and a.conf: amino acids read
+    from z.11x2178.AA.nex (a.G3) or translated from the codons of
+    z.11x2178.nex (datatype = codon-aminoacid: a.G4, a), by the engine's own
+    ingestion; statefrequencies = empirical from those patterns; WAG (a.G3,
+    a) or the fixed matrix of a.G4.start's garli block; 3, 4 or 1 gamma
+    categories at the start file's shape."""
+    d = load()
+    st = d["starts"][start_name]
+    mf = model_fields(st["model"])
+    data = os.path.join(HERE, "golden", "data")
+    if start_name == "a.G3":
+        aln = E.Alignment(os.path.join(data, "z.11x2178.AA.nex"))
+        pats, counts, cs, lc, site = aln.aa_patterns()
+    else:
+        aln = E.Alignment(os.path.join(data, "z.11x2178.nex"))
+        pats, counts, cs, lc, site = aln.translated_patterns()
+    assert aln.names == d["taxa"]
+    rel = np.array(st["garli_block"]["r"] if start_name == "a.G4" else d["wag_rel190"])
+    pi = E.empirical_freqs(pats, counts, 20)
+    ncat = {"a.G3": 3, "a.G4": 4, "a": 1}[start_name]
+    alpha = mf["a"][0] if "a" in mf else 1.0
+    parent, blen = parse_newick(st["newick"], len(d["taxa"]))
+    V, W, lam = E.model_aa(pi, rel)
+    rates = E.discrete_gamma(ncat, alpha) if ncat > 1 else np.ones(1)
+    return sim.Dataset(start_name, len(d["taxa"]), 20, parent, blen, pats, counts, cs, lc, pi,
+                       np.full(ncat, 1.0 / ncat), 0.0, False, [(V, W, lam)], np.zeros(ncat, np.int32), rates,
+                       dict(alpha=alpha, rel=rel, rates=rates, score=st["score"], site=site))

@@ -144,3 +144,41 @@ def test_codon_optimizer_returns_to_the_reference_optimum(gpu, name):
     assert low < -expect - 5.0
     assert abs(-lnl - expect) < 0.01  # runtests.sh's tolerance
     t.close()
+
+
+@pytest.mark.parametrize("name", ["a", "a.G3", "a.G4"])
+def test_amino_acid_scoring_matches_expected_scr(gpu, name, tmp_path):
+    """trunk/tests/scoring/a.conf, a.G3.conf and a.G4.conf (optimizeinputonly
+    = 1): the data ingested, translated and counted by the engine, the gamma
+    shape and branch lengths optimised, the sitelikes file written and summed
+    as runtests.sh does; within 0.01 of expected.scr.  (The start files'
+    [!GarliScore] values came from other data: their printed frequencies
+    differ from the data's in the fourth decimal.)"""
+    ds = refdata.aa_dataset(name)
+    expect = refdata.load()["expected_scr"][name + ".conf"]
+    t = E.Tree(ds.make_instance(E.PREC_FP64), ds.parent, ds.blen)
+    start = E.AaParams(ds.meta["rel"], ds.pi, ds.meta["alpha"])
+    t.set_aa(start)
+    low = t.score()
+    ref0 = O.Oracle(ds).score()
+    assert abs(low - ref0) < 1e-9 * abs(ref0)
+    lnl, opt = t.optimize_aa(start, precision=1e-4)
+    assert lnl >= low - 1e-9
+    site = ds.meta["site"]
+    per_pattern = t.site_likelihoods()
+    per_char = np.where(site >= 0, per_pattern[np.maximum(site, 0)], 0.0)
+    out = tmp_path / ("scr.%s.sitelikes.log" % name)
+    E.write_sitelikes(str(out), lnl, per_char, site >= 0)
+    score = _runtests_score(out)
+    print(name, "start %.5f, lnL %.6f, runtests score %.5f, expected.scr %.5f" % (low, lnl, score, expect), opt,
+          flush=True)
+    assert abs(score - expect) < 0.01
+    at_opt = dataclasses.replace(ds, blen=np.asarray(t.branch_lengths()), meta=dict(ds.meta, alpha=opt.alpha))
+    ref = O.Oracle(at_opt).score()
+    assert abs(lnl - ref) < 1e-9 * abs(ref)
+    t.close()
+    if name == "a.G4":  # the FP32 bulk form reaches the same score within its tolerance
+        t32 = E.Tree(ds.make_instance(E.PREC_FP32), ds.parent, ds.blen)
+        l32, _ = t32.optimize_aa(start, precision=1e-4)
+        assert abs(l32 - lnl) < 2e-3
+        t32.close()
