@@ -187,11 +187,20 @@ def main():
         for _ in range(n_sw):
             of.sweep()
         dt = (time.time() - a) / n_sw
-        cpu = {"value": n_cla * P / dt, "unit": "CLA site-updates/s", "cores": 1, "kind": "port",
-               "sample": "%d full sweeps of the same workload (378 CLA updates x %d patterns + %d edge "
-                         "derivatives each), FP64, one thread, oracle built -O3 -ffast-math, %.1fs total" %
-                         (n_sw, P, n_edges, dt * n_sw)}
-        log("cpu baseline %.3e site-updates/s (%.1fs)" % (cpu["value"], dt))
+        one = n_cla * P / dt
+        log("cpu baseline, one thread: %.3e site-updates/s (%.1fs per sweep)" % (one, dt))
+        del o, of
+        T = _cpu_threads()
+        parts = _pattern_slices(O, ds, T)
+        n_mt = 12
+        wall = _threaded([lambda q=q: [q.sweep() for _ in range(n_mt)] for q in parts])
+        cpu = {"value": n_mt * n_cla * P / wall, "unit": "CLA site-updates/s", "cores": len(parts), "kind": "port",
+               "single_thread_value": one,
+               "sample": "%d full sweeps of the same workload (%d CLA updates x %d patterns + %d edge derivatives "
+                         "each), FP64, patterns split over %d threads as the reference's OpenMP build splits "
+                         "them, oracle built -O3 -ffast-math, %.1fs wall; one thread: %d sweeps, %.1fs" %
+                         (n_mt, n_cla, P, n_edges, len(parts), wall, n_sw, dt * n_sw)}
+        log("cpu baseline, %d threads: %.3e site-updates/s (%.1fs)" % (len(parts), cpu["value"], wall))
 
     traffic = None
     if (prec == E.PREC_FP32 and args.schedule == "levels" and (N, P) == (128, 100000)):
@@ -245,6 +254,38 @@ def _cla_kinds(parent, n_tips):
         for b in nb[a]:
             kinds[sum(1 for c in nb[a] if c != b and 1 <= c <= n_tips)] += 1
     return kinds
+
+
+def _cpu_threads():
+    """Host threads for the CPU baseline: the cores this process may run on,
+    at most 16 (one GPU's share of the host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _pattern_slices(O, ds, T):
+    """One FP64 oracle (reference release flags) per thread over a contiguous
+    slice of the patterns: the reference's OpenMP split of its per-pattern
+    loops (defs.h:29-43), with threads in place of an OpenMP team."""
+    cuts = np.linspace(0, ds.n_patterns, T + 1).astype(int)
+    return [O.Oracle(ds, fast=True, patterns=np.arange(cuts[k], cuts[k + 1])) for k in range(T)
+            if cuts[k + 1] > cuts[k]]
+
+
+def _threaded(fns):
+    """Wall time of running every fn on its own thread (the oracle's ctypes
+    calls release the GIL)."""
+    import threading
+    th = [threading.Thread(target=f) for f in fns]
+    a = time.perf_counter()
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    return time.perf_counter() - a
 
 
 def _population(ds, n, seed):
@@ -354,6 +395,17 @@ def bench_states(args):
     s32, st, served = _serve_rate(ds, E, pop, E.PREC_FP32, slots)
     o = O.Oracle(ds)
     ref = [o.score(parent=p, blen=b) for p, b in pop[:8]]
+    # CPU baseline: the plain FP64 path on the host, each request's full
+    # score with its patterns split over the host threads
+    T = _cpu_threads()
+    parts = _pattern_slices(O, ds, T)
+    n_cpu = 4
+    wall = _threaded([lambda q=q: [q.score(parent=p, blen=b) for p, b in pop[:n_cpu]] for q in parts])
+    cpu = {"value": n_cpu / wall, "unit": "lnL evals/s", "cores": len(parts), "kind": "port",
+           "sample": "%d of the same requests scored in full (FP64, oracle built -O3 -ffast-math), patterns "
+                     "split over %d threads as the reference's OpenMP build splits them, %.1fs wall" %
+                     (n_cpu, len(parts), wall)}
+    log("cpu baseline, %d threads: %.2f lnL evals/s" % (len(parts), cpu["value"]))
     acc = {"serve_fp32_max_dlnl": float(max(abs(x - y) for x, y in zip(served[:8], ref))),
            "sweep_fp32_vs_fp64_max_dlnl": float(np.max(np.abs(sweeps["fp32"][:, 0] - sweeps["fp64"][:, 0]))),
            "sweep_fp64_vs_oracle_max_rel": float(np.max(np.abs(sweeps["fp64"][:, 0] - o.sweep()[:, 0]) /
@@ -366,6 +418,7 @@ def bench_states(args):
                       "requests": len(pop)},
            "fp64_plain_evals_per_s": e64, "fp32_plain_evals_per_s": e32, "fp32_served_evals_per_s": s32,
            "served_vs_fp64_plain": s32 / e64, "serve_stats": st, "sweeps": res, "accuracy": acc, "roofline": roof,
+           "cpu_baseline": cpu,
            "serve_config": {"cla_slots": slots, "hbm_budget_GB": slots * ds.n_rates * ((P + 255) // 256 * 256) * (64 if S == 61 else S) * 4 / 1e9,
                             "batch_window_us": SERVE_WINDOW_US}}
     print(json.dumps(out), flush=True)
