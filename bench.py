@@ -361,7 +361,9 @@ def bench_states(args):
         n_l, ms_l, bytes_l = inst.kernel_stats(0)
         sweeps[tag] = sw
         res[tag] = {"ms_per_sweep": ms, "site_updates_per_s": n_cla * P / (ms / 1000.0),
-                    "cla_kernel_GBs": bytes_l / max(ms_l, 1e-12) / 1e6, "cla_kernel_ms_per_sweep": ms_l / args.steps}
+                    "cla_kernel_GBs": bytes_l / max(ms_l, 1e-12) / 1e6, "cla_kernel_ms_per_sweep": ms_l / args.steps,
+                    "cla_launches": n_l, "cla_algorithmic_bytes_per_launch": bytes_l / max(n_l, 1),
+                    "cla_avg_launch_ms": ms_l / max(n_l, 1)}
         t.close()
         inst.close()
         log("%s sweep %.3f ms, CLA kernels %.1f GB/s" % (tag, ms, res[tag]["cla_kernel_GBs"]))
@@ -376,15 +378,26 @@ def bench_states(args):
         log("codon CLA MFMA: %.1f TFLOP/s (fp32 form) of %.1f published, %.1f measured register-only ceiling"
             % (res["fp32"]["codon_mfma_TFLOPs"], PEAK_FP64_TFLOPS, ceiling))
         roof = {"bound": "mfma", "achieved": res["fp32"]["codon_mfma_TFLOPs"], "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": res["fp32"]["codon_mfma_TFLOPs"] / PEAK_FP64_TFLOPS, "traffic": None,
+                "unit": "TFLOP/s", "frac": res["fp32"]["codon_mfma_TFLOPs"] / PEAK_FP64_TFLOPS,
+                "traffic": measured_traffic("k_cla_codon<float,3>|codon 32x5000 levels"),
+                "algorithmic_bytes_per_launch": res["fp32"]["cla_algorithmic_bytes_per_launch"],
+                "avg_launch_ms": res["fp32"]["cla_avg_launch_ms"], "launches": res["fp32"]["cla_launches"],
                 "measured_ceiling": ceiling, "frac_of_measured_ceiling": res["fp32"]["codon_mfma_TFLOPs"] / ceiling,
                 "kernel": "k_cla_codon<float,3>", "useful_flops_per_sweep": flops}
     else:
         gbs = res["fp32"]["cla_kernel_GBs"]
         roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                "traffic": None, "kernel": "k_cla_aa<float,4>"}
+                "traffic": measured_traffic("k_cla_aa<float,4>|aa 64x20000 levels"),
+                "algorithmic_bytes_per_launch": res["fp32"]["cla_algorithmic_bytes_per_launch"],
+                "avg_launch_ms": res["fp32"]["cla_avg_launch_ms"], "launches": res["fp32"]["cla_launches"],
+                "kernel": "k_cla_aa<float,4>"}
     if args.quick:
-        print(json.dumps({"workload": name, "sweeps": res, "roofline": roof}), flush=True)
+        copy = None
+        if args.calibrate > 0:
+            nbytes = int(args.calibrate * (1 << 30))
+            cms = E.copy_calibrate(nbytes, 10)
+            copy = {"bytes": nbytes, "ms": cms, "GB/s": 2 * nbytes / cms / 1e6}
+        print(json.dumps({"workload": name, "sweeps": res, "roofline": roof, "copy": copy}), flush=True)
         return
     pop = _population(ds, 64, 11)
     e64, lat64 = _eval_rates(ds, E, pop, E.PREC_FP64)

@@ -469,13 +469,16 @@ __device__ __forceinline__ unsigned mix32(unsigned h) {
   h ^= h >> 16;
   return h;
 }
-__device__ __forceinline__ float round_dither(double x, unsigned key) {
+// x * 2^scale rounded to FP32 with the dither drawn from x itself, so a value
+// scaled before its store rounds as the unscaled value would (FP32 normals).
+__device__ __forceinline__ float round_dither(double x, unsigned key, int scale = 0) {
   if (x == 0.0) return 0.0f;
   const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(x));
   const unsigned h = mix32(key * 0x9e3779b1u ^ unsigned(b >> 29));
-  const int e = __builtin_amdgcn_frexp_exp(x);  // x = m * 2^e, m in [0.5, 1)
+  const double y = scale ? __builtin_amdgcn_ldexp(x, scale) : x;
+  const int e = __builtin_amdgcn_frexp_exp(y);  // y = m * 2^e, m in [0.5, 1)
   const double ulp = __builtin_amdgcn_ldexp(1.0, e - 24 > -149 ? e - 24 : -149);
-  return float(x + double(int(h)) * 0x1p-32 * ulp);
+  return float(y + double(int(h)) * 0x1p-32 * ulp);
 }
 // The same dithered rounding done on the bit pattern, for kernels that store
 // many values per pattern: adding an integer k to the bits of a positive normal
@@ -641,9 +644,9 @@ __global__ __launch_bounds__(256) void k_cla_nuc(T *__restrict__ cla, int *__res
 // One child at a time, the live registers are its 20 sums plus the other
 // child's; the sums run in the
 // reference's q order, so the FP64 form keeps its mul-then-add sequence
-// (tree.cpp:7843-7910, 8082-8160, 8537-8599).  Each output is stored as
-// formed; a rescale (per pattern, over the R waves through a small LDS table)
-// re-reads and scales what the lane stored.
+// (tree.cpp:7843-7910, 8082-8160, 8537-8599).  The 20 products are held until
+// the op's rescale decision (per pattern, over the R waves through a small
+// LDS table) and then stored once, scaled where needed.
 // =====================================================================
 template <typename T, int R>
 __global__ __launch_bounds__(64 * R, sizeof(T) == 4 ? 1 : 3) void k_cla_aa(T *__restrict__ cla, int *__restrict__ ua,
@@ -722,24 +725,17 @@ __global__ __launch_bounds__(64 * R, sizeof(T) == 4 ? 1 : 3) void k_cla_aa(T *__
     child(op.c1, op.pm1, 0, a1);
     child(op.c2, op.pm2, 1, a2);
     T *dst = cla + op.dest * slot + off;
-    double m = 0.0;
-    unsigned seed = 0;  // FP32 form: dither seed of this (pattern, category) and op
+    double o[S], m = 0.0;
 #pragma unroll
     for (int f = 0; f < S; f++) {
-      const double o = mul_nc(a1[f], a2[f]);
-      m = fmax(m, o);
-      if constexpr (kF32) {
-        if (f == 0)
-          seed = mix32((unsigned(p) * R + unsigned(r)) * 0x9e3779b1u ^
-                       unsigned(static_cast<unsigned long long>(__double_as_longlong(o)) >> 29));
-      }
-      if (live) {
-        if constexpr (kF32)
-          dst[64 * f] = round_dither_bits(o, dither_stream(seed, unsigned(f)));
-        else
-          dst[64 * f] = o;
-      }
+      o[f] = mul_nc(a1[f], a2[f]);
+      m = fmax(m, o[f]);
     }
+    // The rescale is decided before anything is stored (per pattern, over
+    // the R waves through a small LDS table), so each value is written once.
+    int e = 0;           // FP32 form: exact power-of-two scale
+    double mult = 1.0;   // FP64 form: the reference's table multiplier
+    bool scale = false;
     if (op.flags & kOpRescale) {
       smax[r][lane] = m;
       __syncthreads();
@@ -748,21 +744,32 @@ __global__ __launch_bounds__(64 * R, sizeof(T) == 4 ? 1 : 3) void k_cla_aa(T *__
       if (live) {
         if constexpr (kF32) {
           if (m < 0x1p-32 && m > 0.0) {
-            const int e = 1 - __builtin_amdgcn_frexp_exp(m);  // max * 2^e in [1, 2)
-#pragma unroll
-            for (int f = 0; f < S; f++) dst[64 * f] = ldexpf(dst[64 * f], e);
+            e = 1 - __builtin_amdgcn_frexp_exp(m);  // max * 2^e in [1, 2)
             u += e;
+            scale = true;
           }
         } else if (m < c_rescale_below) {
           // RescaleRateHet / RescaleRateHetNState (tree.cpp:3423-3471, 3492-3598)
           if (m < c_reduce_below && r == 0) atomicOr(flags, m < c_bail_below ? 3 : 1);
           int idx = 0;
           while (idx + 1 < kRescaleLen && c_thresh[idx + 1] > m) idx++;
-          const double mult = c_mult[idx];
-#pragma unroll
-          for (int f = 0; f < S; f++) dst[64 * f] = mul_nc(dst[64 * f], mult);
+          mult = c_mult[idx];
           u += c_incr[idx];
+          scale = true;
         }
+      }
+    }
+    if (live) {
+      if constexpr (kF32) {
+        // dither seed of this (pattern, category) and op, from the unscaled first value
+        const unsigned seed = mix32((unsigned(p) * R + unsigned(r)) * 0x9e3779b1u ^
+                                    unsigned(static_cast<unsigned long long>(__double_as_longlong(o[0])) >> 29));
+#pragma unroll
+        for (int f = 0; f < S; f++)
+          dst[64 * f] = round_dither_bits(scale ? __builtin_amdgcn_ldexp(o[f], e) : o[f], dither_stream(seed, unsigned(f)));
+      } else {
+#pragma unroll
+        for (int f = 0; f < S; f++) dst[64 * f] = scale ? mul_nc(o[f], mult) : o[f];
       }
     }
     if (live && r == 0) ua[op.dest * size_t(Ppad) + p] = u;
@@ -877,9 +884,9 @@ __device__ __forceinline__ void codon_child(const T *__restrict__ cla, const dou
 // grid (Ppad/16, fused ? 1 : n_ops); a block is R waves over the same 16
 // patterns and wave r takes category r, so that a wave's serial MFMA chain is
 // one category long (small dependency levels fill the GPU R times faster).
-// Each product is stored as soon as it is formed; a rescale takes each
-// pattern's maximum over its 16 lanes and over the R waves (through LDS) and
-// each wave re-reads and scales what it stored.
+// The products are held until the op's rescale decision (each pattern's
+// maximum over its 16 lanes and over the R waves, through LDS) and stored
+// once, scaled where needed.
 template <typename T, int R>
 __global__ __launch_bounds__(64 * R, 3) void k_cla_codon(T *__restrict__ cla, int *__restrict__ ua,
                                                          const unsigned char *__restrict__ tips,
@@ -901,28 +908,19 @@ __global__ __launch_bounds__(64 * R, 3) void k_cla_codon(T *__restrict__ cla, in
     if (op.c1 < 0) codon_tip_states(tips, op.c1, Ppad, P, g, p0, ts1);
     if (op.c2 < 0) codon_tip_states(tips, op.c2, Ppad, P, g, p0, ts2);
     double mx[4] = {0.0, 0.0, 0.0, 0.0};
+    double ov[4][4];  // [pattern row v][state tile t], held until the rescale decision
     {
       gle_doublex4 a1[4], a2[4];
       codon_child(cla, pmpad + (size_t(op.pm1) * 3 * R + r) * 4096, op.c1, ts1, slot, R, r, l, p0, P, a1);
       codon_child(cla, pmpad + (size_t(op.pm2) * 3 * R + r) * 4096, op.c2, ts2, slot, R, r, l, p0, P, a2);
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int p = p0 + g + 4 * v;
+      for (int v = 0; v < 4; v++)
 #pragma unroll
         for (int t = 0; t < 4; t++) {
-          const int st = 16 * t + lo;
-          const double o = st < 61 ? mul_nc(a1[t][v], a2[t][v]) : 0.0;
+          const double o = 16 * t + lo < 61 ? mul_nc(a1[t][v], a2[t][v]) : 0.0;
+          ov[v][t] = o;
           mx[v] = fmax(mx[v], o);
-          if (p < P) {
-            T ov;
-            if constexpr (kF32)
-              ov = round_dither(o, unsigned(p) * 512u + unsigned(r * 64 + st));
-            else
-              ov = o;
-            dst[(size_t(p) * R + r) * 64 + st] = ov;
-          }
         }
-      }
     }
     if (op.flags & kOpRescale) {
 #pragma unroll
@@ -948,31 +946,37 @@ __global__ __launch_bounds__(64 * R, 3) void k_cla_codon(T *__restrict__ cla, in
         if (op.c1 >= 0) u += ua[size_t(op.c1) * Ppad + p];
         if (op.c2 >= 0) u += ua[size_t(op.c2) * Ppad + p];
       }
-      if (op.flags & kOpRescale) {
+      // the pattern's rescale (over its R x 64 values) is applied as the values are stored
+      int e = 0;
+      double mult = 1.0;
+      bool scale = false;
+      if ((op.flags & kOpRescale) && p < P) {
         const double m = mx[v];
-        if (p < P) {
-          if constexpr (kF32) {
-            if (m < 0x1p-32 && m > 0.0) {
-              const int e = 1 - __builtin_amdgcn_frexp_exp(m);
-#pragma unroll
-              for (int t = 0; t < 4; t++) {
-                T *q = dst + (size_t(p) * R + r) * 64 + 16 * t + lo;
-                *q = ldexpf(*q, e);
-              }
-              u += e;
-            }
-          } else if (m < c_rescale_below) {
-            if (m < c_reduce_below && lo == 0 && r == 0) atomicOr(flags, m < c_bail_below ? 3 : 1);
-            int idx = 0;
-            while (idx + 1 < kRescaleLen && c_thresh[idx + 1] > m) idx++;
-            const double mult = c_mult[idx];
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-              T *q = dst + (size_t(p) * R + r) * 64 + 16 * t + lo;
-              *q = mul_nc(*q, mult);
-            }
-            u += c_incr[idx];
+        if constexpr (kF32) {
+          if (m < 0x1p-32 && m > 0.0) {
+            e = 1 - __builtin_amdgcn_frexp_exp(m);
+            u += e;
+            scale = true;
           }
+        } else if (m < c_rescale_below) {
+          if (m < c_reduce_below && lo == 0 && r == 0) atomicOr(flags, m < c_bail_below ? 3 : 1);
+          int idx = 0;
+          while (idx + 1 < kRescaleLen && c_thresh[idx + 1] > m) idx++;
+          mult = c_mult[idx];
+          u += c_incr[idx];
+          scale = true;
+        }
+      }
+      if (p < P) {
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          const int st = 16 * t + lo;
+          T out;
+          if constexpr (kF32)
+            out = round_dither(ov[v][t], unsigned(p) * 512u + unsigned(r * 64 + st), scale ? e : 0);
+          else
+            out = scale ? mul_nc(ov[v][t], mult) : ov[v][t];
+          dst[(size_t(p) * R + r) * 64 + st] = out;
         }
       }
       if (p < P && lo == 0 && r == 0) ua[size_t(op.dest) * Ppad + p] = u;
