@@ -466,9 +466,10 @@ def bench_serve(args):
     # FP64 plain per-request drop-in
     insts64 = [ds.make_instance(E.PREC_FP64) for ds in dss]
     a = time.perf_counter()
+    plain = []
     for w, p, b in reqs:
         t = E.Tree(insts64[w], p, b)
-        t.score()
+        plain.append(t.score())
         t.close()
     t64 = time.perf_counter() - a
     for i in insts64:
@@ -497,13 +498,38 @@ def bench_serve(args):
     t32 = time.perf_counter() - a
     st = srv.stats()
     srv.close()
+    # every served FP32 result against the FP64 drop-in's score of the same request
+    dl = [abs(x - y) for x, y in zip(lnls, plain)]
+    acc = {"served_fp32_vs_fp64_max_dlnl": float(max(dl)), "served_fp32_vs_fp64_mean_dlnl": float(np.mean(dl))}
+    log("served fp32 vs fp64 plain: max |dlnL| %.2e over %d requests" % (max(dl), n_req))
+    cpu = None
+    if not args.no_cpu:
+        # CPU baseline: one request of each instance scored in full by the
+        # FP64 oracle, each request's patterns split over the host threads;
+        # the stream's rate is the inverse of the mean time over its mix
+        from oracle import oracle as O
+        T = _cpu_threads()
+        t_of = {}
+        for w, p, b in reqs:
+            if w in t_of:
+                continue
+            parts = _pattern_slices(O, dss[w], T)
+            t_of[w] = _threaded([lambda q=q, p=p, b=b: q.score(parent=p, blen=b) for q in parts])
+        mean_t = sum(t_of[w] for w, _, _ in reqs) / n_req
+        cpu = {"value": 1.0 / mean_t, "unit": "lnL evals/s", "cores": T, "kind": "port",
+               "sample": "one request of each of the %d instances scored in full (FP64, oracle built -O3 "
+                         "-ffast-math), patterns split over %d threads as the reference's OpenMP build splits them, "
+                         "%.2fs wall in all; weighted by the stream's mix of instances" % (len(t_of), T,
+                                                                                         sum(t_of.values()))}
+        log("cpu baseline, %d threads: %.2f lnL evals/s" % (T, cpu["value"]))
     out = {"metric": METRIC, "value": n_req / t32, "unit": "lnL evals/s", "n_gpus": 1, "steps": n_req,
            "warmup": 0, "ms_per_step": 1000.0 * t32 / n_req, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded simulations)",
            "config": {"workload": "GA request stream, Poisson arrivals, %d requests over %d instances" %
                       (n_req, len(dss)), "instances": [list(x) for x in specs], "arrival_rate_per_s": rate},
            "fp64_plain_evals_per_s": n_req / t64, "served_vs_fp64_plain": t64 / t32,
-           "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_batch": st["mean_batch"]}
+           "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_batch": st["mean_batch"], "accuracy": acc,
+           "cpu_baseline": cpu}
     log("serve: %.1f evals/s (plain fp64 %.1f), p50 %.2f ms p99 %.2f ms, mean batch %.1f" %
         (n_req / t32, n_req / t64, st["p50_ms"], st["p99_ms"], st["mean_batch"]))
     print(json.dumps(out), flush=True)
