@@ -1038,7 +1038,8 @@ __device__ __forceinline__ void block_reduce3(double a, double b, double c, doub
 // FP32 form: power-of-two counts; the invariant term is added at true scale.
 template <bool POW2>
 __device__ __forceinline__ void site_terms(double L, double D1, double D2, int u, double pinv_term, bool nstate,
-                                           int count, double &lnl, double &g1, double &g2, double &ul) {
+                                           int count, double &lnl, double &g1, double &g2, double &ul, double &r1o,
+                                           double &r2o) {
 #pragma clang fp contract(off)
   double sL, r1, r2;
   if (!POW2) {
@@ -1058,6 +1059,8 @@ __device__ __forceinline__ void site_terms(double L, double D1, double D2, int u
     r1 = D1 * sc * rs;
     r2 = D2 * sc * rs - r1 * r1;
   }
+  r1o = r1;
+  r2o = r2;
   if (nstate && !(ul < 0.0)) {  // NState scoring skips non-negative site lnL (tree.cpp:6473)
     lnl = g1 = g2 = 0.0;
     return;
@@ -1065,6 +1068,18 @@ __device__ __forceinline__ void site_terms(double L, double D1, double D2, int u
   lnl = double(count) * ul;
   g1 = double(count) * r1;
   g2 = double(count) * r2;
+}
+
+// Per-pattern values of an edge evaluated with site output, in five sections
+// of Ppad doubles: site lnL, d1 = D1/L, d2 = D2/L - d1^2, and the rescale
+// counts of the partial side and of the child side (-1 for a tip child).
+__device__ __forceinline__ void site_out(double *site, int Ppad, int p, double ul, double r1, double r2, int u1,
+                                         int u2) {
+  site[p] = ul;
+  site[Ppad + p] = r1;
+  site[2 * size_t(Ppad) + p] = r2;
+  site[3 * size_t(Ppad) + p] = double(u1);
+  site[4 * size_t(Ppad) + p] = double(u2);
 }
 
 // One thread per pattern over all categories.  P, P', P'' of the op and the
@@ -1089,7 +1104,8 @@ __global__ __launch_bounds__(256) void k_edge_nuc(const T *__restrict__ cla, con
   double lnl = 0, g1 = 0, g2 = 0;
   if (p < P) {
     double L[4] = {0, 0, 0, 0}, G1[4] = {0, 0, 0, 0}, G2[4] = {0, 0, 0, 0};
-    int u = ua[op.partial * size_t(Ppad) + p];
+    const int u1 = ua[op.partial * size_t(Ppad) + p];
+    int u2 = -1;
     const T *xp = cla + op.partial * slot + nuc_off(p, 0, R);
     const bool internal = op.child >= 0;
     int m = 15;
@@ -1100,7 +1116,7 @@ __global__ __launch_bounds__(256) void k_edge_nuc(const T *__restrict__ cla, con
       const T *yp = cla + op.child * slot + nuc_off(p, 0, R);
 #pragma unroll
       for (int r = 0; r < R; r++) load4(yp + 256 * r, yt[r]);
-      u += ua[op.child * size_t(Ppad) + p];
+      u2 = ua[op.child * size_t(Ppad) + p];
     } else {
       m = tips[size_t(-op.child - 1) * Ppad + p];
     }
@@ -1156,9 +1172,9 @@ __global__ __launch_bounds__(256) void k_edge_nuc(const T *__restrict__ cla, con
         if (cs & (1 << s)) btot += pi[s];
       pt = pinv * btot;
     }
-    double ul;
-    site_terms<sizeof(T) == 4>(Lt, D1, D2, u, pt, false, counts[p], lnl, g1, g2, ul);
-    if (site && blockIdx.y == 0) site[p] = ul;
+    double ul, r1, r2;
+    site_terms<sizeof(T) == 4>(Lt, D1, D2, u1 + (internal ? u2 : 0), pt, false, counts[p], lnl, g1, g2, ul, r1, r2);
+    if (site && blockIdx.y == 0) site_out(site, Ppad, p, ul, r1, r2, u1, u2);
   }
   block_reduce3(lnl, g1, g2, partials + (size_t(blockIdx.y) * gridDim.x + blockIdx.x) * 3);
 }
@@ -1244,17 +1260,18 @@ __global__ __launch_bounds__(64 * R) void k_edge_codon(const T *__restrict__ cla
     for (int v = 0; v < 4; v++) {
       const int p = p0 + g + 4 * v;
       if (p >= P) continue;
-      int u = ua[size_t(op.partial) * Ppad + p];
-      if (op.child >= 0) u += ua[size_t(op.child) * Ppad + p];
+      const int u1 = ua[size_t(op.partial) * Ppad + p];
+      const int u2 = op.child >= 0 ? ua[size_t(op.child) * Ppad + p] : -1;
       double pt = 0.0;
       const int cs = cst[p];
       if (pinv_in_model && cs >= 0 && cs < 61) pt = pinv * pi[cs];
-      double l1, d1, d2, ul;
-      site_terms<sizeof(T) == 4>(sL[v], s1[v], s2[v], u, pt, true, counts[p], l1, d1, d2, ul);
+      double l1, d1, d2, ul, r1, r2;
+      site_terms<sizeof(T) == 4>(sL[v], s1[v], s2[v], u1 + (u2 > 0 ? u2 : 0), pt, true, counts[p], l1, d1, d2, ul,
+                                 r1, r2);
       lnl += l1;
       g1 += d1;
       g2 += d2;
-      if (site && blockIdx.y == 0) site[p] = ul;
+      if (site && blockIdx.y == 0) site_out(site, Ppad, p, ul, r1, r2, u1, u2);
     }
   }
   block_reduce3(lnl, g1, g2, partials + (size_t(blockIdx.y) * gridDim.x + blockIdx.x) * 3);
@@ -1376,14 +1393,14 @@ __global__ __launch_bounds__(64 * R) void k_edge_aa(const T *__restrict__ cla, c
       s1 += sred[1][k][lane] * w[k];
       s2 += sred[2][k][lane] * w[k];
     }
-    int u = ua[op.partial * size_t(Ppad) + p];
-    if (op.child >= 0) u += ua[op.child * size_t(Ppad) + p];
+    const int u1 = ua[op.partial * size_t(Ppad) + p];
+    const int u2 = op.child >= 0 ? ua[op.child * size_t(Ppad) + p] : -1;
     double pt = 0.0;
     const int cs = cst[p];
     if (pinv_in_model && cs >= 0 && cs < S) pt = pinv * pi[cs];
-    double ul;
-    site_terms<sizeof(T) == 4>(sL, s1, s2, u, pt, true, counts[p], lnl, g1, g2, ul);
-    if (site && blockIdx.y == 0) site[p] = ul;
+    double ul, r1, r2;
+    site_terms<sizeof(T) == 4>(sL, s1, s2, u1 + (u2 > 0 ? u2 : 0), pt, true, counts[p], lnl, g1, g2, ul, r1, r2);
+    if (site && blockIdx.y == 0) site_out(site, Ppad, p, ul, r1, r2, u1, u2);
   }
   block_reduce3(lnl, g1, g2, partials + (size_t(blockIdx.y) * gridDim.x + blockIdx.x) * 3);
 }
@@ -1859,35 +1876,57 @@ static int enqueue_edges(Instance *I, const DevEdgeOp *ops, int n, bool site) {
   return GLE_SUCCESS;
 }
 
-static int copy_edges_out(Instance *I, int n, double *out, double *site_lnl) {
+// site5: host buffer of 5 x P values (lnL, d1, d2, partial and child rescale counts)
+static int copy_edges_out(Instance *I, int n, double *out, double *site5) {
   GLE_HIP(hipMemcpyAsync(out, I->d_eres, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, I->stream));
-  if (site_lnl)
-    GLE_HIP(hipMemcpyAsync(site_lnl, I->d_site, sizeof(double) * I->P, hipMemcpyDeviceToHost, I->stream));
+  if (site5)
+    for (int k = 0; k < 5; k++)
+      GLE_HIP(hipMemcpyAsync(site5 + size_t(k) * I->P, I->d_site + size_t(k) * I->Ppad, sizeof(double) * I->P,
+                             hipMemcpyDeviceToHost, I->stream));
   return GLE_SUCCESS;
 }
 
-// Edge results of an instance and its heavy-pattern part, in the caller's pattern order.
-int engine_edges(Instance *I, const DevEdgeOp *ops, int n, double *out, double *site_lnl) {
+// Edge results of an instance and its heavy-pattern part; with site5 (5 x P,
+// the caller's pattern order) also the per-pattern values of the ops' first
+// edge.
+static int edges_impl(Instance *I, const DevEdgeOp *ops, int n, double *out, double *site5) {
   std::lock_guard<std::mutex> lk_(I->mu);
   if (n <= 0) return GLE_SUCCESS;
   Instance *H = I->heavy.get();
-  const bool scatter = site_lnl && !I->light_idx.empty();
-  std::vector<double> site_l(scatter ? std::max(I->P, 1) : 0), site_h(H && site_lnl ? std::max(H->P, 1) : 0);
+  const bool scatter = site5 && H;  // the two parts' patterns go back to the caller's order
+  std::vector<double> site_l(scatter ? 5 * size_t(std::max(I->P, 1)) : 0),
+      site_h(H && site5 ? 5 * size_t(std::max(H->P, 1)) : 0);
   std::vector<double> out_h(H ? 3 * size_t(n) : 0, 0.0);
   // kernels of both parts first; the copies to host memory may wait for them
-  int rc = enqueue_edges(I, ops, n, site_lnl != nullptr);
-  if (!rc && H) rc = enqueue_edges(H, ops, n, site_lnl != nullptr);
-  if (!rc) rc = copy_edges_out(I, n, out, site_lnl ? (scatter ? site_l.data() : site_lnl) : nullptr);
-  if (!rc && H) rc = copy_edges_out(H, n, out_h.data(), site_lnl ? site_h.data() : nullptr);
+  int rc = enqueue_edges(I, ops, n, site5 != nullptr);
+  if (!rc && H) rc = enqueue_edges(H, ops, n, site5 != nullptr);
+  if (!rc) rc = copy_edges_out(I, n, out, site5 ? (scatter ? site_l.data() : site5) : nullptr);
+  if (!rc && H) rc = copy_edges_out(H, n, out_h.data(), site5 ? site_h.data() : nullptr);
   rc = finish_both(I, rc);
   if (rc) return rc;
   if (H)
     for (int i = 0; i < 3 * n; i++) out[i] += out_h[i];
   if (scatter) {
-    for (size_t k = 0; k < I->light_idx.size(); k++) site_lnl[I->light_idx[k]] = site_l[k];
-    for (size_t k = 0; k < I->heavy_idx.size(); k++) site_lnl[I->heavy_idx[k]] = site_h[k];
+    const size_t Pt = I->light_idx.size() + I->heavy_idx.size();
+    for (int k = 0; k < 5; k++) {
+      for (size_t j = 0; j < I->light_idx.size(); j++) site5[k * Pt + I->light_idx[j]] = site_l[k * size_t(I->P) + j];
+      for (size_t j = 0; j < I->heavy_idx.size(); j++) site5[k * Pt + I->heavy_idx[j]] = site_h[k * size_t(H->P) + j];
+    }
   }
   return GLE_SUCCESS;
+}
+
+int engine_edges(Instance *I, const DevEdgeOp *ops, int n, double *out, double *site_lnl) {
+  if (!site_lnl) return edges_impl(I, ops, n, out, nullptr);
+  const size_t Pt = size_t(instance_patterns(I));
+  std::vector<double> s5(5 * std::max<size_t>(Pt, 1));
+  int rc = edges_impl(I, ops, n, out, s5.data());
+  if (!rc) std::copy(s5.begin(), s5.begin() + Pt, site_lnl);
+  return rc;
+}
+
+int engine_edge_sites(Instance *I, const DevEdgeOp &op, double *out3, double *site5) {
+  return edges_impl(I, &op, 1, out3, site5);
 }
 
 // Rescale flags of an instance and its heavy part.  flags may be null (reset
@@ -2057,7 +2096,7 @@ static int create_internal(const gle_instance_desc *d, std::unique_ptr<Instance>
   GLE_ALLOC(I->d_Vinv, sizeof(double) * size_t(I->n_eig) * S * S);
   GLE_ALLOC(I->d_lam, sizeof(double) * size_t(I->n_eig) * S);
   GLE_ALLOC(I->d_model, sizeof(double) * (S + 2 * R));
-  GLE_ALLOC(I->d_site, sizeof(double) * I->Ppad);
+  GLE_ALLOC(I->d_site, sizeof(double) * 5 * size_t(I->Ppad));
 #undef GLE_ALLOC
   // zero everything the kernels may read before the caller sets it
   const unsigned char missing = (unsigned char)(S == 4 ? 15 : S);
@@ -2387,6 +2426,25 @@ extern "C" int gle_edge_likelihoods(int h, const gle_edge_op *ops, int n, double
   std::vector<DevEdgeOp> d(n);
   for (int i = 0; i < n; i++) d[i] = {ops[i].partial, ops[i].child, ops[i].pmat, ops[i].derivs ? 1 : 0};
   return engine_edges(I, d.data(), n, out, site);
+}
+
+extern "C" int gle_edge_site_values(int h, const gle_edge_op *op, double *out3, double *site_lnl, double *site_d1,
+                                    double *site_d2, int *under_partial, int *under_child) {
+  GET_INSTANCE(h);
+  if (!op || !out3) return set_error(GLE_ERR_ARG, "null argument");
+  const DevEdgeOp d = {op->partial, op->child, op->pmat, 1};
+  const size_t P = size_t(instance_patterns(I));
+  std::vector<double> s5(5 * std::max<size_t>(P, 1));
+  const int rc = engine_edge_sites(I, d, out3, s5.data());
+  if (rc) return rc;
+  for (size_t q = 0; q < P; q++) {
+    if (site_lnl) site_lnl[q] = s5[q];
+    if (site_d1) site_d1[q] = s5[P + q];
+    if (site_d2) site_d2[q] = s5[2 * P + q];
+    if (under_partial) under_partial[q] = int(s5[3 * P + q]);
+    if (under_child) under_child[q] = int(s5[4 * P + q]);
+  }
+  return GLE_SUCCESS;
 }
 
 extern "C" int gle_get_rescale_flags(int h, int *flags, int reset) {

@@ -63,6 +63,9 @@ int engine_update_pmats(Instance *, const int *idx, const double *t, int n, int 
 int engine_update_partials(Instance *, const DevClaOp *ops, int n_ops, const int *level_start, int n_levels,
                            bool fused);
 int engine_edges(Instance *, const DevEdgeOp *ops, int n, double *out, double *site_lnl);
+// one edge with its per-pattern values: site5 = 5 x patterns (lnL, d1, d2,
+// partial-side and child-side rescale counts, -1 for a tip child)
+int engine_edge_sites(Instance *, const DevEdgeOp &op, double *out3, double *site5);
 int engine_flags(Instance *, int *flags, int reset);
 int engine_get_partials(Instance *, int slot, double *out, int *u);
 int engine_sync(Instance *);

@@ -662,3 +662,38 @@ extern "C" int gle_write_sitelikes(const char *path, int append, int tree_num, d
   const bool ok = std::fclose(f) == 0;
   return ok ? GLE_SUCCESS : set_error(GLE_ERR_ARG, "write to %s failed", path);
 }
+
+// Tree::OutputSiteDerivatives (tree.cpp:5567-5595): an ostream at precision
+// 10 in its default float format, i.e. %.10g.
+extern "C" int gle_write_site_derivatives(const char *ordered_path, const char *packed_path, int subset, int nchar,
+                                          const int *site_pattern, int npat, const double *lnl, const double *d1,
+                                          const double *d2, const int *u1, const int *u2) {
+  if (nchar < 0 || npat < 0 || (npat > 0 && (!lnl || !d1 || !d2 || !u1)) || (nchar > 0 && !site_pattern))
+    return set_error(GLE_ERR_ARG, "bad argument");
+  for (int c = 0; c < nchar; c++)
+    if (site_pattern[c] < -1 || site_pattern[c] >= npat) return set_error(GLE_ERR_ARG, "character %d: bad pattern", c);
+  auto row = [&](std::FILE *f, int label, int q) {
+    std::fprintf(f, "%d\t%.10g\t%.10g\t%.10g\t%d", label, lnl[q], d1[q], d2[q], u1[q]);
+    if (u2) std::fprintf(f, "\t%d\n", u2[q]);
+    else std::fputs("\t-\n", f);
+  };
+  if (ordered_path) {
+    std::FILE *f = std::fopen(ordered_path, "w");
+    if (!f) return set_error(GLE_ERR_ARG, "cannot write %s", ordered_path);
+    std::fprintf(f, "Partition subset %d\nsite#\ttruelnL\td1\td2\tunder1\tunder2\n", subset);
+    for (int c = 0; c < nchar; c++) {
+      if (site_pattern[c] < 0) std::fprintf(f, "%d\tgap\t-\t-\t-\t-", c + 1);
+      else row(f, c + 1, site_pattern[c]);
+    }
+    if (std::fclose(f)) return set_error(GLE_ERR_ARG, "write to %s failed", ordered_path);
+  }
+  if (packed_path) {
+    std::FILE *f = std::fopen(packed_path, "w");
+    if (!f) return set_error(GLE_ERR_ARG, "cannot write %s", packed_path);
+    std::fprintf(f, "Partition subset %d\npackedIndex\ttruelnL\td1\td2\tunder1\tunder2\n", subset);
+    for (int q = 0; q < npat; q++) row(f, q, q);
+    if (std::fclose(f)) return set_error(GLE_ERR_ARG, "write to %s failed", packed_path);
+  }
+  return GLE_SUCCESS;
+}
+

@@ -259,7 +259,7 @@ static int child_code(Tree &t, int c, int a) {
 
 // One attempt of an edge evaluation.  Returns GLE_RESCALE_RETRY when the
 // reference would have thrown from RescaleRateHet.
-static int edge_attempt(Tree &t, int pa, int pb, bool derivs, double *res, double *site) {
+static int edge_attempt(Tree &t, int pa, int pb, bool derivs, double *res, double *site, double *site5) {
   std::vector<std::array<int, 3>> want;
   collect(t, pa, pb, want);
   collect(t, pb, pa, want);
@@ -275,7 +275,7 @@ static int edge_attempt(Tree &t, int pa, int pb, bool derivs, double *res, doubl
   op.child = child_code(t, pb, pa);
   op.pm = t.pm[en];
   op.derivs = derivs ? 1 : 0;
-  rc = engine_edges(t.I, &op, 1, res, site);
+  rc = site5 ? engine_edge_sites(t.I, op, res, site5) : engine_edges(t.I, &op, 1, res, site);
   if (rc) return rc;
   rc = engine_flags(t.I, &flags, 1);
   if (rc) return rc;
@@ -288,9 +288,9 @@ static int edge_attempt(Tree &t, int pa, int pb, bool derivs, double *res, doubl
 }
 
 // Retry loop of Tree::Score / BranchLike (tree.cpp:4077-4090)
-static int edge_eval(Tree &t, int pa, int pb, bool derivs, double *res, double *site) {
+static int edge_eval(Tree &t, int pa, int pb, bool derivs, double *res, double *site, double *site5 = nullptr) {
   for (;;) {
-    const int rc = edge_attempt(t, pa, pb, derivs, res, site);
+    const int rc = edge_attempt(t, pa, pb, derivs, res, site, site5);
     if (rc != GLE_RESCALE_RETRY) return rc;
     dirty_all(t);
     t.rescale_every -= 2;
@@ -529,6 +529,27 @@ extern "C" int gle_tree_derivatives(int h, int node, double *lnl, double *d1, do
   if (!lnl || !d1 || !d2) return set_error(GLE_ERR_ARG, "null output");
   GET_TREE(h);
   return tree_derivs(*T, node, lnl, d1, d2);
+}
+
+extern "C" int gle_tree_site_derivatives(int h, int node, double *out3, double *site_lnl, double *site_d1,
+                                         double *site_d2, int *under_partial, int *under_child) {
+  if (!out3) return set_error(GLE_ERR_ARG, "null output");
+  GET_TREE(h);
+  Tree &t = *T;
+  if (node <= 0 || node >= t.nn) return set_error(GLE_ERR_ARG, "node %d has no parent edge", node);
+  const size_t P = size_t(instance_patterns(t.I));
+  std::vector<double> s5(5 * std::max<size_t>(P, 1));
+  const int rc = edge_eval(t, t.parent[node], node, true, out3, nullptr, s5.data());
+  if (rc) return rc;
+  t.lnl = out3[0];
+  for (size_t q = 0; q < P; q++) {
+    if (site_lnl) site_lnl[q] = s5[q];
+    if (site_d1) site_d1[q] = s5[P + q];
+    if (site_d2) site_d2[q] = s5[2 * P + q];
+    if (under_partial) under_partial[q] = int(s5[3 * P + q]);
+    if (under_child) under_child[q] = int(s5[4 * P + q]);
+  }
+  return GLE_SUCCESS;
 }
 
 extern "C" int gle_tree_optimize_all_branches(int h, double prec, double *improve, double *lnl) {

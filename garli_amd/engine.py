@@ -78,6 +78,8 @@ SIGNATURES = {
     "gle_empirical_nuc_freqs": ([C.c_int, C.c_int, C.POINTER(C.c_uint8), C.POINTER(C.c_int), _D], C.c_int),
     "gle_codon_freqs_fx4": ([_D, C.c_int, _D], C.c_int),
     "gle_write_sitelikes": ([C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_int, _D, C.POINTER(C.c_int)], C.c_int),
+    "gle_write_site_derivatives": ([C.c_char_p, C.c_char_p, C.c_int, C.c_int, _I, C.c_int, _D, _D, _D, _I, _I],
+                                   C.c_int),
     "gle_create_instance": ([C.POINTER(InstanceDesc), _I], C.c_int),
     "gle_destroy_instance": ([C.c_int], C.c_int),
     "gle_set_tip_ambig_stream": ([C.c_int, C.c_int, _S8, C.c_int], C.c_int),
@@ -95,6 +97,7 @@ SIGNATURES = {
     "gle_update_partials": ([C.c_int, C.POINTER(ClaOp), C.c_int, _I, C.c_int], C.c_int),
     "gle_get_partials": ([C.c_int, C.c_int, _D, _I], C.c_int),
     "gle_edge_likelihoods": ([C.c_int, C.POINTER(EdgeOp), C.c_int, _D, _D], C.c_int),
+    "gle_edge_site_values": ([C.c_int, C.POINTER(EdgeOp), _D, _D, _D, _D, _I, _I], C.c_int),
     "gle_get_rescale_flags": ([C.c_int, _I, C.c_int], C.c_int),
     "gle_synchronize": ([C.c_int], C.c_int),
     "gle_set_timing": ([C.c_int, C.c_int], C.c_int),
@@ -110,6 +113,7 @@ SIGNATURES = {
     "gle_tree_score": ([C.c_int, _D], C.c_int),
     "gle_tree_site_likelihoods": ([C.c_int, _D], C.c_int),
     "gle_tree_derivatives": ([C.c_int, C.c_int, _D, _D, _D], C.c_int),
+    "gle_tree_site_derivatives": ([C.c_int, C.c_int, _D, _D, _D, _D, _I, _I], C.c_int),
     "gle_tree_optimize_all_branches": ([C.c_int, C.c_double, _D, _D], C.c_int),
     "gle_tree_sweep": ([C.c_int, _D], C.c_int),
     "gle_tree_get_cla": ([C.c_int, C.c_int, C.c_int, _D, _I], C.c_int),
@@ -380,6 +384,31 @@ def codon_freqs_fx4(base_freqs, f3x4=False):
     return pi
 
 
+def _site_values(call, P):
+    out = np.zeros(3)
+    v = {k: np.zeros(P) for k in ("lnl", "d1", "d2")}
+    u = {k: np.zeros(P, dtype=np.int32) for k in ("under_partial", "under_child")}
+    _check(call(_p(out, C.c_double), _p(v["lnl"], C.c_double), _p(v["d1"], C.c_double), _p(v["d2"], C.c_double),
+                _p(u["under_partial"], C.c_int), _p(u["under_child"], C.c_int)))
+    v.update(u)
+    return tuple(out), v
+
+
+def write_site_derivatives(ordered_path, packed_path, site_pattern, values, subset=1):
+    """orderedSiteDerivs / packedSiteDerivs logs from Tree.site_derivatives'
+    values; site_pattern maps each character to its pattern (-1: gap)."""
+    sp = np.ascontiguousarray(site_pattern, dtype=np.int32)
+    lnl, d1, d2 = (np.ascontiguousarray(values[k], dtype=np.float64) for k in ("lnl", "d1", "d2"))
+    u1 = np.ascontiguousarray(values["under_partial"], dtype=np.int32)
+    u2 = np.ascontiguousarray(values["under_child"], dtype=np.int32)
+    tip_child = bool(np.all(u2 < 0))
+    _check(lib().gle_write_site_derivatives(
+        None if ordered_path is None else os.fsencode(ordered_path),
+        None if packed_path is None else os.fsencode(packed_path), int(subset), len(sp), _p(sp, C.c_int), len(lnl),
+        _p(lnl, C.c_double), _p(d1, C.c_double), _p(d2, C.c_double), _p(u1, C.c_int),
+        None if tip_child else _p(u2, C.c_int)))
+
+
 def write_sitelikes(path, lnl, site_lnl, included, tree_num=1, append=False):
     """<ofprefix>.sitelikes.log in GARLI's format (per character -lnL or "-",
     then the tree's -lnL)."""
@@ -514,6 +543,12 @@ class Instance:
                                           _p(site, C.c_double) if want_sites else None))
         return (out.reshape(-1, 3), site) if want_sites else out.reshape(-1, 3)
 
+    def edge_site_values(self, op):
+        """One edge (partial, child, pmat) with derivatives: ((lnL, d1, d2),
+        per-pattern dict of lnl, d1, d2, under_partial, under_child)."""
+        e = EdgeOp(op[0], op[1], op[2], 1)
+        return _site_values(lambda *a: lib().gle_edge_site_values(self.h, C.byref(e), *a), self.n_patterns)
+
     def set_timing(self, enable=True):
         _check(lib().gle_set_timing(self.h, int(enable)))
 
@@ -577,6 +612,11 @@ class Tree:
         a, b, c = C.c_double(0), C.c_double(0), C.c_double(0)
         _check(lib().gle_tree_derivatives(self.h, node, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
+
+    def site_derivatives(self, node):
+        """Derivatives on the edge above `node` with their per-pattern values:
+        ((lnL, d1, d2), dict of lnl, d1, d2, under_partial, under_child)."""
+        return _site_values(lambda *a: lib().gle_tree_site_derivatives(self.h, int(node), *a), self.inst.n_patterns)
 
     def set_branch_length(self, node, v):
         _check(lib().gle_tree_set_branch_length(self.h, node, float(v)))

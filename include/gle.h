@@ -131,6 +131,15 @@ int gle_codon_freqs_fx4(const double *base_freqs, int f3x4, double *pi61);
  * line per character (-lnL, or "-" where included[s] == 0), then
  * "tree_num <tab> -lnl" (Tree::OutputSiteLikelihoods, tree.cpp:5501-5565;
  * population.cpp:3459-3478). */
+/* orderedSiteDerivs / packedSiteDerivs logs in Tree::OutputSiteDerivatives's
+ * layout (tree.cpp:5567-5595): "Partition subset <subset>" and a header, then
+ * per character (site_pattern[c] = its pattern, or -1: "gap") and per pattern
+ * the site lnL, d1, d2 (10 significant digits) and the rescale counts
+ * (under_child NULL: "-").  As in the reference, a gap line ends without a
+ * newline.  Either path may be NULL. */
+int gle_write_site_derivatives(const char *ordered_path, const char *packed_path, int subset, int nchar,
+                               const int *site_pattern, int n_patterns, const double *site_lnl, const double *site_d1,
+                               const double *site_d2, const int *under_partial, const int *under_child);
 int gle_write_sitelikes(const char *path, int append, int tree_num, double lnl, int nchar, const double *site_lnl,
                         const int *included);
 
@@ -213,6 +222,16 @@ typedef struct gle_edge_op {
   int partial, child, pmat, derivs;
 } gle_edge_op;
 int gle_edge_likelihoods(int h, const gle_edge_op *ops, int n, double *out, double *site_lnl);
+/* One edge with derivatives and its per-pattern values, the inputs of
+ * Tree::OutputSiteDerivatives (tree.cpp:5567-5595; filled under
+ * OUTPUT_SITEDERIVS in GetDerivsPartialTerminal / Internal,
+ * optimization.cpp:2412-3228): out3 = {lnL, d1, d2}; per pattern (any output
+ * may be NULL) the unscaled site lnL, d1 = D1/L and d2 = D2/L - d1^2 (not
+ * multiplied by the pattern count) and the rescale counts of the partial and
+ * child sides (-1 for a tip child).  Counts are in the instance's units: the
+ * reference's for the FP64 form, powers of two for the FP32 form. */
+int gle_edge_site_values(int h, const gle_edge_op *op, double *out3, double *site_lnl, double *site_d1,
+                         double *site_d2, int *under_partial, int *under_child);
 /* rescale flags accumulated since the last reset (see GLE_FLAG_*) */
 int gle_get_rescale_flags(int h, int *flags, int reset);
 int gle_synchronize(int h);
@@ -254,6 +273,10 @@ int gle_tree_score(int tree, double *lnl);
 int gle_tree_site_likelihoods(int tree, double *site_lnl);
 /* Tree::CalcDerivativesRateHet(nd->anc, nd) (optimization.cpp:1099-1194) */
 int gle_tree_derivatives(int tree, int node, double *lnl, double *d1, double *d2);
+/* As gle_tree_derivatives, with the per-pattern values of gle_edge_site_values
+ * for the edge above `node` (partial side: its parent). */
+int gle_tree_site_derivatives(int tree, int node, double *out3, double *site_lnl, double *site_d1, double *site_d2,
+                              int *under_partial, int *under_child);
 /* Tree::OptimizeAllBranches (optimization.cpp:97-105, 1359-1972) */
 int gle_tree_optimize_all_branches(int tree, double precision, double *improve, double *lnl);
 /* Whole-tree sweep: every directional CLA, then (lnL, d1, d2) on every edge

@@ -112,3 +112,29 @@ def test_sitelikes_file_and_runtests_sum(tmp_path):
     assert col2 / 2 == pytest.approx((12.0 + 1.5 + 3.0 + 4.125) / 2)
     E.write_sitelikes(str(path), -12.0, site, inc, tree_num=2, append=True)
     assert path.read_text().count("Tree\t-lnL") == 1 and path.read_text().splitlines()[-1] == "2\t12"
+
+
+def test_site_derivative_logs(tmp_path):
+    """Tree::OutputSiteDerivatives' two files (tree.cpp:5567-5595): the
+    ordered one per character, "gap" (with no line end, as the reference
+    writes it) for a character without a pattern; the packed one per pattern;
+    %.10g values, "-" for the child count of a tip child."""
+    vals = {"lnl": np.array([-1.25, -2.5, -0.123456789012]), "d1": np.array([0.5, -1e-5, 3.0]),
+            "d2": np.array([-0.25, -2.0, 1.0 / 3.0]), "under_partial": np.array([0, 7, 14], np.int32),
+            "under_child": np.array([-1, -1, -1], np.int32)}
+    o, pk = tmp_path / "orderedSiteDerivs.log", tmp_path / "packedSiteDerivs.log"
+    E.write_site_derivatives(str(o), str(pk), [2, -1, 0, 1], vals)
+    assert o.read_text() == ("Partition subset 1\nsite#\ttruelnL\td1\td2\tunder1\tunder2\n"
+                             "1\t-0.123456789\t3\t0.3333333333\t14\t-\n"
+                             "2\tgap\t-\t-\t-\t-"
+                             "3\t-1.25\t0.5\t-0.25\t0\t-\n"
+                             "4\t-2.5\t-1e-05\t-2\t7\t-\n")
+    assert pk.read_text().splitlines()[1:] == ["packedIndex\ttruelnL\td1\td2\tunder1\tunder2",
+                                               "0\t-1.25\t0.5\t-0.25\t0\t-", "1\t-2.5\t-1e-05\t-2\t7\t-",
+                                               "2\t-0.123456789\t3\t0.3333333333\t14\t-"]
+    vals["under_child"] = np.array([3, 0, 5], np.int32)
+    E.write_site_derivatives(None, str(pk), [0, 1, 2], vals, subset=2)
+    assert pk.read_text().splitlines()[0] == "Partition subset 2"
+    assert pk.read_text().splitlines()[2] == "0\t-1.25\t0.5\t-0.25\t0\t3"
+    with pytest.raises(E.GleError):
+        E.write_site_derivatives(str(o), None, [0, 5], vals)
