@@ -1104,8 +1104,7 @@ __global__ __launch_bounds__(256) void k_edge_nuc(const T *__restrict__ cla, con
   double lnl = 0, g1 = 0, g2 = 0;
   if (p < P) {
     double L[4] = {0, 0, 0, 0}, G1[4] = {0, 0, 0, 0}, G2[4] = {0, 0, 0, 0};
-    const int u1 = ua[op.partial * size_t(Ppad) + p];
-    int u2 = -1;
+    int u = ua[op.partial * size_t(Ppad) + p];
     const T *xp = cla + op.partial * slot + nuc_off(p, 0, R);
     const bool internal = op.child >= 0;
     int m = 15;
@@ -1116,7 +1115,7 @@ __global__ __launch_bounds__(256) void k_edge_nuc(const T *__restrict__ cla, con
       const T *yp = cla + op.child * slot + nuc_off(p, 0, R);
 #pragma unroll
       for (int r = 0; r < R; r++) load4(yp + 256 * r, yt[r]);
-      u2 = ua[op.child * size_t(Ppad) + p];
+      u += ua[op.child * size_t(Ppad) + p];
     } else {
       m = tips[size_t(-op.child - 1) * Ppad + p];
     }
@@ -1173,8 +1172,12 @@ __global__ __launch_bounds__(256) void k_edge_nuc(const T *__restrict__ cla, con
       pt = pinv * btot;
     }
     double ul, r1, r2;
-    site_terms<sizeof(T) == 4>(Lt, D1, D2, u1 + (internal ? u2 : 0), pt, false, counts[p], lnl, g1, g2, ul, r1, r2);
-    if (site && blockIdx.y == 0) site_out(site, Ppad, p, ul, r1, r2, u1, u2);
+    site_terms<sizeof(T) == 4>(Lt, D1, D2, u, pt, false, counts[p], lnl, g1, g2, ul, r1, r2);
+    // the two sides' counts are read again here rather than kept live (the
+    // kernel sits just under the 128-VGPR line)
+    if (site && blockIdx.y == 0)
+      site_out(site, Ppad, p, ul, r1, r2, ua[op.partial * size_t(Ppad) + p],
+               internal ? ua[op.child * size_t(Ppad) + p] : -1);
   }
   block_reduce3(lnl, g1, g2, partials + (size_t(blockIdx.y) * gridDim.x + blockIdx.x) * 3);
 }
@@ -1876,57 +1879,53 @@ static int enqueue_edges(Instance *I, const DevEdgeOp *ops, int n, bool site) {
   return GLE_SUCCESS;
 }
 
-// site5: host buffer of 5 x P values (lnL, d1, d2, partial and child rescale counts)
-static int copy_edges_out(Instance *I, int n, double *out, double *site5) {
+// site: host buffer of nsec x P values, the first nsec of (lnL, d1, d2,
+// partial and child rescale counts)
+static int copy_edges_out(Instance *I, int n, double *out, double *site, int nsec) {
   GLE_HIP(hipMemcpyAsync(out, I->d_eres, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, I->stream));
-  if (site5)
-    for (int k = 0; k < 5; k++)
-      GLE_HIP(hipMemcpyAsync(site5 + size_t(k) * I->P, I->d_site + size_t(k) * I->Ppad, sizeof(double) * I->P,
+  if (site)
+    for (int k = 0; k < nsec; k++)
+      GLE_HIP(hipMemcpyAsync(site + size_t(k) * I->P, I->d_site + size_t(k) * I->Ppad, sizeof(double) * I->P,
                              hipMemcpyDeviceToHost, I->stream));
   return GLE_SUCCESS;
 }
 
-// Edge results of an instance and its heavy-pattern part; with site5 (5 x P,
-// the caller's pattern order) also the per-pattern values of the ops' first
-// edge.
-static int edges_impl(Instance *I, const DevEdgeOp *ops, int n, double *out, double *site5) {
+// Edge results of an instance and its heavy-pattern part; with site (nsec x
+// patterns, the caller's pattern order) also the per-pattern values of the
+// ops' first edge: its lnL (nsec = 1) or all five sections (nsec = 5).
+static int edges_impl(Instance *I, const DevEdgeOp *ops, int n, double *out, double *site, int nsec) {
   std::lock_guard<std::mutex> lk_(I->mu);
   if (n <= 0) return GLE_SUCCESS;
   Instance *H = I->heavy.get();
-  const bool scatter = site5 && H;  // the two parts' patterns go back to the caller's order
-  std::vector<double> site_l(scatter ? 5 * size_t(std::max(I->P, 1)) : 0),
-      site_h(H && site5 ? 5 * size_t(std::max(H->P, 1)) : 0);
+  const bool scatter = site && H;  // the two parts' patterns go back to the caller's order
+  std::vector<double> site_l(scatter ? nsec * size_t(std::max(I->P, 1)) : 0),
+      site_h(scatter ? nsec * size_t(std::max(H->P, 1)) : 0);
   std::vector<double> out_h(H ? 3 * size_t(n) : 0, 0.0);
   // kernels of both parts first; the copies to host memory may wait for them
-  int rc = enqueue_edges(I, ops, n, site5 != nullptr);
-  if (!rc && H) rc = enqueue_edges(H, ops, n, site5 != nullptr);
-  if (!rc) rc = copy_edges_out(I, n, out, site5 ? (scatter ? site_l.data() : site5) : nullptr);
-  if (!rc && H) rc = copy_edges_out(H, n, out_h.data(), site5 ? site_h.data() : nullptr);
+  int rc = enqueue_edges(I, ops, n, site != nullptr);
+  if (!rc && H) rc = enqueue_edges(H, ops, n, site != nullptr);
+  if (!rc) rc = copy_edges_out(I, n, out, site ? (scatter ? site_l.data() : site) : nullptr, nsec);
+  if (!rc && H) rc = copy_edges_out(H, n, out_h.data(), site ? site_h.data() : nullptr, nsec);
   rc = finish_both(I, rc);
   if (rc) return rc;
   if (H)
     for (int i = 0; i < 3 * n; i++) out[i] += out_h[i];
   if (scatter) {
     const size_t Pt = I->light_idx.size() + I->heavy_idx.size();
-    for (int k = 0; k < 5; k++) {
-      for (size_t j = 0; j < I->light_idx.size(); j++) site5[k * Pt + I->light_idx[j]] = site_l[k * size_t(I->P) + j];
-      for (size_t j = 0; j < I->heavy_idx.size(); j++) site5[k * Pt + I->heavy_idx[j]] = site_h[k * size_t(H->P) + j];
+    for (int k = 0; k < nsec; k++) {
+      for (size_t j = 0; j < I->light_idx.size(); j++) site[k * Pt + I->light_idx[j]] = site_l[k * size_t(I->P) + j];
+      for (size_t j = 0; j < I->heavy_idx.size(); j++) site[k * Pt + I->heavy_idx[j]] = site_h[k * size_t(H->P) + j];
     }
   }
   return GLE_SUCCESS;
 }
 
 int engine_edges(Instance *I, const DevEdgeOp *ops, int n, double *out, double *site_lnl) {
-  if (!site_lnl) return edges_impl(I, ops, n, out, nullptr);
-  const size_t Pt = size_t(instance_patterns(I));
-  std::vector<double> s5(5 * std::max<size_t>(Pt, 1));
-  int rc = edges_impl(I, ops, n, out, s5.data());
-  if (!rc) std::copy(s5.begin(), s5.begin() + Pt, site_lnl);
-  return rc;
+  return edges_impl(I, ops, n, out, site_lnl, 1);
 }
 
 int engine_edge_sites(Instance *I, const DevEdgeOp &op, double *out3, double *site5) {
-  return edges_impl(I, &op, 1, out3, site5);
+  return edges_impl(I, &op, 1, out3, site5, 5);
 }
 
 // Rescale flags of an instance and its heavy part.  flags may be null (reset
