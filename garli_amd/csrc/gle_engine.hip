@@ -87,6 +87,7 @@ struct Instance {
   void *d_cla = nullptr;
   double *d_pm64 = nullptr;
   double *d_pmpad = nullptr;  // codons (padded) and amino acids (transposed) only
+  double *d_pdiag = nullptr;  // amino acids: exp(s t), s exp(s t), s^2 exp(s t) per [pmat][R][3][S], s = lam * rate
   double *d_V = nullptr, *d_Vinv = nullptr, *d_lam = nullptr, *d_model = nullptr;  // model: pi[S] w[R] mult[R]
   // staging
   DevClaOp *d_ops = nullptr;
@@ -359,8 +360,8 @@ __global__ __launch_bounds__(256) void k_pmat(int S, int Sp, int R, const int *_
                                               const double *__restrict__ tt, const int *__restrict__ eig_of_rate,
                                               const double *__restrict__ rate_mult, const double *__restrict__ V,
                                               const double *__restrict__ Vinv, const double *__restrict__ lam,
-                                              double *__restrict__ pm64, double *__restrict__ pmpad, int derivs,
-                                              int layout) {
+                                              double *__restrict__ pm64, double *__restrict__ pmpad,
+                                              double *__restrict__ pdiag, int derivs, int layout) {
 #pragma clang fp contract(off)
   extern __shared__ double sm[];
   const int SS = S * S, op = blockIdx.x, r = blockIdx.y;
@@ -371,14 +372,20 @@ __global__ __launch_bounds__(256) void k_pmat(int S, int Sp, int R, const int *_
     sW[q] = Vinv[size_t(e) * SS + q];
   }
   const double t = tt[op], mult = rate_mult[r];
+  const int slot = idx[op];
   for (int k = threadIdx.x; k < S; k += blockDim.x) {
     const double s = lam[size_t(e) * S + k] * mult;
     ex[k] = exp(s * t);
     e1[k] = s * ex[k];
     e2[k] = s * e1[k];
+    if (pdiag) {
+      double *d = pdiag + (size_t(slot) * R + r) * 3 * S;
+      d[k] = ex[k];
+      d[S + k] = e1[k];
+      d[2 * S + k] = e2[k];
+    }
   }
   __syncthreads();
-  const int slot = idx[op];
   double *P = pm64 + (size_t(slot) * 3 * R + r) * SS;
   double *D1 = pm64 + (size_t(slot) * 3 * R + R + r) * SS;
   double *D2 = pm64 + (size_t(slot) * 3 * R + 2 * R + r) * SS;
@@ -1288,7 +1295,10 @@ __global__ __launch_bounds__(64 * R) void k_edge_codon(const T *__restrict__ cla
 template <typename T, int R>
 __global__ __launch_bounds__(64 * R) void k_edge_aa(const T *__restrict__ cla, const int *__restrict__ ua,
                                                     const unsigned char *__restrict__ tips,
-                                                    const double *__restrict__ pmT, const double *__restrict__ model,
+                                                    const double *__restrict__ pmT, const double *__restrict__ pdiag,
+                                                    const double *__restrict__ Vg, const double *__restrict__ Wg,
+                                                    const int *__restrict__ eig_of_rate,
+                                                    const double *__restrict__ model,
                                                     const int *__restrict__ counts, const int *__restrict__ cst,
                                                     const DevEdgeOp *__restrict__ ops, double *__restrict__ partials,
                                                     double *__restrict__ site, double pinv, int pinv_in_model,
@@ -1308,7 +1318,22 @@ __global__ __launch_bounds__(64 * R) void k_edge_aa(const T *__restrict__ cla, c
   // transposed P, P', P'' of category r: PT[(mi * R) * S * S + q * S + f] = M_mi[f][q]
   const double *PT = pmT + (size_t(op.pm) * 3 * R + r) * S * S;
   const double *pi = model, *w = model + S;
-  {
+  // FP32 form, derivatives on an internal child: spectral route.  With
+  // P = V diag(ex) V^-1, z = V^-1 y and a = V^T (x * pi), the three sums are
+  // sum_k a_k z_k {ex_k, s ex_k, s^2 ex_k}: two 20 x 20 products per pattern
+  // and category instead of three (P' y and P'' y need no matrices of their
+  // own).  The FP64 form keeps the reference's order of operations.
+  const bool spectral = sizeof(T) == 4 && op.derivs && op.child >= 0;
+  if (spectral) {
+    // sM[r][0]: V (row f = V[f][0..S)); sM[r][1]: V^-1 transposed (row q = V^-1[0..S)[q])
+    const int e = eig_of_rate[r];
+    const double *Ve = Vg + size_t(e) * S * S, *We = Wg + size_t(e) * S * S;
+    for (int k = lane; k < S * S; k += 64) {
+      sM[r][0][k] = Ve[k];
+      sM[r][1][(k % S) * S + k / S] = We[k];
+    }
+    __syncthreads();
+  } else {
     const int nmat = op.derivs ? 3 : 1;
     for (int mi = 0; mi < nmat; mi++) {
       const double2 *g = reinterpret_cast<const double2 *>(PT + size_t(mi) * R * S * S);
@@ -1324,7 +1349,44 @@ __global__ __launch_bounds__(64 * R) void k_edge_aa(const T *__restrict__ cla, c
 #pragma unroll
     for (int f = 0; f < S; f++) x[f] = cla[op.partial * slot + off + 64 * f];
     const int nm = op.derivs ? 3 : 1;
-    if (op.child >= 0) {
+    if (spectral) {
+      const double *Vs = sM[r][0], *WT = sM[r][1];
+      const double *dg = pdiag + (size_t(op.pm) * R + r) * 3 * S;  // ex, s ex, s^2 ex
+      double z[S];
+      {
+        T y[S];
+#pragma unroll
+        for (int q = 0; q < S; q++) y[q] = cla[op.child * slot + off + 64 * q];
+#pragma unroll
+        for (int k = 0; k < S; k++) z[k] = 0.0;
+#pragma unroll
+        for (int q = 0; q < S; q++) {
+          const double yq = double(y[q]);
+#pragma unroll
+          for (int k = 0; k < S; k++) z[k] = __builtin_fma(WT[q * S + k], yq, z[k]);
+        }
+      }
+      double av[S];
+#pragma unroll
+      for (int k = 0; k < S; k++) av[k] = 0.0;
+#pragma unroll
+      for (int f = 0; f < S; f++) {
+        const double xf = double(x[f]) * pi[f];
+#pragma unroll
+        for (int k = 0; k < S; k++) av[k] = __builtin_fma(Vs[f * S + k], xf, av[k]);
+      }
+      double l0 = 0.0, l1 = 0.0, l2 = 0.0;
+#pragma unroll
+      for (int k = 0; k < S; k++) {
+        const double c = av[k] * z[k];
+        l0 = __builtin_fma(c, dg[k], l0);
+        l1 = __builtin_fma(c, dg[S + k], l1);
+        l2 = __builtin_fma(c, dg[2 * S + k], l2);
+      }
+      rr[0] = l0;
+      rr[1] = l1;
+      rr[2] = l2;
+    } else if (op.child >= 0) {
       T y[S];
 #pragma unroll
       for (int q = 0; q < S; q++) y[q] = cla[op.child * slot + off + 64 * q];
@@ -1536,7 +1598,7 @@ static int enqueue_pmats(Instance *I, const int *idx, const double *t, int n, in
   timed_begin(I, 2, double(n) * I->R * S * S * 8.0 * (derivs ? 3 : 1));
   GLE_LAUNCH(I, k_pmat, dim3(n, I->R), dim3(256), shmem, S, I->Sp, I->R, I->d_pmidx, I->d_pmt,
                      I->d_eig_of_rate, I->d_model + S + I->R, I->d_V, I->d_Vinv, I->d_lam, I->d_pm64, I->d_pmpad,
-                     derivs, S == 20 ? kPadTransposed : S == 61 ? kPadCodonFrag : kPadRowMajor);
+                     I->d_pdiag, derivs, S == 20 ? kPadTransposed : S == 61 ? kPadCodonFrag : kPadRowMajor);
   timed_end(I);
   GLE_HIP(hipGetLastError());
   return GLE_SUCCESS;
@@ -1796,8 +1858,9 @@ static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk, bo
 #undef GLE_EDGE_NUC
   } else if (I->S == 20) {
 #define GLE_EDGE_AA(RR)                                                                                           \
-  GLE_LAUNCH(I, (k_edge_aa<T, RR>), grid, dim3(64 * RR), 0, a.cla, a.u, a.tips, I->d_pmpad, a.model,            \
-                     a.counts, a.cst, d_ops, a.partials, a.site, a.pinv, a.pinv_in_model, a.Ppad, a.P)
+  GLE_LAUNCH(I, (k_edge_aa<T, RR>), grid, dim3(64 * RR), 0, a.cla, a.u, a.tips, I->d_pmpad, I->d_pdiag,         \
+                     I->d_V, I->d_Vinv, I->d_eig_of_rate, a.model, a.counts, a.cst, d_ops, a.partials, a.site, a.pinv, \
+                     a.pinv_in_model, a.Ppad, a.P)
     switch (I->R) {
       case 1: GLE_EDGE_AA(1); break;
       case 2: GLE_EDGE_AA(2); break;
@@ -2014,7 +2077,7 @@ static int destroy(Instance *I) {
     I->heavy.reset();
   }
   void *ptrs[] = {I->d_tips, I->d_counts, I->d_cst, I->d_u, I->d_flags, I->d_eig_of_rate, I->d_cla,
-                  I->d_pm64, I->d_pmpad,  I->d_V,   I->d_Vinv, I->d_lam, I->d_model, I->d_ops,
+                  I->d_pm64, I->d_pmpad,  I->d_pdiag, I->d_V, I->d_Vinv, I->d_lam, I->d_model, I->d_ops,
                   I->d_eops, I->d_part,  I->d_eres, I->d_site, I->d_pmidx, I->d_pmt};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -2091,6 +2154,7 @@ static int create_internal(const gle_instance_desc *d, std::unique_ptr<Instance>
   GLE_ALLOC(I->d_cla, I->slot_elems() * I->elem_bytes() * I->n_slots);
   GLE_ALLOC(I->d_pm64, sizeof(double) * size_t(I->n_pm) * 3 * R * S * S);
   if (S == 61 || S == 20) GLE_ALLOC(I->d_pmpad, sizeof(double) * size_t(I->n_pm) * 3 * R * I->Sp * I->Sp);
+  if (S == 20 && I->prec == GLE_PREC_FP32) GLE_ALLOC(I->d_pdiag, sizeof(double) * size_t(I->n_pm) * R * 3 * S);
   GLE_ALLOC(I->d_V, sizeof(double) * size_t(I->n_eig) * S * S);
   GLE_ALLOC(I->d_Vinv, sizeof(double) * size_t(I->n_eig) * S * S);
   GLE_ALLOC(I->d_lam, sizeof(double) * size_t(I->n_eig) * S);
@@ -2107,7 +2171,8 @@ static int create_internal(const gle_instance_desc *d, std::unique_ptr<Instance>
       hipMemset(I->d_cla, 0, I->slot_elems() * I->elem_bytes() * I->n_slots) != hipSuccess ||
       hipMemset(I->d_pm64, 0, sizeof(double) * size_t(I->n_pm) * 3 * R * S * S) != hipSuccess ||
       (I->d_pmpad && hipMemset(I->d_pmpad, 0, sizeof(double) * size_t(I->n_pm) * 3 * R * I->Sp * I->Sp) !=
-                         hipSuccess))
+                         hipSuccess) ||
+      (I->d_pdiag && hipMemset(I->d_pdiag, 0, sizeof(double) * size_t(I->n_pm) * R * 3 * S) != hipSuccess))
     return fail(set_error(GLE_ERR_DEVICE, "device memset failed"));
   I->h_tips.assign(size_t(I->N) * I->P_orig, missing);
   I->h_pi.assign(S, 1.0 / S);
