@@ -188,7 +188,10 @@ int gle_set_fused_threshold(int h, int min_patterns);
 int gle_set_graph_replay(int h, int enable, int *replays);
 /* state freqs [S], category weights [R] (Model::GetRateProbs, model.h:2051-2064), pinv */
 int gle_set_model(int h, const double *pi, const double *rate_probs, double pinv, int pinv_in_model);
-/* eigensystem e: V, Vinv [S*S], lam [S] (already scaled) */
+/* eigensystem e: V, Vinv [S*S], lam [S] (already scaled).  Transition
+ * matrices computed before the call describe the old model: recompute them
+ * (gle_update_transition_matrices) before the next edge or CLA update, as
+ * the reference's Model::UpdateQMat / CalcPmats order requires. */
 int gle_set_eigen(int h, int e, const double *V, const double *Vinv, const double *lam);
 /* per category: which eigensystem and the multiplier on lam
  * (rateMult/(1-pinv) with pinv in the model, rateMult otherwise, 1 for omega classes;
