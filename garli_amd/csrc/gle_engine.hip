@@ -87,7 +87,9 @@ struct Instance {
   void *d_cla = nullptr;
   double *d_pm64 = nullptr;
   double *d_pmpad = nullptr;  // codons (padded) and amino acids (transposed) only
-  double *d_pdiag = nullptr;  // amino acids: exp(s t), s exp(s t), s^2 exp(s t) per [pmat][R][3][S], s = lam * rate
+  double *d_pdiag = nullptr;  // FP32 amino acids / codons: exp(s t), s exp(s t), s^2 exp(s t) per [pmat][R][3][S],
+                              // s = lam * rate
+  double *d_eigfrag = nullptr;  // FP32 codons: V^-1 and V^T of each eigensystem in codon fragment order
   double *d_V = nullptr, *d_Vinv = nullptr, *d_lam = nullptr, *d_model = nullptr;  // model: pi[S] w[R] mult[R]
   // staging
   DevClaOp *d_ops = nullptr;
@@ -1200,7 +1202,10 @@ __global__ __launch_bounds__(256) void k_edge_nuc(const T *__restrict__ cla, con
 template <typename T, int R>
 __global__ __launch_bounds__(64 * R) void k_edge_codon(const T *__restrict__ cla, const int *__restrict__ ua,
                                                    const unsigned char *__restrict__ tips,
-                                                   const double *__restrict__ pmpad, const double *__restrict__ model,
+                                                   const double *__restrict__ pmpad, const double *__restrict__ pdiag,
+                                                   const double *__restrict__ eigfrag,
+                                                   const int *__restrict__ eig_of_rate,
+                                                   const double *__restrict__ model,
                                                    const int *__restrict__ counts, const int *__restrict__ cst,
                                                    const DevEdgeOp *__restrict__ ops, double *__restrict__ partials,
                                                    double *__restrict__ site, double pinv, int pinv_in_model,
@@ -1220,8 +1225,39 @@ __global__ __launch_bounds__(64 * R) void k_edge_codon(const T *__restrict__ cla
     double acc[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};  // [matrix][v]: sum over this lane's states
     double y[16];
     if (op.child >= 0) codon_load_x(cla, op.child, slot, R, r, lo, g, p0, P, y);
+    // FP32 form, derivatives on an internal child: spectral route.  With
+    // P = V diag(ex) V^-1, z = V^-1 y and a = V^T (x * pi) are two MFMA
+    // contractions per tile (instead of P y, P' y and P'' y), and each cell
+    // adds a z times ex, s ex and s^2 ex.  The FP64 form keeps the P matrices.
+    const bool spectral = sizeof(T) == 4 && op.derivs && op.child >= 0;
+    if (spectral) {
+      double xa[16];
+      codon_load_x(cla, op.partial, slot, R, r, lo, g, p0, P, xa);
+#pragma unroll
+      for (int kk = 0; kk < 16; kk++) {
+        const int st = 16 * g + kk;
+        xa[kk] *= st < 61 ? pi[st < 61 ? st : 0] : 0.0;
+      }
+      const double *Wf = eigfrag + size_t(eig_of_rate[r]) * 8192, *VTf = Wf + 4096;
+      const double *dg = pdiag + (size_t(op.pm) * R + r) * 3 * 61;
 #pragma unroll 1
-    for (int t = 0; t < 4; t++) {
+      for (int t = 0; t < 4; t++) {
+        const int st = 16 * t + lo, sc = st < 61 ? st : 0;
+        const double on = st < 61 ? 1.0 : 0.0;
+        const gle_doublex4 Z = codon_mfma_tile(Wf, y, t, l);
+        const gle_doublex4 A = codon_mfma_tile(VTf, xa, t, l);
+        const double f0 = on * dg[sc], f1 = on * dg[61 + sc], f2 = on * dg[122 + sc];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const double c = A[v] * Z[v];
+          acc[0][v] += c * f0;
+          acc[1][v] += c * f1;
+          acc[2][v] += c * f2;
+        }
+      }
+    }
+#pragma unroll 1
+    for (int t = 0; t < (spectral ? 0 : 4); t++) {
       const int st = 16 * t + lo;
       double xw[4];  // x * pi of the partial side for this lane's cells
 #pragma unroll
@@ -1875,7 +1911,8 @@ static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk, bo
 #undef GLE_EDGE_AA
   } else if (I->S == 61) {
 #define GLE_EDGE_CODON(RR)                                                                                        \
-  GLE_LAUNCH(I, (k_edge_codon<T, RR>), grid, dim3(64 * RR), 0, a.cla, a.u, a.tips, I->d_pmpad, a.model,              \
+  GLE_LAUNCH(I, (k_edge_codon<T, RR>), grid, dim3(64 * RR), 0, a.cla, a.u, a.tips, I->d_pmpad, I->d_pdiag,           \
+                     I->d_eigfrag, I->d_eig_of_rate, a.model,                                                       \
                      a.counts, a.cst, d_ops, a.partials, a.site, a.pinv, a.pinv_in_model, a.Ppad, a.P)
     switch (I->R) {
       case 1: GLE_EDGE_CODON(1); break;
@@ -2077,7 +2114,8 @@ static int destroy(Instance *I) {
     I->heavy.reset();
   }
   void *ptrs[] = {I->d_tips, I->d_counts, I->d_cst, I->d_u, I->d_flags, I->d_eig_of_rate, I->d_cla,
-                  I->d_pm64, I->d_pmpad,  I->d_pdiag, I->d_V, I->d_Vinv, I->d_lam, I->d_model, I->d_ops,
+                  I->d_pm64, I->d_pmpad,  I->d_pdiag, I->d_eigfrag, I->d_V, I->d_Vinv, I->d_lam, I->d_model,
+                  I->d_ops,
                   I->d_eops, I->d_part,  I->d_eres, I->d_site, I->d_pmidx, I->d_pmt};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -2154,7 +2192,9 @@ static int create_internal(const gle_instance_desc *d, std::unique_ptr<Instance>
   GLE_ALLOC(I->d_cla, I->slot_elems() * I->elem_bytes() * I->n_slots);
   GLE_ALLOC(I->d_pm64, sizeof(double) * size_t(I->n_pm) * 3 * R * S * S);
   if (S == 61 || S == 20) GLE_ALLOC(I->d_pmpad, sizeof(double) * size_t(I->n_pm) * 3 * R * I->Sp * I->Sp);
-  if (S == 20 && I->prec == GLE_PREC_FP32) GLE_ALLOC(I->d_pdiag, sizeof(double) * size_t(I->n_pm) * R * 3 * S);
+  if ((S == 20 || S == 61) && I->prec == GLE_PREC_FP32)
+    GLE_ALLOC(I->d_pdiag, sizeof(double) * size_t(I->n_pm) * R * 3 * S);
+  if (S == 61 && I->prec == GLE_PREC_FP32) GLE_ALLOC(I->d_eigfrag, sizeof(double) * size_t(I->n_eig) * 2 * 4096);
   GLE_ALLOC(I->d_V, sizeof(double) * size_t(I->n_eig) * S * S);
   GLE_ALLOC(I->d_Vinv, sizeof(double) * size_t(I->n_eig) * S * S);
   GLE_ALLOC(I->d_lam, sizeof(double) * size_t(I->n_eig) * S);
@@ -2172,7 +2212,8 @@ static int create_internal(const gle_instance_desc *d, std::unique_ptr<Instance>
       hipMemset(I->d_pm64, 0, sizeof(double) * size_t(I->n_pm) * 3 * R * S * S) != hipSuccess ||
       (I->d_pmpad && hipMemset(I->d_pmpad, 0, sizeof(double) * size_t(I->n_pm) * 3 * R * I->Sp * I->Sp) !=
                          hipSuccess) ||
-      (I->d_pdiag && hipMemset(I->d_pdiag, 0, sizeof(double) * size_t(I->n_pm) * R * 3 * S) != hipSuccess))
+      (I->d_pdiag && hipMemset(I->d_pdiag, 0, sizeof(double) * size_t(I->n_pm) * R * 3 * S) != hipSuccess) ||
+      (I->d_eigfrag && hipMemset(I->d_eigfrag, 0, sizeof(double) * size_t(I->n_eig) * 2 * 4096) != hipSuccess))
     return fail(set_error(GLE_ERR_DEVICE, "device memset failed"));
   I->h_tips.assign(size_t(I->N) * I->P_orig, missing);
   I->h_pi.assign(S, 1.0 / S);
@@ -2231,6 +2272,18 @@ static int push_model(Instance *X, const Instance *M) {
   GLE_HIP(hipMemcpy(X->d_V, M->h_V.data(), sizeof(double) * M->h_V.size(), hipMemcpyHostToDevice));
   GLE_HIP(hipMemcpy(X->d_Vinv, M->h_Vinv.data(), sizeof(double) * M->h_Vinv.size(), hipMemcpyHostToDevice));
   GLE_HIP(hipMemcpy(X->d_lam, M->h_lam.data(), sizeof(double) * M->h_lam.size(), hipMemcpyHostToDevice));
+  if (X->d_eigfrag) {
+    // V^-1 (output k, input q) and V^T (output k, input f) as zero-padded 64 x 64
+    // blocks in fragment order, the layout codon_mfma_tile reads
+    std::vector<double> f(size_t(X->n_eig) * 2 * 4096, 0.0);
+    for (int e = 0; e < X->n_eig; e++)
+      for (int i = 0; i < 61; i++)
+        for (int j = 0; j < 61; j++) {
+          f[size_t(e) * 8192 + codon_frag(i, j)] = M->h_Vinv[size_t(e) * 3721 + i * 61 + j];
+          f[size_t(e) * 8192 + 4096 + codon_frag(i, j)] = M->h_V[size_t(e) * 3721 + j * 61 + i];
+        }
+    GLE_HIP(hipMemcpy(X->d_eigfrag, f.data(), sizeof(double) * f.size(), hipMemcpyHostToDevice));
+  }
   X->pinv = M->pinv;
   X->pinv_in_model = M->pinv_in_model;
   return GLE_SUCCESS;
