@@ -232,7 +232,8 @@ int gle_edge_likelihoods(int h, const gle_edge_op *ops, int n, double *out, doub
  * may be NULL) the unscaled site lnL, d1 = D1/L and d2 = D2/L - d1^2 (not
  * multiplied by the pattern count) and the rescale counts of the partial and
  * child sides (-1 for a tip child).  Counts are in the instance's units: the
- * reference's for the FP64 form, powers of two for the FP32 form. */
+ * reference's for the FP64 form, powers of two for the FP32 form.  The
+ * op's transition-matrix slot must hold P' and P'' (computed with_derivs). */
 int gle_edge_site_values(int h, const gle_edge_op *op, double *out3, double *site_lnl, double *site_d1,
                          double *site_d2, int *under_partial, int *under_child);
 /* rescale flags accumulated since the last reset (see GLE_FLAG_*) */
