@@ -1550,6 +1550,21 @@ __global__ __launch_bounds__(256) void k_mfma_calibrate(double *__restrict__ out
   }
   out[size_t(blockIdx.x) * blockDim.x + threadIdx.x] = (d0[0] + d1[1]) + (d2[2] + d3[3]);
 }
+// The same for v_mfma_f64_4x4x4_4b_f64 (four 4x4x4 blocks, 512 flops), eight chains.
+__global__ __launch_bounds__(256) void k_mfma_calibrate44(double *__restrict__ out, int iters) {
+  double d[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) d[c] = 0.0;
+  const double a = threadIdx.x * 1e-3, b = blockIdx.x * 1e-3 + 1.0;
+  for (int i = 0; i < iters; i++) {
+#pragma unroll
+    for (int c = 0; c < 8; c++) d[c] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, d[c], 0, 0, 0);
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int c = 0; c < 8; c++) s += d[c];
+  out[size_t(blockIdx.x) * blockDim.x + threadIdx.x] = s;
+}
 
 // =====================================================================
 // Host-side launchers
@@ -2628,8 +2643,8 @@ extern "C" int gle_copy_calibrate(size_t bytes, int reps, double *ms_per_copy) {
   return GLE_SUCCESS;
 }
 
-extern "C" int gle_mfma_calibrate(double *tflops) {
-  if (!tflops) return set_error(GLE_ERR_ARG, "bad calibration request");
+extern "C" int gle_mfma_calibrate_shape(int shape, double *tflops) {
+  if (!tflops || (shape != 0 && shape != 1)) return set_error(GLE_ERR_ARG, "bad calibration request");
   {
     std::lock_guard<std::mutex> lk(g_mu);
     int rc = ensure_device();
@@ -2641,14 +2656,16 @@ extern "C" int gle_mfma_calibrate(double *tflops) {
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
-  hipLaunchKernelGGL(k_mfma_calibrate, dim3(kBlocks), dim3(kThreads), 0, nullptr, out, kIters);
+  auto k = shape == 0 ? k_mfma_calibrate : k_mfma_calibrate44;
+  hipLaunchKernelGGL(k, dim3(kBlocks), dim3(kThreads), 0, nullptr, out, kIters);
   (void)hipEventRecord(e0, nullptr);
-  hipLaunchKernelGGL(k_mfma_calibrate, dim3(kBlocks), dim3(kThreads), 0, nullptr, out, kIters);
+  hipLaunchKernelGGL(k, dim3(kBlocks), dim3(kThreads), 0, nullptr, out, kIters);
   (void)hipEventRecord(e1, nullptr);
   const hipError_t err = hipEventSynchronize(e1);
   float ms = 0.0f;
   (void)hipEventElapsedTime(&ms, e0, e1);
-  const double flops = double(kBlocks) * (kThreads / 64) * kIters * 4 * 2048.0;
+  // flops per wave and iteration: 4 chains x 2048 (16x16x4) or 8 chains x 512 (4x4x4, 4 blocks)
+  const double flops = double(kBlocks) * (kThreads / 64) * kIters * (shape == 0 ? 4 * 2048.0 : 8 * 512.0);
   *tflops = flops / (double(ms) * 1e-3) / 1e12;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -2656,6 +2673,8 @@ extern "C" int gle_mfma_calibrate(double *tflops) {
   if (err != hipSuccess) return set_error(GLE_ERR_DEVICE, "MFMA calibration failed: %s", hipGetErrorString(err));
   return GLE_SUCCESS;
 }
+
+extern "C" int gle_mfma_calibrate(double *tflops) { return gle_mfma_calibrate_shape(0, tflops); }
 
 extern "C" int gle_synchronize(int h) {
   GET_INSTANCE(h);

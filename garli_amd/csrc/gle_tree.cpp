@@ -41,6 +41,10 @@ struct Tree {
   int schedule = 0;
   unsigned long long gen = 0;  // instance generation the cached CLAs and P(t) belong to
   double lnl = 0.0;
+  // The last score is current (nothing changed since): its site lnL is
+  // computed when asked for, on the same edge, instead of copied back with
+  // every score.
+  bool site_ready = false;
   std::vector<double> site;
   // scratch
   std::vector<int> level;
@@ -70,6 +74,7 @@ static void release(Tree &t, int a, int k) {
   s = -1;
 }
 static void dirty_all(Tree &t) {
+  t.site_ready = false;
   for (int a = 0; a < t.nn; a++)
     if (!is_tip(t, a))
       for (int k = 0; k < t.nnb[a]; k++) release(t, a, k);
@@ -84,6 +89,7 @@ static void dirty_from(Tree &t, int from, int came) {
   }
 }
 static void dirty_edge(Tree &t, int n) {
+  t.site_ready = false;
   const int a = t.parent[n];
   dirty_from(t, n, a);
   dirty_from(t, a, n);
@@ -312,9 +318,10 @@ static int tree_score(Tree &t, double *lnl) {
   int pa, pb;
   root_edge(t, pa, pb);
   double res[3];
-  t.site.resize(instance_patterns(t.I));
-  const int rc = edge_eval(t, pa, pb, false, res, t.site.data());
+  t.site.clear();
+  const int rc = edge_eval(t, pa, pb, false, res, nullptr);
   if (rc) return rc;
+  t.site_ready = true;
   t.lnl = res[0];
   *lnl = res[0];
   return GLE_SUCCESS;
@@ -520,8 +527,19 @@ extern "C" int gle_tree_score(int h, double *lnl) {
 extern "C" int gle_tree_site_likelihoods(int h, double *out) {
   if (!out) return set_error(GLE_ERR_ARG, "null output");
   GET_TREE(h);
-  if (T->site.empty()) return set_error(GLE_ERR_STATE, "score the tree first");
-  std::copy(T->site.begin(), T->site.end(), out);
+  Tree &t = *T;
+  if (!t.site_ready) return set_error(GLE_ERR_STATE, "score the tree first");
+  if (t.site.empty()) {
+    int pa, pb;
+    root_edge(t, pa, pb);
+    std::vector<double> site(size_t(instance_patterns(t.I)));
+    double res[3];
+    const int rc = edge_eval(t, pa, pb, false, res, site.data());
+    if (rc) return rc;
+    t.site.swap(site);
+    t.site_ready = true;  // a rescale retry inside edge_eval changes no value
+  }
+  std::copy(t.site.begin(), t.site.end(), out);
   return GLE_SUCCESS;
 }
 
@@ -586,6 +604,7 @@ extern "C" int gle_tree_set_rescale_every(int h, int every) {
   GET_TREE(h);
   if (every < 2) return set_error(GLE_ERR_ARG, "rescaleEvery must be >= 2");
   T->rescale_every = every;
+  T->site_ready = false;
   return GLE_SUCCESS;
 }
 

@@ -103,6 +103,7 @@ SIGNATURES = {
     "gle_set_timing": ([C.c_int, C.c_int], C.c_int),
     "gle_copy_calibrate": ([C.c_size_t, C.c_int, _D], C.c_int),
     "gle_mfma_calibrate": ([_D], C.c_int),
+    "gle_mfma_calibrate_shape": ([C.c_int, _D], C.c_int),
     "gle_get_kernel_stats": ([C.c_int, C.c_int, C.POINTER(C.c_longlong), _D, _D, C.c_int], C.c_int),
     "gle_tree_create": ([C.c_int, _I, _D, _I], C.c_int),
     "gle_tree_copy": ([C.c_int, _I], C.c_int),
@@ -175,10 +176,11 @@ def device_count():
     return lib().gle_device_count()
 
 
-def mfma_calibrate():
-    """Achievable FP64 MFMA TFLOP/s (register operands, 8 waves per SIMD)."""
+def mfma_calibrate(shape=0):
+    """Achievable FP64 MFMA TFLOP/s (register operands, 8 waves per SIMD) of
+    v_mfma_f64_16x16x4_f64 (shape 0) or v_mfma_f64_4x4x4_4b_f64 (shape 1)."""
     tf = C.c_double(0)
-    _check(lib().gle_mfma_calibrate(C.byref(tf)))
+    _check(lib().gle_mfma_calibrate_shape(int(shape), C.byref(tf)))
     return tf.value
 
 

@@ -253,6 +253,9 @@ int gle_copy_calibrate(size_t bytes, int reps, double *ms_per_copy);
  * register operands.  No reference counterpart: it calibrates the codon
  * kernels' MFMA utilisation. */
 int gle_mfma_calibrate(double *tflops);
+/* As gle_mfma_calibrate for shape 0 (v_mfma_f64_16x16x4_f64, four chains
+ * per wave) or shape 1 (v_mfma_f64_4x4x4_4b_f64, eight chains per wave). */
+int gle_mfma_calibrate_shape(int shape, double *tflops);
 int gle_get_kernel_stats(int h, int kind, long long *launches, double *ms, double *bytes, int reset);
 
 /* ---------------------------------------------------------------------------
@@ -273,7 +276,9 @@ int gle_tree_get_branch_lengths(int tree, double *blen);
 int gle_tree_invalidate(int tree);
 /* Tree::Score (tree.cpp:4003-4094), with the rescale retry; *lnl = Tree::lnL */
 int gle_tree_score(int tree, double *lnl);
-/* per-pattern site lnL of the last score (OutputSiteLikelihoods source) */
+/* per-pattern site lnL of the last score (OutputSiteLikelihoods source),
+ * computed on this call on the score's edge; GLE_ERR_STATE if the tree has
+ * changed since that score */
 int gle_tree_site_likelihoods(int tree, double *site_lnl);
 /* Tree::CalcDerivativesRateHet(nd->anc, nd) (optimization.cpp:1099-1194) */
 int gle_tree_derivatives(int tree, int node, double *lnl, double *d1, double *d2);
