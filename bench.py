@@ -92,7 +92,7 @@ def main():
     inst = ds.make_instance(prec)
     n_heavy = inst.heavy_patterns()
     if n_heavy:
-        log("%d patterns with count > 4 run in FP64 beside the FP32 bulk" % n_heavy)
+        log("%d heavily weighted patterns run in FP64 beside the FP32 bulk" % n_heavy)
     tree = E.Tree(inst, ds.parent, ds.blen)
     tree.set_schedule(1 if args.schedule == "fused" else 0)
 

@@ -126,7 +126,10 @@ struct Instance {
   // FP32 instances: patterns whose count exceeds heavy_threshold run in an FP64
   // sub-instance that receives the same slots, P(t) and op lists, so FP32
   // rounding is never multiplied by a large pattern weight.
-  int heavy_threshold = 4;  // 0 keeps every pattern in FP32
+  // A pattern of count c carries c times its FP32 per-site error (at most
+  // about 1e-6 in lnL here): above 100 that could exceed 1e-4, so such
+  // patterns run in the FP64 part.  0 keeps every pattern in FP32.
+  int heavy_threshold = 100;
   int fused_min_patterns = 1 << 19;  // smallest padded pattern count that runs fused walks
   // bumped by every change of data or model: cached CLAs and P(t) of trees and
   // servers built before it are stale (MakeAllNodesDirty, tree.cpp)

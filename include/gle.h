@@ -169,7 +169,8 @@ int gle_set_tip_states(int h, int tip, const unsigned char *states);
 /* counts per pattern; constant-pattern states (nuc bitmask / state index) for
  * patterns 0..last_const (DataPartition::GetCounts/LastConstant/GetConstStates) */
 int gle_set_patterns(int h, const int *counts, int last_const, const int *const_states);
-/* FP32 instances keep patterns whose count exceeds `threshold` (default 4) in an
+/* FP32 instances keep patterns whose count exceeds `threshold` (default 100:
+ * count times the FP32 per-site error, about 1e-6 at most, stays under 1e-4) in an
  * FP64 part that runs the same op lists, so FP32 rounding is never multiplied
  * by a large pattern weight; 0 keeps every pattern in FP32.  No reference
  * counterpart (the reference has one precision per build, defs.h:86-112). */

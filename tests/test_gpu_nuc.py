@@ -112,6 +112,7 @@ def test_fp32_heavy_pattern_split(gpu):
     f64.score()
     site64 = f64.site_likelihoods()
     inst = ds.make_instance(E.PREC_FP32)
+    inst.set_heavy_split(4)
     n_heavy = inst.heavy_patterns()
     assert 0 < n_heavy < ds.n_patterns
     t = E.Tree(inst, ds.parent, ds.blen)

@@ -75,6 +75,7 @@ def test_site_values_fp32_heavy_split(gpu):
     assert np.any(ds.counts > 4)
     t64 = E.Tree(ds.make_instance(E.PREC_FP64), ds.parent, ds.blen)
     inst = ds.make_instance(E.PREC_FP32)
+    inst.set_heavy_split(4)
     assert inst.heavy_patterns() > 0
     t32 = E.Tree(inst, ds.parent, ds.blen)
     node = ds.n_tips + 3
