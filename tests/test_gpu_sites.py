@@ -85,3 +85,28 @@ def test_site_values_fp32_heavy_split(gpu):
     np.testing.assert_allclose(v32["d1"], v64["d1"], atol=1e-4 * max(1.0, np.abs(v64["d1"]).max()))
     t32.close()
     t64.close()
+
+
+def test_site_likelihoods_follow_the_last_score(gpu):
+    """Site likelihoods are computed on request for the last score: their
+    count-weighted sum is that score, and a tree changed since the score has
+    none until it is scored again."""
+    ds = sim.gtr_gamma(n_tips=12, n_sites=800, pinv=0.1, seed=306)
+    t = E.Tree(ds.make_instance(E.PREC_FP64), ds.parent, ds.blen)
+    with pytest.raises(E.GleError):
+        t.site_likelihoods()
+    lnl = t.score()
+    site = t.site_likelihoods()
+    assert abs(np.dot(ds.counts, site) - lnl) < 1e-10 * abs(lnl)
+    np.testing.assert_array_equal(t.site_likelihoods(), site)  # asked again: the same values
+    t.derivatives(ds.n_tips + 2)  # an evaluation elsewhere changes nothing
+    np.testing.assert_array_equal(t.site_likelihoods(), site)
+    t.set_branch_length(3, ds.blen[3] * 1.5)
+    with pytest.raises(E.GleError):
+        t.site_likelihoods()
+    lnl2 = t.score()
+    site2 = t.site_likelihoods()
+    assert abs(np.dot(ds.counts, site2) - lnl2) < 1e-10 * abs(lnl2) and lnl2 != lnl
+    ref = O.Oracle(ds).score(blen=t.branch_lengths(), sites=True)[1]
+    np.testing.assert_allclose(site2, ref, rtol=1e-9)
+    t.close()
