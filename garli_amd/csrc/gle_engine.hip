@@ -78,6 +78,7 @@ __constant__ double c_mult[kRescaleLen];
 __constant__ int c_incr[kRescaleLen];
 __constant__ double c_rescale_below, c_reduce_below, c_bail_below;
 
+
 // --------------------------------------------------------------- instance
 struct Instance {
   int N = 0, P = 0, Ppad = 0, S = 0, Sp = 0, R = 0, n_eig = 0, n_slots = 0, n_pm = 0, prec = 0;
@@ -217,13 +218,13 @@ std::mutex &pool_mutex() {
   return m;
 }
 
-static int ensure_device() {
+static int ensure_device(int dev = 0) {
   if (g_device >= 0) return GLE_SUCCESS;
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
     return set_error(GLE_ERR_DEVICE, "no HIP device visible: the GARLI engine runs only on a GPU");
-  GLE_HIP(hipSetDevice(0));
-  g_device = 0;
+  GLE_HIP(hipSetDevice(dev));
+  g_device = dev;
   const RescaleTables &t = rescale_tables();
   GLE_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_thresh), t.thresh, sizeof t.thresh));
   GLE_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_mult), t.mult, sizeof t.mult));
@@ -1818,41 +1819,47 @@ static int enqueue_partials(Instance *I, const DevClaOp *ops, int n_ops, const i
       I->g_cache.push_back(std::move(g));
     }
   }
-  const bool capture = entry != nullptr;
-  if (capture) GLE_HIP(hipStreamBeginCapture(I->stream, hipStreamCaptureModeThreadLocal));
-  if (fused) {
-    timed_begin(I, 0, bytes_of(0, n_ops));
-    rc = I->prec == GLE_PREC_FP64 ? launch_cla<double>(I, I->d_ops, n_ops, true)
-                                  : launch_cla<float>(I, I->d_ops, n_ops, true);
-    timed_end(I);
-    if (rc) return rc;
-  } else {
+  auto launch_all = [&]() -> int {
+    if (fused) {
+      timed_begin(I, 0, bytes_of(0, n_ops));
+      const int r = I->prec == GLE_PREC_FP64 ? launch_cla<double>(I, I->d_ops, n_ops, true)
+                                             : launch_cla<float>(I, I->d_ops, n_ops, true);
+      timed_end(I);
+      return r;
+    }
     const int nl = n_levels > 0 ? n_levels : n_ops;
     for (int l = 0; l < nl; l++) {
       const int b = n_levels > 0 ? level_start[l] : l, e = n_levels > 0 ? level_start[l + 1] : l + 1;
       if (e <= b) continue;
       timed_begin(I, 0, bytes_of(b, e));
-      rc = I->prec == GLE_PREC_FP64 ? launch_cla<double>(I, I->d_ops + b, e - b, false)
-                                    : launch_cla<float>(I, I->d_ops + b, e - b, false);
+      const int r = I->prec == GLE_PREC_FP64 ? launch_cla<double>(I, I->d_ops + b, e - b, false)
+                                             : launch_cla<float>(I, I->d_ops + b, e - b, false);
       timed_end(I);
-      if (rc) break;
+      if (r) return r;
     }
-  }
-  if (capture) {
+    return GLE_SUCCESS;
+  };
+  bool captured = false;
+  if (entry && hipStreamBeginCapture(I->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+    rc = launch_all();
     hipGraph_t graph = nullptr;
     const hipError_t ec = hipStreamEndCapture(I->stream, &graph);
     hipError_t ei = hipErrorUnknown;
     if (ec == hipSuccess && !rc) ei = hipGraphInstantiate(&entry->exec, graph, nullptr, nullptr, 0);
     if (graph) (void)hipGraphDestroy(graph);
-    if (rc) return rc;
-    if (ec != hipSuccess || ei != hipSuccess) {
-      entry->exec = nullptr;
-      I->drop_graphs();
-      I->graphs = false;  // capture unavailable here: stay eager from now on
-      return set_error(GLE_ERR_DEVICE, "graph capture of the level launches failed");
-    }
-    GLE_HIP(hipGraphLaunch(entry->exec, I->stream));
+    if (rc) return rc;  // a launch argument was refused before anything ran
+    captured = ec == hipSuccess && ei == hipSuccess;
+    if (captured) GLE_HIP(hipGraphLaunch(entry->exec, I->stream));
   }
+  if (entry && !captured) {
+    // capture is unavailable here: nothing ran, so launch eagerly, and stay
+    // eager from now on
+    (void)hipGetLastError();
+    entry->exec = nullptr;
+    I->drop_graphs();
+    I->graphs = false;
+  }
+  if (!captured) rc = launch_all();
   if (rc) return rc;
   GLE_HIP(hipGetLastError());
   return GLE_SUCCESS;
@@ -1870,7 +1877,7 @@ int engine_update_partials(Instance *I, const DevClaOp *ops, int n_ops, const in
 // derivs: some op of the launch wants (d1, d2); the nucleotide kernel then
 // computes them for every op (the others ignore them).
 template <typename T>
-static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk, bool derivs) {
+static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk, bool derivs, double *d_part) {
   EdgeArgs<T> a;
   a.cla = reinterpret_cast<const T *>(I->d_cla);
   a.u = I->d_u;
@@ -1887,7 +1894,7 @@ static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk, bo
   a.R = I->R;
   a.S = I->S;
   a.Sp = I->Sp;
-  a.partials = I->d_part;
+  a.partials = d_part;
   a.site = I->d_site;
   dim3 grid(nblk, n);
   if (I->S == 4) {
@@ -1953,8 +1960,18 @@ static int edge_blocks(const Instance *I) {
   return I->S == 4 ? I->Ppad / 256 : I->S == 61 ? I->Ppad / 16 : I->Ppad / 64;
 }
 
-// Edge kernels and the per-op reduction into d_eres; d_site when `site`.
-static int enqueue_edges(Instance *I, const DevEdgeOp *ops, int n, bool site) {
+// Device buffers of one edge launch: the uploaded op list, the per-block
+// partial sums [op][block][3] and the per-op results [op][3].
+struct EdgeBufs {
+  DevEdgeOp *ops;
+  double *part, *res;
+};
+
+// Edge kernels and the per-op reduction into the results; d_site when `site`.
+// With ext the launch uses those buffers as they are (nothing is allocated,
+// the op list is always uploaded and the launch is not timed); otherwise the
+// instance's own buffers, grown as needed.
+static int enqueue_edges(Instance *I, const DevEdgeOp *ops, int n, bool site, const EdgeBufs *ext = nullptr) {
   for (int i = 0; i < n; i++) {
     const DevEdgeOp &o = ops[i];
     if (o.partial < 0 || o.partial >= I->n_slots || o.child >= I->n_slots || (o.child < 0 && -o.child - 1 >= I->N) ||
@@ -1962,6 +1979,20 @@ static int enqueue_edges(Instance *I, const DevEdgeOp *ops, int n, bool site) {
       return set_error(GLE_ERR_ARG, "edge op %d out of range", i);
   }
   const int nblk = edge_blocks(I);
+  bool derivs = false;
+  for (int i = 0; i < n; i++) derivs = derivs || ops[i].derivs;
+  if (ext) {
+    GLE_HIP(hipMemcpyAsync(ext->ops, ops, sizeof(DevEdgeOp) * n, hipMemcpyHostToDevice, I->stream));
+    double *site_save = I->d_site;
+    if (!site) I->d_site = nullptr;
+    const int rc = I->prec == GLE_PREC_FP64 ? launch_edges<double>(I, ext->ops, n, nblk, derivs, ext->part)
+                                            : launch_edges<float>(I, ext->ops, n, nblk, derivs, ext->part);
+    I->d_site = site_save;
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_edge_reduce, dim3(n), dim3(64), 0, I->stream, ext->part, nblk, ext->res);
+    GLE_HIP(hipGetLastError());
+    return GLE_SUCCESS;
+  }
   const int eops_cap0 = I->eops_cap;
   int rc = grow(I->d_eops, I->eops_cap, n);
   if (rc) return rc;
@@ -1985,10 +2016,8 @@ static int enqueue_edges(Instance *I, const DevEdgeOp *ops, int n, bool site) {
   double eb = 0.0;
   for (int i = 0; i < n; i++) eb += edge_op_bytes(I, ops[i]);
   timed_begin(I, 1, eb);
-  bool derivs = false;
-  for (int i = 0; i < n; i++) derivs = derivs || ops[i].derivs;
-  rc = I->prec == GLE_PREC_FP64 ? launch_edges<double>(I, I->d_eops, n, nblk, derivs)
-                                : launch_edges<float>(I, I->d_eops, n, nblk, derivs);
+  rc = I->prec == GLE_PREC_FP64 ? launch_edges<double>(I, I->d_eops, n, nblk, derivs, I->d_part)
+                                : launch_edges<float>(I, I->d_eops, n, nblk, derivs, I->d_part);
   timed_end(I);
   I->d_site = site_save;
   if (rc) return rc;
@@ -1999,8 +2028,8 @@ static int enqueue_edges(Instance *I, const DevEdgeOp *ops, int n, bool site) {
 
 // site: host buffer of nsec x P values, the first nsec of (lnL, d1, d2,
 // partial and child rescale counts)
-static int copy_edges_out(Instance *I, int n, double *out, double *site, int nsec) {
-  GLE_HIP(hipMemcpyAsync(out, I->d_eres, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, I->stream));
+static int copy_edges_out(Instance *I, int n, double *out, double *site, int nsec, const double *d_res = nullptr) {
+  GLE_HIP(hipMemcpyAsync(out, d_res ? d_res : I->d_eres, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, I->stream));
   if (site)
     for (int k = 0; k < nsec; k++)
       GLE_HIP(hipMemcpyAsync(site + size_t(k) * I->P, I->d_site + size_t(k) * I->Ppad, sizeof(double) * I->P,
@@ -2168,12 +2197,8 @@ extern "C" int gle_init(int device) {
   if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
     return set_error(GLE_ERR_DEVICE, "no HIP device visible: the GARLI engine runs only on a GPU");
   if (device < 0 || device >= n) return set_error(GLE_ERR_ARG, "device %d out of range (%d visible)", device, n);
-  GLE_HIP(hipSetDevice(device));
   g_device = -1;
-  int rc = ensure_device();
-  if (rc) return rc;
-  g_device = device;
-  return GLE_SUCCESS;
+  return ensure_device(device);
 }
 
 static int create_internal(const gle_instance_desc *d, std::unique_ptr<Instance> &out) {
@@ -2538,15 +2563,50 @@ extern "C" int gle_get_transition_matrices(int h, int pm, double *out) {
   return GLE_SUCCESS;
 }
 
-extern "C" int gle_update_partials(int h, const gle_cla_op *ops, int n, const int *level_start, int n_levels) {
-  GET_INSTANCE(h);
-  if (n < 0 || (n > 0 && !ops) || (n_levels > 0 && !level_start)) return set_error(GLE_ERR_ARG, "bad op list");
-  if (n_levels > 0 && (level_start[0] != 0 || level_start[n_levels] != n))
+// The caller's level ranges: non-decreasing, covering [0, n); the ops of one
+// level run concurrently, so none of them may read or overwrite a CLA that
+// another op of the same level writes (BlockingOperationsSet semantics,
+// calculationmanager.h:118-123).
+static int check_levels(const gle_cla_op *ops, int n, const int *level_start, int n_levels) {
+  if (level_start[0] != 0 || level_start[n_levels] != n)
     return set_error(GLE_ERR_ARG, "level ranges must cover the op list");
+  for (int l = 0; l < n_levels; l++)
+    if (level_start[l + 1] < level_start[l] || level_start[l + 1] > n)
+      return set_error(GLE_ERR_ARG, "level %d range [%d, %d) is not inside [0, %d) in order", l, level_start[l],
+                       level_start[l + 1], n);
+  std::unordered_set<int> dest;
+  for (int l = 0; l < n_levels; l++) {
+    dest.clear();
+    for (int i = level_start[l]; i < level_start[l + 1]; i++)
+      if (!dest.insert(ops[i].dest).second)
+        return set_error(GLE_ERR_ARG, "level %d writes slot %d twice", l, ops[i].dest);
+    for (int i = level_start[l]; i < level_start[l + 1]; i++)
+      for (int c : {ops[i].child1, ops[i].child2})
+        if (c >= 0 && dest.count(c))
+          return set_error(GLE_ERR_ARG, "op %d reads slot %d, which its own level %d writes", i, c, l);
+  }
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_update_partials(int h, const gle_cla_op *ops, int n, const int *level_start, int n_levels) {
+  if (n < 0 || (n > 0 && !ops) || (n_levels > 0 && !level_start) || n_levels < 0)
+    return set_error(GLE_ERR_ARG, "bad op list");
+  if (n_levels > 0) {
+    const int rc = check_levels(ops, n, level_start, n_levels);
+    if (rc) return rc;
+  }
+  GET_INSTANCE(h);
   std::vector<DevClaOp> d(n);
   for (int i = 0; i < n; i++)
     d[i] = {ops[i].dest, ops[i].child1, ops[i].pmat1, ops[i].child2, ops[i].pmat2, ops[i].rescale ? kOpRescale : 0};
-  return engine_update_partials(I, d.data(), n, level_start, n_levels, false);
+  int rc = engine_update_partials(I, d.data(), n, level_start, n_levels, false);
+  if (rc || I->prec != GLE_PREC_FP64) return rc;
+  // RescaleRateHet's throw(1) (tree.cpp:3436-3449): a rescaled site whose
+  // maximum fell below 1e-190 since the flags were last reset
+  int flags = 0;
+  rc = engine_flags(I, &flags, 0);
+  if (rc) return rc;
+  return (flags & GLE_FLAG_REDUCE_RESCALE) ? GLE_RESCALE_RETRY : GLE_SUCCESS;
 }
 
 extern "C" int gle_get_partials(int h, int slot, double *out, int *u) {

@@ -5,17 +5,23 @@
 // pass (Individual::Mutate, individual.cpp:127-222).  The service takes such
 // requests from any number of client threads and runs them together:
 //
-//  * CLA cache.  Every directional CLA X(a|b) is keyed by a 128-bit signature
-//    of the subtree on a's side (tip identities, topology, exact branch-length
-//    bits).  Any request that needs a subtree some earlier request computed --
-//    the unmodified part of a perturbed parent, or a subtree shared by other
-//    individuals -- reuses the device slot.  This generalises ClaManager's
-//    holder sharing (clamanager.h:96-441): a changed branch changes the
-//    signatures of exactly the CLAs SweepDirtynessOverTree would dirty.
+//  * CLA cache.  Every directional CLA X(a|b) is named by an exact subtree
+//    id: ids are interned from (id of one child, bits of its branch length,
+//    id of the other child, bits of its branch length), the pair ordered, and
+//    tips are their node numbers.  Two CLAs get the same id exactly when
+//    their subtrees are identical (tips, topology, branch-length bits), so no
+//    collision can hand a request another subtree's CLA.  Any request that
+//    needs a subtree some earlier request computed -- the unmodified part of a
+//    perturbed parent, or a subtree shared by other individuals -- reuses the
+//    device slot.  This generalises ClaManager's holder sharing
+//    (clamanager.h:96-441): a changed branch changes the ids of exactly the
+//    CLAs SweepDirtynessOverTree would dirty.
 //  * P(t) cache keyed by branch-length bits (P, P', P'' per slot).
 //  * Admission by HBM budget: a request is admitted when its working set fits
 //    in the instance's CLA slots alongside the requests already running;
-//    least-recently-used cache entries are evicted on demand.
+//    least-recently-used cache entries are evicted on demand.  A tick whose
+//    new CLAs or P(t) would not fit runs the requests that do and keeps the
+//    rest for the next tick.
 //  * Continuous batching: each tick gathers the next step of every admitted
 //    request (a score, or one Newton iteration of a branch-length pass),
 //    dedupes shared CLAs, and issues one P(t) launch, one CLA launch per
@@ -46,14 +52,6 @@
 
 namespace gle {
 
-struct Sig {
-  uint64_t a = 0, b = 0;
-  bool operator==(const Sig &o) const { return a == o.a && b == o.b; }
-};
-struct SigHash {
-  size_t operator()(const Sig &s) const { return size_t(s.a ^ (s.b * 0x9E3779B97F4A7C15ull)); }
-};
-
 static inline uint64_t mix64(uint64_t x) {  // SplitMix64 finaliser
   x += 0x9E3779B97F4A7C15ull;
   x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -65,25 +63,31 @@ static inline uint64_t dbits(double v) {
   std::memcpy(&u, &v, sizeof u);
   return u;
 }
-static Sig tip_sig(int tip) {
-  Sig s;
-  s.a = mix64(0x7469700000000000ull + uint64_t(tip));
-  s.b = mix64(s.a ^ 0x5eed5eed5eed5eedull);
-  return s;
-}
-// order-independent combination of two (child signature, branch length) pairs
-static Sig combine(Sig x, double bx, Sig y, double by) {
-  uint64_t hx = mix64(x.a ^ mix64(dbits(bx))), hy = mix64(y.a ^ mix64(dbits(by)));
-  uint64_t gx = mix64(x.b ^ mix64(dbits(bx) + 1)), gy = mix64(y.b ^ mix64(dbits(by) + 1));
-  if (hx > hy || (hx == hy && gx > gy)) {
-    std::swap(hx, hy);
-    std::swap(gx, gy);
+
+// The exact name of an internal subtree: its two (child id, branch-length
+// bits) pairs in a fixed order.
+struct SubKey {
+  uint64_t x, lx, y, ly;
+  bool operator==(const SubKey &o) const { return x == o.x && lx == o.lx && y == o.y && ly == o.ly; }
+};
+struct SubKeyHash {
+  size_t operator()(const SubKey &k) const {
+    return size_t(mix64(k.x ^ mix64(k.lx ^ mix64(k.y ^ mix64(k.ly)))));
   }
-  Sig s;
-  s.a = mix64(hx * 3 + mix64(hy));
-  s.b = mix64(gx * 5 + mix64(gy ^ 0xabcdef));
-  return s;
+};
+static SubKey sub_key(uint64_t ix, double lx, uint64_t iy, double ly) {
+  SubKey k{ix, dbits(lx), iy, dbits(ly)};
+  if (k.x > k.y || (k.x == k.y && k.lx > k.ly)) {
+    std::swap(k.x, k.y);
+    std::swap(k.lx, k.ly);
+  }
+  return k;
 }
+// Internal subtree ids start above every tip number.
+constexpr uint64_t kFirstSubtreeId = uint64_t(1) << 40;
+// Interned names are kept while their CLAs come and go; past this many the
+// table and the CLA cache are dropped together (a rare full refill).
+constexpr size_t kInternCap = size_t(1) << 22;
 
 struct CacheEntry {
   int slot = -1, rank = 0;
@@ -94,13 +98,36 @@ struct ServeInstance {
   int h = -1;
   Instance *I = nullptr;
   int rescale_every = 16;
-  std::unordered_map<Sig, CacheEntry, SigHash> cla;
-  std::unordered_map<uint64_t, CacheEntry> pm;  // blen bits -> P, P', P'' slot
+  std::unordered_map<SubKey, uint64_t, SubKeyHash> intern;  // subtree name -> id
+  uint64_t next_id = kFirstSubtreeId;
+  std::unordered_map<uint64_t, CacheEntry> cla;  // subtree id -> CLA slot
+  std::unordered_map<uint64_t, CacheEntry> pm;   // blen bits -> P, P', P'' slot
   int reserved = 0;                              // working-set slots of admitted requests
   unsigned long long gen = 0;                    // instance generation the caches belong to
+
+  uint64_t intern_id(const SubKey &k) {
+    auto it = intern.find(k);
+    if (it != intern.end()) return it->second;
+    const uint64_t id = next_id++;
+    intern.emplace(k, id);
+    return id;
+  }
+  uint64_t find_id(const SubKey &k) const {
+    auto it = intern.find(k);
+    return it == intern.end() ? 0 : it->second;
+  }
+  // every cached CLA back to the pool (caller holds pool_mutex)
+  void drop_clas(ClaPool *pool) {
+    for (auto &c : cla)
+      if (--pool->refs[c.second.slot] == 0) pool->free_slot(c.second.slot);
+    cla.clear();
+  }
 };
 
 enum { kReqScore = 0, kReqOptimize = 1 };
+
+// per-request memo of subtree ids: kUnset = not looked up yet, 0 = no id yet
+constexpr uint64_t kUnset = ~uint64_t(0);
 
 struct Req {
   int id = 0, inst = 0, kind = 0, N = 0, nn = 0;
@@ -115,11 +142,12 @@ struct Req {
   bool scoring = true;  // final score phase
   double lnl = 0.0, improve = 0.0;
   int rc = GLE_SUCCESS, need = 0;
+  int waits = 0;          // ticks in a row this request was deferred (goes first next time)
   bool done = false;      // set by the worker while it owns the request
   bool finished = false;  // published under Server::mu once the worker let go
   std::chrono::steady_clock::time_point t_submit, t_done;
   // per-tick scratch
-  std::vector<std::array<Sig, 3>> sig;
+  std::vector<std::array<uint64_t, 3>> sid;
   int edge_a = 0, edge_b = 0;
   bool derivs = false;
 };
@@ -136,7 +164,7 @@ struct Server {
   int next_id = 1;
   uint64_t tick = 0;
   std::vector<double> latency_ms;
-  long long batched_requests = 0, ticks = 0;
+  long long batched_requests = 0, ticks = 0, deferred = 0;
   int batch_window_us = 0;  // an idle server waits this long for a burst to gather
   // where tick time goes: building op lists on the host, and from the first
   // launch until the results are back
@@ -179,23 +207,27 @@ static int req_nbk(const Req &r, int a, int b) {
   return -1;
 }
 static double req_len(const Req &r, int a, int b) { return r.parent[a] == b ? r.blen[a] : r.blen[b]; }
-
-// signature of X(a|b), memoised in r.sig (a.a == 0 && a.b == 0 means unset)
-static Sig req_sig(Req &r, int a, int b) {
-  if (req_tip(r, a)) return tip_sig(a - 1);
-  const int k = req_nbk(r, a, b);
-  Sig &s = r.sig[a][k];
-  if (s.a || s.b) return s;
-  int x = -1, y = -1;
+static void req_children(const Req &r, int a, int b, int &x, int &y) {
+  x = y = -1;
   for (int j = 0; j < r.nnb[a]; j++) {
     const int c = r.nb[a][j];
     if (c == b) continue;
     if (x < 0) x = c;
     else y = c;
   }
-  s = combine(req_sig(r, x, a), req_len(r, a, x), req_sig(r, y, a), req_len(r, a, y));
-  if (!s.a && !s.b) s.a = 1;
-  return s;
+}
+
+// The id X(a|b) already has (0: its subtree was never named, so no CLA of it
+// can be cached), memoised per tick in r.sid.
+static uint64_t known_id(Req &r, const ServeInstance &si, int a, int b) {
+  if (req_tip(r, a)) return uint64_t(a);
+  uint64_t &m = r.sid[a][req_nbk(r, a, b)];
+  if (m != kUnset) return m;
+  int x, y;
+  req_children(r, a, b, x, y);
+  const uint64_t ix = known_id(r, si, x, a), iy = ix ? known_id(r, si, y, a) : 0;
+  m = (ix && iy) ? si.find_id(sub_key(ix, req_len(r, a, x), iy, req_len(r, a, y))) : 0;
+  return m;
 }
 
 // ------------------------------------------------------------ one tick
@@ -207,12 +239,39 @@ struct TickBuilder {
   ServeInstance *si;
   ClaPool *pool;
   uint64_t tick;
-  std::unordered_map<Sig, TickNew, SigHash> fresh;
+  std::unordered_map<uint64_t, TickNew> fresh;
+  std::vector<uint64_t> fresh_order;          // ids in the order they joined `fresh`
   std::vector<std::pair<int, DevClaOp>> ops;  // (level, op)
   std::vector<int> pm_idx;
   std::vector<double> pm_len;
   std::unordered_map<uint64_t, int> pm_fresh;
   int err = GLE_SUCCESS;
+
+  struct Mark {
+    size_t fresh, ops, pm;
+  };
+  Mark mark() const { return {fresh_order.size(), ops.size(), pm_idx.size()}; }
+  // undo everything one request added since m: its new CLA slots and P(t)
+  // slots go back to the pool
+  void rollback(const Mark &m) {
+    for (size_t i = m.fresh; i < fresh_order.size(); i++) {
+      auto it = fresh.find(fresh_order[i]);
+      const int s = it->second.slot;
+      if (--pool->refs[s] == 0) pool->free_slot(s);
+      fresh.erase(it);
+    }
+    fresh_order.resize(m.fresh);
+    ops.resize(m.ops);
+    for (size_t i = m.pm; i < pm_idx.size(); i++) {
+      pm_fresh.erase(dbits(pm_len[i]));
+      pool->pm_free.push_back(pm_idx[i]);
+    }
+    pm_idx.resize(m.pm);
+    pm_len.resize(m.pm);
+    err = GLE_SUCCESS;
+  }
+  // every new CLA and P(t) slot of the tick back to the pool (a failed tick)
+  void release_all() { rollback(Mark{0, 0, 0}); }
 
   int take_slot() {
     if (!pool->has_free()) {  // evict the least recently used entry not used this tick
@@ -257,43 +316,46 @@ struct TickBuilder {
     return s;
   }
 
-  // device reference of X(a|b): CLA slot, or tip code -(tip)-1; fills rank/level
+  // device reference of X(a|b): CLA slot, or tip code -(tip)-1; fills rank,
+  // level and the subtree id (memoised in r.sid)
   int resolve(Req &r, int a, int b, int &rank, int &level) {
     if (req_tip(r, a)) {
       rank = 0;
       level = -1;
       return -a;
     }
-    const Sig s = req_sig(r, a, b);
-    auto it = si->cla.find(s);
-    if (it != si->cla.end()) {
-      it->second.last = tick;
-      rank = it->second.rank;
-      level = -1;
-      return it->second.slot;
+    const uint64_t known = known_id(r, *si, a, b);
+    if (known) {
+      auto it = si->cla.find(known);
+      if (it != si->cla.end()) {
+        it->second.last = tick;
+        rank = it->second.rank;
+        level = -1;
+        return it->second.slot;
+      }
+      auto f = fresh.find(known);
+      if (f != fresh.end()) {
+        rank = f->second.rank;
+        level = f->second.level;
+        return f->second.slot;
+      }
     }
-    auto f = fresh.find(s);
-    if (f != fresh.end()) {
-      rank = f->second.rank;
-      level = f->second.level;
-      return f->second.slot;
-    }
-    int x = -1, y = -1;
-    for (int j = 0; j < r.nnb[a]; j++) {
-      const int c = r.nb[a][j];
-      if (c == b) continue;
-      if (x < 0) x = c;
-      else y = c;
-    }
+    int x, y;
+    req_children(r, a, b, x, y);
     int rx, ry, lx, ly;
     DevClaOp op;
     op.c1 = resolve(r, x, a, rx, lx);
+    if (err) return 0;
     op.c2 = resolve(r, y, a, ry, ly);
     if (err) return 0;
+    const uint64_t ix = req_tip(r, x) ? uint64_t(x) : r.sid[x][req_nbk(r, x, a)];
+    const uint64_t iy = req_tip(r, y) ? uint64_t(y) : r.sid[y][req_nbk(r, y, a)];
+    const uint64_t id = si->intern_id(sub_key(ix, req_len(r, a, x), iy, req_len(r, a, y)));
+    r.sid[a][req_nbk(r, a, b)] = id;
     op.pm1 = pm_slot(req_len(r, a, x));
-    op.pm2 = pm_slot(req_len(r, a, y));
-    const int slot = take_slot();
-    if (slot < 0 || op.pm1 < 0 || op.pm2 < 0) {
+    op.pm2 = op.pm1 < 0 ? -1 : pm_slot(req_len(r, a, y));
+    const int slot = op.pm2 < 0 ? -1 : take_slot();
+    if (slot < 0) {
       err = set_error(GLE_ERR_NOMEM, "serving cache exhausted: raise n_cla_slots / n_pmat_slots");
       return 0;
     }
@@ -305,7 +367,8 @@ struct TickBuilder {
     }
     op.dest = slot;
     level = std::max(lx, ly) + 1;
-    fresh[s] = {slot, rank, level};
+    fresh[id] = {slot, rank, level};
+    fresh_order.push_back(id);
     ops.push_back({level, op});
     return slot;
   }
@@ -324,8 +387,8 @@ static void choose_score_edge(Req &r, const TickBuilder &tb) {
     if (req_tip(r, a)) return 0;
     int &m = memo[a][req_nbk(r, a, b)];
     if (m >= 0) return m;
-    const Sig s = req_sig(r, a, b);
-    if (tb.si->cla.count(s) || tb.fresh.count(s)) return m = 0;
+    const uint64_t id = known_id(r, *tb.si, a, b);
+    if (id && (tb.si->cla.count(id) || tb.fresh.count(id))) return m = 0;
     int c = 1;
     for (int j = 0; j < r.nnb[a]; j++)
       if (r.nb[a][j] != b) c += cost(r.nb[a][j], a);
@@ -375,94 +438,131 @@ static void advance(Req &r, const double *res) {
   }
 }
 
-// Runs one batched step of every request in `batch` (all on instance si).
+// Runs one batched step of the requests in `batch` (all on instance si).  A
+// request whose new CLAs or P(t) do not fit beside those the tick already
+// holds waits for the next tick (it stays active and unchanged); one that
+// does not fit on its own fails.  On a device error every request of the tick
+// fails and every slot the tick took goes back to the pool.  Returns how many
+// requests were deferred.
 static int run_tick(Server &sv, ServeInstance &si, std::vector<Req *> &batch, double &host_ms, double &device_ms) {
   using clock = std::chrono::steady_clock;
   auto ms_since = [](clock::time_point a) { return std::chrono::duration<double, std::milli>(clock::now() - a).count(); };
+  auto fail_all = [](std::vector<Req *> &v, int rc) {
+    for (Req *r : v) {
+      r->rc = rc;
+      r->done = true;
+    }
+  };
   for (;;) {
     const auto t_build = clock::now();
     auto t_launch = t_build;
     std::vector<DevClaOp> ops;
     std::vector<int> level_start;
     std::vector<DevEdgeOp> eops;
-    std::vector<std::pair<Sig, TickNew>> fresh;
-    {
-      std::lock_guard<std::mutex> pk(pool_mutex());
-      ClaPool *pool = pool_for(si.I);
-      if (si.gen != instance_generation(si.I)) {  // data or model changed: drop both caches
-        for (auto &c : si.cla)
-          if (--pool->refs[c.second.slot] == 0) pool->free_slot(c.second.slot);
-        si.cla.clear();
-        for (auto &c : si.pm) pool->pm_free.push_back(c.second.slot);
-        si.pm.clear();
-        si.gen = instance_generation(si.I);
-      }
-      TickBuilder tb{&si, pool, sv.tick, {}, {}, {}, {}, {}, GLE_SUCCESS};
-      for (Req *r : batch) {
-        r->sig.assign(r->nn, {Sig(), Sig(), Sig()});
-        next_edge(*r, tb);
-        int ra, la, rb, lb;
-        DevEdgeOp e;
-        e.partial = tb.resolve(*r, r->edge_a, r->edge_b, ra, la);
-        e.child = tb.resolve(*r, r->edge_b, r->edge_a, rb, lb);
-        e.pm = tb.pm_slot(req_len(*r, r->edge_a, r->edge_b));
-        e.derivs = r->derivs ? 1 : 0;
-        if (tb.err || e.pm < 0) return tb.err ? tb.err : set_error(GLE_ERR_NOMEM, "out of pmat slots");
-        eops.push_back(e);
-      }
-      std::stable_sort(tb.ops.begin(), tb.ops.end(),
-                       [](const std::pair<int, DevClaOp> &x, const std::pair<int, DevClaOp> &y) {
-                         return x.first < y.first;
-                       });
-      int cur = -1;
-      for (auto &lo : tb.ops) {
-        if (lo.first != cur) {
-          level_start.push_back(int(ops.size()));
-          cur = lo.first;
-        }
-        ops.push_back(lo.second);
-      }
-      level_start.push_back(int(ops.size()));
-      for (auto &f : tb.fresh) fresh.push_back(f);
-      host_ms += ms_since(t_build);
-      t_launch = clock::now();
-      // P, P', P'' for new branch lengths (one launch)
-      int rc = engine_update_pmats(si.I, tb.pm_idx.data(), tb.pm_len.data(), int(tb.pm_idx.size()), 1);
-      if (rc) return rc;
-      for (size_t i = 0; i < tb.pm_idx.size(); i++) si.pm[dbits(tb.pm_len[i])] = {tb.pm_idx[i], 0, sv.tick};
+    std::vector<Req *> in_tick;
+    int deferred = 0;
+    std::unique_lock<std::mutex> pk(pool_mutex());
+    ClaPool *pool = pool_for(si.I);
+    if (si.gen != instance_generation(si.I) || si.intern.size() > kInternCap) {
+      // data or model changed (or the name table is full): drop both caches
+      si.drop_clas(pool);
+      for (auto &c : si.pm) pool->pm_free.push_back(c.second.slot);
+      si.pm.clear();
+      si.intern.clear();
+      si.gen = instance_generation(si.I);
     }
+    TickBuilder tb{&si, pool, sv.tick, {}, {}, {}, {}, {}, {}, GLE_SUCCESS};
+    for (Req *r : batch) {
+      const TickBuilder::Mark m = tb.mark();
+      r->sid.assign(r->nn, {kUnset, kUnset, kUnset});
+      next_edge(*r, tb);
+      int ra, la, rb, lb;
+      DevEdgeOp e;
+      e.partial = tb.resolve(*r, r->edge_a, r->edge_b, ra, la);
+      e.child = tb.err ? 0 : tb.resolve(*r, r->edge_b, r->edge_a, rb, lb);
+      e.pm = tb.err ? -1 : tb.pm_slot(req_len(*r, r->edge_a, r->edge_b));
+      e.derivs = r->derivs ? 1 : 0;
+      if (!tb.err && e.pm < 0) tb.err = set_error(GLE_ERR_NOMEM, "serving cache exhausted: out of pmat slots");
+      if (tb.err) {
+        const int err = tb.err;
+        tb.rollback(m);
+        if (in_tick.empty()) {  // does not fit even alone
+          r->rc = err;
+          r->done = true;
+        } else {
+          deferred++;  // next tick, with the slots this tick holds free again
+          r->waits++;
+        }
+        continue;
+      }
+      eops.push_back(e);
+      in_tick.push_back(r);
+      r->waits = 0;
+    }
+    if (in_tick.empty()) return deferred;
+    std::stable_sort(tb.ops.begin(), tb.ops.end(),
+                     [](const std::pair<int, DevClaOp> &x, const std::pair<int, DevClaOp> &y) {
+                       return x.first < y.first;
+                     });
+    int cur = -1;
+    for (auto &lo : tb.ops) {
+      if (lo.first != cur) {
+        level_start.push_back(int(ops.size()));
+        cur = lo.first;
+      }
+      ops.push_back(lo.second);
+    }
+    level_start.push_back(int(ops.size()));
+    host_ms += ms_since(t_build);
+    t_launch = clock::now();
+    // P, P', P'' for new branch lengths (one launch)
+    int rc = engine_update_pmats(si.I, tb.pm_idx.data(), tb.pm_len.data(), int(tb.pm_idx.size()), 1);
+    if (rc) {
+      tb.release_all();
+      fail_all(in_tick, rc);
+      return deferred;
+    }
+    for (size_t i = 0; i < tb.pm_idx.size(); i++) si.pm[dbits(tb.pm_len[i])] = {tb.pm_idx[i], 0, sv.tick};
+    tb.pm_idx.clear();  // now owned by the P(t) cache
+    tb.pm_len.clear();
+    tb.pm_fresh.clear();
+    pk.unlock();
     int flags = 0;
-    int rc = engine_flags(si.I, nullptr, 1);
-    if (rc) return rc;
-    rc = engine_update_partials(si.I, ops.data(), int(ops.size()), level_start.data(),
-                                int(level_start.size()) - 1, false);
-    if (rc) return rc;
+    rc = engine_flags(si.I, nullptr, 1);
+    if (!rc)
+      rc = engine_update_partials(si.I, ops.data(), int(ops.size()), level_start.data(),
+                                  int(level_start.size()) - 1, false);
     std::vector<double> res(3 * eops.size());
-    rc = engine_edges(si.I, eops.data(), int(eops.size()), res.data(), nullptr);
-    if (rc) return rc;
-    rc = engine_flags(si.I, &flags, 1);
-    if (rc) return rc;
+    if (!rc) rc = engine_edges(si.I, eops.data(), int(eops.size()), res.data(), nullptr);
+    if (!rc) rc = engine_flags(si.I, &flags, 1);
     device_ms += ms_since(t_launch);
+    pk.lock();
+    if (rc) {
+      tb.release_all();
+      fail_all(in_tick, rc);
+      return deferred;
+    }
     const bool thrown = instance_precision(si.I) == GLE_PREC_FP64 &&
                         (((flags & GLE_FLAG_REDUCE_RESCALE) && si.rescale_every > 2) ||
                          ((flags & GLE_FLAG_BAIL_OUT) && si.rescale_every <= 2));
-    std::lock_guard<std::mutex> pk(pool_mutex());
-    ClaPool *pool = pool_for(si.I);
     if (thrown) {
       // Tree::Score's retry: drop every cached CLA (they were built with the old
       // rescaleEvery) and run the step again with rescaleEvery - 2.
-      for (auto &f : fresh)
-        if (--pool->refs[f.second.slot] == 0) pool->free_slot(f.second.slot);
-      for (auto &c : si.cla)
-        if (--pool->refs[c.second.slot] == 0) pool->free_slot(c.second.slot);
-      si.cla.clear();
+      tb.release_all();
+      si.drop_clas(pool);
       si.rescale_every -= 2;
-      if (si.rescale_every < 2) return set_error(GLE_ERR_RESCALE_FATAL, "Problem with rescaling in serving.");
+      if (si.rescale_every < 2) {
+        fail_all(in_tick, set_error(GLE_ERR_RESCALE_FATAL, "Problem with rescaling in serving."));
+        return deferred;
+      }
       continue;
     }
-    for (auto &f : fresh) si.cla[f.first] = {f.second.slot, f.second.rank, sv.tick};
-    for (size_t i = 0; i < batch.size(); i++) advance(*batch[i], &res[3 * i]);
-    return GLE_SUCCESS;
+    for (uint64_t id : tb.fresh_order) {
+      const TickNew &f = tb.fresh.at(id);
+      si.cla[id] = {f.slot, f.rank, sv.tick};
+    }
+    for (size_t i = 0; i < in_tick.size(); i++) advance(*in_tick[i], &res[3 * i]);
+    return deferred;
   }
 }
 
@@ -473,6 +573,7 @@ static int run_tick(Server &sv, ServeInstance &si, std::vector<Req *> &batch, do
 static int working_set(const Req &r) { return r.kind == kReqScore ? r.N - 2 : 2 * (r.N - 2) + 4; }
 
 static void worker_loop(Server *sv) {
+  std::vector<std::pair<ServeInstance *, std::vector<Req *>>> by_inst;
   for (;;) {
     std::vector<Req *> work;
     {
@@ -507,25 +608,30 @@ static void worker_loop(Server *sv) {
       }
       work = sv->active;
       sv->tick++;
+      // the instances are looked up here, under the lock that submissions
+      // insert under (map references stay valid across later inserts)
+      for (Req *r : work) {
+        ServeInstance *si = &sv->insts[r->inst];
+        auto it = std::find_if(by_inst.begin(), by_inst.end(), [&](const auto &e) { return e.first == si; });
+        if (it == by_inst.end()) by_inst.push_back({si, {r}});
+        else it->second.push_back(r);
+      }
     }
-    std::map<int, std::vector<Req *>> by_inst;
-    for (Req *r : work) by_inst[r->inst].push_back(r);
     double host_ms = 0.0, device_ms = 0.0;
+    int deferred = 0;
     for (auto &kv : by_inst) {
-      ServeInstance &si = sv->insts[kv.first];
-      const int rc = run_tick(*sv, si, kv.second, host_ms, device_ms);
-      if (rc)
-        for (Req *r : kv.second) {
-          r->rc = rc;
-          r->done = true;
-        }
+      // requests deferred last time go first, so none waits behind the same others forever
+      std::stable_sort(kv.second.begin(), kv.second.end(), [](const Req *a, const Req *b) { return a->waits > b->waits; });
+      deferred += run_tick(*sv, *kv.first, kv.second, host_ms, device_ms);
     }
+    by_inst.clear();
     {
       std::lock_guard<std::mutex> lk(sv->mu);
       sv->ticks++;
       sv->host_ms += host_ms;
       sv->device_ms += device_ms;
-      sv->batched_requests += long(work.size());
+      sv->batched_requests += long(work.size()) - deferred;
+      sv->deferred += deferred;
       std::vector<Req *> still;
       const auto now = std::chrono::steady_clock::now();
       for (Req *r : sv->active) {
@@ -651,6 +757,10 @@ extern "C" int gle_serve_submit(int h, int inst, const int *parent, const double
     if (r->need > instance_cla_slots(I))
       return set_error(GLE_ERR_NOMEM, "request needs %d CLA slots, instance has %d", r->need,
                        instance_cla_slots(I));
+    // a step may need a P(t) for every one of its 2N - 3 branch lengths
+    if (instance_pmat_slots(I) < 2 * r->N - 3)
+      return set_error(GLE_ERR_NOMEM, "request needs %d transition-matrix slots, instance has %d", 2 * r->N - 3,
+                       instance_pmat_slots(I));
     r->id = sv->next_id++;
     *req = r->id;
     sv->pending.push_back(r.get());

@@ -11,7 +11,9 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstring>
 #include <map>
+#include <unordered_map>
 #include <memory>
 #include <mutex>
 #include <vector>
@@ -45,7 +47,9 @@ struct Tree {
   // computed when asked for, on the same edge, instead of copied back with
   // every score.
   bool site_ready = false;
+  int score_pa = 0, score_pb = 0;  // the edge the last score was taken on
   std::vector<double> site;
+  long long cla_updates = 0;  // CLA updates this tree has computed (all calls)
   // scratch
   std::vector<int> level;
   std::vector<DevClaOp> ops;
@@ -95,33 +99,30 @@ static void dirty_edge(Tree &t, int n) {
   dirty_from(t, a, n);
 }
 
-static int build_tree(Tree &t, int inst_h, Instance *I, const int *parent, const double *blen) {
-  const int N = instance_tips(I), nn = 2 * N - 2;
-  t.inst_h = inst_h;
-  t.I = I;
-  t.pool = pool_for(I);
-  t.N = N;
-  t.nn = nn;
-  t.parent.assign(parent, parent + nn);
-  t.blen.assign(blen, blen + nn);
-  t.nb.assign(nn, {-1, -1, -1});
-  t.nnb.assign(nn, 0);
+// Topology of a parent array: neighbours (anc first, then children
+// ascending) and degrees, checked to be an unrooted binary tree on the
+// instance's tips with node 0 as the root.
+static int topology(int N, const int *parent, const double *blen, std::vector<std::array<int, 3>> &nb,
+                    std::vector<int> &nnb) {
+  const int nn = 2 * N - 2;
+  nb.assign(nn, {-1, -1, -1});
+  nnb.assign(nn, 0);
   if (parent[0] != -1) return set_error(GLE_ERR_ARG, "node 0 must be the root (parent -1)");
   for (int n = 1; n < nn; n++) {
     const int a = parent[n];
     if (a < 0 || a >= nn || a == n || (a >= 1 && a <= N))
       return set_error(GLE_ERR_ARG, "node %d: bad parent %d", n, a);
     if (!(blen[n] >= 0.0)) return set_error(GLE_ERR_ARG, "node %d: bad branch length", n);
-    t.nb[n][t.nnb[n]++] = a;
+    nb[n][nnb[n]++] = a;
   }
   for (int n = 1; n < nn; n++) {
     const int a = parent[n];
-    if (t.nnb[a] >= 3) return set_error(GLE_ERR_ARG, "node %d has too many children", a);
-    t.nb[a][t.nnb[a]++] = n;
+    if (nnb[a] >= 3) return set_error(GLE_ERR_ARG, "node %d has too many children", a);
+    nb[a][nnb[a]++] = n;
   }
   for (int n = 0; n < nn; n++) {
-    const int want = is_tip(t, n) ? 1 : 3;
-    if (t.nnb[n] != want) return set_error(GLE_ERR_ARG, "node %d has degree %d, expected %d", n, t.nnb[n], want);
+    const int want = (n >= 1 && n <= N) ? 1 : 3;
+    if (nnb[n] != want) return set_error(GLE_ERR_ARG, "node %d has degree %d, expected %d", n, nnb[n], want);
   }
   // every node must reach the root (no cycles)
   for (int n = 1; n < nn; n++) {
@@ -132,6 +133,20 @@ static int build_tree(Tree &t, int inst_h, Instance *I, const int *parent, const
     }
     if (x != 0) return set_error(GLE_ERR_ARG, "node %d does not reach the root", n);
   }
+  return GLE_SUCCESS;
+}
+
+static int build_tree(Tree &t, int inst_h, Instance *I, const int *parent, const double *blen) {
+  const int N = instance_tips(I), nn = 2 * N - 2;
+  t.inst_h = inst_h;
+  t.I = I;
+  t.pool = pool_for(I);
+  t.N = N;
+  t.nn = nn;
+  t.parent.assign(parent, parent + nn);
+  t.blen.assign(blen, blen + nn);
+  int rc = topology(N, parent, blen, t.nb, t.nnb);
+  if (rc) return rc;
   t.slot.assign(nn, {-1, -1, -1});
   if (int(t.pool->pm_free.size()) < nn) return set_error(GLE_ERR_NOMEM, "out of transition-matrix slots");
   t.pm.assign(nn, -1);
@@ -190,8 +205,17 @@ static int refresh_pmats(Tree &t, const std::vector<int> &edges, bool derivs) {
 }
 
 // Compute the listed directional CLAs (already in dependency order with levels).
+// On any failure the slots handed out here go back to the pool and the CLAs
+// stay dirty, so a later call recomputes them; the levels collect() recorded
+// are cleared on every path.
 static int run_ops(Tree &t, std::vector<std::array<int, 3>> &want) {
   if (want.empty()) return GLE_SUCCESS;
+  std::vector<std::array<int, 2>> assigned;  // (a, k) whose slot this call set
+  auto fail = [&](int rc) {
+    for (auto &ak : assigned) release(t, ak[0], ak[1]);
+    for (auto &w : want) t.level[size_t(w[0]) * 3 + w[1]] = -1;
+    return rc;
+  };
   const bool fused = t.schedule == 1 && engine_supports_fused(t.I);
   // Level schedule: sort by dependency level.  Fused schedule: keep collect()'s
   // depth-first post-order, so a CLA is consumed soon after it is written.
@@ -204,7 +228,7 @@ static int run_ops(Tree &t, std::vector<std::array<int, 3>> &want) {
       if (t.nb[a][j] != b) edges.push_back(edge_node(t, a, t.nb[a][j]));
   }
   int rc = refresh_pmats(t, edges, false);
-  if (rc) return rc;
+  if (rc) return fail(rc);
   std::vector<DevClaOp> ops;
   std::vector<int> level_start;
   ops.reserve(want.size());
@@ -240,11 +264,12 @@ static int run_ops(Tree &t, std::vector<std::array<int, 3>> &want) {
       op.flags = kOpRescale;
       rank = 0;
     }
-    if (!t.pool->has_free()) return set_error(GLE_ERR_NOMEM, "out of CLA slots");
+    if (!t.pool->has_free()) return fail(set_error(GLE_ERR_NOMEM, "out of CLA slots"));
     const int s = t.pool->take_slot();
     t.pool->refs[s] = 1;
     t.pool->rank[s] = rank;
     t.slot[a][k] = s;
+    assigned.push_back({a, k});
     op.dest = s;
     if (w[2] != cur) {
       level_start.push_back(int(ops.size()));
@@ -253,9 +278,12 @@ static int run_ops(Tree &t, std::vector<std::array<int, 3>> &want) {
     ops.push_back(op);
   }
   level_start.push_back(int(ops.size()));
+  rc = engine_update_partials(t.I, ops.data(), int(ops.size()), level_start.data(), int(level_start.size()) - 1,
+                              fused);
+  if (rc) return fail(rc);
   for (auto &w : want) t.level[size_t(w[0]) * 3 + w[1]] = -1;
-  return engine_update_partials(t.I, ops.data(), int(ops.size()), level_start.data(), int(level_start.size()) - 1,
-                                fused);
+  t.cla_updates += static_cast<long long>(ops.size());
+  return GLE_SUCCESS;
 }
 
 static int child_code(Tree &t, int c, int a) {
@@ -266,11 +294,11 @@ static int child_code(Tree &t, int c, int a) {
 // One attempt of an edge evaluation.  Returns GLE_RESCALE_RETRY when the
 // reference would have thrown from RescaleRateHet.
 static int edge_attempt(Tree &t, int pa, int pb, bool derivs, double *res, double *site, double *site5) {
-  std::vector<std::array<int, 3>> want;
-  collect(t, pa, pb, want);
-  collect(t, pb, pa, want);
   int flags = 0, rc = engine_flags(t.I, nullptr, 1);
   if (rc) return rc;
+  std::vector<std::array<int, 3>> want;  // collect() levels are cleared by run_ops on every path
+  collect(t, pa, pb, want);
+  collect(t, pb, pa, want);
   rc = run_ops(t, want);
   if (rc) return rc;
   const int en = edge_node(t, pa, pb);
@@ -319,9 +347,12 @@ static int tree_score(Tree &t, double *lnl) {
   root_edge(t, pa, pb);
   double res[3];
   t.site.clear();
+  t.site_ready = false;
   const int rc = edge_eval(t, pa, pb, false, res, nullptr);
   if (rc) return rc;
   t.site_ready = true;
+  t.score_pa = pa;  // site likelihoods are taken on this same edge later
+  t.score_pb = pb;
   t.lnl = res[0];
   *lnl = res[0];
   return GLE_SUCCESS;
@@ -386,12 +417,12 @@ static int optimize_all(Tree &t, double prec, double *improve, double *lnl) {
 static int sweep(Tree &t, double *out) {
   for (int attempt = 0;; attempt++) {
     dirty_all(t);
+    int flags = 0, rc = engine_flags(t.I, nullptr, 1);
+    if (rc) return rc;
     std::vector<std::array<int, 3>> want;
     for (int a = 0; a < t.nn; a++)
       if (!is_tip(t, a))
         for (int k = 0; k < t.nnb[a]; k++) collect(t, a, t.nb[a][k], want);
-    int flags = 0, rc = engine_flags(t.I, nullptr, 1);
-    if (rc) return rc;
     rc = run_ops(t, want);
     if (rc) return rc;
     std::vector<int> edges;
@@ -420,6 +451,101 @@ static void destroy_tree(Tree &t) {
   dirty_all(t);
   for (int n = 1; n < t.nn; n++)
     if (t.pm[n] >= 0) t.pool->pm_free.push_back(t.pm[n]);
+}
+
+// Exact names of directional CLAs: X(a|b) is named by its two (child name,
+// branch-length bits) pairs in a fixed order, tips by their node numbers, and
+// names are interned into integers.  Two CLAs get the same name exactly when
+// the subtrees on their side are identical (tips, topology, lengths), which
+// is exactly when one can stand for the other.
+struct CladeNames {
+  struct Key {
+    uint64_t x, lx, y, ly;
+    bool operator==(const Key &o) const { return x == o.x && lx == o.lx && y == o.y && ly == o.ly; }
+  };
+  struct KeyHash {
+    size_t operator()(const Key &k) const {
+      uint64_t h = k.x * 0x9E3779B97F4A7C15ull ^ (k.lx + 0x632BE59BD9B4E019ull);
+      h = (h ^ (h >> 29)) * 0xBF58476D1CE4E5B9ull ^ (k.y * 0x94D049BB133111EBull) ^ k.ly;
+      return size_t(h ^ (h >> 31));
+    }
+  };
+  std::unordered_map<Key, uint64_t, KeyHash> ids;
+  uint64_t next = uint64_t(1) << 40;  // above every tip number
+  uint64_t name(uint64_t ix, double lx, uint64_t iy, double ly) {
+    uint64_t bx, by;
+    std::memcpy(&bx, &lx, 8);
+    std::memcpy(&by, &ly, 8);
+    Key k{ix, bx, iy, by};
+    if (k.x > k.y || (k.x == k.y && k.lx > k.ly)) {
+      std::swap(k.x, k.y);
+      std::swap(k.lx, k.ly);
+    }
+    auto it = ids.find(k);
+    if (it != ids.end()) return it->second;
+    ids.emplace(k, next);
+    return next++;
+  }
+};
+
+static uint64_t clade_name(const Tree &t, int a, int b, CladeNames &cn, std::vector<std::array<uint64_t, 3>> &memo) {
+  if (is_tip(t, a)) return uint64_t(a);
+  uint64_t &m = memo[a][nb_index(t, a, b)];
+  if (m) return m;
+  int x = -1, y = -1;
+  for (int j = 0; j < t.nnb[a]; j++) {
+    const int c = t.nb[a][j];
+    if (c == b) continue;
+    if (x < 0) x = c;
+    else y = c;
+  }
+  const uint64_t ix = clade_name(t, x, a, cn, memo), iy = clade_name(t, y, a, cn, memo);
+  m = cn.name(ix, t.blen[edge_node(t, a, x)], iy, t.blen[edge_node(t, a, y)]);
+  return m;
+}
+
+// A topology move (SPRMutate / NNIMutate, tree.cpp:2571-2708, 6970) followed
+// by SweepDirtynessOverTree (tree.cpp:4247): the tree takes the new parent
+// array and lengths and keeps every clean CLA whose side of the tree the move
+// left unchanged -- however the nodes were renumbered -- so only the CLAs on
+// the paths through the changed edges are recomputed.  P(t) slots stay with
+// their nodes and are refreshed where a node's length changed.
+static int set_topology(Tree &t, const int *parent, const double *blen) {
+  std::vector<std::array<int, 3>> nb;
+  std::vector<int> nnb;
+  int rc = topology(t.N, parent, blen, nb, nnb);
+  if (rc) return rc;
+  CladeNames cn;
+  std::unordered_map<uint64_t, int> clean;  // name -> slot of the old tree's clean CLAs
+  {
+    std::vector<std::array<uint64_t, 3>> memo(t.nn, {0, 0, 0});
+    for (int a = 0; a < t.nn; a++)
+      if (!is_tip(t, a))
+        for (int k = 0; k < t.nnb[a]; k++)
+          if (t.slot[a][k] >= 0) clean.emplace(clade_name(t, a, t.nb[a][k], cn, memo), t.slot[a][k]);
+  }
+  const std::vector<std::array<int, 3>> old_slot = t.slot;
+  t.parent.assign(parent, parent + t.nn);
+  t.blen.assign(blen, blen + t.nn);
+  t.nb.swap(nb);
+  t.nnb.swap(nnb);
+  std::vector<std::array<uint64_t, 3>> memo(t.nn, {0, 0, 0});
+  for (int a = 0; a < t.nn; a++) {
+    t.slot[a] = {-1, -1, -1};
+    if (is_tip(t, a)) continue;
+    for (int k = 0; k < t.nnb[a]; k++) {
+      auto it = clean.find(clade_name(t, a, t.nb[a][k], cn, memo));
+      if (it != clean.end()) {
+        t.slot[a][k] = it->second;
+        t.pool->refs[it->second]++;
+      }
+    }
+  }
+  for (const auto &row : old_slot)  // the old tree's references
+    for (int s : row)
+      if (s >= 0 && --t.pool->refs[s] == 0) t.pool->free_slot(s);
+  t.site_ready = false;
+  return GLE_SUCCESS;
 }
 
 }  // namespace gle
@@ -505,6 +631,19 @@ extern "C" int gle_tree_set_branch_length(int h, int node, double v) {
   return GLE_SUCCESS;
 }
 
+extern "C" int gle_tree_set_topology(int h, const int *parent, const double *blen) {
+  if (!parent || !blen) return set_error(GLE_ERR_ARG, "null argument");
+  GET_TREE(h);
+  return set_topology(*T, parent, blen);
+}
+
+extern "C" int gle_tree_cla_updates(int h, long long *count) {
+  if (!count) return set_error(GLE_ERR_ARG, "null output");
+  GET_TREE(h);
+  *count = T->cla_updates;
+  return GLE_SUCCESS;
+}
+
 extern "C" int gle_tree_get_branch_lengths(int h, double *out) {
   if (!out) return set_error(GLE_ERR_ARG, "null output");
   GET_TREE(h);
@@ -530,11 +669,11 @@ extern "C" int gle_tree_site_likelihoods(int h, double *out) {
   Tree &t = *T;
   if (!t.site_ready) return set_error(GLE_ERR_STATE, "score the tree first");
   if (t.site.empty()) {
-    int pa, pb;
-    root_edge(t, pa, pb);
+    // the score's own edge: root_edge() now could pick another one, since
+    // later derivative calls may have made other root CLAs clean
     std::vector<double> site(size_t(instance_patterns(t.I)));
     double res[3];
-    const int rc = edge_eval(t, pa, pb, false, res, site.data());
+    const int rc = edge_eval(t, t.score_pa, t.score_pb, false, res, site.data());
     if (rc) return rc;
     t.site.swap(site);
     t.site_ready = true;  // a rescale retry inside edge_eval changes no value

@@ -106,6 +106,8 @@ SIGNATURES = {
     "gle_mfma_calibrate_shape": ([C.c_int, _D], C.c_int),
     "gle_get_kernel_stats": ([C.c_int, C.c_int, C.POINTER(C.c_longlong), _D, _D, C.c_int], C.c_int),
     "gle_tree_create": ([C.c_int, _I, _D, _I], C.c_int),
+    "gle_tree_set_topology": ([C.c_int, _I, _D], C.c_int),
+    "gle_tree_cla_updates": ([C.c_int, C.POINTER(C.c_longlong)], C.c_int),
     "gle_tree_copy": ([C.c_int, _I], C.c_int),
     "gle_tree_destroy": ([C.c_int], C.c_int),
     "gle_tree_set_branch_length": ([C.c_int, C.c_int, C.c_double], C.c_int),
@@ -524,12 +526,15 @@ class Instance:
         return out.reshape(3, R, S, S)
 
     def update_partials(self, ops, level_start=None):
+        """One batch of CLA ops (dest, child1, pmat1, child2, pmat2, rescale);
+        returns RESCALE_RETRY when the reference would have thrown from
+        RescaleRateHet (FP64), SUCCESS otherwise."""
         arr = (ClaOp * len(ops))(*[ClaOp(*o) for o in ops])
         if level_start is None:
-            _check(lib().gle_update_partials(self.h, arr, len(ops), None, 0))
-        else:
-            ls = np.ascontiguousarray(level_start, dtype=np.int32)
-            _check(lib().gle_update_partials(self.h, arr, len(ops), _p(ls, C.c_int), len(ls) - 1))
+            return _check(lib().gle_update_partials(self.h, arr, len(ops), None, 0), allow=(RESCALE_RETRY,))
+        ls = np.ascontiguousarray(level_start, dtype=np.int32)
+        return _check(lib().gle_update_partials(self.h, arr, len(ops), _p(ls, C.c_int), len(ls) - 1),
+                      allow=(RESCALE_RETRY,))
 
     def get_partials(self, slot):
         out = np.zeros(self.n_patterns * self.n_rates * self.n_states)
@@ -622,6 +627,20 @@ class Tree:
 
     def set_branch_length(self, node, v):
         _check(lib().gle_tree_set_branch_length(self.h, node, float(v)))
+
+    def set_topology(self, parent, blen):
+        """A topology move: the tree takes the new parent array and lengths and
+        keeps every CLA whose side of the tree is unchanged."""
+        par = np.ascontiguousarray(parent, dtype=np.int32)
+        bl = _d(blen)
+        assert len(par) == 2 * self.inst.n_tips - 2 and len(bl) == len(par)
+        _check(lib().gle_tree_set_topology(self.h, _p(par, C.c_int), _p(bl, C.c_double)))
+
+    def cla_updates(self):
+        """CLA updates this tree has computed so far."""
+        n = C.c_longlong(0)
+        _check(lib().gle_tree_cla_updates(self.h, C.byref(n)))
+        return n.value
 
     def branch_lengths(self):
         out = np.zeros(2 * self.inst.n_tips - 2)

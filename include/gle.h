@@ -212,8 +212,14 @@ typedef struct gle_cla_op {
   int dest, child1, pmat1, child2, pmat2, rescale;
 } gle_cla_op;
 /* ops sorted by dependency level; level_start[n_levels+1] gives the ranges
- * (BlockingOperationsSet, calculationmanager.h:118-123).  n_levels = 0 means
- * one op per level. */
+ * (BlockingOperationsSet, calculationmanager.h:118-123): non-decreasing from 0
+ * to n_ops, and no op may read or overwrite a slot another op of its own
+ * level writes (GLE_ERR_ARG otherwise).  n_levels = 0: the engine derives the
+ * levels of a topologically ordered list itself.  FP64 instances return
+ * GLE_RESCALE_RETRY, with every CLA written, when the rescale flags gathered
+ * since their last reset (gle_get_rescale_flags) hold GLE_FLAG_REDUCE_RESCALE:
+ * the reference's throw(1) from RescaleRateHet (tree.cpp:3436-3449), after
+ * which the caller lowers its rescale frequency and recomputes. */
 int gle_update_partials(int h, const gle_cla_op *ops, int n_ops, const int *level_start, int n_levels);
 /* debugging/parity: one CLA slot back in the reference layout [P][R][S] + u[P] */
 int gle_get_partials(int h, int slot, double *out, int *u);
@@ -273,6 +279,15 @@ int gle_tree_destroy(int tree);
 /* Tree::SetBranchLength (tree.h:425) + SweepDirtynessOverTree (tree.cpp:4247) */
 int gle_tree_set_branch_length(int tree, int node, double blen);
 int gle_tree_get_branch_lengths(int tree, double *blen);
+/* A topology move: Tree::SPRMutate / NNIMutate (tree.cpp:2571-2708, 6970)
+ * followed by SweepDirtynessOverTree (tree.cpp:4247).  The tree takes the new
+ * parent array and lengths (as in gle_tree_create) and keeps every clean CLA
+ * whose side of the tree is unchanged -- matched by the exact subtree (tips,
+ * topology and branch-length bits), whatever the node numbers -- so only the
+ * CLAs on the paths through the changed edges are recomputed. */
+int gle_tree_set_topology(int tree, const int *parent, const double *blen);
+/* CLA updates the tree has computed so far (every call that ran ops). */
+int gle_tree_cla_updates(int tree, long long *count);
 /* Tree::MakeAllNodesDirty (tree.cpp): every CLA of this tree recomputed on next use */
 int gle_tree_invalidate(int tree);
 /* Tree::Score (tree.cpp:4003-4094), with the rescale retry; *lnl = Tree::lnL */

@@ -44,3 +44,22 @@ def test_instance_description_checked_before_the_device():
     if E.device_count() == 0:  # no GPU here: a valid description fails on the device
         assert _create() == -2
         assert b"device" in E.lib().gle_last_error().lower() or b"gpu" in E.lib().gle_last_error().lower()
+
+
+def test_level_ranges_checked_before_the_device():
+    """gle_update_partials refuses level ranges that skip, reorder or overrun
+    the op list, and levels whose ops read a CLA written beside them, before
+    it looks at the instance (so this runs without a GPU)."""
+    ops = (E.ClaOp * 3)(E.ClaOp(0, -1, 0, -2, 1, 0), E.ClaOp(1, -3, 2, -4, 3, 0), E.ClaOp(2, 0, 4, 1, 5, 1))
+    lib = E.lib()
+
+    def call(levels):
+        ls = (E.C.c_int * len(levels))(*levels)
+        return lib.gle_update_partials(12345, ops, 3, ls, len(levels) - 1), lib.gle_last_error().decode()
+
+    for bad, what in (([0, 4, 3], "range"), ([0, 2, 1, 3], "range"), ([1, 3], "cover"), ([0, 3], "own level"),
+                      ([0, -1, 3], "range")):
+        rc, msg = call(bad)
+        assert rc == -1 and what in msg, (bad, msg)
+    rc, msg = call([0, 2, 3])  # valid levels: now the bad handle is what fails
+    assert rc == -1 and "instance" in msg
