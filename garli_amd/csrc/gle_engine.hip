@@ -78,6 +78,32 @@ __constant__ double c_mult[kRescaleLen];
 __constant__ int c_incr[kRescaleLen];
 __constant__ double c_rescale_below, c_reduce_below, c_bail_below;
 
+// Scratch of the allocation-free edge entry point (gle_edge_likelihoods_noalloc):
+// the uploaded op list, the per-block partial sums and the per-op results of an
+// instance and of its FP64 heavy part, in place of the buffers
+// gle_edge_likelihoods grows with hipMalloc.  Every GPU the library runs on
+// holds its own copy until the process ends, and every such call on one GPU
+// shares it.  In host code `scratch` only names the symbol: kernels and copies
+// use the device address hipGetSymbolAddress gives for the current device,
+// kept in g_scratch_dev.
+constexpr size_t kNoAllocScratchBytes = size_t(8) << 20;
+constexpr int kNoAllocMaxOps = 1024;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static unsigned char *g_scratch_dev[kMaxDevices];
+
+// Once per device, with that device current (called from device set-up).
+static int scratch_setup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return set_error(GLE_ERR_DEVICE, "device %d beyond the scratch table", dev);
+  if (g_scratch_dev[dev]) return GLE_SUCCESS;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return set_error(GLE_ERR_DEVICE, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                     hipGetErrorString(e));
+  g_scratch_dev[dev] = static_cast<unsigned char *>(p);
+  return GLE_SUCCESS;
+}
 
 // --------------------------------------------------------------- instance
 struct Instance {
@@ -218,12 +244,16 @@ std::mutex &pool_mutex() {
   return m;
 }
 
+static int scratch_setup(int dev);
+
 static int ensure_device(int dev = 0) {
   if (g_device >= 0) return GLE_SUCCESS;
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
     return set_error(GLE_ERR_DEVICE, "no HIP device visible: the GARLI engine runs only on a GPU");
   GLE_HIP(hipSetDevice(dev));
+  const int rs = scratch_setup(dev);
+  if (rs) return rs;
   g_device = dev;
   const RescaleTables &t = rescale_tables();
   GLE_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_thresh), t.thresh, sizeof t.thresh));
@@ -2621,6 +2651,67 @@ extern "C" int gle_edge_likelihoods(int h, const gle_edge_op *ops, int n, double
   std::vector<DevEdgeOp> d(n);
   for (int i = 0; i < n; i++) d[i] = {ops[i].partial, ops[i].child, ops[i].pmat, ops[i].derivs ? 1 : 0};
   return engine_edges(I, d.data(), n, out, site);
+}
+
+static size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
+// Scratch bytes one part (an instance or its heavy part) needs for n edge ops,
+// and its buffers carved from `base` in the same order.
+static size_t edge_scratch_bytes(const Instance *X, int n) {
+  return round256(sizeof(DevEdgeOp) * size_t(n)) + round256(sizeof(double) * 3 * size_t(n)) +
+         round256(sizeof(double) * 3 * size_t(n) * size_t(edge_blocks(X)));
+}
+static EdgeBufs carve_edge_scratch(unsigned char *base, int n) {
+  EdgeBufs b;
+  b.ops = reinterpret_cast<DevEdgeOp *>(base);
+  base += round256(sizeof(DevEdgeOp) * size_t(n));
+  b.res = reinterpret_cast<double *>(base);
+  base += round256(sizeof(double) * 3 * size_t(n));
+  b.part = reinterpret_cast<double *>(base);
+  return b;
+}
+
+// gle_edge_likelihoods with every device buffer taken from the library's
+// scratch array: no allocation on any path.  Each block writes its own
+// partial sums and each op its own result, so nothing has to start at zero.
+extern "C" int gle_edge_likelihoods_noalloc(int h, const gle_edge_op *ops, int n, double *out, double *site) {
+  GET_INSTANCE(h);
+  if (n < 0 || (n > 0 && (!ops || !out))) return set_error(GLE_ERR_ARG, "bad edge op list");
+  if (n > kNoAllocMaxOps)
+    return set_error(GLE_ERR_ARG, "%d edge ops: the allocation-free entry point takes at most %d", n,
+                     kNoAllocMaxOps);
+  if (n == 0) return GLE_SUCCESS;
+  DevEdgeOp d[kNoAllocMaxOps];
+  for (int i = 0; i < n; i++) d[i] = {ops[i].partial, ops[i].child, ops[i].pmat, ops[i].derivs ? 1 : 0};
+  std::lock_guard<std::mutex> lk_(I->mu);
+  Instance *H = I->heavy.get();
+  if (site && H)
+    return set_error(GLE_ERR_UNSUPPORTED,
+                     "site values of a split FP32 instance need gle_edge_likelihoods (host-side reordering)");
+  const size_t main_bytes = edge_scratch_bytes(I, n);
+  const size_t need = main_bytes + (H ? edge_scratch_bytes(H, n) : 0);
+  if (need > kNoAllocScratchBytes)
+    return set_error(GLE_ERR_ARG, "%d edge ops over %d patterns need %zu bytes of scratch; the maximum is %zu", n,
+                     I->P_orig, need, kNoAllocScratchBytes);
+  int dev = 0;
+  GLE_HIP(hipGetDevice(&dev));
+  int rc = scratch_setup(dev);  // set up already, unless this device was never initialised by the library
+  if (rc) return rc;
+  unsigned char *base = g_scratch_dev[dev];
+  const EdgeBufs bm = carve_edge_scratch(base, n);
+  rc = enqueue_edges(I, d, n, site != nullptr, &bm);
+  EdgeBufs bh{};
+  if (!rc && H) {
+    bh = carve_edge_scratch(base + main_bytes, n);
+    rc = enqueue_edges(H, d, n, false, &bh);
+  }
+  double out_h[3 * kNoAllocMaxOps];
+  if (!rc) rc = copy_edges_out(I, n, out, site, 1, bm.res);
+  if (!rc && H) rc = copy_edges_out(H, n, out_h, nullptr, 1, bh.res);
+  rc = finish_both(I, rc);
+  if (rc) return rc;
+  if (H)
+    for (int i = 0; i < 3 * n; i++) out[i] += out_h[i];
+  return GLE_SUCCESS;
 }
 
 extern "C" int gle_edge_site_values(int h, const gle_edge_op *op, double *out3, double *site_lnl, double *site_d1,

@@ -97,6 +97,7 @@ SIGNATURES = {
     "gle_update_partials": ([C.c_int, C.POINTER(ClaOp), C.c_int, _I, C.c_int], C.c_int),
     "gle_get_partials": ([C.c_int, C.c_int, _D, _I], C.c_int),
     "gle_edge_likelihoods": ([C.c_int, C.POINTER(EdgeOp), C.c_int, _D, _D], C.c_int),
+    "gle_edge_likelihoods_noalloc": ([C.c_int, C.POINTER(EdgeOp), C.c_int, _D, _D], C.c_int),
     "gle_edge_site_values": ([C.c_int, C.POINTER(EdgeOp), _D, _D, _D, _D, _I, _I], C.c_int),
     "gle_get_rescale_flags": ([C.c_int, _I, C.c_int], C.c_int),
     "gle_synchronize": ([C.c_int], C.c_int),
@@ -548,6 +549,16 @@ class Instance:
         site = np.zeros(self.n_patterns) if want_sites else None
         _check(lib().gle_edge_likelihoods(self.h, arr, len(ops), _p(out, C.c_double),
                                           _p(site, C.c_double) if want_sites else None))
+        return (out.reshape(-1, 3), site) if want_sites else out.reshape(-1, 3)
+
+    def edge_likelihoods_noalloc(self, ops, want_sites=False):
+        """edge_likelihoods through the allocation-free entry point (device
+        buffers from the library's per-device scratch array)."""
+        arr = (EdgeOp * len(ops))(*[EdgeOp(*o) for o in ops])
+        out = np.zeros(3 * len(ops))
+        site = np.zeros(self.n_patterns) if want_sites else None
+        _check(lib().gle_edge_likelihoods_noalloc(self.h, arr, len(ops), _p(out, C.c_double),
+                                                  _p(site, C.c_double) if want_sites else None))
         return (out.reshape(-1, 3), site) if want_sites else out.reshape(-1, 3)
 
     def edge_site_values(self, op):

@@ -232,6 +232,16 @@ typedef struct gle_edge_op {
   int partial, child, pmat, derivs;
 } gle_edge_op;
 int gle_edge_likelihoods(int h, const gle_edge_op *ops, int n, double *out, double *site_lnl);
+/* gle_edge_likelihoods for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc); it has no counterpart in the reference.  The op list, the
+ * per-block partial sums and the results live in the library's 8 MiB
+ * per-device scratch array instead of buffers grown with hipMalloc: at most
+ * 1024 ops, and ops x ceil(patterns / block) x 24 bytes must fit beside them
+ * (block = 256 patterns for 4 states, 64 for 20, 16 for 61), else GLE_ERR_ARG.
+ * site_lnl is refused (GLE_ERR_UNSUPPORTED) for an FP32 instance with an FP64
+ * heavy part.  Calls that overlap on one GPU race on the scratch array. */
+int gle_edge_likelihoods_noalloc(int h, const gle_edge_op *ops, int n, double *out, double *site_lnl);
 /* One edge with derivatives and its per-pattern values, the inputs of
  * Tree::OutputSiteDerivatives (tree.cpp:5567-5595; filled under
  * OUTPUT_SITEDERIVS in GetDerivsPartialTerminal / Internal,
