@@ -55,6 +55,8 @@ def main():
     ap.add_argument("--schedule", choices=["levels", "fused"], default="levels")
     ap.add_argument("--evals", type=int, default=60, help="timed full-tree lnL evaluations")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle legs")
+    ap.add_argument("--no-codon", action="store_true",
+                    help="skip the codon served-vs-plain leg of the default workload")
     ap.add_argument("--patterns", type=int, default=100000)
     ap.add_argument("--taxa", type=int, default=128)
     ap.add_argument("--quick", action="store_true",
@@ -74,7 +76,8 @@ def main():
     from garli_amd import sim
 
     if args.workload in ("aa", "codon"):
-        return bench_states(args)
+        print(json.dumps(bench_states(args)), flush=True)
+        return
     if args.workload == "serve":
         return bench_serve(args)
 
@@ -134,6 +137,7 @@ def main():
         copy = {"bytes": nbytes, "ms": ms, "GB/s": 2 * nbytes / ms / 1e6}
         log("copy calibration: %d bytes in %.3f ms -> %.1f GB/s (read+write)" % (nbytes, ms, copy["GB/s"]))
     if args.quick:
+        args.no_codon = True
         print(json.dumps({"metric": METRIC, "value": site_updates, "unit": "CLA site-updates/s",
                           "ms_per_step": ms_step, "cla_avg_launch_ms": avg_ms, "cla_GBs": achieved,
                           "launches": n_l, "copy": copy}), flush=True)
@@ -195,7 +199,8 @@ def main():
         n_mt = 12
         wall = _threaded([lambda q=q: [q.sweep() for _ in range(n_mt)] for q in parts])
         cpu = {"value": n_mt * n_cla * P / wall, "unit": "CLA site-updates/s", "cores": len(parts), "kind": "port",
-               "single_thread_value": one,
+               "single_thread_value": one, "host_cores_allowed": _host_cores()[0],
+               "host_cores_reported": _host_cores()[1],
                "sample": "%d full sweeps of the same workload (%d CLA updates x %d patterns + %d edge derivatives "
                          "each), FP64, patterns split over %d threads as the reference's OpenMP build splits "
                          "them, oracle built -O3 -ffast-math, %.1fs wall; one thread: %d sweeps, %.1fs" %
@@ -236,6 +241,19 @@ def main():
                      "bytes_per_site_update": 3 * 4 * 4 * (4 if prec == E.PREC_FP32 else 8) + 12},
         "cpu_baseline": cpu,
     }
+    if not args.no_codon:
+        # the serving target's own configuration, timed in the same run: codon
+        # GY94 M3, 32 taxa x 5,000 patterns, 64 GA offspring, FP32-bulk served
+        # against the plain FP64 drop-in scoring each request on its own
+        c = bench_states(args, workload="codon", cpu_leg=False)
+        out["codon_served_evals_per_s"] = c["fp32_served_evals_per_s"]
+        out["codon_fp64_plain_evals_per_s"] = c["fp64_plain_evals_per_s"]
+        out["served_vs_fp64_plain"] = c["served_vs_fp64_plain"]
+        out["codon_served_p50_ms"] = c["serve_stats"]["p50_ms"]
+        out["codon_served_p99_ms"] = c["serve_stats"]["p99_ms"]
+        out["codon_served_max_dlnl_vs_oracle"] = c["accuracy"]["serve_fp32_max_dlnl"]
+        out["codon_roofline"] = c["roofline"]
+        out["codon_config"] = c["config"]
     print(json.dumps(out), flush=True)
 
 
@@ -256,14 +274,20 @@ def _cla_kinds(parent, n_tips):
     return kinds
 
 
+def _host_cores():
+    """(cores this process may run on, cores the host reports)."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    return aff, os.cpu_count() or aff
+
+
 def _cpu_threads():
     """Host threads for the CPU baseline: the cores this process may run on,
-    at most 16 (one GPU's share of the host)."""
-    try:
-        n = len(os.sched_getaffinity(0))
-    except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+    at most 16 -- the host share that goes with one GPU on these machines
+    (worker pools are sized to it whatever the host reports)."""
+    return max(1, min(16, _host_cores()[0]))
 
 
 def _pattern_slices(O, ds, T):
@@ -327,14 +351,16 @@ def _serve_rate(ds, E, pop, prec, slots):
     return len(pop) / dt, st, res
 
 
-def bench_states(args):
+def bench_states(args, workload=None, cpu_leg=True):
     """AA (20 states, G4, 64 taxa, 20k patterns) or codon (GY94 M3, 32 taxa, 5k
     patterns): sweeps in both forms, lnL evaluations per request (FP64 plain
-    drop-in) and batched through the serving layer (FP32 bulk)."""
+    drop-in) and batched through the serving layer (FP32 bulk).  Returns the
+    record (printed by main for --workload aa|codon)."""
     from garli_amd import engine as E
     from garli_amd import sim
     from oracle import oracle as O
-    if args.workload == "aa":
+    workload = workload or args.workload
+    if workload == "aa":
         ds = sim.aa_gamma(n_tips=64, n_patterns=20000, seed=3)
         name, S = "amino-acid 20-state G4 (random exchangeabilities), 64 taxa, 20000 patterns", 20
     else:
@@ -394,14 +420,13 @@ def bench_states(args):
                 "algorithmic_bytes_per_launch": res["fp32"]["cla_algorithmic_bytes_per_launch"],
                 "avg_launch_ms": res["fp32"]["cla_avg_launch_ms"], "launches": res["fp32"]["cla_launches"],
                 "kernel": "k_cla_aa<float,4>"}
-    if args.quick:
+    if args.quick and workload == args.workload:
         copy = None
         if args.calibrate > 0:
             nbytes = int(args.calibrate * (1 << 30))
             cms = E.copy_calibrate(nbytes, 10)
             copy = {"bytes": nbytes, "ms": cms, "GB/s": 2 * nbytes / cms / 1e6}
-        print(json.dumps({"workload": name, "sweeps": res, "roofline": roof, "copy": copy}), flush=True)
-        return
+        return {"workload": name, "sweeps": res, "roofline": roof, "copy": copy}
     pop = _population(ds, 64, 11)
     e64, lat64 = _eval_rates(ds, E, pop, E.PREC_FP64)
     e32, _ = _eval_rates(ds, E, pop, E.PREC_FP32)
@@ -411,17 +436,19 @@ def bench_states(args):
     s32, st, served = _serve_rate(ds, E, pop, E.PREC_FP32, slots)
     o = O.Oracle(ds)
     ref = [o.score(parent=p, blen=b) for p, b in pop[:8]]
-    # CPU baseline: the plain FP64 path on the host, each request's full
-    # score with its patterns split over the host threads
-    T = _cpu_threads()
-    parts = _pattern_slices(O, ds, T)
-    n_cpu = 4
-    wall = _threaded([lambda q=q: [q.score(parent=p, blen=b) for p, b in pop[:n_cpu]] for q in parts])
-    cpu = {"value": n_cpu / wall, "unit": "lnL evals/s", "cores": len(parts), "kind": "port",
-           "sample": "%d of the same requests scored in full (FP64, oracle built -O3 -ffast-math), patterns "
-                     "split over %d threads as the reference's OpenMP build splits them, %.1fs wall" %
-                     (n_cpu, len(parts), wall)}
-    log("cpu baseline, %d threads: %.2f lnL evals/s" % (len(parts), cpu["value"]))
+    cpu = None
+    if cpu_leg:
+        # CPU baseline: the plain FP64 path on the host, each request's full
+        # score with its patterns split over the host threads
+        T = _cpu_threads()
+        parts = _pattern_slices(O, ds, T)
+        n_cpu = 4
+        wall = _threaded([lambda q=q: [q.score(parent=p, blen=b) for p, b in pop[:n_cpu]] for q in parts])
+        cpu = {"value": n_cpu / wall, "unit": "lnL evals/s", "cores": len(parts), "kind": "port",
+               "sample": "%d of the same requests scored in full (FP64, oracle built -O3 -ffast-math), patterns "
+                         "split over %d threads as the reference's OpenMP build splits them, %.1fs wall" %
+                         (n_cpu, len(parts), wall)}
+        log("cpu baseline, %d threads: %.2f lnL evals/s" % (len(parts), cpu["value"]))
     acc = {"serve_fp32_max_dlnl": float(max(abs(x - y) for x, y in zip(served[:8], ref))),
            "sweep_fp32_vs_fp64_max_dlnl": float(np.max(np.abs(sweeps["fp32"][:, 0] - sweeps["fp64"][:, 0]))),
            "sweep_fp64_vs_oracle_max_rel": float(np.max(np.abs(sweeps["fp64"][:, 0] - o.sweep()[:, 0]) /
@@ -437,48 +464,54 @@ def bench_states(args):
            "cpu_baseline": cpu,
            "serve_config": {"cla_slots": slots, "hbm_budget_GB": slots * ds.n_rates * ((P + 255) // 256 * 256) * (64 if S == 61 else S) * 4 / 1e9,
                             "batch_window_us": SERVE_WINDOW_US}}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_serve(args):
-    """Config 5: a stream of GA requests with Poisson arrivals over uneven
-    instances (nucleotide and amino-acid, 16-256 taxa, 1k-100k patterns), each
-    a parent tree plus 1-3 NNI/SPR moves; FP32 bulk served vs FP64 plain
-    per-request drop-in."""
+    """The serving workload: a stream of 256 GA requests with Poisson arrivals
+    over uneven instances (nucleotide and amino-acid, 16-512 taxa, 1k-200k
+    patterns), each a parent tree plus 1-3 NNI/SPR moves; one in eight asks for
+    a branch-length pass before its score (Individual::Mutate's
+    optimisation, individual.cpp:127-222).  FP32 bulk served against the FP64
+    plain drop-in running the same requests one at a time."""
     import threading
     from garli_amd import engine as E
     from garli_amd import sim
     rng = np.random.default_rng(99)
     E.init(0)
     specs = [("gtr", 16, 1000), ("aa", 24, 2000), ("gtr", 64, 20000), ("aa", 48, 5000),
-             ("gtr", 128, 50000), ("gtr", 256, 100000), ("aa", 96, 10000), ("gtr", 32, 5000)]
+             ("gtr", 128, 50000), ("gtr", 512, 4000), ("aa", 96, 10000), ("gtr", 32, 200000)]
     dss = []
+    t0 = time.time()
     for k, (kind, n, p) in enumerate(specs):
         if kind == "gtr":
             dss.append(sim.gtr_gamma(n_tips=n, n_sites=None, n_patterns=p, mean_blen=0.05, pinv=0.1,
                                      ambig_frac=0.0, seed=200 + k))
         else:
             dss.append(sim.aa_gamma(n_tips=n, n_patterns=p, seed=200 + k))
+    log("simulated %d instances in %.1fs" % (len(dss), time.time() - t0))
     n_req = 256
     which = rng.integers(len(dss), size=n_req)
     reqs = []
     for k in range(n_req):
         ds = dss[which[k]]
         p, b = sim.perturb(ds.parent, ds.blen, rng, ds.n_tips, int(rng.integers(1, 4)))
-        reqs.append((int(which[k]), p, b))
+        # branch-length passes on the instances of up to 128 taxa
+        kind = E.REQ_OPTIMIZE if (k % 8 == 3 and ds.n_tips <= 128) else E.REQ_SCORE
+        reqs.append((int(which[k]), p, b, kind))
+    n_opt = sum(1 for r in reqs if r[3] == E.REQ_OPTIMIZE)
     # FP64 plain per-request drop-in
     insts64 = [ds.make_instance(E.PREC_FP64) for ds in dss]
     a = time.perf_counter()
     plain = []
-    for w, p, b in reqs:
+    for w, p, b, kind in reqs:
         t = E.Tree(insts64[w], p, b)
-        plain.append(t.score())
+        plain.append(t.score() if kind == E.REQ_SCORE else t.optimize_all_branches(0.01)[1])
         t.close()
     t64 = time.perf_counter() - a
-    for i in insts64:
-        i.close()
+    log("plain fp64: %d requests (%d branch passes) in %.2fs" % (n_req, n_opt, t64))
     # FP32 bulk served, Poisson arrivals at a rate above the plain path's capacity
-    budget = 16 << 30
+    budget = 32 << 30
     insts = []
     for ds in dss:
         slot_bytes = ds.n_patterns * ds.n_rates * ds.n_states * 4
@@ -487,53 +520,69 @@ def bench_serve(args):
     srv = E.Server()
     rate = 4.0 * n_req / t64
     ids = [None] * n_req
+    gaps = rng.exponential(1.0 / rate, size=n_req)
 
     def producer():
-        for k, (w, p, b) in enumerate(reqs):
-            ids[k] = srv.submit(insts[w], p, b)
-            time.sleep(float(rng.exponential(1.0 / rate)))
+        for k, (w, p, b, kind) in enumerate(reqs):
+            ids[k] = srv.submit(insts[w], p, b, kind=kind, precision=0.01)
+            time.sleep(float(gaps[k]))
 
     a = time.perf_counter()
     th = threading.Thread(target=producer)
     th.start()
     th.join()
-    lnls = [srv.wait(i)[0] for i in ids]
+    served = [srv.wait(i, n_nodes=len(reqs[k][1])) for k, i in enumerate(ids)]
     t32 = time.perf_counter() - a
     st = srv.stats()
+    tm = srv.timing()
     srv.close()
-    # every served FP32 result against the FP64 drop-in's score of the same request
-    dl = [abs(x - y) for x, y in zip(lnls, plain)]
-    acc = {"served_fp32_vs_fp64_max_dlnl": float(max(dl)), "served_fp32_vs_fp64_mean_dlnl": float(np.mean(dl))}
+    # every served FP32 result against the FP64 drop-in on the same request:
+    # scores directly, branch passes rescored at the served lengths in FP64
+    dl, dopt = [], []
+    for (w, p, b, kind), (lnl, _, bl), ref in zip(reqs, served, plain):
+        if kind == E.REQ_SCORE:
+            dl.append(abs(lnl - ref))
+        else:
+            t = E.Tree(insts64[w], p, bl)
+            dl.append(abs(lnl - t.score()))
+            dopt.append(lnl - ref)
+            t.close()
+    for i in insts64:
+        i.close()
+    acc = {"served_fp32_vs_fp64_max_dlnl": float(max(dl)), "served_fp32_vs_fp64_mean_dlnl": float(np.mean(dl)),
+           "branch_pass_served_minus_fp64_pass": [float(min(dopt)), float(max(dopt))] if dopt else None}
     log("served fp32 vs fp64 plain: max |dlnL| %.2e over %d requests" % (max(dl), n_req))
     cpu = None
     if not args.no_cpu:
-        # CPU baseline: one request of each instance scored in full by the
-        # FP64 oracle, each request's patterns split over the host threads;
-        # the stream's rate is the inverse of the mean time over its mix
+        # CPU baseline: one score request of each instance in full by the FP64
+        # oracle, patterns split over the host threads; the stream's score rate
+        # is the inverse of the mean time over its mix of instances
         from oracle import oracle as O
         T = _cpu_threads()
         t_of = {}
-        for w, p, b in reqs:
-            if w in t_of:
+        for w, p, b, kind in reqs:
+            if w in t_of or kind != E.REQ_SCORE:
                 continue
             parts = _pattern_slices(O, dss[w], T)
             t_of[w] = _threaded([lambda q=q, p=p, b=b: q.score(parent=p, blen=b) for q in parts])
-        mean_t = sum(t_of[w] for w, _, _ in reqs) / n_req
-        cpu = {"value": 1.0 / mean_t, "unit": "lnL evals/s", "cores": T, "kind": "port",
-               "sample": "one request of each of the %d instances scored in full (FP64, oracle built -O3 "
+        n_sc = sum(1 for r in reqs if r[3] == E.REQ_SCORE and r[0] in t_of)
+        mean_t = sum(t_of[w] for w, _, _, kind in reqs if kind == E.REQ_SCORE and w in t_of) / max(n_sc, 1)
+        cpu = {"value": 1.0 / mean_t, "unit": "lnL evals/s (score requests)", "cores": T, "kind": "port",
+               "sample": "one score request of each of the %d instances in full (FP64, oracle built -O3 "
                          "-ffast-math), patterns split over %d threads as the reference's OpenMP build splits them, "
-                         "%.2fs wall in all; weighted by the stream's mix of instances" % (len(t_of), T,
-                                                                                         sum(t_of.values()))}
+                         "%.2fs wall in all; weighted by the stream's mix of score requests (branch passes "
+                         "not timed on the CPU)" % (len(t_of), T, sum(t_of.values()))}
         log("cpu baseline, %d threads: %.2f lnL evals/s" % (T, cpu["value"]))
-    out = {"metric": METRIC, "value": n_req / t32, "unit": "lnL evals/s", "n_gpus": 1, "steps": n_req,
+    out = {"metric": METRIC, "value": n_req / t32, "unit": "requests/s", "n_gpus": 1, "steps": n_req,
            "warmup": 0, "ms_per_step": 1000.0 * t32 / n_req, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded simulations)",
-           "config": {"workload": "GA request stream, Poisson arrivals, %d requests over %d instances" %
-                      (n_req, len(dss)), "instances": [list(x) for x in specs], "arrival_rate_per_s": rate},
-           "fp64_plain_evals_per_s": n_req / t64, "served_vs_fp64_plain": t64 / t32,
-           "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_batch": st["mean_batch"], "accuracy": acc,
-           "cpu_baseline": cpu}
-    log("serve: %.1f evals/s (plain fp64 %.1f), p50 %.2f ms p99 %.2f ms, mean batch %.1f" %
+           "config": {"workload": "GA request stream, Poisson arrivals, %d requests (%d with a branch-length pass) "
+                      "over %d instances" % (n_req, n_opt, len(dss)),
+                      "instances": [list(x) for x in specs], "arrival_rate_per_s": rate},
+           "fp64_plain_requests_per_s": n_req / t64, "served_vs_fp64_plain": t64 / t32,
+           "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_batch": st["mean_batch"],
+           "serve_timing_ms": tm, "accuracy": acc, "cpu_baseline": cpu}
+    log("serve: %.1f requests/s (plain fp64 %.1f), p50 %.2f ms p99 %.2f ms, mean batch %.1f" %
         (n_req / t32, n_req / t64, st["p50_ms"], st["p99_ms"], st["mean_batch"]))
     print(json.dumps(out), flush=True)
 
