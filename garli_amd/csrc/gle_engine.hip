@@ -185,6 +185,18 @@ struct Instance {
       if (g.exec) (void)hipGraphExecDestroy(g.exec);
     g_cache.clear();
   }
+  // asynchronous serving ticks (engine_tick_enqueue / engine_tick_collect):
+  // pinned staging areas used in turn, each with the event its tick ends on
+  struct TickStage {
+    unsigned char *h = nullptr;  // pinned: P(t) list, op lists, flags and results
+    unsigned char *d = nullptr;  // the same layout on the device
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+    int n_edges = 0;
+    double *res = nullptr;
+    int *flags = nullptr;
+    bool busy = false;
+  } stage[kTickStages];
   std::unique_ptr<Instance> heavy;
   int P_orig = 0;
   std::vector<int> light_idx, heavy_idx;  // original pattern indices of each part
@@ -1777,7 +1789,7 @@ static int launch_cla(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
 }
 
 static int enqueue_partials(Instance *I, const DevClaOp *ops, int n_ops, const int *level_start, int n_levels,
-                            bool fused) {
+                            bool fused, bool allow_graph = true) {
   if (n_ops <= 0) return GLE_SUCCESS;
   fused = fused && engine_supports_fused(I);
   std::vector<DevClaOp> lv_ops;
@@ -1816,7 +1828,7 @@ static int enqueue_partials(Instance *I, const DevClaOp *ops, int n_ops, const i
   // holds kernel nodes only; the op list is copied in above, outside it.
   std::vector<int> levels;
   if (!fused && n_levels > 0) levels.assign(level_start, level_start + n_levels + 1);
-  const bool graphable = I->graphs && !I->timing && !fused && n_levels > 0;
+  const bool graphable = allow_graph && I->graphs && !I->timing && !fused && n_levels > 0;
   Instance::GraphEntry *entry = nullptr;
   if (graphable) {
     for (auto &g : I->g_cache)
@@ -2097,6 +2109,170 @@ static int edges_impl(Instance *I, const DevEdgeOp *ops, int n, double *out, dou
   return GLE_SUCCESS;
 }
 
+// ---------------------------------------------------------------- serving ticks
+// One tick of the serving layer on one part (the instance or its heavy
+// part), enqueued without any wait: P, P', P'' for the new lengths, the
+// rescale-flag reset, the CLA levels, the edges and their reduction, then the
+// results and flags copied into pinned memory, and an event.  The lists are
+// staged in pinned memory first, so every copy is asynchronous.
+static int tick_enqueue_part(Instance *X, int sidx, const int *pm_idx, const double *pm_len, int n_pm,
+                             int pm_derivs, const DevClaOp *ops, int n_ops, const int *level_start, int n_levels,
+                             const DevEdgeOp *eops, int n_e) {
+  Instance::TickStage &st = X->stage[sidx];
+  if (st.busy) return set_error(GLE_ERR_STATE, "tick stage %d still in flight", sidx);
+  for (int i = 0; i < n_pm; i++)
+    if (pm_idx[i] < 0 || pm_idx[i] >= X->n_pm || !(pm_len[i] >= 0.0))
+      return set_error(GLE_ERR_ARG, "tick pmat %d out of range", i);
+  for (int i = 0; i < n_ops; i++) {
+    const DevClaOp &o = ops[i];
+    auto bad = [&](int c) { return c >= X->n_slots || (c < 0 && -c - 1 >= X->N); };
+    if (o.dest < 0 || o.dest >= X->n_slots || bad(o.c1) || bad(o.c2) || o.pm1 < 0 || o.pm1 >= X->n_pm ||
+        o.pm2 < 0 || o.pm2 >= X->n_pm || o.dest == o.c1 || o.dest == o.c2)
+      return set_error(GLE_ERR_ARG, "tick cla op %d out of range", i);
+  }
+  for (int i = 0; i < n_e; i++) {
+    const DevEdgeOp &o = eops[i];
+    if (o.partial < 0 || o.partial >= X->n_slots || o.child >= X->n_slots || (o.child < 0 && -o.child - 1 >= X->N) ||
+        o.pm < 0 || o.pm >= X->n_pm || o.child == o.partial)
+      return set_error(GLE_ERR_ARG, "tick edge op %d out of range", i);
+  }
+  // one staging layout in pinned host memory and on the device:
+  // [P(t) slots | lengths | CLA ops | edge ops | flags | results]; the
+  // upload carries everything up to the (zeroed) flags, the download the
+  // flags and the results
+  auto a8 = [](size_t b) { return (b + 7) & ~size_t(7); };
+  const size_t o_len = a8(sizeof(int) * n_pm), o_ops = o_len + sizeof(double) * n_pm,
+               o_eops = o_ops + a8(sizeof(DevClaOp) * n_ops), o_flags = o_eops + a8(sizeof(DevEdgeOp) * n_e),
+               o_res = o_flags + 8, need = o_res + sizeof(double) * 3 * n_e;
+  if (need > st.cap) {
+    const size_t cap = std::max(need + need / 2, size_t(64) << 10);
+    if (st.h) GLE_HIP(hipHostFree(st.h));
+    if (st.d) GLE_HIP(hipFree(st.d));
+    st.h = st.d = nullptr;
+    st.cap = 0;
+    GLE_HIP(hipHostMalloc(reinterpret_cast<void **>(&st.h), cap));
+    GLE_HIP(hipMalloc(reinterpret_cast<void **>(&st.d), cap));
+    st.cap = cap;
+  }
+  if (!st.done) GLE_HIP(hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
+  std::copy(pm_idx, pm_idx + n_pm, reinterpret_cast<int *>(st.h));
+  std::copy(pm_len, pm_len + n_pm, reinterpret_cast<double *>(st.h + o_len));
+  std::copy(ops, ops + n_ops, reinterpret_cast<DevClaOp *>(st.h + o_ops));
+  std::copy(eops, eops + n_e, reinterpret_cast<DevEdgeOp *>(st.h + o_eops));
+  std::memset(st.h + o_flags, 0, 8);
+  st.res = reinterpret_cast<double *>(st.h + o_res);
+  st.flags = reinterpret_cast<int *>(st.h + o_flags);
+  GLE_HIP(hipMemcpyAsync(st.d, st.h, o_res, hipMemcpyHostToDevice, X->stream));
+  const DevClaOp *d_ops = reinterpret_cast<const DevClaOp *>(st.d + o_ops);
+  const DevEdgeOp *d_eops = reinterpret_cast<const DevEdgeOp *>(st.d + o_eops);
+  int *d_flags = reinterpret_cast<int *>(st.d + o_flags);
+  if (n_pm > 0) {
+    const int S = X->S;
+    const size_t shmem = sizeof(double) * (2 * S * S + 3 * S);
+    timed_begin(X, 2, double(n_pm) * X->R * S * S * 8.0 * (pm_derivs ? 3 : 1));
+    GLE_LAUNCH(X, k_pmat, dim3(n_pm, X->R), dim3(256), shmem, S, X->Sp, X->R,
+               reinterpret_cast<const int *>(st.d), reinterpret_cast<const double *>(st.d + o_len),
+               X->d_eig_of_rate, X->d_model + S + X->R, X->d_V, X->d_Vinv, X->d_lam, X->d_pm64, X->d_pmpad,
+               X->d_pdiag, pm_derivs, S == 20 ? kPadTransposed : S == 61 ? kPadCodonFrag : kPadRowMajor);
+    timed_end(X);
+  }
+  // the CLA kernels raise their rescale flags in this tick's own word
+  int *flags_save = X->d_flags;
+  X->d_flags = d_flags;
+  int rc = GLE_SUCCESS;
+  for (int l = 0; l < n_levels && !rc; l++) {
+    const int b = level_start[l], e = level_start[l + 1];
+    if (e <= b) continue;
+    double bytes = 0.0;
+    for (int i = b; i < e; i++) bytes += cla_op_bytes(X, ops[i]);
+    timed_begin(X, 0, bytes);
+    rc = X->prec == GLE_PREC_FP64 ? launch_cla<double>(X, d_ops + b, e - b, false)
+                                  : launch_cla<float>(X, d_ops + b, e - b, false);
+    timed_end(X);
+  }
+  X->d_flags = flags_save;
+  if (rc) return rc;
+  if (n_e > 0) {
+    const int nblk = edge_blocks(X);
+    const size_t part = size_t(n_e) * nblk * 3;
+    if (part > X->part_cap) {
+      if (X->d_part) GLE_HIP(hipFree(X->d_part));
+      X->d_part = nullptr;
+      X->part_cap = 0;
+      GLE_HIP(hipMalloc(&X->d_part, sizeof(double) * part));
+      X->part_cap = part;
+    }
+    bool derivs = false;
+    double eb = 0.0;
+    for (int i = 0; i < n_e; i++) {
+      derivs = derivs || eops[i].derivs;
+      eb += edge_op_bytes(X, eops[i]);
+    }
+    double *site_save = X->d_site;
+    X->d_site = nullptr;
+    timed_begin(X, 1, eb);
+    rc = X->prec == GLE_PREC_FP64 ? launch_edges<double>(X, d_eops, n_e, nblk, derivs, X->d_part)
+                                  : launch_edges<float>(X, d_eops, n_e, nblk, derivs, X->d_part);
+    timed_end(X);
+    X->d_site = site_save;
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_edge_reduce, dim3(n_e), dim3(64), 0, X->stream, X->d_part, nblk,
+                       reinterpret_cast<double *>(st.d + o_res));
+  }
+  GLE_HIP(hipGetLastError());
+  GLE_HIP(hipMemcpyAsync(st.h + o_flags, st.d + o_flags, 8 + sizeof(double) * 3 * n_e, hipMemcpyDeviceToHost,
+                         X->stream));
+  GLE_HIP(hipEventRecord(st.done, X->stream));
+  st.n_edges = n_e;
+  st.busy = true;
+  return GLE_SUCCESS;
+}
+
+int engine_tick_enqueue(Instance *I, int sidx, const int *pm_idx, const double *pm_len, int n_pm, int pm_derivs,
+                        const DevClaOp *ops, int n_ops, const int *level_start, int n_levels,
+                        const DevEdgeOp *eops, int n_e) {
+  std::lock_guard<std::mutex> lk_(I->mu);
+  if (sidx < 0 || sidx >= kTickStages) return set_error(GLE_ERR_ARG, "tick stage %d", sidx);
+  int rc = tick_enqueue_part(I, sidx, pm_idx, pm_len, n_pm, pm_derivs, ops, n_ops, level_start, n_levels, eops, n_e);
+  if (!rc && I->heavy)
+    rc = tick_enqueue_part(I->heavy.get(), sidx, pm_idx, pm_len, n_pm, pm_derivs, ops, n_ops, level_start, n_levels,
+                           eops, n_e);
+  if (rc) {  // drain what was queued so the stages can be reused
+    (void)hipStreamSynchronize(I->stream);
+    if (I->heavy) (void)hipStreamSynchronize(I->heavy->stream);
+    I->stage[sidx].busy = false;
+    if (I->heavy) I->heavy->stage[sidx].busy = false;
+  }
+  return rc;
+}
+
+// Waits for the tick in stage sidx and returns its n x (lnL, d1, d2) (both
+// parts summed) and the OR of the parts' rescale flags.
+int engine_tick_collect(Instance *I, int sidx, double *res, int *flags) {
+  std::lock_guard<std::mutex> lk_(I->mu);
+  int rc = GLE_SUCCESS, f = 0;
+  for (Instance *X : {I, I->heavy.get()}) {
+    if (!X) continue;
+    Instance::TickStage &st = X->stage[sidx];
+    if (!st.busy) continue;
+    const hipError_t e = hipEventSynchronize(st.done);
+    st.busy = false;
+    if (e != hipSuccess) {
+      rc = set_error(GLE_ERR_DEVICE, "serving tick failed: %s", hipGetErrorString(e));
+      continue;
+    }
+    timed_collect(X);
+    for (int i = 0; i < 3 * st.n_edges; i++) res[i] = X == I ? st.res[i] : res[i] + st.res[i];
+    f |= st.flags[0];
+  }
+  if (flags) *flags = f;
+  if (!rc) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = set_error(GLE_ERR_DEVICE, "serving tick: %s", hipGetErrorString(e));
+  }
+  return rc;
+}
+
 int engine_edges(Instance *I, const DevEdgeOp *ops, int n, double *out, double *site_lnl) {
   return edges_impl(I, ops, n, out, site_lnl, 1);
 }
@@ -2197,6 +2373,11 @@ static int destroy(Instance *I) {
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (I->h_stage) (void)hipHostFree(I->h_stage);
+  for (auto &st : I->stage) {
+    if (st.h) (void)hipHostFree(st.h);
+    if (st.d) (void)hipFree(st.d);
+    if (st.done) (void)hipEventDestroy(st.done);
+  }
   for (auto &p : I->pending) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);

@@ -67,6 +67,17 @@ int engine_edges(Instance *, const DevEdgeOp *ops, int n, double *out, double *s
 // partial-side and child-side rescale counts, -1 for a tip child)
 int engine_edge_sites(Instance *, const DevEdgeOp &op, double *out3, double *site5);
 int engine_flags(Instance *, int *flags, int reset);
+// Staging areas per instance for serving ticks in flight at once.
+constexpr int kTickStages = 3;
+// One serving tick on an instance (and its heavy part), queued with no wait
+// into staging area sidx (0 .. kTickStages - 1, one per tick in flight): P (and
+// P', P'' when pm_derivs) for the listed lengths, the rescale-flag reset, the CLA ops by level, the
+// edges.  engine_tick_collect waits for it and returns n_e x (lnL, d1, d2)
+// and the rescale flags.
+int engine_tick_enqueue(Instance *, int sidx, const int *pm_idx, const double *pm_len, int n_pm, int pm_derivs,
+                        const DevClaOp *ops, int n_ops, const int *level_start, int n_levels, const DevEdgeOp *eops,
+                        int n_e);
+int engine_tick_collect(Instance *, int sidx, double *res, int *flags);
 int engine_get_partials(Instance *, int slot, double *out, int *u);
 int engine_sync(Instance *);
 bool engine_supports_fused(const Instance *);

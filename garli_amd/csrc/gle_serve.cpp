@@ -40,6 +40,7 @@
 #include <cstring>
 #include <deque>
 #include <functional>
+#include <list>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -70,11 +71,6 @@ struct SubKey {
   uint64_t x, lx, y, ly;
   bool operator==(const SubKey &o) const { return x == o.x && lx == o.lx && y == o.y && ly == o.ly; }
 };
-struct SubKeyHash {
-  size_t operator()(const SubKey &k) const {
-    return size_t(mix64(k.x ^ mix64(k.lx ^ mix64(k.y ^ mix64(k.ly)))));
-  }
-};
 static SubKey sub_key(uint64_t ix, double lx, uint64_t iy, double ly) {
   SubKey k{ix, dbits(lx), iy, dbits(ly)};
   if (k.x > k.y || (k.x == k.y && k.lx > k.ly)) {
@@ -87,40 +83,130 @@ static SubKey sub_key(uint64_t ix, double lx, uint64_t iy, double ly) {
 constexpr uint64_t kFirstSubtreeId = uint64_t(1) << 40;
 // Interned names are kept while their CLAs come and go; past this many the
 // table and the CLA cache are dropped together (a rare full refill).
-constexpr size_t kInternCap = size_t(1) << 22;
+constexpr size_t kInternCap = size_t(1) << 20;
+
+// Interned subtree names: an open-addressing table (linear probing,
+// power-of-two capacity, at most half full); id 0 marks an empty cell.
+class NameTable {
+  struct Cell {
+    SubKey k;
+    uint64_t id;
+  };
+  std::vector<Cell> t_;
+  size_t n_ = 0, mask_ = 0;
+  static size_t hash(const SubKey &k) {
+    return size_t(mix64(k.x * 0x9E3779B97F4A7C15ull ^ k.lx) ^ mix64(k.y * 0xC2B2AE3D27D4EB4Full + k.ly));
+  }
+  void grow() {
+    std::vector<Cell> old;
+    old.swap(t_);
+    t_.assign(std::max<size_t>(1024, 2 * old.size()), Cell{{0, 0, 0, 0}, 0});
+    mask_ = t_.size() - 1;
+    for (const Cell &c : old)
+      if (c.id)
+        for (size_t i = hash(c.k) & mask_;; i = (i + 1) & mask_)
+          if (!t_[i].id) {
+            t_[i] = c;
+            break;
+          }
+  }
+
+ public:
+  size_t size() const { return n_; }
+  void clear() {
+    std::vector<Cell>().swap(t_);
+    n_ = mask_ = 0;
+  }
+  uint64_t find(const SubKey &k) const {
+    if (t_.empty()) return 0;
+    for (size_t i = hash(k) & mask_;; i = (i + 1) & mask_) {
+      if (!t_[i].id) return 0;
+      if (t_[i].k == k) return t_[i].id;
+    }
+  }
+  uint64_t intern(const SubKey &k, uint64_t &next) {
+    if (2 * (n_ + 1) > t_.size()) grow();
+    for (size_t i = hash(k) & mask_;; i = (i + 1) & mask_) {
+      if (!t_[i].id) {
+        t_[i] = Cell{k, next};
+        n_++;
+        return next++;
+      }
+      if (t_[i].k == k) return t_[i].id;
+    }
+  }
+};
 
 struct CacheEntry {
   int slot = -1, rank = 0;
   uint64_t last = 0;
+  std::list<uint64_t>::iterator pos;  // place in the cache's use order
+};
+
+// A device-slot cache in least-recently-used order: a hit moves its entry to
+// the back, so eviction takes the front (O(1) instead of a scan).
+struct LruCache {
+  std::unordered_map<uint64_t, CacheEntry> map;
+  std::list<uint64_t> order;
+  CacheEntry *find(uint64_t key) {
+    auto it = map.find(key);
+    return it == map.end() ? nullptr : &it->second;
+  }
+  bool has(uint64_t key) const { return map.count(key) != 0; }
+  void touch(CacheEntry *e, uint64_t tick) {
+    e->last = tick;
+    order.splice(order.end(), order, e->pos);
+  }
+  void put(uint64_t key, int slot, int rank, uint64_t tick) {
+    auto it = map.find(key);
+    if (it != map.end()) order.erase(it->second.pos);
+    order.push_back(key);
+    CacheEntry &e = map[key];
+    e.slot = slot;
+    e.rank = rank;
+    e.last = tick;
+    e.pos = std::prev(order.end());
+  }
+  // slot of the least recently used entry not used in tick `tick`, removed; -1 if none
+  int evict(uint64_t tick) {
+    if (order.empty()) return -1;
+    auto it = map.find(order.front());
+    if (it->second.last >= tick) return -1;
+    const int s = it->second.slot;
+    order.pop_front();
+    map.erase(it);
+    return s;
+  }
+  template <typename F>
+  void clear(F &&release) {
+    for (auto &kv : map) release(kv.second.slot);
+    map.clear();
+    order.clear();
+  }
 };
 
 struct ServeInstance {
   int h = -1;
   Instance *I = nullptr;
   int rescale_every = 16;
-  std::unordered_map<SubKey, uint64_t, SubKeyHash> intern;  // subtree name -> id
+  NameTable intern;  // subtree name -> id
   uint64_t next_id = kFirstSubtreeId;
-  std::unordered_map<uint64_t, CacheEntry> cla;  // subtree id -> CLA slot
-  std::unordered_map<uint64_t, CacheEntry> pm;   // blen bits -> P, P', P'' slot
+  LruCache cla;  // subtree id -> CLA slot
+  LruCache pm;   // blen bits -> P, P', P'' slot
   int reserved = 0;                              // working-set slots of admitted requests
   unsigned long long gen = 0;                    // instance generation the caches belong to
+  unsigned long long epoch = 0;                  // bumped whenever the CLA cache is dropped
 
-  uint64_t intern_id(const SubKey &k) {
-    auto it = intern.find(k);
-    if (it != intern.end()) return it->second;
-    const uint64_t id = next_id++;
-    intern.emplace(k, id);
-    return id;
-  }
-  uint64_t find_id(const SubKey &k) const {
-    auto it = intern.find(k);
-    return it == intern.end() ? 0 : it->second;
-  }
+  uint64_t intern_id(const SubKey &k) { return intern.intern(k, next_id); }
+  uint64_t find_id(const SubKey &k) const { return intern.find(k); }
   // every cached CLA back to the pool (caller holds pool_mutex)
   void drop_clas(ClaPool *pool) {
-    for (auto &c : cla)
-      if (--pool->refs[c.second.slot] == 0) pool->free_slot(c.second.slot);
-    cla.clear();
+    cla.clear([pool](int s) {
+      if (--pool->refs[s] == 0) pool->free_slot(s);
+    });
+  }
+  void drop_pmats(ClaPool *pool) {
+    pm.clear([pool](int s) { pool->pm_free.push_back(s); });
   }
 };
 
@@ -143,6 +229,7 @@ struct Req {
   double lnl = 0.0, improve = 0.0;
   int rc = GLE_SUCCESS, need = 0;
   int waits = 0;          // ticks in a row this request was deferred (goes first next time)
+  bool in_flight = false; // its step is queued on the device (worker only)
   bool done = false;      // set by the worker while it owns the request
   bool finished = false;  // published under Server::mu once the worker let go
   std::chrono::steady_clock::time_point t_submit, t_done;
@@ -246,6 +333,8 @@ struct TickBuilder {
   std::vector<double> pm_len;
   std::unordered_map<uint64_t, int> pm_fresh;
   int err = GLE_SUCCESS;
+  std::vector<char> pm_new;  // per pm_idx entry: 1 a new slot, 0 a cached one refreshed with P', P''
+  bool derivs = false;       // the tick computes P' and P'' as well
 
   struct Mark {
     size_t fresh, ops, pm;
@@ -264,55 +353,58 @@ struct TickBuilder {
     ops.resize(m.ops);
     for (size_t i = m.pm; i < pm_idx.size(); i++) {
       pm_fresh.erase(dbits(pm_len[i]));
-      pool->pm_free.push_back(pm_idx[i]);
+      if (pm_new[i]) pool->pm_free.push_back(pm_idx[i]);
     }
     pm_idx.resize(m.pm);
     pm_len.resize(m.pm);
+    pm_new.resize(m.pm);
     err = GLE_SUCCESS;
   }
   // every new CLA and P(t) slot of the tick back to the pool (a failed tick)
   void release_all() { rollback(Mark{0, 0, 0}); }
 
   int take_slot() {
-    if (!pool->has_free()) {  // evict the least recently used entry not used this tick
-      auto victim = si->cla.end();
-      for (auto it = si->cla.begin(); it != si->cla.end(); ++it)
-        if (it->second.last < tick && (victim == si->cla.end() || it->second.last < victim->second.last))
-          victim = it;
-      if (victim == si->cla.end()) return -1;
-      const int s = victim->second.slot;
+    while (!pool->has_free()) {  // evict least recently used entries not used this tick
+      const int s = si->cla.evict(tick);
+      if (s < 0) return -1;
       if (--pool->refs[s] == 0) pool->free_slot(s);
-      si->cla.erase(victim);
-      if (!pool->has_free()) return -1;
     }
     const int s = pool->take_slot();
     pool->refs[s] = 1;
     return s;
   }
 
-  int pm_slot(double len) {
+  // P(t) slot of a branch length; need_derivs when an edge's Newton step
+  // reads P' and P'' from it.  The tick computes the derivative matrices
+  // only when one of its steps needs them (P(t) cache entries remember
+  // whether they hold them, in CacheEntry::rank).
+  int pm_slot(double len, bool need_derivs = false) {
     const uint64_t key = dbits(len);
-    auto it = si->pm.find(key);
-    if (it != si->pm.end()) {
-      it->second.last = tick;
-      return it->second.slot;
+    if (need_derivs) derivs = true;
+    if (CacheEntry *e = si->pm.find(key)) {
+      si->pm.touch(e, tick);
+      if (!need_derivs || e->rank) return e->slot;
+      if (!pm_fresh.count(key)) {  // cached without P', P'': recompute it in place
+        pm_fresh[key] = e->slot;
+        pm_idx.push_back(e->slot);
+        pm_len.push_back(len);
+        pm_new.push_back(0);
+      }
+      return e->slot;
     }
     auto f = pm_fresh.find(key);
     if (f != pm_fresh.end()) return f->second;
     if (pool->pm_free.empty()) {
-      auto victim = si->pm.end();
-      for (auto jt = si->pm.begin(); jt != si->pm.end(); ++jt)
-        if (jt->second.last < tick && (victim == si->pm.end() || jt->second.last < victim->second.last))
-          victim = jt;
-      if (victim == si->pm.end()) return -1;
-      pool->pm_free.push_back(victim->second.slot);
-      si->pm.erase(victim);
+      const int v = si->pm.evict(tick);
+      if (v < 0) return -1;
+      pool->pm_free.push_back(v);
     }
     const int s = pool->pm_free.back();
     pool->pm_free.pop_back();
     pm_fresh[key] = s;
     pm_idx.push_back(s);
     pm_len.push_back(len);
+    pm_new.push_back(1);
     return s;
   }
 
@@ -326,12 +418,11 @@ struct TickBuilder {
     }
     const uint64_t known = known_id(r, *si, a, b);
     if (known) {
-      auto it = si->cla.find(known);
-      if (it != si->cla.end()) {
-        it->second.last = tick;
-        rank = it->second.rank;
+      if (CacheEntry *e = si->cla.find(known)) {
+        si->cla.touch(e, tick);
+        rank = e->rank;
         level = -1;
-        return it->second.slot;
+        return e->slot;
       }
       auto f = fresh.find(known);
       if (f != fresh.end()) {
@@ -381,19 +472,23 @@ struct TickBuilder {
 // offspring whose parent's CLAs are cached.  Ties keep the reference's edge
 // (root, middle child).  The choice depends on what is cached, so a result
 // alone and in a mix agree to the form's rounding, not bit for bit.
+// CLAs missing for X(a|b), memoised in memo
+static int missing_clas(Req &r, const TickBuilder &tb, std::vector<std::array<int, 3>> &memo, int a, int b) {
+  if (req_tip(r, a)) return 0;
+  int &m = memo[a][req_nbk(r, a, b)];
+  if (m >= 0) return m;
+  const uint64_t id = known_id(r, *tb.si, a, b);
+  if (id && (tb.si->cla.has(id) || tb.fresh.count(id))) return m = 0;
+  int c = 1;
+  for (int j = 0; j < r.nnb[a]; j++)
+    if (r.nb[a][j] != b) c += missing_clas(r, tb, memo, r.nb[a][j], a);
+  return m = c;
+}
+
 static void choose_score_edge(Req &r, const TickBuilder &tb) {
-  std::vector<std::array<int, 3>> memo(r.nn, {-1, -1, -1});
-  std::function<int(int, int)> cost = [&](int a, int b) -> int {  // CLAs missing for X(a|b)
-    if (req_tip(r, a)) return 0;
-    int &m = memo[a][req_nbk(r, a, b)];
-    if (m >= 0) return m;
-    const uint64_t id = known_id(r, *tb.si, a, b);
-    if (id && (tb.si->cla.count(id) || tb.fresh.count(id))) return m = 0;
-    int c = 1;
-    for (int j = 0; j < r.nnb[a]; j++)
-      if (r.nb[a][j] != b) c += cost(r.nb[a][j], a);
-    return m = c;
-  };
+  static thread_local std::vector<std::array<int, 3>> memo;
+  memo.assign(r.nn, {-1, -1, -1});
+  auto cost = [&](int a, int b) { return missing_clas(r, tb, memo, a, b); };
   r.edge_a = 0;
   r.edge_b = r.nb[0][1];
   int best = cost(r.edge_a, r.edge_b) + cost(r.edge_b, r.edge_a);
@@ -438,132 +533,154 @@ static void advance(Req &r, const double *res) {
   }
 }
 
-// Runs one batched step of the requests in `batch` (all on instance si).  A
-// request whose new CLAs or P(t) do not fit beside those the tick already
-// holds waits for the next tick (it stays active and unchanged); one that
-// does not fit on its own fails.  On a device error every request of the tick
-// fails and every slot the tick took goes back to the pool.  Returns how many
-// requests were deferred.
-static int run_tick(Server &sv, ServeInstance &si, std::vector<Req *> &batch, double &host_ms, double &device_ms) {
+// One instance's share of a tick: the requests it carries, the CLAs it
+// computes (named, so a failed tick can take them back out of the cache) and
+// the cache epoch it was built on.
+struct TickPart {
+  ServeInstance *si = nullptr;
+  std::vector<Req *> reqs;
+  std::vector<uint64_t> fresh_ids;
+  unsigned long long epoch = 0;
+  int n_edges = 0;
+};
+// A tick queued on the device: up to two are in flight, so the host builds
+// the next while the device runs the current one.  Every tick's work is on
+// the instances' own streams, so a later tick that reads CLAs an earlier one
+// writes, or reuses a slot an earlier one read, runs after it.
+struct Flight {
+  uint64_t tick = 0;
+  int stage = 0;
+  std::vector<TickPart> parts;
+  std::chrono::steady_clock::time_point t_launch;
+};
+
+// Builds one instance's part of a tick from `batch` and queues it on the
+// device (no wait).  A request whose new CLAs or P(t) do not fit beside those
+// the part already holds waits for a later tick, unchanged and holding
+// nothing; one that does not fit on its own fails.  The part's new CLAs and
+// P(t) enter the caches at once, so the next tick can use them.
+static int build_part(Server &sv, ServeInstance &si, std::vector<Req *> &batch, uint64_t tick, int stage,
+                      TickPart &part, double &host_ms) {
   using clock = std::chrono::steady_clock;
-  auto ms_since = [](clock::time_point a) { return std::chrono::duration<double, std::milli>(clock::now() - a).count(); };
-  auto fail_all = [](std::vector<Req *> &v, int rc) {
-    for (Req *r : v) {
-      r->rc = rc;
-      r->done = true;
-    }
-  };
-  for (;;) {
-    const auto t_build = clock::now();
-    auto t_launch = t_build;
-    std::vector<DevClaOp> ops;
-    std::vector<int> level_start;
-    std::vector<DevEdgeOp> eops;
-    std::vector<Req *> in_tick;
-    int deferred = 0;
-    std::unique_lock<std::mutex> pk(pool_mutex());
-    ClaPool *pool = pool_for(si.I);
-    if (si.gen != instance_generation(si.I) || si.intern.size() > kInternCap) {
-      // data or model changed (or the name table is full): drop both caches
-      si.drop_clas(pool);
-      for (auto &c : si.pm) pool->pm_free.push_back(c.second.slot);
-      si.pm.clear();
-      si.intern.clear();
-      si.gen = instance_generation(si.I);
-    }
-    TickBuilder tb{&si, pool, sv.tick, {}, {}, {}, {}, {}, {}, GLE_SUCCESS};
-    for (Req *r : batch) {
-      const TickBuilder::Mark m = tb.mark();
-      r->sid.assign(r->nn, {kUnset, kUnset, kUnset});
-      next_edge(*r, tb);
-      int ra, la, rb, lb;
-      DevEdgeOp e;
-      e.partial = tb.resolve(*r, r->edge_a, r->edge_b, ra, la);
-      e.child = tb.err ? 0 : tb.resolve(*r, r->edge_b, r->edge_a, rb, lb);
-      e.pm = tb.err ? -1 : tb.pm_slot(req_len(*r, r->edge_a, r->edge_b));
-      e.derivs = r->derivs ? 1 : 0;
-      if (!tb.err && e.pm < 0) tb.err = set_error(GLE_ERR_NOMEM, "serving cache exhausted: out of pmat slots");
-      if (tb.err) {
-        const int err = tb.err;
-        tb.rollback(m);
-        if (in_tick.empty()) {  // does not fit even alone
-          r->rc = err;
-          r->done = true;
-        } else {
-          deferred++;  // next tick, with the slots this tick holds free again
-          r->waits++;
-        }
-        continue;
-      }
-      eops.push_back(e);
-      in_tick.push_back(r);
-      r->waits = 0;
-    }
-    if (in_tick.empty()) return deferred;
-    std::stable_sort(tb.ops.begin(), tb.ops.end(),
-                     [](const std::pair<int, DevClaOp> &x, const std::pair<int, DevClaOp> &y) {
-                       return x.first < y.first;
-                     });
-    int cur = -1;
-    for (auto &lo : tb.ops) {
-      if (lo.first != cur) {
-        level_start.push_back(int(ops.size()));
-        cur = lo.first;
-      }
-      ops.push_back(lo.second);
-    }
-    level_start.push_back(int(ops.size()));
-    host_ms += ms_since(t_build);
-    t_launch = clock::now();
-    // P, P', P'' for new branch lengths (one launch)
-    int rc = engine_update_pmats(si.I, tb.pm_idx.data(), tb.pm_len.data(), int(tb.pm_idx.size()), 1);
-    if (rc) {
-      tb.release_all();
-      fail_all(in_tick, rc);
-      return deferred;
-    }
-    for (size_t i = 0; i < tb.pm_idx.size(); i++) si.pm[dbits(tb.pm_len[i])] = {tb.pm_idx[i], 0, sv.tick};
-    tb.pm_idx.clear();  // now owned by the P(t) cache
-    tb.pm_len.clear();
-    tb.pm_fresh.clear();
-    pk.unlock();
-    int flags = 0;
-    rc = engine_flags(si.I, nullptr, 1);
-    if (!rc)
-      rc = engine_update_partials(si.I, ops.data(), int(ops.size()), level_start.data(),
-                                  int(level_start.size()) - 1, false);
-    std::vector<double> res(3 * eops.size());
-    if (!rc) rc = engine_edges(si.I, eops.data(), int(eops.size()), res.data(), nullptr);
-    if (!rc) rc = engine_flags(si.I, &flags, 1);
-    device_ms += ms_since(t_launch);
-    pk.lock();
-    if (rc) {
-      tb.release_all();
-      fail_all(in_tick, rc);
-      return deferred;
-    }
-    const bool thrown = instance_precision(si.I) == GLE_PREC_FP64 &&
-                        (((flags & GLE_FLAG_REDUCE_RESCALE) && si.rescale_every > 2) ||
-                         ((flags & GLE_FLAG_BAIL_OUT) && si.rescale_every <= 2));
-    if (thrown) {
-      // Tree::Score's retry: drop every cached CLA (they were built with the old
-      // rescaleEvery) and run the step again with rescaleEvery - 2.
-      tb.release_all();
-      si.drop_clas(pool);
-      si.rescale_every -= 2;
-      if (si.rescale_every < 2) {
-        fail_all(in_tick, set_error(GLE_ERR_RESCALE_FATAL, "Problem with rescaling in serving."));
-        return deferred;
+  const auto t_build = clock::now();
+  std::vector<DevClaOp> ops;
+  std::vector<int> level_start;
+  std::vector<DevEdgeOp> eops;
+  int deferred = 0;
+  std::lock_guard<std::mutex> pk(pool_mutex());
+  ClaPool *pool = pool_for(si.I);
+  if (si.gen != instance_generation(si.I) || si.intern.size() > kInternCap) {
+    // data or model changed (or the name table is full): drop both caches
+    si.drop_clas(pool);
+    si.drop_pmats(pool);
+    si.intern.clear();
+    si.gen = instance_generation(si.I);
+    si.epoch++;
+  }
+  TickBuilder tb{&si, pool, tick, {}, {}, {}, {}, {}, {}, GLE_SUCCESS};
+  part.si = &si;
+  part.epoch = si.epoch;
+  for (Req *r : batch) {
+    const TickBuilder::Mark m = tb.mark();
+    r->sid.assign(r->nn, {kUnset, kUnset, kUnset});
+    next_edge(*r, tb);
+    int ra, la, rb, lb;
+    DevEdgeOp e;
+    e.partial = tb.resolve(*r, r->edge_a, r->edge_b, ra, la);
+    e.child = tb.err ? 0 : tb.resolve(*r, r->edge_b, r->edge_a, rb, lb);
+    e.pm = tb.err ? -1 : tb.pm_slot(req_len(*r, r->edge_a, r->edge_b), r->derivs);
+    e.derivs = r->derivs ? 1 : 0;
+    if (!tb.err && e.pm < 0) tb.err = set_error(GLE_ERR_NOMEM, "serving cache exhausted: out of pmat slots");
+    if (tb.err) {
+      const int err = tb.err;
+      tb.rollback(m);
+      if (part.reqs.empty()) {  // does not fit even alone
+        r->rc = err;
+        r->done = true;
+      } else {
+        deferred++;
+        r->waits++;
       }
       continue;
     }
-    for (uint64_t id : tb.fresh_order) {
-      const TickNew &f = tb.fresh.at(id);
-      si.cla[id] = {f.slot, f.rank, sv.tick};
-    }
-    for (size_t i = 0; i < in_tick.size(); i++) advance(*in_tick[i], &res[3 * i]);
+    eops.push_back(e);
+    part.reqs.push_back(r);
+    r->waits = 0;
+  }
+  if (part.reqs.empty()) {
+    host_ms += std::chrono::duration<double, std::milli>(clock::now() - t_build).count();
     return deferred;
   }
+  std::stable_sort(tb.ops.begin(), tb.ops.end(),
+                   [](const std::pair<int, DevClaOp> &x, const std::pair<int, DevClaOp> &y) { return x.first < y.first; });
+  int cur = -1;
+  for (auto &lo : tb.ops) {
+    if (lo.first != cur) {
+      level_start.push_back(int(ops.size()));
+      cur = lo.first;
+    }
+    ops.push_back(lo.second);
+  }
+  level_start.push_back(int(ops.size()));
+  const int rc = engine_tick_enqueue(si.I, stage, tb.pm_idx.data(), tb.pm_len.data(), int(tb.pm_idx.size()),
+                                     tb.derivs ? 1 : 0, ops.data(), int(ops.size()), level_start.data(),
+                                     int(level_start.size()) - 1, eops.data(), int(eops.size()));
+  if (rc) {
+    tb.release_all();
+    for (Req *r : part.reqs) {
+      r->rc = rc;
+      r->done = true;
+    }
+    part.reqs.clear();
+  } else {
+    for (size_t i = 0; i < tb.pm_idx.size(); i++) si.pm.put(dbits(tb.pm_len[i]), tb.pm_idx[i], tb.derivs, tick);
+    for (uint64_t id : tb.fresh_order) {
+      const TickNew &f = tb.fresh.at(id);
+      si.cla.put(id, f.slot, f.rank, tick);
+      part.fresh_ids.push_back(id);
+    }
+    part.n_edges = int(eops.size());
+    for (Req *r : part.reqs) r->in_flight = true;
+  }
+  host_ms += std::chrono::duration<double, std::milli>(clock::now() - t_build).count();
+  return deferred;
+}
+
+// Waits for one part and applies its results.  A part built on a cache the
+// instance has since dropped is discarded (its requests run their step
+// again).  A device error fails the part's requests and drops the caches;
+// the reference's rescale throw (tree.cpp:3436-3449, FP64) drops the caches
+// and lowers rescaleEvery by 2 (Tree::Score's retry, tree.cpp:4077-4090) and
+// the requests run their step again.
+static void collect_part(Flight &f, TickPart &part) {
+  ServeInstance &si = *part.si;
+  std::vector<double> res(3 * size_t(std::max(part.n_edges, 1)));
+  int flags = 0;
+  const int rc = engine_tick_collect(si.I, f.stage, res.data(), &flags);
+  for (Req *r : part.reqs) r->in_flight = false;
+  const bool stale = part.epoch != si.epoch;
+  const bool thrown = !rc && instance_precision(si.I) == GLE_PREC_FP64 &&
+                      (((flags & GLE_FLAG_REDUCE_RESCALE) && si.rescale_every > 2) ||
+                       ((flags & GLE_FLAG_BAIL_OUT) && si.rescale_every <= 2));
+  if (rc || thrown) {
+    std::lock_guard<std::mutex> pk(pool_mutex());
+    if (!stale) {
+      si.drop_clas(pool_for(si.I));
+      si.epoch++;
+    }
+    if (thrown && !stale) si.rescale_every -= 2;
+    const bool fatal = thrown && si.rescale_every < 2;
+    if (rc || fatal) {
+      const int e = rc ? rc : set_error(GLE_ERR_RESCALE_FATAL, "Problem with rescaling in serving.");
+      for (Req *r : part.reqs) {
+        r->rc = e;
+        r->done = true;
+      }
+    }
+    return;
+  }
+  if (stale) return;
+  for (size_t i = 0; i < part.reqs.size(); i++) advance(*part.reqs[i], &res[3 * i]);
 }
 
 // CLA slots a request can hold in one tick: a step evaluates one edge and
@@ -572,15 +689,46 @@ static int run_tick(Server &sv, ServeInstance &si, std::vector<Req *> &batch, do
 // CLAs survive between Newton steps.
 static int working_set(const Req &r) { return r.kind == kReqScore ? r.N - 2 : 2 * (r.N - 2) + 4; }
 
+// Admission by HBM budget: per instance in arrival order, while the working
+// sets fit the instance's CLA slots (the first request of an idle instance
+// always goes in).  A request that waits holds back later ones of its own
+// instance only.  Caller holds sv.mu.
+static void admit(Server &sv) {
+  std::vector<int> blocked;
+  for (auto it = sv.pending.begin(); it != sv.pending.end();) {
+    Req *r = *it;
+    if (std::find(blocked.begin(), blocked.end(), r->inst) != blocked.end()) {
+      ++it;
+      continue;
+    }
+    ServeInstance &si = sv.insts[r->inst];
+    if (si.reserved + r->need > instance_cla_slots(si.I) && si.reserved > 0) {
+      blocked.push_back(r->inst);
+      ++it;
+      continue;
+    }
+    si.reserved += r->need;
+    sv.active.push_back(r);
+    it = sv.pending.erase(it);
+  }
+}
+
 static void worker_loop(Server *sv) {
-  std::vector<std::pair<ServeInstance *, std::vector<Req *>>> by_inst;
+  std::deque<Flight> flights;
+  int next_stage = 0;
   for (;;) {
-    std::vector<Req *> work;
+    std::vector<std::pair<ServeInstance *, std::vector<Req *>>> by_inst;
+    bool stop = false;
     {
       std::unique_lock<std::mutex> lk(sv->mu);
-      sv->cv_work.wait(lk, [&] { return sv->stop || !sv->pending.empty() || !sv->active.empty(); });
-      if (sv->stop) return;
-      if (sv->active.empty() && sv->batch_window_us > 0) {
+      auto has_ready = [&] {
+        for (Req *r : sv->active)
+          if (!r->in_flight && !r->done) return true;
+        return false;
+      };
+      sv->cv_work.wait(lk, [&] { return sv->stop || !sv->pending.empty() || !flights.empty() || has_ready(); });
+      stop = sv->stop;
+      if (!stop && sv->active.empty() && flights.empty() && sv->batch_window_us > 0) {
         // idle until now: let a burst of submissions gather into the first
         // tick, until no request has arrived for a fifth of the window (at
         // least 20 us) or the window has passed
@@ -591,61 +739,92 @@ static void worker_loop(Server *sv) {
         for (;;) {
           const auto until = std::min(deadline, clock::now() + gap);
           sv->cv_work.wait_until(lk, until, [&] { return sv->stop || sv->pending.size() != seen; });
-          if (sv->stop) return;
+          if (sv->stop) break;
           if (sv->pending.size() == seen || clock::now() >= deadline) break;
           seen = sv->pending.size();
         }
+        stop = sv->stop;
       }
-      // admission by HBM budget: FIFO while the working sets fit the instance's CLA slots
-      while (!sv->pending.empty()) {
-        Req *r = sv->pending.front();
-        ServeInstance &si = sv->insts[r->inst];
-        const int cap = instance_cla_slots(si.I);
-        if (si.reserved + r->need > cap && !sv->active.empty()) break;
-        si.reserved += r->need;
-        sv->pending.pop_front();
-        sv->active.push_back(r);
+      if (!stop) {
+        admit(*sv);
+        if (flights.size() < 2) {
+          // One tick runs while the next is built and queued.  A large burst
+          // (a generation's offspring) reaching an idle device goes out in two
+          // halves, so the second half is built while the first runs.
+          int ready = 0;
+          for (Req *r : sv->active) ready += !(r->in_flight || r->done);
+          const int chunk = (flights.empty() && ready >= 16) ? (ready + 1) / 2 : ready;
+          int taken = 0;
+          // the instances are looked up here, under the lock that submissions
+          // insert under (map references stay valid across later inserts)
+          for (Req *r : sv->active) {
+            if (r->in_flight || r->done || taken >= chunk) continue;
+            taken++;
+            ServeInstance *si = &sv->insts[r->inst];
+            auto it = std::find_if(by_inst.begin(), by_inst.end(), [&](const auto &e) { return e.first == si; });
+            if (it == by_inst.end()) by_inst.push_back({si, {r}});
+            else it->second.push_back(r);
+          }
+        }
+        if (!by_inst.empty()) sv->tick++;
       }
-      work = sv->active;
-      sv->tick++;
-      // the instances are looked up here, under the lock that submissions
-      // insert under (map references stay valid across later inserts)
-      for (Req *r : work) {
-        ServeInstance *si = &sv->insts[r->inst];
-        auto it = std::find_if(by_inst.begin(), by_inst.end(), [&](const auto &e) { return e.first == si; });
-        if (it == by_inst.end()) by_inst.push_back({si, {r}});
-        else it->second.push_back(r);
-      }
+    }
+    if (stop) {  // let the queued work finish before the caches are released
+      for (Flight &f : flights)
+        for (TickPart &p : f.parts) collect_part(f, p);
+      return;
     }
     double host_ms = 0.0, device_ms = 0.0;
+    long long in_tick = 0;
     int deferred = 0;
-    for (auto &kv : by_inst) {
-      // requests deferred last time go first, so none waits behind the same others forever
-      std::stable_sort(kv.second.begin(), kv.second.end(), [](const Req *a, const Req *b) { return a->waits > b->waits; });
-      deferred += run_tick(*sv, *kv.first, kv.second, host_ms, device_ms);
+    if (!by_inst.empty()) {
+      Flight f;
+      f.tick = sv->tick;
+      f.stage = next_stage;
+      f.t_launch = std::chrono::steady_clock::now();
+      for (auto &kv : by_inst) {
+        // requests deferred last time go first, so none waits behind the same others forever
+        std::stable_sort(kv.second.begin(), kv.second.end(),
+                         [](const Req *a, const Req *b) { return a->waits > b->waits; });
+        TickPart part;
+        deferred += build_part(*sv, *kv.first, kv.second, f.tick, f.stage, part, host_ms);
+        in_tick += long(part.reqs.size());
+        if (!part.reqs.empty()) f.parts.push_back(std::move(part));
+      }
+      if (!f.parts.empty()) {
+        flights.push_back(std::move(f));
+        next_stage = (next_stage + 1) % kTickStages;
+      }
+    } else if (!flights.empty()) {
+      Flight f = std::move(flights.front());
+      flights.pop_front();
+      const auto t0 = std::chrono::steady_clock::now();
+      for (TickPart &p : f.parts) collect_part(f, p);
+      device_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
-    by_inst.clear();
     {
       std::lock_guard<std::mutex> lk(sv->mu);
-      sv->ticks++;
+      if (in_tick) sv->ticks++;
       sv->host_ms += host_ms;
       sv->device_ms += device_ms;
-      sv->batched_requests += long(work.size()) - deferred;
+      sv->batched_requests += in_tick;
       sv->deferred += deferred;
       std::vector<Req *> still;
       const auto now = std::chrono::steady_clock::now();
+      bool any_done = false;
       for (Req *r : sv->active) {
-        if (r->done) {
+        if (r->done && !r->in_flight) {
           r->t_done = now;
           r->finished = true;
+          any_done = true;
           sv->insts[r->inst].reserved -= r->need;
           sv->latency_ms.push_back(std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count());
         } else
           still.push_back(r);
       }
       sv->active.swap(still);
+      if (any_done) sv->cv_done.notify_all();
     }
-    sv->cv_done.notify_all();
   }
 }
 
@@ -702,9 +881,8 @@ extern "C" int gle_serve_destroy(int h) {
     for (auto &kv : sv->insts) {
       if (get_instance(kv.second.h) != kv.second.I) continue;  // destroyed: its pool went with it
       ClaPool *pool = pool_for(kv.second.I);
-      for (auto &c : kv.second.cla)
-        if (--pool->refs[c.second.slot] == 0) pool->free_slot(c.second.slot);
-      for (auto &p : kv.second.pm) pool->pm_free.push_back(p.second.slot);
+      kv.second.drop_clas(pool);
+      kv.second.drop_pmats(pool);
     }
   }
   std::lock_guard<std::mutex> lk(g_servers_mu);
@@ -712,13 +890,10 @@ extern "C" int gle_serve_destroy(int h) {
   return GLE_SUCCESS;
 }
 
-extern "C" int gle_serve_submit(int h, int inst, const int *parent, const double *blen, int kind, double precision,
-                                int *req) {
-  Server *sv = get_server(h);
-  if (!sv) return set_error(GLE_ERR_ARG, "bad server handle %d", h);
-  Instance *I = get_instance(inst);
-  if (!I || !parent || !blen || !req || (kind != kReqScore && kind != kReqOptimize) ||
-      (kind == kReqOptimize && !(precision > 0.0)))
+// A request for instance I (validated), not yet queued.
+static int make_request(Instance *I, int inst, const int *parent, const double *blen, int kind, double precision,
+                        std::unique_ptr<Req> &out) {
+  if (!parent || !blen || (kind != kReqScore && kind != kReqOptimize) || (kind == kReqOptimize && !(precision > 0.0)))
     return set_error(GLE_ERR_ARG, "bad request");
   auto r = std::make_unique<Req>();
   r->inst = inst;
@@ -730,6 +905,12 @@ extern "C" int gle_serve_submit(int h, int inst, const int *parent, const double
   int rc = tree_topology(*r);
   if (rc) return rc;
   r->need = working_set(*r);
+  if (r->need > instance_cla_slots(I))
+    return set_error(GLE_ERR_NOMEM, "request needs %d CLA slots, instance has %d", r->need, instance_cla_slots(I));
+  // a step may need a P(t) for every one of its 2N - 3 branch lengths
+  if (instance_pmat_slots(I) < 2 * r->N - 3)
+    return set_error(GLE_ERR_NOMEM, "request needs %d transition-matrix slots, instance has %d", 2 * r->N - 3,
+                     instance_pmat_slots(I));
   if (kind == kReqOptimize) {
     // OptimizeAllBranches order: preorder from each root child (optimization.cpp:97-105)
     std::vector<int> stack;
@@ -744,27 +925,62 @@ extern "C" int gle_serve_submit(int h, int inst, const int *parent, const double
     r->scoring = false;
     r->newton.start(r->blen[r->order[0]], precision);
   }
-  r->t_submit = std::chrono::steady_clock::now();
+  out = std::move(r);
+  return GLE_SUCCESS;
+}
+
+// Queues validated requests (caller holds sv.mu).
+static void enqueue_requests(Server &sv, int inst, Instance *I, std::vector<std::unique_ptr<Req>> &rs, int *ids) {
+  ServeInstance &si = sv.insts[inst];
+  if (!si.I) {
+    si.h = inst;
+    si.I = I;
+    si.rescale_every = instance_precision(I) == GLE_PREC_FP64 ? 16 : 1;
+    si.gen = instance_generation(I);
+  }
+  const auto now = std::chrono::steady_clock::now();
+  for (size_t k = 0; k < rs.size(); k++) {
+    Req *r = rs[k].get();
+    r->t_submit = now;
+    r->id = sv.next_id++;
+    ids[k] = r->id;
+    sv.pending.push_back(r);
+    sv.reqs[r->id] = std::move(rs[k]);
+  }
+}
+
+extern "C" int gle_serve_submit(int h, int inst, const int *parent, const double *blen, int kind, double precision,
+                                int *req) {
+  Server *sv = get_server(h);
+  if (!sv) return set_error(GLE_ERR_ARG, "bad server handle %d", h);
+  Instance *I = get_instance(inst);
+  if (!I || !req) return set_error(GLE_ERR_ARG, "bad request");
+  std::vector<std::unique_ptr<Req>> rs(1);
+  const int rc = make_request(I, inst, parent, blen, kind, precision, rs[0]);
+  if (rc) return rc;
   {
     std::lock_guard<std::mutex> lk(sv->mu);
-    ServeInstance &si = sv->insts[inst];
-    if (!si.I) {
-      si.h = inst;
-      si.I = I;
-      si.rescale_every = instance_precision(I) == GLE_PREC_FP64 ? 16 : 1;
-      si.gen = instance_generation(I);
-    }
-    if (r->need > instance_cla_slots(I))
-      return set_error(GLE_ERR_NOMEM, "request needs %d CLA slots, instance has %d", r->need,
-                       instance_cla_slots(I));
-    // a step may need a P(t) for every one of its 2N - 3 branch lengths
-    if (instance_pmat_slots(I) < 2 * r->N - 3)
-      return set_error(GLE_ERR_NOMEM, "request needs %d transition-matrix slots, instance has %d", 2 * r->N - 3,
-                       instance_pmat_slots(I));
-    r->id = sv->next_id++;
-    *req = r->id;
-    sv->pending.push_back(r.get());
-    sv->reqs[r->id] = std::move(r);
+    enqueue_requests(*sv, inst, I, rs, req);
+  }
+  sv->cv_work.notify_one();
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_serve_submit_batch(int h, int inst, int n, const int *parents, const double *blens, int kind,
+                                      double precision, int *reqs) {
+  Server *sv = get_server(h);
+  if (!sv) return set_error(GLE_ERR_ARG, "bad server handle %d", h);
+  Instance *I = get_instance(inst);
+  if (!I || n < 0 || (n > 0 && (!parents || !blens || !reqs))) return set_error(GLE_ERR_ARG, "bad request batch");
+  const size_t nn = size_t(2 * instance_tips(I) - 2);
+  std::vector<std::unique_ptr<Req>> rs(n);
+  for (int k = 0; k < n; k++) {
+    const int rc = make_request(I, inst, parents + k * nn, blens + k * nn, kind, precision, rs[k]);
+    if (rc) return rc;  // nothing of the batch was queued
+  }
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    enqueue_requests(*sv, inst, I, rs, reqs);
   }
   sv->cv_work.notify_one();
   return GLE_SUCCESS;

@@ -133,6 +133,7 @@ SIGNATURES = {
     "gle_serve_create": ([_I], C.c_int),
     "gle_serve_destroy": ([C.c_int], C.c_int),
     "gle_serve_submit": ([C.c_int, C.c_int, _I, _D, C.c_int, C.c_double, _I], C.c_int),
+    "gle_serve_submit_batch": ([C.c_int, C.c_int, C.c_int, _I, _D, C.c_int, C.c_double, _I], C.c_int),
     "gle_serve_poll": ([C.c_int, C.c_int, _I], C.c_int),
     "gle_serve_wait": ([C.c_int, C.c_int, _D, _D, _D], C.c_int),
     "gle_serve_stats": ([C.c_int, C.POINTER(C.c_longlong), _D, _D, _D, C.c_int], C.c_int),
@@ -863,6 +864,17 @@ class Server:
         _check(lib().gle_serve_submit(self.h, inst.h, _p(par, C.c_int), _p(bl, C.c_double), int(kind),
                                       float(precision), C.byref(rid)))
         return rid.value
+
+    def submit_batch(self, inst, trees, kind=REQ_SCORE, precision=0.0):
+        """Submit [(parent, blen), ...] on one instance in one call; returns the ids."""
+        nn = 2 * inst.n_tips - 2
+        par = np.ascontiguousarray(np.concatenate([np.asarray(p, np.int32) for p, _ in trees]), dtype=np.int32)
+        bl = _d(np.concatenate([np.asarray(b, np.float64) for _, b in trees]))
+        assert len(par) == nn * len(trees)
+        ids = np.zeros(len(trees), dtype=np.int32)
+        _check(lib().gle_serve_submit_batch(self.h, inst.h, len(trees), _p(par, C.c_int), _p(bl, C.c_double),
+                                            int(kind), float(precision), _p(ids, C.c_int)))
+        return [int(i) for i in ids]
 
     def poll(self, rid):
         d = C.c_int(0)

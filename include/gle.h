@@ -419,6 +419,12 @@ int gle_serve_destroy(int server);
 /* parent/blen as in gle_tree_create; returns a request id */
 int gle_serve_submit(int server, int inst, const int *parent, const double *blen, int kind, double precision,
                      int *req);
+/* n requests on one instance in one call (a GA generation's offspring,
+ * Population::NextGeneration, population.cpp:4532-4589): parents and blens
+ * hold n consecutive arrays of 2N - 2 entries; reqs[n] receives the ids.  If
+ * any request is invalid, none is queued. */
+int gle_serve_submit_batch(int server, int inst, int n, const int *parents, const double *blens, int kind,
+                           double precision, int *reqs);
 int gle_serve_poll(int server, int req, int *done);
 /* blocks until done; blen_out (optional) receives the final branch lengths */
 int gle_serve_wait(int server, int req, double *lnl, double *blen_out, double *latency_ms);
