@@ -340,23 +340,30 @@ def _eval_rates(ds, E, pop, prec):
     return 1000.0 / lat.mean(), lat
 
 
-SERVE_WINDOW_US = 250  # an idle server gathers a burst of offspring (ends after a 50 us lull)
+SERVE_REPEATS = 5  # cold-cache bursts timed; the median is reported
 
 
 def _serve_rate(ds, E, pop, prec, slots):
-    inst = ds.make_instance(prec, n_cla_slots=slots, n_pmat_slots=4 * slots)
-    _warm(inst)
-    srv = E.Server()
-    srv.set_batch_window(SERVE_WINDOW_US)
-    a = time.perf_counter()
-    ids = srv.submit_batch(inst, pop)  # the generation's offspring in one call
-    res = [srv.wait(i)[0] for i in ids]
-    dt = time.perf_counter() - a
-    st = srv.stats()
-    st.update(srv.timing())
-    srv.close()
-    inst.close()
-    return len(pop) / dt, st, res
+    """The generation's offspring submitted in one call to a server over a
+    fresh instance (cold cache); the median of SERVE_REPEATS such bursts."""
+    runs = []
+    for _ in range(SERVE_REPEATS):
+        inst = ds.make_instance(prec, n_cla_slots=slots, n_pmat_slots=4 * slots)
+        _warm(inst)
+        srv = E.Server()
+        a = time.perf_counter()
+        ids = srv.submit_batch(inst, pop)
+        res = [srv.wait(i)[0] for i in ids]
+        dt = time.perf_counter() - a
+        st = srv.stats()
+        st.update(srv.timing())
+        srv.close()
+        inst.close()
+        runs.append((len(pop) / dt, st, res))
+    runs.sort(key=lambda x: x[0])
+    rate, st, res = runs[len(runs) // 2]
+    st["bursts_evals_per_s"] = [r[0] for r in runs]
+    return rate, st, res
 
 
 def bench_states(args, workload=None, cpu_leg=True):
@@ -471,7 +478,7 @@ def bench_states(args, workload=None, cpu_leg=True):
            "served_vs_fp64_plain": s32 / e64, "serve_stats": st, "sweeps": res, "accuracy": acc, "roofline": roof,
            "cpu_baseline": cpu,
            "serve_config": {"cla_slots": slots, "hbm_budget_GB": slots * ds.n_rates * ((P + 255) // 256 * 256) * (64 if S == 61 else S) * 4 / 1e9,
-                            "batch_window_us": SERVE_WINDOW_US}}
+                            "submit": "one gle_serve_submit_batch call per burst", "bursts": SERVE_REPEATS}}
     return out
 
 

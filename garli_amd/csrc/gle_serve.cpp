@@ -749,11 +749,12 @@ static void worker_loop(Server *sv) {
         admit(*sv);
         if (flights.size() < 2) {
           // One tick runs while the next is built and queued.  A large burst
-          // (a generation's offspring) reaching an idle device goes out in two
-          // halves, so the second half is built while the first runs.
+          // (a generation's offspring) reaching an idle device starts with a
+          // quarter of it, so the device starts early and the rest is built
+          // while that quarter runs.
           int ready = 0;
           for (Req *r : sv->active) ready += !(r->in_flight || r->done);
-          const int chunk = (flights.empty() && ready >= 16) ? (ready + 1) / 2 : ready;
+          const int chunk = (flights.empty() && ready >= 16) ? (ready + 3) / 4 : ready;
           int taken = 0;
           // the instances are looked up here, under the lock that submissions
           // insert under (map references stay valid across later inserts)
