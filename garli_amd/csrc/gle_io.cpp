@@ -16,8 +16,12 @@
 // population.cpp:3459-3478, 3702-3727): a header, one line per character
 // (site number and -lnL, or "-" for a character that is excluded or carries
 // no data), then the tree's line with its -lnL.
+//
+// Reads the trees of a NEXUS TREES block (a starting tree file) with its
+// TRANSLATE table and GarliScore / GarliModel comments.
 #include <algorithm>
 #include <cctype>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -697,3 +701,442 @@ extern "C" int gle_write_site_derivatives(const char *ordered_path, const char *
   return GLE_SUCCESS;
 }
 
+
+// --------------------------------------------------------------- start trees
+// The TREES blocks of a NEXUS file as GarliReader reads them through NCL for
+// a starting tree (garlireader.cpp:793, 1245; Population::
+// SeedPopulationWithStartingTree, population.cpp:1186): a TRANSLATE table
+// and tree statements whose comments may carry the tree's [&U] flag, its
+// [!GarliScore s] and its [!GarliModel ...] parameters.
+namespace gle {
+
+struct TreeFile {
+  std::map<std::string, std::string> translate;  // token -> taxon label
+  struct Entry {
+    std::string name, text;  // the newick text after '=' (comments kept)
+  };
+  std::vector<Entry> trees;
+};
+
+static std::mutex g_tf_mu;
+static std::vector<std::unique_ptr<TreeFile>> g_tfs;
+
+static bool ieq_at(const std::string &s, size_t p, const char *w) {
+  const size_t n = std::strlen(w);
+  if (p + n > s.size()) return false;
+  for (size_t i = 0; i < n; i++)
+    if (std::tolower(static_cast<unsigned char>(s[p + i])) != w[i]) return false;
+  return true;
+}
+
+// A newick string: nested children, labels (quoted or bare, '_' read as a
+// blank), ":length", [comments] anywhere.
+struct NewickNode {
+  std::string label;
+  double len = -1.0;  // -1: not given
+  std::vector<int> kids;
+};
+
+class NewickParser {
+ public:
+  explicit NewickParser(const std::string &s) : s_(s) {}
+  int parse(std::vector<NewickNode> &out, std::string &err) {
+    out.clear();
+    const int root = node(out, err);
+    if (!err.empty()) return -1;
+    skip();
+    if (p_ < s_.size() && s_[p_] == ';') ++p_;
+    skip();
+    if (p_ != s_.size()) {
+      err = "text after the tree";
+      return -1;
+    }
+    return root;
+  }
+
+ private:
+  void skip() {
+    while (p_ < s_.size()) {
+      if (std::isspace(static_cast<unsigned char>(s_[p_]))) {
+        ++p_;
+      } else if (s_[p_] == '[') {
+        int depth = 0;
+        do {
+          if (s_[p_] == '[') depth++;
+          else if (s_[p_] == ']') depth--;
+          ++p_;
+        } while (p_ < s_.size() && depth > 0);
+      } else {
+        break;
+      }
+    }
+  }
+  std::string label() {
+    skip();
+    std::string out;
+    if (p_ < s_.size() && s_[p_] == '\'') {
+      ++p_;
+      while (p_ < s_.size()) {
+        if (s_[p_] == '\'') {
+          if (p_ + 1 < s_.size() && s_[p_ + 1] == '\'') {
+            out += '\'';
+            p_ += 2;
+            continue;
+          }
+          ++p_;
+          break;
+        }
+        out += s_[p_++];
+      }
+      return out;
+    }
+    while (p_ < s_.size() && !std::strchr("(),:;[", s_[p_]) && !std::isspace(static_cast<unsigned char>(s_[p_])))
+      out += s_[p_++];
+    std::replace(out.begin(), out.end(), '_', ' ');
+    return out;
+  }
+  int node(std::vector<NewickNode> &out, std::string &err) {
+    skip();
+    NewickNode nd;
+    if (p_ < s_.size() && s_[p_] == '(') {
+      ++p_;
+      for (;;) {
+        const int c = node(out, err);
+        if (!err.empty()) return -1;
+        nd.kids.push_back(c);
+        skip();
+        if (p_ < s_.size() && s_[p_] == ',') {
+          ++p_;
+          continue;
+        }
+        if (p_ < s_.size() && s_[p_] == ')') {
+          ++p_;
+          break;
+        }
+        err = "expected ',' or ')'";
+        return -1;
+      }
+    }
+    nd.label = label();
+    skip();
+    if (p_ < s_.size() && s_[p_] == ':') {
+      ++p_;
+      skip();
+      const char *b = s_.c_str() + p_;
+      char *e = nullptr;
+      nd.len = std::strtod(b, &e);
+      if (e == b) {
+        err = "bad branch length";
+        return -1;
+      }
+      p_ += size_t(e - b);
+    }
+    if (nd.kids.empty() && nd.label.empty()) {
+      err = "a tip without a label";
+      return -1;
+    }
+    out.push_back(nd);
+    return int(out.size()) - 1;
+  }
+  const std::string &s_;
+  size_t p_ = 0;
+};
+
+static int read_tree_file(const std::string &text, TreeFile &F) {
+  size_t p = 0;
+  const size_t n = text.size();
+  auto skip_ws = [&] {
+    while (p < n && std::isspace(static_cast<unsigned char>(text[p]))) ++p;
+  };
+  auto skip_comment = [&] {
+    int depth = 0;
+    do {
+      if (text[p] == '[') depth++;
+      else if (text[p] == ']') depth--;
+      ++p;
+    } while (p < n && depth > 0);
+  };
+  // the end of a statement: the next ';' outside comments and quotes
+  auto statement_end = [&](size_t from) {
+    size_t q = from;
+    while (q < n) {
+      if (text[q] == '[') {
+        int depth = 0;
+        do {
+          if (text[q] == '[') depth++;
+          else if (text[q] == ']') depth--;
+          ++q;
+        } while (q < n && depth > 0);
+        continue;
+      }
+      if (text[q] == '\'') {
+        ++q;
+        while (q < n && text[q] != '\'') ++q;
+        ++q;
+        continue;
+      }
+      if (text[q] == ';') return q;
+      ++q;
+    }
+    return n;
+  };
+  bool in_trees = false;
+  while (p < n) {
+    skip_ws();
+    if (p >= n) break;
+    if (text[p] == '[') {
+      skip_comment();
+      continue;
+    }
+    if (!in_trees) {
+      if (ieq_at(text, p, "#nexus")) {
+        p += 6;
+        continue;
+      }
+      if (ieq_at(text, p, "begin")) {
+        p += 5;
+        skip_ws();
+        const bool trees = ieq_at(text, p, "trees");
+        p = statement_end(p) + 1;
+        in_trees = trees;
+        continue;
+      }
+      p = statement_end(p) + 1;
+      continue;
+    }
+    if (ieq_at(text, p, "endblock") || ieq_at(text, p, "end")) {
+      in_trees = false;
+      p = statement_end(p) + 1;
+      continue;
+    }
+    if (ieq_at(text, p, "translate")) {
+      const size_t e = statement_end(p);
+      std::string body = text.substr(p + 9, e - p - 9);
+      size_t b = 0;
+      while (b <= body.size()) {
+        size_t c = body.find(',', b);
+        if (c == std::string::npos) c = body.size();
+        std::istringstream is(body.substr(b, c - b));
+        std::string tok, lab;
+        if (is >> tok) {
+          std::getline(is, lab);
+          const size_t s0 = lab.find_first_not_of(" \t\r\n"), s1 = lab.find_last_not_of(" \t\r\n");
+          lab = s0 == std::string::npos ? "" : lab.substr(s0, s1 - s0 + 1);
+          if (lab.size() >= 2 && lab.front() == '\'' && lab.back() == '\'') lab = lab.substr(1, lab.size() - 2);
+          else std::replace(lab.begin(), lab.end(), '_', ' ');
+          F.translate[tok] = lab;
+        }
+        b = c + 1;
+      }
+      p = e + 1;
+      continue;
+    }
+    if (ieq_at(text, p, "utree") || ieq_at(text, p, "tree")) {
+      const size_t e = statement_end(p);
+      const std::string st = text.substr(p, e - p);
+      const size_t eq = st.find('=');
+      if (eq == std::string::npos) return set_error(GLE_ERR_ARG, "tree statement without '='");
+      std::string name = st.substr(ieq_at(st, 0, "utree") ? 5 : 4, eq - (ieq_at(st, 0, "utree") ? 5 : 4));
+      const size_t s0 = name.find_first_not_of(" \t\r\n*"), s1 = name.find_last_not_of(" \t\r\n");
+      name = s0 == std::string::npos ? "" : name.substr(s0, s1 - s0 + 1);
+      F.trees.push_back({name, st.substr(eq + 1)});
+      p = e + 1;
+      continue;
+    }
+    p = statement_end(p) + 1;
+  }
+  if (F.trees.empty()) return set_error(GLE_ERR_ARG, "no tree statement in a TREES block");
+  return GLE_SUCCESS;
+}
+
+static TreeFile *get_tf(int h) {
+  std::lock_guard<std::mutex> lk(g_tf_mu);
+  if (h < 0 || h >= int(g_tfs.size()) || !g_tfs[h]) return nullptr;
+  return g_tfs[h].get();
+}
+
+// The value after "!key" in the tree's comments ("" if absent).
+static std::string comment_value(const std::string &text, const char *key) {
+  const std::string k = std::string("!") + key;
+  size_t b = 0;
+  while ((b = text.find('[', b)) != std::string::npos) {
+    const size_t e = text.find(']', b);
+    if (e == std::string::npos) break;
+    std::string c = text.substr(b + 1, e - b - 1);
+    size_t s = c.find_first_not_of(" \t\r\n");
+    if (s != std::string::npos && c.compare(s, k.size(), k) == 0) {
+      c = c.substr(s + k.size());
+      const size_t a = c.find_first_not_of(" \t\r\n"), z = c.find_last_not_of(" \t\r\n");
+      return a == std::string::npos ? "" : c.substr(a, z - a + 1);
+    }
+    b = e + 1;
+  }
+  return "";
+}
+
+}  // namespace gle
+
+extern "C" int gle_treefile_read(const char *path, int *handle) {
+  if (!path || !handle) return set_error(GLE_ERR_ARG, "null argument");
+  std::ifstream in(path, std::ios::binary);
+  if (!in) return set_error(GLE_ERR_ARG, "cannot open %s", path);
+  const std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+  auto F = std::make_unique<TreeFile>();
+  const int rc = read_tree_file(text, *F);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_tf_mu);
+  g_tfs.push_back(std::move(F));
+  *handle = int(g_tfs.size()) - 1;
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_treefile_count(int h) {
+  TreeFile *F = get_tf(h);
+  if (!F) return set_error(GLE_ERR_ARG, "bad tree file handle %d", h);
+  return int(F->trees.size());
+}
+
+extern "C" int gle_treefile_model(int h, int k, char *buf, int cap) {
+  TreeFile *F = get_tf(h);
+  if (!F || k < 0 || k >= int(F->trees.size()) || !buf || cap < 1) return set_error(GLE_ERR_ARG, "bad argument");
+  const std::string v = comment_value(F->trees[k].text, "GarliModel");
+  const size_t m = std::min(v.size(), size_t(cap - 1));
+  std::memcpy(buf, v.data(), m);
+  buf[m] = 0;
+  return int(v.size());
+}
+
+// Tree k as the tree layer's arrays, following Tree::Tree(const char *, ..)
+// (tree.cpp:309-576): internal nodes numbered n_taxa + 1.. in the order the
+// description opens them; a missing length is the starting length 0.05
+// (DEF_STARTING_BRLEN, defs.h:108) and a given one is held to [1e-8, 100]
+// (DEF_MIN_BRLEN / DEF_MAX_BRLEN, defs.h:106-107); a bifurcating top is made
+// a trifurcation by MakeTrifurcatingRoot's rule (tree.cpp:761-797, the left
+// child dissolved when internal, else the right one, its length added to its
+// sibling's) and the dissolved node's number given up (EliminateNode).  A
+// polytomy is resolved as ArbitrarilyBifurcate does (tree.cpp:855-923), one
+// new node of the starting length per step, but joining the node's first two
+// children where the reference draws the pair at random; the return value is
+// then 1.
+extern "C" int gle_treefile_tree(int h, int k, int n_taxa, const char *const *names, int *parent, double *blen,
+                                 double *score) {
+  constexpr double kStartLen = 0.05, kMinLen = 1e-8, kMaxLen = 100.0;
+  TreeFile *F = get_tf(h);
+  if (!F || k < 0 || k >= int(F->trees.size()) || n_taxa < 3 || !parent || !blen)
+    return set_error(GLE_ERR_ARG, "bad argument");
+  std::vector<NewickNode> nodes;
+  std::string err;
+  const int root = NewickParser(F->trees[k].text).parse(nodes, err);
+  if (root < 0) return set_error(GLE_ERR_ARG, "tree %s: %s", F->trees[k].name.c_str(), err.c_str());
+  for (int u = 0; u < int(nodes.size()); u++) {
+    if (u == root) continue;
+    double &l = nodes[u].len;
+    l = l < 0.0 ? kStartLen : std::min(std::max(l, kMinLen), kMaxLen);
+    if (nodes[u].kids.size() == 1) return set_error(GLE_ERR_ARG, "an internal node with one child");
+  }
+  // tip label -> taxon number 1..N: its TRANSLATE label matched by name among
+  // the given taxa (a blank matching '_'), else the label itself as a number
+  // 1..N (a TRANSLATE table's tokens are the taxa's numbers)
+  auto as_number = [&](const std::string &l) -> int {
+    char *e = nullptr;
+    const long v = std::strtol(l.c_str(), &e, 10);
+    return (!l.empty() && e && *e == 0 && v >= 1 && v <= n_taxa) ? int(v) : -1;
+  };
+  auto taxon = [&](const std::string &lab) -> int {
+    std::string l = lab;
+    auto it = F->translate.find(lab);
+    if (it != F->translate.end()) l = it->second;
+    if (names)
+      for (int i = 0; i < n_taxa; i++) {
+        std::string alt = names[i];
+        std::replace(alt.begin(), alt.end(), '_', ' ');
+        if (l == names[i] || l == alt) return i + 1;
+      }
+    const int v = as_number(lab);
+    return v > 0 ? v : as_number(l);
+  };
+  // the node dissolved by a bifurcating top
+  int gone = -1;
+  if (nodes[root].kids.size() == 2) {
+    const int a = nodes[root].kids[0], b = nodes[root].kids[1];
+    gone = !nodes[a].kids.empty() ? a : (!nodes[b].kids.empty() ? b : -1);
+    if (gone < 0) return set_error(GLE_ERR_ARG, "a two-taxon tree");
+  }
+  if (nodes[root].kids.size() < 2) return set_error(GLE_ERR_ARG, "the tree's top node has one child");
+  // numbers: internal nodes in the order the description opens them (a
+  // preorder of the tree as written), tips by taxon
+  const int nn = 2 * n_taxa - 2;
+  std::vector<int> num(nodes.size(), -1);
+  int next = n_taxa + 1;
+  {
+    std::vector<int> stack = {root};
+    while (!stack.empty()) {
+      const int u = stack.back();
+      stack.pop_back();
+      if (u == root) {
+        num[u] = 0;
+      } else if (nodes[u].kids.empty()) {
+        num[u] = taxon(nodes[u].label);
+        if (num[u] < 0) return set_error(GLE_ERR_ARG, "tip %s is not a taxon of the data", nodes[u].label.c_str());
+      } else if (u != gone) {
+        num[u] = next++;
+      }
+      for (auto c = nodes[u].kids.rbegin(); c != nodes[u].kids.rend(); ++c) stack.push_back(*c);
+    }
+  }
+  if (gone >= 0) {
+    const int other = nodes[root].kids[0] == gone ? nodes[root].kids[1] : nodes[root].kids[0];
+    nodes[other].len += nodes[gone].len;
+    std::vector<int> kids = {other};
+    kids.insert(kids.end(), nodes[gone].kids.begin(), nodes[gone].kids.end());
+    nodes[root].kids = kids;
+  }
+  // polytomies: join the first two children under a new node until resolved
+  bool resolved = false;
+  for (int u = 0; u < int(nodes.size()); u++) {
+    if (u == gone || nodes[u].kids.empty()) continue;
+    const size_t want = u == root ? 3 : 2;
+    while (nodes[u].kids.size() > want) {
+      NewickNode j;
+      j.len = kStartLen;
+      j.kids = {nodes[u].kids[0], nodes[u].kids[1]};
+      nodes.push_back(j);
+      num.push_back(next++);
+      std::vector<int> kids = {int(nodes.size()) - 1};
+      kids.insert(kids.end(), nodes[u].kids.begin() + 2, nodes[u].kids.end());
+      nodes[u].kids = kids;
+      resolved = true;
+    }
+  }
+  if (next != nn) return set_error(GLE_ERR_ARG, "tree has %d internal nodes, %d taxa need %d", next - n_taxa - 1,
+                                   n_taxa, n_taxa - 2);
+  std::vector<int> par(nn, -2);
+  std::vector<double> len(nn, 0.0);
+  std::vector<std::pair<int, int>> stack = {{root, -1}};
+  while (!stack.empty()) {
+    const auto [u, pu] = stack.back();
+    stack.pop_back();
+    const int me = num[u];
+    if (me < 0 || me >= nn) return set_error(GLE_ERR_ARG, "tree does not fit %d taxa", n_taxa);
+    if (par[me] != -2) return set_error(GLE_ERR_ARG, "taxon %d appears twice", me);
+    par[me] = pu < 0 ? -1 : num[pu];
+    len[me] = pu < 0 ? 0.0 : nodes[u].len;
+    for (int c : nodes[u].kids) stack.push_back({c, u});
+  }
+  for (int i = 0; i < nn; i++)
+    if (par[i] == -2) return set_error(GLE_ERR_ARG, "the tree does not hold all %d taxa", n_taxa);
+  std::copy(par.begin(), par.end(), parent);
+  std::copy(len.begin(), len.end(), blen);
+  if (score) {
+    const std::string v = comment_value(F->trees[k].text, "GarliScore");
+    *score = v.empty() ? std::nan("") : std::strtod(v.c_str(), nullptr);
+  }
+  return resolved ? 1 : GLE_SUCCESS;
+}
+
+extern "C" int gle_treefile_free(int h) {
+  std::lock_guard<std::mutex> lk(g_tf_mu);
+  if (h < 0 || h >= int(g_tfs.size()) || !g_tfs[h]) return set_error(GLE_ERR_ARG, "bad tree file handle %d", h);
+  g_tfs[h].reset();
+  return GLE_SUCCESS;
+}

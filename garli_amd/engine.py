@@ -80,6 +80,11 @@ SIGNATURES = {
     "gle_write_sitelikes": ([C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_int, _D, C.POINTER(C.c_int)], C.c_int),
     "gle_write_site_derivatives": ([C.c_char_p, C.c_char_p, C.c_int, C.c_int, _I, C.c_int, _D, _D, _D, _I, _I],
                                    C.c_int),
+    "gle_treefile_read": ([C.c_char_p, C.POINTER(C.c_int)], C.c_int),
+    "gle_treefile_count": ([C.c_int], C.c_int),
+    "gle_treefile_tree": ([C.c_int, C.c_int, C.c_int, C.POINTER(C.c_char_p), _I, _D, _D], C.c_int),
+    "gle_treefile_model": ([C.c_int, C.c_int, C.c_char_p, C.c_int], C.c_int),
+    "gle_treefile_free": ([C.c_int], C.c_int),
     "gle_create_instance": ([C.POINTER(InstanceDesc), _I], C.c_int),
     "gle_destroy_instance": ([C.c_int], C.c_int),
     "gle_set_tip_ambig_stream": ([C.c_int, C.c_int, _S8, C.c_int], C.c_int),
@@ -257,6 +262,39 @@ def ambig_string_to_masks(stream, n):
 
 
 DATA_DNA, DATA_PROTEIN, DATA_OTHER = 0, 1, 2
+
+
+def read_start_trees(path, taxon_names=None, n_taxa=None):
+    """The trees of a NEXUS TREES block (a GARLI starting-tree file) read by
+    gle_treefile_read: a list of dicts with the tree's name-free arrays
+    parent / blen (node 0 the root, taxon i node i, in taxon_names' order),
+    score (the GarliScore comment, NaN without one), model (the GarliModel
+    comment, "" without one) and polytomies_resolved."""
+    if n_taxa is None:
+        n_taxa = len(taxon_names)
+    h = C.c_int(-1)
+    _check(lib().gle_treefile_read(os.fsencode(path), C.byref(h)))
+    try:
+        n = lib().gle_treefile_count(h)
+        _check(n, allow=range(1 << 30))
+        names = None
+        if taxon_names is not None:
+            names = (C.c_char_p * n_taxa)(*[t.encode() for t in taxon_names])
+        out = []
+        buf = C.create_string_buffer(1 << 16)
+        for k in range(n):
+            parent = np.zeros(2 * n_taxa - 2, dtype=np.int32)
+            blen = np.zeros(2 * n_taxa - 2)
+            score = C.c_double()
+            rc = _check(lib().gle_treefile_tree(h, k, n_taxa, names, _p(parent, C.c_int), _p(blen, C.c_double),
+                                                C.byref(score)), allow=(1,))
+            m = lib().gle_treefile_model(h, k, buf, len(buf))
+            _check(m, allow=range(1 << 30))
+            out.append(dict(parent=parent, blen=blen, score=score.value, model=buf.value.decode(),
+                            polytomies_resolved=rc == 1))
+        return out
+    finally:
+        lib().gle_treefile_free(h)
 
 
 class Alignment:

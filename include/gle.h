@@ -127,10 +127,6 @@ int gle_empirical_nuc_freqs(int ntax, int n_patterns, const unsigned char *masks
 /* F1x4 (f3x4 = 0) or F3x4 codon frequencies from gle_alignment_codons'
  * base_freqs (CalcF1x4Freqs / CalcF3x4Freqs, sequencedata.cpp:605-660). */
 int gle_codon_freqs_fx4(const double *base_freqs, int f3x4, double *pi61);
-/* <ofprefix>.sitelikes.log as GARLI writes it: the header unless append, one
- * line per character (-lnL, or "-" where included[s] == 0), then
- * "tree_num <tab> -lnl" (Tree::OutputSiteLikelihoods, tree.cpp:5501-5565;
- * population.cpp:3459-3478). */
 /* orderedSiteDerivs / packedSiteDerivs logs in Tree::OutputSiteDerivatives's
  * layout (tree.cpp:5567-5595): "Partition subset <subset>" and a header, then
  * per character (site_pattern[c] = its pattern, or -1: "gap") and per pattern
@@ -140,8 +136,33 @@ int gle_codon_freqs_fx4(const double *base_freqs, int f3x4, double *pi61);
 int gle_write_site_derivatives(const char *ordered_path, const char *packed_path, int subset, int nchar,
                                const int *site_pattern, int n_patterns, const double *site_lnl, const double *site_d1,
                                const double *site_d2, const int *under_partial, const int *under_child);
+/* <ofprefix>.sitelikes.log as GARLI writes it: the header unless append, one
+ * line per character (-lnL, or "-" where included[s] == 0), then
+ * "tree_num <tab> -lnl" (Tree::OutputSiteLikelihoods, tree.cpp:5501-5565;
+ * population.cpp:3459-3478). */
 int gle_write_sitelikes(const char *path, int append, int tree_num, double lnl, int nchar, const double *site_lnl,
                         const int *included);
+/* Starting trees from a NEXUS TREES block, as GarliReader hands them to
+ * Population::SeedPopulationWithStartingTree (garlireader.cpp:793, 1245;
+ * population.cpp:1186): the TRANSLATE table, then each "tree name = newick;".
+ * gle_treefile_tree gives tree k as the tree layer's arrays for n_taxa taxa
+ * the way Tree::Tree(const char *, ..) builds it (tree.cpp:309-576): node 0
+ * the root; taxon i (names[i - 1], or the label read as a number when names
+ * is NULL or holds no match) node i; internal nodes n_taxa + 1.. in the order
+ * the description opens them; parent[2 n_taxa - 2] (-1 at the root) and
+ * blen (missing: 0.05, DEF_STARTING_BRLEN; given: held to [1e-8, 100],
+ * defs.h:106-108).  A bifurcating top becomes a trifurcation by
+ * MakeTrifurcatingRoot's rule (tree.cpp:761-797).  A polytomy is resolved as
+ * ArbitrarilyBifurcate does (tree.cpp:855-923) with the node's first two
+ * children joined where the reference draws the pair at random; the call
+ * then returns 1.  score: the [!GarliScore s] comment, NaN without one.
+ * gle_treefile_model copies the [!GarliModel ..] comment into buf[cap] and
+ * returns its length (0 without one). */
+int gle_treefile_read(const char *path, int *handle);
+int gle_treefile_count(int h);
+int gle_treefile_tree(int h, int k, int n_taxa, const char *const *names, int *parent, double *blen, double *score);
+int gle_treefile_model(int h, int k, char *buf, int cap);
+int gle_treefile_free(int h);
 
 /* ---------------------------------------------------------------------------
  * Instance layer.

@@ -7,8 +7,8 @@ and tests/data/expected.scr; and, from garli/trunk/src/model.cpp, the numbers
 of the published WAG amino-acid exchangeability matrix (Whelan & Goldman
 2001) that ratematrix = wag multiplies into Q (Model::MultiplyByWAGAAMatrix),
 as 190 values in upper-triangle row order over ACDEFGHIKLMNPQRSTVWY.  Writes
-tests/golden/garli_testdata.json and copies the alignment files the ingestion
-tests read into tests/golden/data/.
+tests/golden/garli_testdata.json and copies the alignment and starting-tree
+files the ingestion tests read into tests/golden/data/.
 
     python tests/golden/make_fixtures.py [/path/to/garli/trunk/tests/data]
 """
@@ -21,7 +21,10 @@ DEFAULT = "/root/reference/garli/trunk/tests/data"
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "garli_testdata.json")
 DATA_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 DATA_FILES = ["z.11x2178.nex", "z.11x2178.wtset.nex", "z.11x2178.AA.nex", "z.11x30.phy", "z.11x30.AA.fas",
-              "z.11x30.wackyNames.nex", "z.11x30.stop.nex"]
+              "z.11x30.wackyNames.nex", "z.11x30.stop.nex", "moore.matK90-120.nex",
+              # starting-tree files the tree-file reader reads
+              "a.G3.start", "a.G4.start", "a.start", "c.M3x2.start", "c.start", "n.G4.start", "n.G5.start",
+              "n.start", "p.3diff.start", "L.start", "moore.start"]
 
 
 def read_matrix(path):

@@ -96,14 +96,21 @@ def model_fields(model):
     return out
 
 
+def start_tree(start_name, taxa):
+    """The first tree of the start file <start_name>.start (a copy under
+    tests/golden/data) as the library's tree-file reader gives it: parent,
+    blen, score and model, tips numbered by the order of taxa."""
+    return E.read_start_trees(os.path.join(HERE, "golden", "data", start_name + ".start"), taxa)[0]
+
+
 def nucleotide_dataset(start_name, ncat=None):
     d = load()
-    st = d["starts"][start_name]
+    st = start_tree(start_name, d["taxa"])
     mf = model_fields(st["model"])
     ntax = len(d["taxa"])
     codes = np.array([[E.nuc_char_to_datum(c) for c in seq] for seq in d["nuc_matrix"]], dtype=np.uint8)
     pats, counts, cs, lc, _ = E.compress_patterns(codes, 4)
-    parent, blen = parse_newick(st["newick"], ntax)
+    parent, blen = st["parent"], st["blen"]
     pi = np.array(mf["e"])
     r = np.array(mf["r"])
     rel = r[:5] / r[5]
@@ -171,12 +178,12 @@ def empirical_codon_freqs(codons):
 
 def codon_dataset(start_name):
     d = load()
-    st = d["starts"][start_name]
+    st = start_tree(start_name, d["taxa"])
     mf = model_fields(st["model"])
     ntax = len(d["taxa"])
     codons = codon_matrix(d["nuc_matrix"])
     pats, counts, cs, lc, _ = E.compress_patterns(codons, 61)
-    parent, blen = parse_newick(st["newick"], ntax)
+    parent, blen = st["parent"], st["blen"]
     pi = empirical_codon_freqs(codons)
     rel6 = np.array(mf["r"])
     o = mf["o"]
@@ -197,7 +204,7 @@ def aa_dataset(start_name):
     a) or the fixed matrix of a.G4.start's garli block; 3, 4 or 1 gamma
     categories at the start file's shape."""
     d = load()
-    st = d["starts"][start_name]
+    st = start_tree(start_name, d["taxa"])
     mf = model_fields(st["model"])
     data = os.path.join(HERE, "golden", "data")
     if start_name == "a.G3":
@@ -207,11 +214,11 @@ def aa_dataset(start_name):
         aln = E.Alignment(os.path.join(data, "z.11x2178.nex"))
         pats, counts, cs, lc, site = aln.translated_patterns()
     assert aln.names == d["taxa"]
-    rel = np.array(st["garli_block"]["r"] if start_name == "a.G4" else d["wag_rel190"])
+    rel = np.array(d["starts"]["a.G4"]["garli_block"]["r"] if start_name == "a.G4" else d["wag_rel190"])
     pi = E.empirical_freqs(pats, counts, 20)
     ncat = {"a.G3": 3, "a.G4": 4, "a": 1}[start_name]
     alpha = mf["a"][0] if "a" in mf else 1.0
-    parent, blen = parse_newick(st["newick"], len(d["taxa"]))
+    parent, blen = st["parent"], st["blen"]
     V, W, lam = E.model_aa(pi, rel)
     rates = E.discrete_gamma(ncat, alpha) if ncat > 1 else np.ones(1)
     return sim.Dataset(start_name, len(d["taxa"]), 20, parent, blen, pats, counts, cs, lc, pi,

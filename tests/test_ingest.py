@@ -3,6 +3,7 @@
 The files under tests/golden/data are the reference's own test data
 (garli/trunk/tests/data), copied by tests/golden/make_fixtures.py."""
 import os
+import re
 
 import numpy as np
 import pytest
@@ -138,3 +139,177 @@ def test_site_derivative_logs(tmp_path):
     assert pk.read_text().splitlines()[2] == "0\t-1.25\t0.5\t-0.25\t0\t3"
     with pytest.raises(E.GleError):
         E.write_site_derivatives(str(o), None, [0, 5], vals)
+
+
+# ------------------------------------------------------------ starting trees
+def _splits(parent, n_taxa):
+    """The bipartitions of a tree given as a parent array, each as the
+    frozenset of taxa on the side without taxon 1."""
+    nn = len(parent)
+    below = {n: ({n} if 1 <= n <= n_taxa else set()) for n in range(nn)}
+    order = sorted(range(1, nn), key=lambda n: -_depth(parent, n))
+    for n in order:
+        below[int(parent[n])] |= below[n]
+    out = set()
+    for n in range(n_taxa + 1, nn):
+        s = below[n]
+        side = frozenset(s if 1 not in s else set(range(1, n_taxa + 1)) - s)
+        out.add(side)
+    return out
+
+
+def _depth(parent, n):
+    d = 0
+    while n != 0:
+        n = int(parent[n])
+        d += 1
+    return d
+
+
+def _newick_splits(newick, n_taxa):
+    """Bipartitions written in a newick string of taxon numbers, read with a
+    plain stack (no branch lengths needed), the two sides of the top merged."""
+    s = re.sub(r"\[.*?\]", "", newick)
+    s = re.sub(r":[0-9.eE+-]+", "", s).strip().rstrip(";")
+    stack, groups = [[]], []
+    tok = ""
+    for ch in s + ",":
+        if ch == "(":
+            stack.append([])
+        elif ch in ",)":
+            if tok:
+                stack[-1].append({int(tok)})
+                tok = ""
+            if ch == ")":
+                g = set().union(*stack.pop())
+                groups.append(g)
+                stack[-1].append(g)
+        elif not ch.isspace():
+            tok += ch
+    full = set(range(1, n_taxa + 1))
+    out = set()
+    for g in groups:
+        side = frozenset(g if 1 not in g else full - g)
+        if 1 < len(side) < n_taxa - 1:
+            out.add(side)
+    return out
+
+
+def _valid_unrooted(parent, n_taxa):
+    nn = 2 * n_taxa - 2
+    assert len(parent) == nn and parent[0] == -1
+    deg = np.zeros(nn, int)
+    for n in range(1, nn):
+        deg[n] += 1
+        deg[parent[n]] += 1
+    assert all(deg[n] == 1 for n in range(1, n_taxa + 1))
+    assert deg[0] == 3 and all(deg[n] == 3 for n in range(n_taxa + 1, nn))
+    for n in range(1, nn):
+        _depth(parent, n)  # reaches the root
+
+
+@pytest.mark.parametrize("name", ["a.G3", "a.G4", "a", "c.M3x2", "c", "n.G4", "n.G5"])
+def test_start_file_trees_match_the_fixture(name):
+    """Each 11-taxon start file through the library's tree-file reader gives
+    the tree, GarliScore and GarliModel of the fixture (independent newick
+    parse of the same file)."""
+    d = refdata.load()
+    st = d["starts"][name]
+    t = E.read_start_trees(os.path.join(DATA, name + ".start"), d["taxa"])
+    assert len(t) == 1
+    p, b = refdata.parse_newick(st["newick"], len(d["taxa"]))
+    np.testing.assert_array_equal(t[0]["parent"], p)
+    np.testing.assert_array_equal(t[0]["blen"], b)
+    assert t[0]["score"] == st["score"] and t[0]["model"] == st["model"]
+    assert not t[0]["polytomies_resolved"]
+    # without names: the TRANSLATE tokens are the taxon numbers
+    t2 = E.read_start_trees(os.path.join(DATA, name + ".start"), n_taxa=len(d["taxa"]))
+    np.testing.assert_array_equal(t2[0]["parent"], p)
+
+
+def test_start_file_without_a_tree_is_refused():
+    """n.start holds only a garli block (the model) and no TREES block."""
+    with pytest.raises(E.GleError):
+        E.read_start_trees(os.path.join(DATA, "n.start"), n_taxa=11)
+
+
+def test_rooted_start_trees_become_trifurcations():
+    """p.3diff.start and L.start (PAUP trees, bifurcating at the top, no
+    lengths): MakeTrifurcatingRoot dissolves the internal child of the root
+    (tree.cpp:761-797) and every length is the starting length 0.05, the
+    kept tip's doubled."""
+    for fname, ntax in (("p.3diff.start", 11), ("L.start", 30)):
+        txt = open(os.path.join(DATA, fname)).read()
+        newick = re.search(r"tree\s+\S+\s*=\s*(.*?;)", txt, re.S).group(1)
+        t = E.read_start_trees(os.path.join(DATA, fname), n_taxa=ntax)[0]
+        _valid_unrooted(t["parent"], ntax)
+        assert _splits(t["parent"], ntax) == _newick_splits(newick, ntax)
+        # "(1,(2,..." : tip 1 is kept, the right child dissolved into the root
+        assert t["parent"][1] == 0 and t["parent"][2] == 0
+        assert t["blen"][1] == pytest.approx(0.1)
+        assert np.all(t["blen"][2:] == 0.05) and np.isnan(t["score"]) and t["model"] == ""
+        # internal nodes numbered as the description opens them, the
+        # dissolved one given up: "(3,((4,11),..." opens ntax+1 under the root
+        assert t["parent"][ntax + 1] == 0 and t["parent"][3] == ntax + 1
+
+
+def test_rooted_tree_with_internal_left_child(tmp_path):
+    """A rooted tree whose left top child is internal: it is the one
+    dissolved, its length added to the right child's; numbers are those of
+    the description with the dissolved node's given up (EliminateNode)."""
+    f = tmp_path / "r.tre"
+    f.write_text("#NEXUS\nbegin trees;\ntree r = (((1:0.1,2:0.2):0.3,3:0.4):0.5,(4:0.6,5:0.7):0.8);\nend;\n")
+    t = E.read_start_trees(str(f), n_taxa=5)[0]
+    # (1,2) -> node 6, (4,5) -> node 7; the root holds 6, 3 and 7
+    np.testing.assert_array_equal(t["parent"], [-1, 6, 6, 0, 7, 7, 0, 0])
+    np.testing.assert_allclose(t["blen"], [0, 0.1, 0.2, 0.4, 0.6, 0.7, 0.3, 1.3])
+
+
+def test_polytomies_are_resolved(tmp_path):
+    """moore.start (86 taxa of moore.matK90-120.nex, PAUP tree with
+    polytomies and a bifurcating top, names through TRANSLATE): fully
+    resolved (the reader returns 1), every split of the description kept,
+    new branches 0.05 long."""
+    aln = _aln("moore.matK90-120.nex")
+    assert len(aln.names) == 86
+    txt = open(os.path.join(DATA, "moore.start")).read()
+    newick = re.search(r"tree\s+\S+\s*=\s*(.*?;)", txt, re.S).group(1)
+    t = E.read_start_trees(os.path.join(DATA, "moore.start"), aln.names)[0]
+    assert t["polytomies_resolved"]
+    _valid_unrooted(t["parent"], 86)
+    want = _newick_splits(newick, 86)
+    got = _splits(t["parent"], 86)
+    assert want < got and len(got) == 86 - 3
+    # rooted as well ("(1,(((..."): tip 1 carries both top lengths
+    assert t["blen"][1] == pytest.approx(0.1) and np.all(t["blen"][2:] == 0.05)
+    # a trifurcating top with four children: one join
+    f = tmp_path / "p.tre"
+    f.write_text("#NEXUS\nbegin trees;\ntree p = (1:0.1,2:0.2,3:0.3,(4:0.4,5:0.5):0.6);\nend;\n")
+    t = E.read_start_trees(str(f), n_taxa=5)[0]
+    assert t["polytomies_resolved"]
+    np.testing.assert_array_equal(t["parent"], [-1, 7, 7, 0, 6, 6, 0, 0])
+    np.testing.assert_allclose(t["blen"], [0, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.05])
+
+
+def test_tree_labels_lengths_and_errors(tmp_path):
+    """Names matched through TRANSLATE or directly (quoted, '_' as a blank),
+    lengths held to [1e-8, 100], several trees in a block, and the errors
+    Tree::Tree raises (tree.cpp:309-576): a tip not in the data, a taxon
+    twice, a taxon missing."""
+    names = ["Homo sapiens", "Pan", "Gorilla", "Pongo"]
+    f = tmp_path / "n.tre"
+    f.write_text("#NEXUS\n[comment; with ; semicolons]\nbegin trees;\n translate a Pan, b 'Homo sapiens',"
+                 " c Gorilla, d Pongo;\n tree one = [&U] (a:0,b:1000,(c:0.25,d:0.5):0.125);\n"
+                 " tree two = [&U][!GarliScore -12.5] ('Homo sapiens':1,Pan:2,(Gorilla:3,Pongo:4):5);\nend;\n")
+    t = E.read_start_trees(str(f), names)
+    assert len(t) == 2
+    np.testing.assert_array_equal(t[0]["parent"], [-1, 0, 0, 5, 5, 0])
+    np.testing.assert_allclose(t[0]["blen"], [0, 100.0, 1e-8, 0.25, 0.5, 0.125])
+    np.testing.assert_array_equal(t[1]["parent"], [-1, 0, 0, 5, 5, 0])
+    np.testing.assert_allclose(t[1]["blen"], [0, 1, 2, 3, 4, 5])
+    assert t[1]["score"] == -12.5 and np.isnan(t[0]["score"])
+    for body in ("(1,2,(3,9));", "(1,2,(3,3));", "(1,2,3);", "(1,2,(3,4)"):
+        g = tmp_path / "bad.tre"
+        g.write_text("#NEXUS\nbegin trees;\ntree x = %s\nend;\n" % body)
+        with pytest.raises(E.GleError):
+            E.read_start_trees(str(g), n_taxa=4)
