@@ -63,6 +63,8 @@ def main():
                     help="sweeps only (for profiler runs): no evals, accuracy or CPU legs")
     ap.add_argument("--calibrate", type=float, default=0.0,
                     help="also run a plain copy of this many GiB (counter calibration)")
+    ap.add_argument("--leg", choices=["serve", "plain"], default=None,
+                    help="aa / codon: run only the served or only the plain FP64 request leg (profiler runs)")
     ap.add_argument("--workload", choices=["gtr", "aa", "codon", "serve"], default="gtr",
                     help="gtr: the metric's configuration (default); aa / codon: the 20- and 61-state "
                          "configurations; serve: Poisson-arrival GA requests through the serving layer")
@@ -76,7 +78,7 @@ def main():
     from garli_amd import sim
 
     if args.workload in ("aa", "codon"):
-        print(json.dumps(bench_states(args)), flush=True)
+        print(json.dumps(bench_leg(args) if args.leg else bench_states(args)), flush=True)
         return
     if args.workload == "serve":
         return bench_serve(args)
@@ -366,6 +368,32 @@ def _serve_rate(ds, E, pop, prec, slots):
     return rate, st, res
 
 
+def _states_dataset(workload):
+    from garli_amd import sim
+    if workload == "aa":
+        return sim.aa_gamma(n_tips=64, n_patterns=20000, seed=3), \
+            "amino-acid 20-state G4 (random exchangeabilities), 64 taxa, 20000 patterns", 20
+    return sim.codon_m3(n_tips=32, n_patterns=5000, seed=4), \
+        "codon GY94 M3 (omega 0.1/1/3, kappa 2.5), 32 taxa, 5000 patterns", 61
+
+
+def bench_leg(args):
+    """One request leg of the aa / codon workload alone, for kernel traces:
+    the 64 offspring served (FP32 bulk, SERVE_REPEATS cold-cache bursts) or
+    scored one at a time by the plain FP64 drop-in."""
+    from garli_amd import engine as E
+    ds, name, _ = _states_dataset(args.workload)
+    E.init(0)
+    pop = _population(ds, 64, 11)
+    if args.leg == "serve":
+        rate, st, _ = _serve_rate(ds, E, pop, E.PREC_FP32, 3 * (ds.n_tips - 2) * 24)
+        log("served %.1f evals/s" % rate)
+        return {"workload": name, "leg": "serve", "evals_per_s": rate, "serve_stats": st}
+    rate, lat = _eval_rates(ds, E, pop, E.PREC_FP64)
+    log("plain fp64 %.1f evals/s" % rate)
+    return {"workload": name, "leg": "plain", "evals_per_s": rate, "mean_ms": float(lat.mean())}
+
+
 def bench_states(args, workload=None, cpu_leg=True):
     """AA (20 states, G4, 64 taxa, 20k patterns) or codon (GY94 M3, 32 taxa, 5k
     patterns): sweeps in both forms, lnL evaluations per request (FP64 plain
@@ -375,12 +403,7 @@ def bench_states(args, workload=None, cpu_leg=True):
     from garli_amd import sim
     from oracle import oracle as O
     workload = workload or args.workload
-    if workload == "aa":
-        ds = sim.aa_gamma(n_tips=64, n_patterns=20000, seed=3)
-        name, S = "amino-acid 20-state G4 (random exchangeabilities), 64 taxa, 20000 patterns", 20
-    else:
-        ds = sim.codon_m3(n_tips=32, n_patterns=5000, seed=4)
-        name, S = "codon GY94 M3 (omega 0.1/1/3, kappa 2.5), 32 taxa, 5000 patterns", 61
+    ds, name, S = _states_dataset(workload)
     E.init(0)
     N, P = ds.n_tips, ds.n_patterns
     n_cla = 3 * (N - 2)
