@@ -1244,8 +1244,10 @@ __global__ __launch_bounds__(256) void k_edge_nuc(const T *__restrict__ cla, con
 // tree.cpp:6431-6480, optimization.cpp:3230-3290).  grid (Ppad/16, n_ops); a
 // block is R waves over the same 16 patterns and wave r takes category r; the
 // per-category row sums meet in LDS, where wave 0 weights them in category
-// order and finishes the sites.
-template <typename T, int R>
+// order and finishes the sites.  D = false: no op of the launch wants
+// derivatives, and the four tiles are unrolled so that a tile's P fragments
+// load during the previous tile's MFMAs.
+template <typename T, int R, bool D>
 __global__ __launch_bounds__(64 * R) void k_edge_codon(const T *__restrict__ cla, const int *__restrict__ ua,
                                                    const unsigned char *__restrict__ tips,
                                                    const double *__restrict__ pmpad, const double *__restrict__ pdiag,
@@ -1263,7 +1265,7 @@ __global__ __launch_bounds__(64 * R) void k_edge_codon(const T *__restrict__ cla
   const size_t slot = size_t(Ppad) * R * 64;
   const DevEdgeOp op = ops[blockIdx.y];
   const double *pi = model, *w = model + 61;
-  const int nm = op.derivs ? 3 : 1;
+  const int nm = D && op.derivs ? 3 : 1;
   double sL[4] = {0, 0, 0, 0}, s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0};
   int ts[4];
   if (op.child < 0) codon_tip_states(tips, op.child, Ppad, P, g, p0, ts);
@@ -1275,7 +1277,7 @@ __global__ __launch_bounds__(64 * R) void k_edge_codon(const T *__restrict__ cla
     // P = V diag(ex) V^-1, z = V^-1 y and a = V^T (x * pi) are two MFMA
     // contractions per tile (instead of P y, P' y and P'' y), and each cell
     // adds a z times ex, s ex and s^2 ex.  The FP64 form keeps the P matrices.
-    const bool spectral = sizeof(T) == 4 && op.derivs && op.child >= 0;
+    const bool spectral = D && sizeof(T) == 4 && op.derivs && op.child >= 0;
     if (spectral) {
       double xa[16];
       codon_load_x(cla, op.partial, slot, R, r, lo, g, p0, P, xa);
@@ -1302,8 +1304,7 @@ __global__ __launch_bounds__(64 * R) void k_edge_codon(const T *__restrict__ cla
         }
       }
     }
-#pragma unroll 1
-    for (int t = 0; t < (spectral ? 0 : 4); t++) {
+    auto tile = [&](int t) {
       const int st = 16 * t + lo;
       double xw[4];  // x * pi of the partial side for this lane's cells
 #pragma unroll
@@ -1320,6 +1321,42 @@ __global__ __launch_bounds__(64 * R) void k_edge_codon(const T *__restrict__ cla
                                              : codon_tip_tile(Pm, ts, t, lo, mi == 0 ? 1.0 : 0.0);
 #pragma unroll
         for (int v = 0; v < 4; v++) acc[mi][v] += Y[v] * xw[v];
+      }
+    };
+    if constexpr (D) {
+#pragma unroll 1
+      for (int t = 0; t < (spectral ? 0 : 4); t++) tile(t);
+    } else {
+      // Y = P y for the four tiles first, with no scheduling barrier between
+      // them, then the x * pi weighting
+      const double *Pm = pmpad + (size_t(op.pm) * 3 * R + r) * 4096;
+      gle_doublex4 Y[4];
+      if (op.child >= 0) {
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          const double *B = Pm + t * 1024 + l;
+          double b[16];
+#pragma unroll
+          for (int kk = 0; kk < 16; kk++) b[kk] = B[kk * 64];
+          gle_doublex4 d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int kk = 0; kk < 16; kk++) d = __builtin_amdgcn_mfma_f64_16x16x4f64(y[kk], b[kk], d, 0, 0, 0);
+          Y[t] = d;
+        }
+      } else {
+#pragma unroll
+        for (int t = 0; t < 4; t++) Y[t] = codon_tip_tile(Pm, ts, t, lo, 1.0);
+      }
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const int st = 16 * t + lo;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int p = p0 + g + 4 * v;
+          const double xw = p < P && st < 61 ? double(cla[op.partial * slot + (size_t(p) * R + r) * 64 + st]) * pi[st]
+                                             : 0.0;
+          acc[0][v] += Y[t][v] * xw;
+        }
       }
     }
 #pragma unroll
@@ -1917,7 +1954,8 @@ int engine_update_partials(Instance *I, const DevClaOp *ops, int n_ops, const in
 }
 
 // derivs: some op of the launch wants (d1, d2); the nucleotide kernel then
-// computes them for every op (the others ignore them).
+// computes them for every op (the others ignore them), the codon kernel takes
+// its derivative-free form only when none does.
 template <typename T>
 static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk, bool derivs, double *d_part) {
   EdgeArgs<T> a;
@@ -1978,9 +2016,14 @@ static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk, bo
 #undef GLE_EDGE_AA
   } else if (I->S == 61) {
 #define GLE_EDGE_CODON(RR)                                                                                        \
-  GLE_LAUNCH(I, (k_edge_codon<T, RR>), grid, dim3(64 * RR), 0, a.cla, a.u, a.tips, I->d_pmpad, I->d_pdiag,           \
-                     I->d_eigfrag, I->d_eig_of_rate, a.model,                                                       \
-                     a.counts, a.cst, d_ops, a.partials, a.site, a.pinv, a.pinv_in_model, a.Ppad, a.P)
+  if (derivs)                                                                                                     \
+    GLE_LAUNCH(I, (k_edge_codon<T, RR, true>), grid, dim3(64 * RR), 0, a.cla, a.u, a.tips, I->d_pmpad,            \
+               I->d_pdiag, I->d_eigfrag, I->d_eig_of_rate, a.model, a.counts, a.cst, d_ops, a.partials, a.site,    \
+               a.pinv, a.pinv_in_model, a.Ppad, a.P);                                                             \
+  else                                                                                                            \
+    GLE_LAUNCH(I, (k_edge_codon<T, RR, false>), grid, dim3(64 * RR), 0, a.cla, a.u, a.tips, I->d_pmpad,           \
+               I->d_pdiag, I->d_eigfrag, I->d_eig_of_rate, a.model, a.counts, a.cst, d_ops, a.partials, a.site,    \
+               a.pinv, a.pinv_in_model, a.Ppad, a.P)
     switch (I->R) {
       case 1: GLE_EDGE_CODON(1); break;
       case 2: GLE_EDGE_CODON(2); break;
