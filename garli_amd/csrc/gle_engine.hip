@@ -402,8 +402,16 @@ __host__ __device__ __forceinline__ int codon_frag(int i, int j) {
 
 // =====================================================================
 // P(t), P', P'' (model.cpp:1271-1358, 1469-1597) -- FP64
-// grid (n, R); V/Vinv of the category's eigensystem staged in LDS.
+// grid (n, R, z); V/Vinv of the category's eigensystem staged in LDS.  Each
+// thread forms a 2x2 tile of the S x S result: per input state k it reads two
+// entries of V's column and two of V^-1's row and updates 4 (P only) or 12
+// independent sums, so the LDS latency of one k overlaps the arithmetic of
+// the others.  Every entry is summed in the reference's order, the product
+// V_ik W_kj first (contraction off), so the values do not depend on the
+// tiling.  The z blocks of a matrix split its (ceil(S/2))^2 tiles 256 apiece.
 // =====================================================================
+__host__ __device__ __forceinline__ int pmat_tiles(int S) { return ((S + 1) / 2) * ((S + 1) / 2); }
+
 __global__ __launch_bounds__(256) void k_pmat(int S, int Sp, int R, const int *__restrict__ idx,
                                               const double *__restrict__ tt, const int *__restrict__ eig_of_rate,
                                               const double *__restrict__ rate_mult, const double *__restrict__ V,
@@ -426,7 +434,7 @@ __global__ __launch_bounds__(256) void k_pmat(int S, int Sp, int R, const int *_
     ex[k] = exp(s * t);
     e1[k] = s * ex[k];
     e2[k] = s * e1[k];
-    if (pdiag) {
+    if (pdiag && blockIdx.z == 0) {
       double *d = pdiag + (size_t(slot) * R + r) * 3 * S;
       d[k] = ex[k];
       d[S + k] = e1[k];
@@ -434,34 +442,60 @@ __global__ __launch_bounds__(256) void k_pmat(int S, int Sp, int R, const int *_
     }
   }
   __syncthreads();
+  const int half = (S + 1) / 2, tile = blockIdx.z * blockDim.x + threadIdx.x;
+  if (tile >= half * half) return;
+  const int i0 = 2 * (tile / half), j0 = 2 * (tile % half);
+  // the second row / column of an edge tile repeats the first (in bounds), and is not stored
+  const int i1 = i0 + 1 < S ? i0 + 1 : i0, j1 = j0 + 1 < S ? j0 + 1 : j0;
+  double sp[2][2] = {{0.0, 0.0}, {0.0, 0.0}}, s1[2][2] = {{0.0, 0.0}, {0.0, 0.0}}, s2[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+  if (derivs) {
+    for (int k = 0; k < S; k++) {
+      const double va[2] = {sV[i0 * S + k], sV[i1 * S + k]}, wb[2] = {sW[k * S + j0], sW[k * S + j1]};
+      const double f0 = ex[k], f1 = e1[k], f2 = e2[k];
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+          const double x = va[a] * wb[b];
+          sp[a][b] += x * f0;
+          s1[a][b] += x * f1;
+          s2[a][b] += x * f2;
+        }
+    }
+  } else {
+    for (int k = 0; k < S; k++) {
+      const double va[2] = {sV[i0 * S + k], sV[i1 * S + k]}, wb[2] = {sW[k * S + j0], sW[k * S + j1]};
+      const double f0 = ex[k];
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) sp[a][b] += (va[a] * wb[b]) * f0;
+    }
+  }
   double *P = pm64 + (size_t(slot) * 3 * R + r) * SS;
   double *D1 = pm64 + (size_t(slot) * 3 * R + R + r) * SS;
   double *D2 = pm64 + (size_t(slot) * 3 * R + 2 * R + r) * SS;
   double *Pp = pmpad ? pmpad + (size_t(slot) * 3 * R + r) * Sp * Sp : nullptr;
-  for (int q = threadIdx.x; q < SS; q += blockDim.x) {
-    const int i = q / S, j = q - i * S;
-    double sp = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int k = 0; k < S; k++) {
-      const double x = sV[i * S + k] * sW[k * S + j];
-      sp += x * ex[k];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      const int i = i0 + a, j = j0 + b;
+      if (i >= S || j >= S) continue;
+      const int q = i * S + j;
+      const double pv = sp[a][b] > 0.0 ? sp[a][b] : 0.0;
+      P[q] = pv;
+      const int pq = layout == kPadTransposed ? j * Sp + i : layout == kPadCodonFrag ? codon_frag(i, j) : i * Sp + j;
+      if (Pp) Pp[pq] = pv;
       if (derivs) {
-        s1 += x * e1[k];
-        s2 += x * e2[k];
+        D1[q] = s1[a][b];
+        D2[q] = s2[a][b];
+        if (Pp) {
+          Pp[size_t(R) * Sp * Sp + pq] = s1[a][b];
+          Pp[size_t(2 * R) * Sp * Sp + pq] = s2[a][b];
+        }
       }
     }
-    const double pv = sp > 0.0 ? sp : 0.0;
-    P[q] = pv;
-    const int pq = layout == kPadTransposed ? j * Sp + i : layout == kPadCodonFrag ? codon_frag(i, j) : i * Sp + j;
-    if (Pp) Pp[pq] = pv;
-    if (derivs) {
-      D1[q] = s1;
-      D2[q] = s2;
-      if (Pp) {
-        Pp[size_t(R) * Sp * Sp + pq] = s1;
-        Pp[size_t(2 * R) * Sp * Sp + pq] = s2;
-      }
-    }
-  }
 }
 
 // 16 wave-uniform doubles (one category's 4x4 P, 128-byte aligned) as two
@@ -1730,7 +1764,7 @@ static int enqueue_pmats(Instance *I, const int *idx, const double *t, int n, in
   const int S = I->S;
   const size_t shmem = sizeof(double) * (2 * S * S + 3 * S);
   timed_begin(I, 2, double(n) * I->R * S * S * 8.0 * (derivs ? 3 : 1));
-  GLE_LAUNCH(I, k_pmat, dim3(n, I->R), dim3(256), shmem, S, I->Sp, I->R, I->d_pmidx, I->d_pmt,
+  GLE_LAUNCH(I, k_pmat, dim3(n, I->R, (pmat_tiles(S) + 255) / 256), dim3(256), shmem, S, I->Sp, I->R, I->d_pmidx, I->d_pmt,
                      I->d_eig_of_rate, I->d_model + S + I->R, I->d_V, I->d_Vinv, I->d_lam, I->d_pm64, I->d_pmpad,
                      I->d_pdiag, derivs, S == 20 ? kPadTransposed : S == 61 ? kPadCodonFrag : kPadRowMajor);
   timed_end(I);
@@ -2213,7 +2247,7 @@ static int tick_enqueue_part(Instance *X, int sidx, const int *pm_idx, const dou
     const int S = X->S;
     const size_t shmem = sizeof(double) * (2 * S * S + 3 * S);
     timed_begin(X, 2, double(n_pm) * X->R * S * S * 8.0 * (pm_derivs ? 3 : 1));
-    GLE_LAUNCH(X, k_pmat, dim3(n_pm, X->R), dim3(256), shmem, S, X->Sp, X->R,
+    GLE_LAUNCH(X, k_pmat, dim3(n_pm, X->R, (pmat_tiles(S) + 255) / 256), dim3(256), shmem, S, X->Sp, X->R,
                reinterpret_cast<const int *>(st.d), reinterpret_cast<const double *>(st.d + o_len),
                X->d_eig_of_rate, X->d_model + S + X->R, X->d_V, X->d_Vinv, X->d_lam, X->d_pm64, X->d_pmpad,
                X->d_pdiag, pm_derivs, S == 20 ? kPadTransposed : S == 61 ? kPadCodonFrag : kPadRowMajor);
