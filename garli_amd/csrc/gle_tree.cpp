@@ -900,6 +900,8 @@ static int line_search(Tree &t, Params &cur, double &best, Apply &&apply, double
 // at most a factor 5 from the current value per round, as the reference
 // limits one pass (maxChangeProportion, tree.cpp:7330)
 static const double kStep = std::log(5.0);
+// codon frequencies: the reference's factor per pass (maxChangeProportion, tree.cpp:7294)
+static const double kFreqStep = std::log(1.2);
 
 // The parameter loop shared by the nucleotide and amino-acid models: rounds
 // of a Newton pass over the branches, then one Brent search per free
@@ -997,10 +999,11 @@ extern "C" int gle_tree_optimize_aa(int h, gle_aa_params *m, int which, double p
 // ------------------------------------------------------------------- codons
 // GARLI's codon models on a fixed tree: nst=6 exchangeabilities between
 // codons one nucleotide apart, an omega class per rate category with its
-// proportion (M0 .. M3, model.cpp:442-687, 1514-1530) and fixed codon
-// frequencies; optimisation of the exchangeabilities (OptimizeRelativeNucRates,
-// tree.cpp:7307) and of the omegas and their proportions
-// (OptimizeOmegaParameters, tree.cpp:7164) with Newton branch passes.
+// proportion (M0 .. M3, model.cpp:442-687, 1514-1530) and codon frequencies;
+// optimisation of the exchangeabilities (OptimizeRelativeNucRates,
+// tree.cpp:7307), of the omegas and their proportions
+// (OptimizeOmegaParameters, tree.cpp:7164) and of the frequencies
+// (OptimizeEquilibriumFreqs, tree.cpp:7288) with Newton branch passes.
 
 static int apply_codon(Tree &t, const gle_codon_params &m) {
   const int R = instance_rates(t.I);
@@ -1061,6 +1064,24 @@ extern "C" int gle_tree_optimize_codon(int h, gle_codon_params *m, int which, do
         rc = line_search(t, cur, best, apply_codon, x0, std::max(x0 - kStep, std::log(1e-4)),
                          std::min(x0 + kStep, std::log(999.0)),
                          [k](gle_codon_params &q, double x) { q.omega[k] = std::exp(x); });
+        if (rc) return rc;
+      }
+    if (which & GLE_OPT_FREQS)
+      // statefrequencies = estimateF (model.h:871-876; "estimate" refuses codon
+      // models, model.h:866-867): OptimizeEquilibriumFreqs (tree.cpp:7288-7303),
+      // each frequency in turn within a factor 1.2 per round, here as the
+      // log-ratio of codon i's frequency to the last codon's
+      for (int i = 0; i < 60; i++) {
+        const double x0 = std::log(cur.pi[i] / cur.pi[60]);
+        rc = line_search(t, cur, best, apply_codon, x0, x0 - kFreqStep, x0 + kFreqStep,
+                         [i](gle_codon_params &q, double x) {
+                           double u[61], sum = 0.0;
+                           for (int k = 0; k < 61; k++) u[k] = q.pi[k] / q.pi[60];
+                           u[i] = std::exp(x);
+                           u[60] = 1.0;
+                           for (int k = 0; k < 61; k++) sum += u[k];
+                           for (int k = 0; k < 61; k++) q.pi[k] = u[k] / sum;
+                         });
         if (rc) return rc;
       }
     if ((which & GLE_OPT_OMEGA_PROBS) && K > 1)  // log-ratio of each proportion to the last class's

@@ -734,7 +734,9 @@ class Tree:
 
     def optimize_codon(self, params, which=None, precision=0.01):
         """Maximise lnL over exchangeabilities, omegas and omega proportions
-        (and branch lengths); returns (lnL, optimised CodonParams)."""
+        (and branch lengths) by default; OPT_FREQS adds the 61 codon
+        frequencies (statefrequencies = estimateF).  Returns (lnL, optimised
+        CodonParams)."""
         if which is None:
             which = OPT_RATES | OPT_OMEGA | OPT_OMEGA_PROBS | OPT_BRANCHES
         c = params.to_c()

@@ -420,9 +420,12 @@ typedef struct gle_codon_params {
  * 827-856). */
 int gle_tree_set_codon(int tree, const gle_codon_params *m);
 /* As gle_tree_optimize_gtr for codon models: GLE_OPT_RATES (AC..CT relative
- * to GT), GLE_OPT_OMEGA, GLE_OPT_OMEGA_PROBS, GLE_OPT_BRANCHES.  Replaces
- * OptimizeRelativeNucRates (tree.cpp:7307) and OptimizeOmegaParameters
- * (tree.cpp:7164) in the OptimizeInputOnly loop (population.cpp:2566). */
+ * to GT), GLE_OPT_OMEGA, GLE_OPT_OMEGA_PROBS, GLE_OPT_FREQS (the 61 codon
+ * frequencies, the reference's statefrequencies = estimateF; its "estimate"
+ * refuses codon models, model.h:866-876), GLE_OPT_BRANCHES.  Replaces
+ * OptimizeRelativeNucRates (tree.cpp:7307), OptimizeOmegaParameters
+ * (tree.cpp:7164) and OptimizeEquilibriumFreqs (tree.cpp:7288) in the
+ * OptimizeInputOnly loop (population.cpp:2566). */
 int gle_tree_optimize_codon(int tree, gle_codon_params *m, int which, double precision, double *lnl);
 
 /* ---------------------------------------------------------------------------
