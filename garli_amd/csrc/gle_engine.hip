@@ -1176,14 +1176,23 @@ __device__ __forceinline__ void site_out(double *site, int Ppad, int p, double u
 // argument is __restrict__).  Per state the categories accumulate in order, as
 // GetScorePartial*RateHet / GetDerivsPartial* do (tree.cpp:6208-6230,
 // optimization.cpp:3123-3158).  grid (Ppad/256, n_ops).
-template <typename T, int R, bool D>
+//
+// FP32 form with derivatives (D): the spectral route of k_edge_aa.  With
+// P = V diag(e) V^-1 (e = exp(s t), s = lambda times the category rate),
+// sum_s pi_s x_s (M y)_s = sum_k a_k z_k f_k for M = P, P', P'' and
+// f = e, s e, s^2 e, where a = V^T (pi x) and z = V^-1 y: two 4 x 4 products
+// and three 4-term sums per category instead of three 4 x 4 products (about a
+// third fewer FP64 instructions in a kernel that is partly bound by their
+// issue).  The FP64 form keeps the reference's order of operations.
+template <typename T, int R, bool D, bool SPEC = false>
 __global__ __launch_bounds__(256) void k_edge_nuc(const T *__restrict__ cla, const int *__restrict__ ua,
                                                   const unsigned char *__restrict__ tips,
                                                   const double *__restrict__ pm64, const double *__restrict__ model,
                                                   const int *__restrict__ counts, const int *__restrict__ cst,
                                                   const DevEdgeOp *__restrict__ ops, double *__restrict__ partials,
                                                   double *__restrict__ site, double pinv, int pinv_in_model,
-                                                  int Ppad, int P) {
+                                                  int Ppad, int P, const double *__restrict__ Vg,
+                                                  const double *__restrict__ Wg, const double *__restrict__ pdiag) {
 #pragma clang fp contract(off)
   const DevEdgeOp op = ops[blockIdx.y];
   const double *Pb = pm64 + size_t(op.pm) * 3 * R * 16;
@@ -1218,10 +1227,56 @@ __global__ __launch_bounds__(256) void k_edge_nuc(const T *__restrict__ cla, con
       else
         return __builtin_fma(q[3], v[3], __builtin_fma(q[2], v[2], __builtin_fma(q[1], v[1], q[0] * v[0])));
     };
+    constexpr bool spectral = SPEC && D && sizeof(T) == 4;
+    double Ls = 0.0, D1s = 0.0, D2s = 0.0;  // spectral route: the category-weighted sums
+    if constexpr (spectral) {
+      // (the host takes this route only when one eigensystem serves every category)
+      double Vm[16], Wm[16];
+      load_uniform16(Vg, Vm);  // V[i][k]
+      load_uniform16(Wg, Wm);  // V^-1[k][j]
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const double *f = pdiag + (size_t(op.pm) * R + r) * 12;
+        double xp[4], a[4], z[4];
+#pragma unroll
+        for (int st = 0; st < 4; st++) xp[st] = double(xt[r][st]) * pi[st];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          a[k] = __builtin_fma(Vm[12 + k], xp[3], __builtin_fma(Vm[8 + k], xp[2],
+                 __builtin_fma(Vm[4 + k], xp[1], Vm[k] * xp[0])));
+        if (internal) {
+          const double y[4] = {double(yt[r][0]), double(yt[r][1]), double(yt[r][2]), double(yt[r][3])};
+#pragma unroll
+          for (int k = 0; k < 4; k++)
+            z[k] = __builtin_fma(Wm[4 * k + 3], y[3], __builtin_fma(Wm[4 * k + 2], y[2],
+                   __builtin_fma(Wm[4 * k + 1], y[1], Wm[4 * k] * y[0])));
+        } else {
+#pragma unroll
+          for (int k = 0; k < 4; k++)
+            z[k] = __builtin_fma(bit[3], Wm[4 * k + 3], __builtin_fma(bit[2], Wm[4 * k + 2],
+                   __builtin_fma(bit[1], Wm[4 * k + 1], bit[0] * Wm[4 * k])));
+        }
+        double l0 = 0.0, l1 = 0.0, l2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const double c = a[k] * z[k];
+          l0 = __builtin_fma(c, f[k], l0);
+          l1 = __builtin_fma(c, f[4 + k], l1);
+          l2 = __builtin_fma(c, f[8 + k], l2);
+        }
+        if (!internal && m == 15) {  // P 1 = 1: the partial side alone, no derivative
+          l0 = xp[0] + xp[1] + xp[2] + xp[3];
+          l1 = l2 = 0.0;
+        }
+        Ls = __builtin_fma(l0, w[r], Ls);
+        D1s = __builtin_fma(l1, w[r], D1s);
+        D2s = __builtin_fma(l2, w[r], D2s);
+      }
+    }
     // P, then P' and P'' when the launch computes derivatives (D)
     constexpr int nm = D ? 3 : 1;
 #pragma unroll
-    for (int r = 0; r < R; r++) {
+    for (int r = 0; r < (spectral ? 0 : R); r++) {
       const double x[4] = {double(xt[r][0]), double(xt[r][1]), double(xt[r][2]), double(xt[r][3])};
       const double y[4] = {double(yt[r][0]), double(yt[r][1]), double(yt[r][2]), double(yt[r][3])};
 #pragma unroll
@@ -1249,9 +1304,9 @@ __global__ __launch_bounds__(256) void k_edge_nuc(const T *__restrict__ cla, con
         }
       }
     }
-    const double Lt = L[0] * pi[0] + L[1] * pi[1] + L[2] * pi[2] + L[3] * pi[3];
-    const double D1 = G1[0] * pi[0] + G1[1] * pi[1] + G1[2] * pi[2] + G1[3] * pi[3];
-    const double D2 = G2[0] * pi[0] + G2[1] * pi[1] + G2[2] * pi[2] + G2[3] * pi[3];
+    const double Lt = spectral ? Ls : L[0] * pi[0] + L[1] * pi[1] + L[2] * pi[2] + L[3] * pi[3];
+    const double D1 = spectral ? D1s : G1[0] * pi[0] + G1[1] * pi[1] + G1[2] * pi[2] + G1[3] * pi[3];
+    const double D2 = spectral ? D2s : G2[0] * pi[0] + G2[1] * pi[1] + G2[2] * pi[2] + G2[3] * pi[3];
     double pt = 0.0;
     const int cs = cst[p];
     if (pinv_in_model && cs >= 0) {
@@ -2012,13 +2067,21 @@ static int launch_edges(Instance *I, const DevEdgeOp *d_ops, int n, int nblk, bo
   a.site = I->d_site;
   dim3 grid(nblk, n);
   if (I->S == 4) {
+    // the FP32 form's spectral derivative route needs one eigensystem for all categories
+    const bool nuc_spec = I->prec == GLE_PREC_FP32 && I->n_eig == 1 && I->d_pdiag;
 #define GLE_EDGE_NUC(RR)                                                                                          \
-  if (derivs)                                                                                                     \
+  if (derivs && nuc_spec)                                                                                         \
+    GLE_LAUNCH(I, (k_edge_nuc<T, RR, true, true>), grid, dim3(256), 0, a.cla, a.u, a.tips, a.pm64,               \
+                       a.model, a.counts, a.cst, d_ops, a.partials, a.site, a.pinv, a.pinv_in_model, a.Ppad, a.P,  \
+                       I->d_V, I->d_Vinv, I->d_pdiag);                                                            \
+  else if (derivs)                                                                                                \
     GLE_LAUNCH(I, (k_edge_nuc<T, RR, true>), grid, dim3(256), 0, a.cla, a.u, a.tips, a.pm64,                     \
-                       a.model, a.counts, a.cst, d_ops, a.partials, a.site, a.pinv, a.pinv_in_model, a.Ppad, a.P); \
+                       a.model, a.counts, a.cst, d_ops, a.partials, a.site, a.pinv, a.pinv_in_model, a.Ppad, a.P,  \
+                       nullptr, nullptr, nullptr);                                                                \
   else                                                                                                            \
     GLE_LAUNCH(I, (k_edge_nuc<T, RR, false>), grid, dim3(256), 0, a.cla, a.u, a.tips, a.pm64,                    \
-                       a.model, a.counts, a.cst, d_ops, a.partials, a.site, a.pinv, a.pinv_in_model, a.Ppad, a.P)
+                       a.model, a.counts, a.cst, d_ops, a.partials, a.site, a.pinv, a.pinv_in_model, a.Ppad, a.P,  \
+                       nullptr, nullptr, nullptr)
     switch (I->R) {
       case 1: GLE_EDGE_NUC(1); break;
       case 2: GLE_EDGE_NUC(2); break;
@@ -2523,8 +2586,7 @@ static int create_internal(const gle_instance_desc *d, std::unique_ptr<Instance>
   GLE_ALLOC(I->d_cla, I->slot_elems() * I->elem_bytes() * I->n_slots);
   GLE_ALLOC(I->d_pm64, sizeof(double) * size_t(I->n_pm) * 3 * R * S * S);
   if (S == 61 || S == 20) GLE_ALLOC(I->d_pmpad, sizeof(double) * size_t(I->n_pm) * 3 * R * I->Sp * I->Sp);
-  if ((S == 20 || S == 61) && I->prec == GLE_PREC_FP32)
-    GLE_ALLOC(I->d_pdiag, sizeof(double) * size_t(I->n_pm) * R * 3 * S);
+  if (I->prec == GLE_PREC_FP32) GLE_ALLOC(I->d_pdiag, sizeof(double) * size_t(I->n_pm) * R * 3 * S);
   if (S == 61 && I->prec == GLE_PREC_FP32) GLE_ALLOC(I->d_eigfrag, sizeof(double) * size_t(I->n_eig) * 2 * 4096);
   GLE_ALLOC(I->d_V, sizeof(double) * size_t(I->n_eig) * S * S);
   GLE_ALLOC(I->d_Vinv, sizeof(double) * size_t(I->n_eig) * S * S);
