@@ -196,6 +196,7 @@ struct ServeInstance {
   int reserved = 0;                              // working-set slots of admitted requests
   unsigned long long gen = 0;                    // instance generation the caches belong to
   unsigned long long epoch = 0;                  // bumped whenever the CLA cache is dropped
+  unsigned long long names = 0;                  // bumped whenever the name table is cleared
 
   uint64_t intern_id(const SubKey &k) { return intern.intern(k, next_id); }
   uint64_t find_id(const SubKey &k) const { return intern.find(k); }
@@ -212,7 +213,10 @@ struct ServeInstance {
 
 enum { kReqScore = 0, kReqOptimize = 1 };
 
-// per-request memo of subtree ids: kUnset = not looked up yet, 0 = no id yet
+// per-request memo of subtree ids: kUnset = not looked up yet, 0 = no id yet.
+// An id is a name, valid until the name table is cleared, so the memo is
+// kept across a request's ticks; a branch-length change forgets the ids
+// whose subtree holds that branch, and a 0 is looked up again each tick.
 constexpr uint64_t kUnset = ~uint64_t(0);
 
 struct Req {
@@ -233,8 +237,9 @@ struct Req {
   bool done = false;      // set by the worker while it owns the request
   bool finished = false;  // published under Server::mu once the worker let go
   std::chrono::steady_clock::time_point t_submit, t_done;
+  std::vector<std::array<uint64_t, 3>> sid;  // X(a|b) id memo, sid[a][index of b among a's neighbours]
+  unsigned long long sid_names = ~0ull;      // the instance's name-table generation sid belongs to
   // per-tick scratch
-  std::vector<std::array<uint64_t, 3>> sid;
   int edge_a = 0, edge_b = 0;
   bool derivs = false;
 };
@@ -305,7 +310,7 @@ static void req_children(const Req &r, int a, int b, int &x, int &y) {
 }
 
 // The id X(a|b) already has (0: its subtree was never named, so no CLA of it
-// can be cached), memoised per tick in r.sid.
+// can be cached), memoised in r.sid.
 static uint64_t known_id(Req &r, const ServeInstance &si, int a, int b) {
   if (req_tip(r, a)) return uint64_t(a);
   uint64_t &m = r.sid[a][req_nbk(r, a, b)];
@@ -516,6 +521,25 @@ static void next_edge(Req &r, const TickBuilder &tb) {
   }
 }
 
+// Forget the ids that name branch n's length: X(x|y) for every x on the far
+// side of y from the branch (x's subtree away from y holds it), found by
+// walking away from each end of the branch.
+static void forget_ids_through(Req &r, int n) {
+  if (r.sid.size() != size_t(r.nn)) return;
+  std::vector<std::pair<int, int>> stack = {{n, r.parent[n]}, {r.parent[n], n}};
+  while (!stack.empty()) {
+    const auto [x, from] = stack.back();
+    stack.pop_back();
+    if (req_tip(r, x)) continue;
+    for (int j = 0; j < r.nnb[x]; j++) {
+      const int y = r.nb[x][j];
+      if (y == from) continue;
+      r.sid[x][j] = kUnset;
+      stack.push_back({y, x});
+    }
+  }
+}
+
 static void advance(Req &r, const double *res) {
   if (r.scoring) {
     r.lnl = res[0];
@@ -525,7 +549,10 @@ static void advance(Req &r, const double *res) {
   const int nd = r.order[r.bi];
   double next = 0.0;
   const NewtonBranch::Action act = r.newton.step(res[0], res[1], res[2], r.blen[nd], &next);
-  if (act != NewtonBranch::kFinish) r.blen[nd] = next;
+  if (act != NewtonBranch::kFinish && next != r.blen[nd]) {
+    r.blen[nd] = next;
+    forget_ids_through(r, nd);
+  }
   if (act != NewtonBranch::kContinue) {
     r.improve += r.newton.total;
     if (++r.bi < r.order.size()) r.newton.start(r.blen[r.order[r.bi]], r.prec);
@@ -574,6 +601,7 @@ static int build_part(Server &sv, ServeInstance &si, std::vector<Req *> &batch, 
     si.drop_clas(pool);
     si.drop_pmats(pool);
     si.intern.clear();
+    si.names++;
     si.gen = instance_generation(si.I);
     si.epoch++;
   }
@@ -582,7 +610,14 @@ static int build_part(Server &sv, ServeInstance &si, std::vector<Req *> &batch, 
   part.epoch = si.epoch;
   for (Req *r : batch) {
     const TickBuilder::Mark m = tb.mark();
-    r->sid.assign(r->nn, {kUnset, kUnset, kUnset});
+    if (r->sid_names != si.names || r->sid.size() != size_t(r->nn)) {
+      r->sid.assign(r->nn, {kUnset, kUnset, kUnset});
+      r->sid_names = si.names;
+    } else {
+      for (auto &e : r->sid)  // another request may have named it since
+        for (uint64_t &v : e)
+          if (v == 0) v = kUnset;
+    }
     next_edge(*r, tb);
     int ra, la, rb, lb;
     DevEdgeOp e;
