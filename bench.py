@@ -57,6 +57,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle legs")
     ap.add_argument("--no-codon", action="store_true",
                     help="skip the codon served-vs-plain leg of the default workload")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the GA request-stream leg of the default workload")
     ap.add_argument("--patterns", type=int, default=100000)
     ap.add_argument("--taxa", type=int, default=128)
     ap.add_argument("--quick", action="store_true",
@@ -81,7 +83,8 @@ def main():
         print(json.dumps(bench_leg(args) if args.leg else bench_states(args)), flush=True)
         return
     if args.workload == "serve":
-        return bench_serve(args)
+        print(json.dumps(bench_serve(args)), flush=True)
+        return
 
     t0 = time.time()
     ds = sim.gtr_gamma(n_tips=args.taxa, n_sites=None, n_patterns=args.patterns, mean_blen=0.05, alpha=None,
@@ -139,7 +142,7 @@ def main():
         copy = {"bytes": nbytes, "ms": ms, "GB/s": 2 * nbytes / ms / 1e6}
         log("copy calibration: %d bytes in %.3f ms -> %.1f GB/s (read+write)" % (nbytes, ms, copy["GB/s"]))
     if args.quick:
-        args.no_codon = True
+        args.no_codon = args.no_serve = True
         print(json.dumps({"metric": METRIC, "value": site_updates, "unit": "CLA site-updates/s",
                           "ms_per_step": ms_step, "cla_avg_launch_ms": avg_ms, "cla_GBs": achieved,
                           "launches": n_l, "copy": copy}), flush=True)
@@ -256,6 +259,17 @@ def main():
         out["codon_served_max_dlnl_vs_oracle"] = c["accuracy"]["serve_fp32_max_dlnl"]
         out["codon_roofline"] = c["roofline"]
         out["codon_config"] = c["config"]
+    if not args.no_serve:
+        # the GA request stream (Poisson arrivals over uneven nucleotide and
+        # amino-acid instances), served FP32 bulk against the plain FP64 drop-in
+        sv = bench_serve(args, cpu_leg=False)
+        out["serve_requests_per_s"] = sv["value"]
+        out["serve_fp64_plain_requests_per_s"] = sv["fp64_plain_requests_per_s"]
+        out["serve_vs_fp64_plain"] = sv["served_vs_fp64_plain"]
+        out["serve_p50_ms"] = sv["p50_ms"]
+        out["serve_p99_ms"] = sv["p99_ms"]
+        out["serve_max_dlnl_vs_fp64"] = sv["accuracy"]["served_fp32_vs_fp64_max_dlnl"]
+        out["serve_config"] = sv["config"]
     print(json.dumps(out), flush=True)
 
 
@@ -505,13 +519,14 @@ def bench_states(args, workload=None, cpu_leg=True):
     return out
 
 
-def bench_serve(args):
+def bench_serve(args, cpu_leg=True):
     """The serving workload: a stream of 256 GA requests with Poisson arrivals
     over uneven instances (nucleotide and amino-acid, 16-512 taxa, 1k-200k
     patterns), each a parent tree plus 1-3 NNI/SPR moves; one in eight asks for
     a branch-length pass before its score (Individual::Mutate's
     optimisation, individual.cpp:127-222).  FP32 bulk served against the FP64
-    plain drop-in running the same requests one at a time."""
+    plain drop-in running the same requests one at a time.  Returns the record
+    (printed by main for --workload serve)."""
     import threading
     from garli_amd import engine as E
     from garli_amd import sim
@@ -591,7 +606,7 @@ def bench_serve(args):
            "branch_pass_served_minus_fp64_pass": [float(min(dopt)), float(max(dopt))] if dopt else None}
     log("served fp32 vs fp64 plain: max |dlnL| %.2e over %d requests" % (max(dl), n_req))
     cpu = None
-    if not args.no_cpu:
+    if cpu_leg and not args.no_cpu:
         # CPU baseline: one score request of each instance in full by the FP64
         # oracle, patterns split over the host threads; the stream's score rate
         # is the inverse of the mean time over its mix of instances
@@ -622,7 +637,7 @@ def bench_serve(args):
            "serve_timing_ms": tm, "accuracy": acc, "cpu_baseline": cpu}
     log("serve: %.1f requests/s (plain fp64 %.1f), p50 %.2f ms p99 %.2f ms, mean batch %.1f" %
         (n_req / t32, n_req / t64, st["p50_ms"], st["p99_ms"], st["mean_batch"]))
-    print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":
