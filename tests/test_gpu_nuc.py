@@ -191,6 +191,35 @@ def test_copy_on_write(gpu):
     assert _rel(la, O.Oracle(ds).score()) < 1e-9
 
 
+def test_out_of_cla_slots_then_recovers(gpu):
+    """Two trees on an instance with CLA slots for about one score: the second
+    tree's score runs out of slots (GLE_ERR_NOMEM) part way through its op
+    list.  The slots it had taken go back and nothing it named stays marked
+    computed, so once the first tree is closed the second scores equal to the
+    oracle, and so do a third and the first tree's topology again."""
+    ds = sim.gtr_gamma(n_tips=14, n_sites=500, seed=93)
+    inst = ds.make_instance(E.PREC_FP64, n_cla_slots=ds.n_tips + 2, n_pmat_slots=4 * ds.n_tips)
+    o = O.Oracle(ds)
+    t1 = E.Tree(inst, ds.parent, ds.blen)
+    l1 = t1.score()
+    assert _rel(l1, o.score()) < 1e-9
+    rng = np.random.default_rng(4)
+    p2, b2 = sim.perturb(ds.parent, ds.blen, rng, ds.n_tips, 3)
+    t2 = E.Tree(inst, p2, b2)
+    with pytest.raises(E.GleError) as ei:
+        t2.score()
+    assert ei.value.code == -3
+    t1.close()
+    assert _rel(t2.score(), o.score(parent=p2, blen=b2)) < 1e-9
+    t2.close()
+    p3, b3 = sim.perturb(ds.parent, ds.blen, rng, ds.n_tips, 2)
+    t3 = E.Tree(inst, p3, b3)
+    assert _rel(t3.score(), o.score(parent=p3, blen=b3)) < 1e-9
+    t3.close()
+    t4 = E.Tree(inst, ds.parent, ds.blen)
+    assert t4.score() == l1
+
+
 @pytest.mark.parametrize("prec", [E.PREC_FP64, E.PREC_FP32])
 def test_graph_replay_equals_eager(gpu, prec):
     """Repeated sweeps replay the captured level launches; every replay gives
