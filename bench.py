@@ -373,6 +373,7 @@ def _serve_rate(ds, E, pop, prec, slots):
         dt = time.perf_counter() - a
         st = srv.stats()
         st.update(srv.timing())
+        st.update(srv.work())
         srv.close()
         inst.close()
         runs.append((len(pop) / dt, st, res))

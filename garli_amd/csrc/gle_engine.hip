@@ -959,29 +959,77 @@ __device__ __forceinline__ gle_doublex4 codon_tip_tile(const double *__restrict_
   return d;
 }
 
-// Tiles 0..3 of Pr . x (internal child c) or of the tip column (tip child c).
-template <typename T>
-__device__ __forceinline__ void codon_child(const T *__restrict__ cla, const double *__restrict__ Pr, int c,
-                                            const int ts[4], size_t slot, int R, int r, int l, int p0, int P,
-                                            gle_doublex4 acc[4]) {
+// Child c's output tiles 0..3 into ov: Pr . x for an internal child, the tip
+// column for a tip child, or (msg) the product an earlier op kept in slot c.
+// ov[t] = tile (MUL false) or ov[t] *= tile (MUL true), so the second child's
+// tiles are never all held.  mkeep >= 0 keeps an internal child's tiles in
+// that slot (a message, read back by later ops with kOpMsg).  Tile t+1's B
+// fragments load while tile t's MFMA chain runs (the loads had been waited for
+// one tile at a time, leaving the MFMA unit idle for most of each wave); the
+// FP32 form holds x in FP32 and widens it at each MFMA, which with the
+// streamed second child makes room for the second buffer.  The FP32 form keeps
+// a message rounded to nearest without the dither: a message is a contraction
+// of stored (dithered) CLA values, so its values differ from pattern to
+// pattern and their roundings are already independent; the dither is for
+// stores whose values repeat across patterns (a tip-tip product).
+template <typename T, bool MUL>
+__device__ __forceinline__ void codon_child_into(T *__restrict__ cla, const double *__restrict__ Pr, int c,
+                                                 bool msg, int mkeep, const int ts[4], size_t slot, int R, int r,
+                                                 int l, int p0, int P, gle_doublex4 ov[4]) {
   const int lo = l & 15, g = l >> 4;
-  if (c >= 0) {
-    double x[16];
-    codon_load_x(cla, c, slot, R, r, lo, g, p0, P, x);
+  auto put = [&](int t, const gle_doublex4 &d) {
+#pragma unroll
+    for (int v = 0; v < 4; v++) ov[t][v] = MUL ? mul_nc(ov[t][v], d[v]) : d[v];
+  };
+  if (msg) {
+    const T *src = cla + c * slot;
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-      const double *B = Pr + t * 1024 + l;
-      double b[16];
+      gle_doublex4 d;
 #pragma unroll
-      for (int kk = 0; kk < 16; kk++) b[kk] = B[kk * 64];
-      gle_doublex4 d = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int kk = 0; kk < 16; kk++) d = __builtin_amdgcn_mfma_f64_16x16x4f64(x[kk], b[kk], d, 0, 0, 0);
-      acc[t] = d;
+      for (int v = 0; v < 4; v++) {
+        const int p = p0 + g + 4 * v;
+        d[v] = p < P ? double(src[(size_t(p) * R + r) * 64 + 16 * t + lo]) : 0.0;
+      }
+      put(t, d);
     }
-  } else {
+    return;
+  }
+  if (c < 0) {
 #pragma unroll
-    for (int t = 0; t < 4; t++) acc[t] = codon_tip_tile(Pr, ts, t, lo, 1.0);
+    for (int t = 0; t < 4; t++) put(t, codon_tip_tile(Pr, ts, t, lo, 1.0));
+    return;
+  }
+  T x[16];
+  {
+    const int p = p0 + lo;
+    if (p < P) {
+      const T *src = cla + c * slot + (size_t(p) * R + r) * 64 + 16 * g;
+#pragma unroll
+      for (int q = 0; q < 16; q += 4) load4(src + q, x + q);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 16; q++) x[q] = T(0);
+    }
+  }
+  double b[2][16];
+  const double *B = Pr + l;
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+#pragma unroll
+      for (int kk = 0; kk < 16; kk++) b[t & 1][kk] = B[t * 1024 + kk * 64];
+    gle_doublex4 d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) d = __builtin_amdgcn_mfma_f64_16x16x4f64(double(x[kk]), b[t & 1][kk], d, 0, 0, 0);
+    if (mkeep >= 0) {
+      T *dst = cla + mkeep * slot;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int p = p0 + g + 4 * v;
+        if (p < P) dst[(size_t(p) * R + r) * 64 + 16 * t + lo] = T(d[v]);
+      }
+    }
+    put(t, d);
   }
 }
 
@@ -991,7 +1039,9 @@ __device__ __forceinline__ void codon_child(const T *__restrict__ cla, const dou
 // The products are held until the op's rescale decision (each pattern's
 // maximum over its 16 lanes and over the R waves, through LDS) and stored
 // once, scaled where needed.
-template <typename T, int R>
+// MSG (serving ticks): ops may read children as kept messages and keep the
+// messages they form (DevClaOp::m1, m2).
+template <typename T, int R, bool MSG>
 __global__ __launch_bounds__(64 * R, 3) void k_cla_codon(T *__restrict__ cla, int *__restrict__ ua,
                                                          const unsigned char *__restrict__ tips,
                                                          const double *__restrict__ pmpad,
@@ -1014,14 +1064,17 @@ __global__ __launch_bounds__(64 * R, 3) void k_cla_codon(T *__restrict__ cla, in
     double mx[4] = {0.0, 0.0, 0.0, 0.0};
     double ov[4][4];  // [pattern row v][state tile t], held until the rescale decision
     {
-      gle_doublex4 a1[4], a2[4];
-      codon_child(cla, pmpad + (size_t(op.pm1) * 3 * R + r) * 4096, op.c1, ts1, slot, R, r, l, p0, P, a1);
-      codon_child(cla, pmpad + (size_t(op.pm2) * 3 * R + r) * 4096, op.c2, ts2, slot, R, r, l, p0, P, a2);
+      gle_doublex4 a1[4];  // child 1's tiles, then the product
+      const bool g1 = MSG && (op.flags & kOpMsg1), g2 = MSG && (op.flags & kOpMsg2);
+      codon_child_into<T, false>(cla, g1 ? nullptr : pmpad + (size_t(op.pm1) * 3 * R + r) * 4096, op.c1, g1,
+                                 MSG ? op.m1 : -1, ts1, slot, R, r, l, p0, P, a1);
+      codon_child_into<T, true>(cla, g2 ? nullptr : pmpad + (size_t(op.pm2) * 3 * R + r) * 4096, op.c2, g2,
+                                MSG ? op.m2 : -1, ts2, slot, R, r, l, p0, P, a1);
 #pragma unroll
       for (int v = 0; v < 4; v++)
 #pragma unroll
         for (int t = 0; t < 4; t++) {
-          const double o = 16 * t + lo < 61 ? mul_nc(a1[t][v], a2[t][v]) : 0.0;
+          const double o = 16 * t + lo < 61 ? a1[t][v] : 0.0;
           ov[v][t] = o;
           mx[v] = fmax(mx[v], o);
         }
@@ -1083,7 +1136,13 @@ __global__ __launch_bounds__(64 * R, 3) void k_cla_codon(T *__restrict__ cla, in
           dst[(size_t(p) * R + r) * 64 + st] = out;
         }
       }
-      if (p < P && lo == 0 && r == 0) ua[size_t(op.dest) * Ppad + p] = u;
+      if (p < P && lo == 0 && r == 0) {
+        ua[size_t(op.dest) * Ppad + p] = u;
+        if constexpr (MSG) {  // a kept message carries its child's rescale counts
+          if (op.m1 >= 0 && op.c1 >= 0) ua[size_t(op.m1) * Ppad + p] = ua[size_t(op.c1) * Ppad + p];
+          if (op.m2 >= 0 && op.c2 >= 0) ua[size_t(op.m2) * Ppad + p] = ua[size_t(op.c2) * Ppad + p];
+        }
+      }
     }
   }
 }
@@ -1874,23 +1933,28 @@ static int launch_nuc_r(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
 }
 
 template <typename T, int R>
-static void launch_codon(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
-  GLE_LAUNCH(I, (k_cla_codon<T, R>), dim3(I->Ppad / 16, fused ? 1 : n), dim3(64 * R), 0,
-                     reinterpret_cast<T *>(I->d_cla), I->d_u, I->d_tips, I->d_pmpad, d_ops, I->d_flags, I->Ppad,
-                     I->P, fused ? n : 0);
+static void launch_codon(Instance *I, const DevClaOp *d_ops, int n, bool fused, bool msg) {
+  if (msg)
+    GLE_LAUNCH(I, (k_cla_codon<T, R, true>), dim3(I->Ppad / 16, fused ? 1 : n), dim3(64 * R), 0,
+               reinterpret_cast<T *>(I->d_cla), I->d_u, I->d_tips, I->d_pmpad, d_ops, I->d_flags, I->Ppad, I->P,
+               fused ? n : 0);
+  else
+    GLE_LAUNCH(I, (k_cla_codon<T, R, false>), dim3(I->Ppad / 16, fused ? 1 : n), dim3(64 * R), 0,
+               reinterpret_cast<T *>(I->d_cla), I->d_u, I->d_tips, I->d_pmpad, d_ops, I->d_flags, I->Ppad, I->P,
+               fused ? n : 0);
 }
 
 template <typename T>
-static int launch_codon_r(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
+static int launch_codon_r(Instance *I, const DevClaOp *d_ops, int n, bool fused, bool msg) {
   switch (I->R) {
-    case 1: launch_codon<T, 1>(I, d_ops, n, fused); break;
-    case 2: launch_codon<T, 2>(I, d_ops, n, fused); break;
-    case 3: launch_codon<T, 3>(I, d_ops, n, fused); break;
-    case 4: launch_codon<T, 4>(I, d_ops, n, fused); break;
-    case 5: launch_codon<T, 5>(I, d_ops, n, fused); break;
-    case 6: launch_codon<T, 6>(I, d_ops, n, fused); break;
-    case 7: launch_codon<T, 7>(I, d_ops, n, fused); break;
-    case 8: launch_codon<T, 8>(I, d_ops, n, fused); break;
+    case 1: launch_codon<T, 1>(I, d_ops, n, fused, msg); break;
+    case 2: launch_codon<T, 2>(I, d_ops, n, fused, msg); break;
+    case 3: launch_codon<T, 3>(I, d_ops, n, fused, msg); break;
+    case 4: launch_codon<T, 4>(I, d_ops, n, fused, msg); break;
+    case 5: launch_codon<T, 5>(I, d_ops, n, fused, msg); break;
+    case 6: launch_codon<T, 6>(I, d_ops, n, fused, msg); break;
+    case 7: launch_codon<T, 7>(I, d_ops, n, fused, msg); break;
+    case 8: launch_codon<T, 8>(I, d_ops, n, fused, msg); break;
     default: return set_error(GLE_ERR_UNSUPPORTED, "rate categories must be 1..8");
   }
   return GLE_SUCCESS;
@@ -1919,12 +1983,13 @@ static int launch_aa_r(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
   return GLE_SUCCESS;
 }
 
+// msg: a serving tick's ops, which may read and keep codon messages
 template <typename T>
-static int launch_cla(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
+static int launch_cla(Instance *I, const DevClaOp *d_ops, int n, bool fused, bool msg = false) {
   switch (I->S) {
     case 4: return launch_nuc_r<T>(I, d_ops, n, fused);
     case 20: return launch_aa_r<T>(I, d_ops, n, fused);
-    case 61: return launch_codon_r<T>(I, d_ops, n, fused);
+    case 61: return launch_codon_r<T>(I, d_ops, n, fused, msg);
     default: return set_error(GLE_ERR_UNSUPPORTED, "state count %d has no kernel", I->S);
   }
 }
@@ -2281,8 +2346,14 @@ static int tick_enqueue_part(Instance *X, int sidx, const int *pm_idx, const dou
   for (int i = 0; i < n_ops; i++) {
     const DevClaOp &o = ops[i];
     auto bad = [&](int c) { return c >= X->n_slots || (c < 0 && -c - 1 >= X->N); };
-    if (o.dest < 0 || o.dest >= X->n_slots || bad(o.c1) || bad(o.c2) || o.pm1 < 0 || o.pm1 >= X->n_pm ||
-        o.pm2 < 0 || o.pm2 >= X->n_pm || o.dest == o.c1 || o.dest == o.c2)
+    // a message child names a slot and needs no P(t); kept messages go to other slots
+    const bool g1 = o.flags & kOpMsg1, g2 = o.flags & kOpMsg2;
+    auto bad_m = [&](int m, int c) { return m != -1 && (m < 0 || m >= X->n_slots || m == o.dest || c < 0); };
+    if (o.dest < 0 || o.dest >= X->n_slots || bad(o.c1) || bad(o.c2) ||
+        (!g1 && (o.pm1 < 0 || o.pm1 >= X->n_pm)) || (!g2 && (o.pm2 < 0 || o.pm2 >= X->n_pm)) ||
+        (g1 && o.c1 < 0) || (g2 && o.c2 < 0) || o.dest == o.c1 || o.dest == o.c2 || bad_m(o.m1, o.c1) ||
+        bad_m(o.m2, o.c2) || (o.m1 >= 0 && (g1 || o.m1 == o.c2 || o.m1 == o.m2)) || (o.m2 >= 0 && (g2 || o.m2 == o.c1)) ||
+        ((g1 || g2 || o.m1 >= 0 || o.m2 >= 0) && X->S != 61))
       return set_error(GLE_ERR_ARG, "tick cla op %d out of range", i);
   }
   for (int i = 0; i < n_e; i++) {
@@ -2341,8 +2412,8 @@ static int tick_enqueue_part(Instance *X, int sidx, const int *pm_idx, const dou
     double bytes = 0.0;
     for (int i = b; i < e; i++) bytes += cla_op_bytes(X, ops[i]);
     timed_begin(X, 0, bytes);
-    rc = X->prec == GLE_PREC_FP64 ? launch_cla<double>(X, d_ops + b, e - b, false)
-                                  : launch_cla<float>(X, d_ops + b, e - b, false);
+    rc = X->prec == GLE_PREC_FP64 ? launch_cla<double>(X, d_ops + b, e - b, false, true)
+                                  : launch_cla<float>(X, d_ops + b, e - b, false, true);
     timed_end(X);
   }
   X->d_flags = flags_save;

@@ -37,14 +37,19 @@ struct RescaleTables {
 const RescaleTables &rescale_tables();
 
 // One CLA update as the device sees it (child < 0 means tip -(child+1)).
+// Codon instances only: with kOpMsg1 (kOpMsg2) set, c1 (c2) is the slot of a
+// stored message P(t) . x of that child instead of the child's CLA, and pm1
+// (pm2) is unused; m1 (m2) >= 0 asks the kernel to keep the message it forms
+// for child 1 (2) in that slot, with the child's rescale counts.
 struct DevClaOp {
   int dest, c1, pm1, c2, pm2, flags;
+  int m1 = -1, m2 = -1;
 };
 // One edge integration.
 struct DevEdgeOp {
   int partial, child, pm, derivs;
 };
-enum : int { kOpRescale = 1 };
+enum : int { kOpRescale = 1, kOpMsg1 = 2, kOpMsg2 = 4 };
 
 // Engine entry points used by the tree and serving layers (gle_engine.hip).
 struct Instance;

@@ -16,6 +16,15 @@
 //    device slot.  This generalises ClaManager's holder sharing
 //    (clamanager.h:96-441): a changed branch changes the ids of exactly the
 //    CLAs SweepDirtynessOverTree would dirty.
+//  * Message cache (codon instances).  A CLA is the product of its two
+//    children's messages P(t) . x; a message is named by (child subtree id,
+//    branch-length bits) and so does not depend on where the child hangs.
+//    The CLA kernel keeps the messages it forms in free slots, and a later CLA
+//    whose child has a kept message reads it instead of contracting the child
+//    again: a CLA on the path of a move has one child off the path, and a
+//    subtree an NNI or SPR moved keeps its message, so most new codon CLAs of
+//    an offspring need one 61 x 61 contraction per category or none instead
+//    of two.  Messages take only free slots and are evicted before CLAs.
 //  * P(t) cache keyed by branch-length bits (P, P', P'' per slot).
 //  * Admission by HBM budget: a request is admitted when its working set fits
 //    in the instance's CLA slots alongside the requests already running;
@@ -79,6 +88,9 @@ static SubKey sub_key(uint64_t ix, double lx, uint64_t iy, double ly) {
   }
   return k;
 }
+// The exact name of a message P(t) . x: (0, 0, child id, length bits), which
+// no subtree name equals (a subtree has two nonzero child ids).
+static SubKey msg_key(uint64_t ic, double len) { return SubKey{0, 0, ic, dbits(len)}; }
 // Internal subtree ids start above every tip number.
 constexpr uint64_t kFirstSubtreeId = uint64_t(1) << 40;
 // Interned names are kept while their CLAs come and go; past this many the
@@ -192,7 +204,9 @@ struct ServeInstance {
   NameTable intern;  // subtree name -> id
   uint64_t next_id = kFirstSubtreeId;
   LruCache cla;  // subtree id -> CLA slot
+  LruCache msg;  // message name -> slot holding P(t) . x of that child (rank = the child's)
   LruCache pm;   // blen bits -> P, P', P'' slot
+  bool msgs = false;  // codon instance: keep and read messages
   int reserved = 0;                              // working-set slots of admitted requests
   unsigned long long gen = 0;                    // instance generation the caches belong to
   unsigned long long epoch = 0;                  // bumped whenever the CLA cache is dropped
@@ -200,11 +214,13 @@ struct ServeInstance {
 
   uint64_t intern_id(const SubKey &k) { return intern.intern(k, next_id); }
   uint64_t find_id(const SubKey &k) const { return intern.find(k); }
-  // every cached CLA back to the pool (caller holds pool_mutex)
+  // every cached CLA and message back to the pool (caller holds pool_mutex)
   void drop_clas(ClaPool *pool) {
-    cla.clear([pool](int s) {
+    auto release = [pool](int s) {
       if (--pool->refs[s] == 0) pool->free_slot(s);
-    });
+    };
+    cla.clear(release);
+    msg.clear(release);
   }
   void drop_pmats(ClaPool *pool) {
     pm.clear([pool](int s) { pool->pm_free.push_back(s); });
@@ -261,6 +277,9 @@ struct Server {
   // where tick time goes: building op lists on the host, and from the first
   // launch until the results are back
   double host_ms = 0.0, device_ms = 0.0;
+  // CLA work: ops, children contracted on the device (internal children not
+  // read from a message), messages read and kept
+  long long n_ops = 0, n_contract = 0, n_msg_read = 0, n_msg_keep = 0;
 };
 
 static std::mutex g_servers_mu;
@@ -340,11 +359,23 @@ struct TickBuilder {
   int err = GLE_SUCCESS;
   std::vector<char> pm_new;  // per pm_idx entry: 1 a new slot, 0 a cached one refreshed with P', P''
   bool derivs = false;       // the tick computes P' and P'' as well
+  // per op (parallel to ops): each child contracted on the device, named by
+  // (id, length) with its rescale rank; id 0 = a tip or a message read
+  struct Kid {
+    uint64_t id = 0;
+    double len = 0.0;
+    int rank = 0;
+    bool cached = false;  // its CLA was in the cache before this tick
+  };
+  std::vector<std::array<Kid, 2>> kids;
+  std::unordered_map<uint64_t, int> uses;  // per subtree id: children contracted from it this tick
+  std::vector<std::pair<uint64_t, TickNew>> fresh_msgs;  // messages kept this tick
 
   struct Mark {
     size_t fresh, ops, pm;
   };
   Mark mark() const { return {fresh_order.size(), ops.size(), pm_idx.size()}; }
+
   // undo everything one request added since m: its new CLA slots and P(t)
   // slots go back to the pool
   void rollback(const Mark &m) {
@@ -356,6 +387,7 @@ struct TickBuilder {
     }
     fresh_order.resize(m.fresh);
     ops.resize(m.ops);
+    kids.resize(m.ops);
     for (size_t i = m.pm; i < pm_idx.size(); i++) {
       pm_fresh.erase(dbits(pm_len[i]));
       if (pm_new[i]) pool->pm_free.push_back(pm_idx[i]);
@@ -365,17 +397,73 @@ struct TickBuilder {
     pm_new.resize(m.pm);
     err = GLE_SUCCESS;
   }
-  // every new CLA and P(t) slot of the tick back to the pool (a failed tick)
-  void release_all() { rollback(Mark{0, 0, 0}); }
+  // every new CLA, message and P(t) slot of the tick back to the pool (a failed tick)
+  void release_all() {
+    for (auto &fm : fresh_msgs)
+      if (--pool->refs[fm.second.slot] == 0) pool->free_slot(fm.second.slot);
+    fresh_msgs.clear();
+    rollback(Mark{0, 0, 0});
+  }
 
   int take_slot() {
-    while (!pool->has_free()) {  // evict least recently used entries not used this tick
-      const int s = si->cla.evict(tick);
+    while (!pool->has_free()) {  // evict least recently used entries not used this tick, messages first
+      int s = si->msg.evict(tick);
+      if (s < 0) s = si->cla.evict(tick);
       if (s < 0) return -1;
       if (--pool->refs[s] == 0) pool->free_slot(s);
     }
     const int s = pool->take_slot();
     pool->refs[s] = 1;
+    return s;
+  }
+
+  // After every request of the tick is laid out: a child the tick contracts
+  // keeps its message in a free slot (or one an unused message frees) when
+  // its subtree is shared -- cached before this tick (the unmodified part of
+  // a parent), or contracted more than once in it -- and the message is not
+  // kept already.  An offspring's own new subtrees are seldom met again, and
+  // their messages would only cost writes.  Messages never take a slot from
+  // a CLA, so admission's working sets are unaffected.
+  void keep_messages() {
+    std::unordered_map<uint64_t, int> kept;
+    for (size_t i = 0; i < ops.size(); i++)
+      for (int k = 0; k < 2; k++) {
+        const Kid &kd = kids[i][k];
+        if (!kd.id || !(kd.cached || uses[kd.id] > 1)) continue;
+        const uint64_t name = si->intern_id(msg_key(kd.id, kd.len));
+        if (si->msg.has(name) || kept.count(name)) continue;
+        if (!pool->has_free()) {
+          const int v = si->msg.evict(tick);
+          if (v < 0) return;
+          if (--pool->refs[v] == 0) pool->free_slot(v);
+          if (!pool->has_free()) return;
+        }
+        const int s = pool->take_slot();
+        pool->refs[s] = 1;
+        (k ? ops[i].second.m2 : ops[i].second.m1) = s;
+        kept[name] = s;
+        fresh_msgs.push_back({name, TickNew{s, kd.rank, 0}});
+      }
+  }
+
+  // A child of a new CLA: a kept message of it (codon instances), else its
+  // CLA slot or tip code as resolve() gives them
+  int child(Req &r, int c, int a, int &rank, int &level, bool &msg) {
+    msg = false;
+    if (si->msgs && !req_tip(r, c)) {
+      const uint64_t ic = known_id(r, *si, c, a);
+      const uint64_t name = ic ? si->find_id(msg_key(ic, req_len(r, a, c))) : 0;
+      if (name)
+        if (CacheEntry *e = si->msg.find(name)) {
+          si->msg.touch(e, tick);
+          rank = e->rank;
+          level = -1;
+          msg = true;
+          return e->slot;
+        }
+    }
+    const int s = resolve(r, c, a, rank, level);
+    if (!err && !req_tip(r, c)) uses[r.sid[c][req_nbk(r, c, a)]]++;
     return s;
   }
 
@@ -439,26 +527,28 @@ struct TickBuilder {
     int x, y;
     req_children(r, a, b, x, y);
     int rx, ry, lx, ly;
+    bool g1, g2;
     DevClaOp op;
-    op.c1 = resolve(r, x, a, rx, lx);
+    op.c1 = child(r, x, a, rx, lx, g1);
     if (err) return 0;
-    op.c2 = resolve(r, y, a, ry, ly);
+    op.c2 = child(r, y, a, ry, ly, g2);
     if (err) return 0;
     const uint64_t ix = req_tip(r, x) ? uint64_t(x) : r.sid[x][req_nbk(r, x, a)];
     const uint64_t iy = req_tip(r, y) ? uint64_t(y) : r.sid[y][req_nbk(r, y, a)];
-    const uint64_t id = si->intern_id(sub_key(ix, req_len(r, a, x), iy, req_len(r, a, y)));
+    const double lenx = req_len(r, a, x), leny = req_len(r, a, y);
+    const uint64_t id = si->intern_id(sub_key(ix, lenx, iy, leny));
     r.sid[a][req_nbk(r, a, b)] = id;
-    op.pm1 = pm_slot(req_len(r, a, x));
-    op.pm2 = op.pm1 < 0 ? -1 : pm_slot(req_len(r, a, y));
+    op.pm1 = g1 ? 0 : pm_slot(lenx);  // a message child needs no P(t)
+    op.pm2 = op.pm1 < 0 ? -1 : g2 ? 0 : pm_slot(leny);
     const int slot = op.pm2 < 0 ? -1 : take_slot();
     if (slot < 0) {
       err = set_error(GLE_ERR_NOMEM, "serving cache exhausted: raise n_cla_slots / n_pmat_slots");
       return 0;
     }
     rank = 2 + rx + ry;
-    op.flags = 0;
+    op.flags = (g1 ? kOpMsg1 : 0) | (g2 ? kOpMsg2 : 0);
     if (rank >= si->rescale_every) {
-      op.flags = kOpRescale;
+      op.flags |= kOpRescale;
       rank = 0;
     }
     op.dest = slot;
@@ -466,6 +556,10 @@ struct TickBuilder {
     fresh[id] = {slot, rank, level};
     fresh_order.push_back(id);
     ops.push_back({level, op});
+    std::array<Kid, 2> kd;
+    if (!g1 && !req_tip(r, x)) kd[0] = Kid{ix, lenx, rx, lx < 0 && !fresh.count(ix)};
+    if (!g2 && !req_tip(r, y)) kd[1] = Kid{iy, leny, ry, ly < 0 && !fresh.count(iy)};
+    kids.push_back(kd);
     return slot;
   }
 };
@@ -586,8 +680,13 @@ struct Flight {
 // the part already holds waits for a later tick, unchanged and holding
 // nothing; one that does not fit on its own fails.  The part's new CLAs and
 // P(t) enter the caches at once, so the next tick can use them.
+// CLA work of the ticks built, published under Server::mu (see Server)
+struct TickWork {
+  long long ops = 0, contract = 0, msg_read = 0, msg_keep = 0;
+};
+
 static int build_part(Server &sv, ServeInstance &si, std::vector<Req *> &batch, uint64_t tick, int stage,
-                      TickPart &part, double &host_ms) {
+                      TickPart &part, double &host_ms, TickWork &work) {
   using clock = std::chrono::steady_clock;
   const auto t_build = clock::now();
   std::vector<DevClaOp> ops;
@@ -646,6 +745,14 @@ static int build_part(Server &sv, ServeInstance &si, std::vector<Req *> &batch, 
     host_ms += std::chrono::duration<double, std::milli>(clock::now() - t_build).count();
     return deferred;
   }
+  if (si.msgs) tb.keep_messages();
+  for (size_t i = 0; i < tb.ops.size(); i++) {
+    const DevClaOp &o = tb.ops[i].second;
+    work.ops++;
+    work.contract += (tb.kids[i][0].id != 0) + (tb.kids[i][1].id != 0);
+    work.msg_read += ((o.flags & kOpMsg1) != 0) + ((o.flags & kOpMsg2) != 0);
+    work.msg_keep += (o.m1 >= 0) + (o.m2 >= 0);
+  }
   std::stable_sort(tb.ops.begin(), tb.ops.end(),
                    [](const std::pair<int, DevClaOp> &x, const std::pair<int, DevClaOp> &y) { return x.first < y.first; });
   int cur = -1;
@@ -674,6 +781,7 @@ static int build_part(Server &sv, ServeInstance &si, std::vector<Req *> &batch, 
       si.cla.put(id, f.slot, f.rank, tick);
       part.fresh_ids.push_back(id);
     }
+    for (auto &fm : tb.fresh_msgs) si.msg.put(fm.first, fm.second.slot, fm.second.rank, tick);
     part.n_edges = int(eops.size());
     for (Req *r : part.reqs) r->in_flight = true;
   }
@@ -811,6 +919,7 @@ static void worker_loop(Server *sv) {
       return;
     }
     double host_ms = 0.0, device_ms = 0.0;
+    TickWork work;
     long long in_tick = 0;
     int deferred = 0;
     if (!by_inst.empty()) {
@@ -823,7 +932,7 @@ static void worker_loop(Server *sv) {
         std::stable_sort(kv.second.begin(), kv.second.end(),
                          [](const Req *a, const Req *b) { return a->waits > b->waits; });
         TickPart part;
-        deferred += build_part(*sv, *kv.first, kv.second, f.tick, f.stage, part, host_ms);
+        deferred += build_part(*sv, *kv.first, kv.second, f.tick, f.stage, part, host_ms, work);
         in_tick += long(part.reqs.size());
         if (!part.reqs.empty()) f.parts.push_back(std::move(part));
       }
@@ -842,6 +951,10 @@ static void worker_loop(Server *sv) {
       std::lock_guard<std::mutex> lk(sv->mu);
       if (in_tick) sv->ticks++;
       sv->host_ms += host_ms;
+      sv->n_ops += work.ops;
+      sv->n_contract += work.contract;
+      sv->n_msg_read += work.msg_read;
+      sv->n_msg_keep += work.msg_keep;
       sv->device_ms += device_ms;
       sv->batched_requests += in_tick;
       sv->deferred += deferred;
@@ -892,6 +1005,19 @@ extern "C" int gle_serve_timing(int h, double *host_ms, double *device_ms, int r
   *host_ms = sv->host_ms;
   *device_ms = sv->device_ms;
   if (reset) sv->host_ms = sv->device_ms = 0.0;
+  return GLE_SUCCESS;
+}
+
+extern "C" int gle_serve_work(int h, long long *cla_ops, long long *contractions, long long *msg_reads,
+                              long long *msg_keeps, int reset) {
+  Server *sv = get_server(h);
+  if (!sv || !cla_ops || !contractions || !msg_reads || !msg_keeps) return set_error(GLE_ERR_ARG, "bad argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  *cla_ops = sv->n_ops;
+  *contractions = sv->n_contract;
+  *msg_reads = sv->n_msg_read;
+  *msg_keeps = sv->n_msg_keep;
+  if (reset) sv->n_ops = sv->n_contract = sv->n_msg_read = sv->n_msg_keep = 0;
   return GLE_SUCCESS;
 }
 
@@ -973,6 +1099,7 @@ static void enqueue_requests(Server &sv, int inst, Instance *I, std::vector<std:
     si.I = I;
     si.rescale_every = instance_precision(I) == GLE_PREC_FP64 ? 16 : 1;
     si.gen = instance_generation(I);
+    si.msgs = instance_states(I) == 61;
   }
   const auto now = std::chrono::steady_clock::now();
   for (size_t k = 0; k < rs.size(); k++) {

@@ -144,6 +144,7 @@ SIGNATURES = {
     "gle_serve_stats": ([C.c_int, C.POINTER(C.c_longlong), _D, _D, _D, C.c_int], C.c_int),
     "gle_serve_set_batch_window": ([C.c_int, C.c_int], C.c_int),
     "gle_serve_timing": ([C.c_int, _D, _D, C.c_int], C.c_int),
+    "gle_serve_work": ([C.c_int] + [C.POINTER(C.c_longlong)] * 4 + [C.c_int], C.c_int),
 }
 
 
@@ -933,6 +934,13 @@ class Server:
         a, b = C.c_double(0), C.c_double(0)
         _check(lib().gle_serve_timing(self.h, C.byref(a), C.byref(b), int(reset)))
         return {"host_ms": a.value, "device_ms": b.value}
+
+    def work(self, reset=False):
+        """CLA work of the ticks so far: ops, children contracted on the device,
+        children read from kept messages (codon instances) and messages kept."""
+        v = [C.c_longlong(0) for _ in range(4)]
+        _check(lib().gle_serve_work(self.h, *[C.byref(x) for x in v], int(reset)))
+        return dict(zip(("cla_ops", "contractions", "msg_reads", "msg_keeps"), [x.value for x in v]))
 
     def set_batch_window(self, microseconds):
         """An idle server waits this long for a burst of submissions to gather."""

@@ -154,6 +154,54 @@ def test_serving_state_spaces(gpu, kind):
     srv.close()
 
 
+def test_codon_messages_are_kept_and_read(gpu):
+    """Codon serving keeps a contracted child's message P(t) . x when its
+    subtree is shared, and later CLAs read it instead of contracting the child
+    again (gle_serve_work counts both).  Results with messages equal the FP64
+    tree layer to 1e-12 relative (FP64 messages are the exact products) and
+    the oracle to 1e-9; the FP32 form stays within 1e-3 of FP64.  A nucleotide
+    instance keeps no messages."""
+    ds = sim.codon_m3(n_tips=12, n_patterns=240, seed=131)
+    pop = _population(ds, 20, 17)
+    o = O.Oracle(ds)
+    ref64 = []
+    inst64 = ds.make_instance(E.PREC_FP64)
+    for p, b in pop:
+        t = E.Tree(inst64, p, b)
+        ref64.append(t.score())
+        t.close()
+    inst64.close()
+    for prec in (E.PREC_FP64, E.PREC_FP32):
+        inst = ds.make_instance(prec, n_cla_slots=400, n_pmat_slots=600)
+        srv = E.Server()
+        srv.wait(srv.submit(inst, ds.parent, ds.blen))  # the parent's CLAs, cached
+        first = [srv.wait(i)[0] for i in srv.submit_batch(inst, pop[:10])]
+        second = [srv.wait(i)[0] for i in srv.submit_batch(inst, pop[10:])]
+        got = first + second
+        w = srv.work()
+        assert w["msg_keeps"] > 0 and w["msg_reads"] > 0, w
+        assert w["contractions"] + w["msg_reads"] > w["msg_reads"]
+        for k, (p, b) in enumerate(pop):
+            if prec == E.PREC_FP64:
+                assert abs(got[k] - ref64[k]) <= 1e-12 * abs(ref64[k]), (k, got[k], ref64[k])
+            else:
+                assert abs(got[k] - ref64[k]) < 1e-3, (k, got[k], ref64[k])
+        for (p, b), g in zip(pop[::5], got[::5]):
+            ref = o.score(parent=p, blen=b)
+            assert abs(g - ref) < (1e-9 * abs(ref) if prec == E.PREC_FP64 else 1e-3)
+        srv.close()
+        inst.close()
+    nuc = sim.gtr_gamma(n_tips=12, n_sites=400, seed=132)
+    inst = nuc.make_instance(E.PREC_FP32, n_cla_slots=300, n_pmat_slots=400)
+    srv = E.Server()
+    npop = _population(nuc, 8, 18)
+    [srv.wait(i) for i in srv.submit_batch(inst, npop)]
+    w = srv.work()
+    assert w["cla_ops"] > 0 and w["msg_keeps"] == 0 and w["msg_reads"] == 0, w
+    srv.close()
+    inst.close()
+
+
 def test_batch_submit_equals_single_submits(gpu):
     """gle_serve_submit_batch queues a generation's offspring in one call; the
     results are those of one-by-one submissions, and a batch holding an
