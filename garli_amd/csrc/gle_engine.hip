@@ -1012,15 +1012,15 @@ __device__ __forceinline__ void codon_child_into(T *__restrict__ cla, const doub
       for (int q = 0; q < 16; q++) x[q] = T(0);
     }
   }
-  double b[2][16];
   const double *B = Pr + l;
 #pragma unroll
   for (int t = 0; t < 4; t++) {
+    double b[16];
 #pragma unroll
-      for (int kk = 0; kk < 16; kk++) b[t & 1][kk] = B[t * 1024 + kk * 64];
+    for (int kk = 0; kk < 16; kk++) b[kk] = B[t * 1024 + kk * 64];
     gle_doublex4 d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int kk = 0; kk < 16; kk++) d = __builtin_amdgcn_mfma_f64_16x16x4f64(double(x[kk]), b[t & 1][kk], d, 0, 0, 0);
+    for (int kk = 0; kk < 16; kk++) d = __builtin_amdgcn_mfma_f64_16x16x4f64(double(x[kk]), b[kk], d, 0, 0, 0);
     if (mkeep >= 0) {
       T *dst = cla + mkeep * slot;
 #pragma unroll
