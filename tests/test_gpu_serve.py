@@ -202,6 +202,36 @@ def test_codon_messages_are_kept_and_read(gpu):
     inst.close()
 
 
+@pytest.mark.parametrize("slots", [10, 24, 60])
+def test_codon_messages_under_slot_pressure(gpu, slots):
+    """Codon serving with few CLA slots: kept messages are evicted before CLAs
+    and their slots reused while later requests read other messages, over
+    several rounds of the same population; every FP64 result still equals
+    the tree layer's to 1e-12 relative (a stale message would not)."""
+    ds = sim.codon_m3(n_tips=12, n_patterns=200, seed=133)
+    pop = _population(ds, 16, 19)
+    inst64 = ds.make_instance(E.PREC_FP64)
+    ref = []
+    for p, b in pop:
+        t = E.Tree(inst64, p, b)
+        ref.append(t.score())
+        t.close()
+    inst64.close()
+    inst = ds.make_instance(E.PREC_FP64, n_cla_slots=slots, n_pmat_slots=400)
+    srv = E.Server()
+    srv.wait(srv.submit(inst, ds.parent, ds.blen))
+    for rnd in range(3):
+        got = [srv.wait(i)[0] for i in srv.submit_batch(inst, pop[rnd::2] if rnd < 2 else pop)]
+        want = ref[rnd::2] if rnd < 2 else ref
+        for g, w in zip(got, want):
+            assert abs(g - w) <= 1e-12 * abs(w), (slots, rnd, g, w)
+    w = srv.work()
+    if slots >= 60:  # with a little room, messages survive to be read
+        assert w["msg_reads"] > 0, w
+    srv.close()
+    inst.close()
+
+
 def test_batch_submit_equals_single_submits(gpu):
     """gle_serve_submit_batch queues a generation's offspring in one call; the
     results are those of one-by-one submissions, and a batch holding an
