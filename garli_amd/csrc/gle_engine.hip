@@ -133,6 +133,7 @@ struct Instance {
   int pm_cap = 0;
   double *h_stage = nullptr;  // pinned
   size_t h_stage_cap = 0;
+  int *h_flag = nullptr;  // pinned: the rescale flags read back with an edge evaluation's results
   double pinv = 0.0;
   int pinv_in_model = 0, last_const = -1;
   // kernel timing (HIP events on this instance's stream): 0 CLA, 1 edge, 2 pmat
@@ -1901,12 +1902,15 @@ static int enqueue_pmats(Instance *I, const int *idx, const double *t, int n, in
   return GLE_SUCCESS;
 }
 
-// The staging buffers are reused by the next call only after the streams drain.
-int engine_update_pmats(Instance *I, const int *idx, const double *t, int n, int derivs) {
+// wait = false queues the work and returns: the caller's next evaluation on
+// the same streams waits for it (the tree layer then synchronises once, with
+// the edge results).  The device copies of idx and t are stream-ordered, so a
+// later call may reuse them at once.
+int engine_update_pmats(Instance *I, const int *idx, const double *t, int n, int derivs, bool wait) {
   std::lock_guard<std::mutex> lk_(I->mu);
   int rc = enqueue_pmats(I, idx, t, n, derivs);
   if (!rc && I->heavy) rc = enqueue_pmats(I->heavy.get(), idx, t, n, derivs);
-  return finish_both(I, rc);
+  return wait ? finish_both(I, rc) : rc;
 }
 
 template <typename T, int R>
@@ -2113,13 +2117,14 @@ static int enqueue_partials(Instance *I, const DevClaOp *ops, int n_ops, const i
   return GLE_SUCCESS;
 }
 
+// wait = false: queued only, as engine_update_pmats
 int engine_update_partials(Instance *I, const DevClaOp *ops, int n_ops, const int *level_start, int n_levels,
-                           bool fused) {
+                           bool fused, bool wait) {
   std::lock_guard<std::mutex> lk_(I->mu);
   // the bulk first: its stream starts working while the small part is issued
   int rc = enqueue_partials(I, ops, n_ops, level_start, n_levels, fused);
   if (!rc && I->heavy) rc = enqueue_partials(I->heavy.get(), ops, n_ops, level_start, n_levels, fused);
-  return finish_both(I, rc);
+  return wait ? finish_both(I, rc) : rc;
 }
 
 // derivs: some op of the launch wants (d1, d2); the nucleotide kernel then
@@ -2302,23 +2307,48 @@ static int copy_edges_out(Instance *I, int n, double *out, double *site, int nse
 // Edge results of an instance and its heavy-pattern part; with site (nsec x
 // patterns, the caller's pattern order) also the per-pattern values of the
 // ops' first edge: its lnL (nsec = 1) or all five sections (nsec = 5).
-static int edges_impl(Instance *I, const DevEdgeOp *ops, int n, double *out, double *site, int nsec) {
+// Pinned room for n results (3 doubles each) of an instance, grown on demand
+// (no copy of the instance is in flight: every evaluation waits before returning).
+static int stage_results(Instance *X, int n) {
+  const size_t need = 3 * size_t(n);
+  if (need <= X->h_stage_cap) return GLE_SUCCESS;
+  if (X->h_stage) GLE_HIP(hipHostFree(X->h_stage));
+  X->h_stage = nullptr;
+  X->h_stage_cap = 0;
+  const size_t cap = std::max(need, size_t(1024));
+  GLE_HIP(hipHostMalloc(reinterpret_cast<void **>(&X->h_stage), sizeof(double) * cap));
+  X->h_stage_cap = cap;
+  return GLE_SUCCESS;
+}
+
+static int edges_impl(Instance *I, const DevEdgeOp *ops, int n, double *out, double *site, int nsec,
+                      int *flags) {
   std::lock_guard<std::mutex> lk_(I->mu);
   if (n <= 0) return GLE_SUCCESS;
   Instance *H = I->heavy.get();
   const bool scatter = site && H;  // the two parts' patterns go back to the caller's order
   std::vector<double> site_l(scatter ? nsec * size_t(std::max(I->P, 1)) : 0),
       site_h(scatter ? nsec * size_t(std::max(H->P, 1)) : 0);
-  std::vector<double> out_h(H ? 3 * size_t(n) : 0, 0.0);
-  // kernels of both parts first; the copies to host memory may wait for them
+  // kernels of both parts first; the results come back through pinned memory,
+  // so the copies queue behind them without a wait each (a copy into pageable
+  // memory had cost a host round trip per part)
   int rc = enqueue_edges(I, ops, n, site != nullptr);
   if (!rc && H) rc = enqueue_edges(H, ops, n, site != nullptr);
-  if (!rc) rc = copy_edges_out(I, n, out, site ? (scatter ? site_l.data() : site) : nullptr, nsec);
-  if (!rc && H) rc = copy_edges_out(H, n, out_h.data(), site ? site_h.data() : nullptr, nsec);
+  if (!rc) rc = stage_results(I, n);
+  if (!rc && H) rc = stage_results(H, n);
+  if (!rc) rc = copy_edges_out(I, n, I->h_stage, site ? (scatter ? site_l.data() : site) : nullptr, nsec);
+  if (!rc && H) rc = copy_edges_out(H, n, H->h_stage, site ? site_h.data() : nullptr, nsec);
+  // the rescale flags of both parts come back with the results (one wait)
+  if (!rc && flags) rc = hipMemcpyAsync(I->h_flag, I->d_flags, sizeof(int), hipMemcpyDeviceToHost, I->stream) ==
+                                 hipSuccess ? GLE_SUCCESS : set_error(GLE_ERR_DEVICE, "flag copy failed");
+  if (!rc && flags && H)
+    rc = hipMemcpyAsync(H->h_flag, H->d_flags, sizeof(int), hipMemcpyDeviceToHost, H->stream) == hipSuccess
+             ? GLE_SUCCESS
+             : set_error(GLE_ERR_DEVICE, "flag copy failed");
   rc = finish_both(I, rc);
   if (rc) return rc;
-  if (H)
-    for (int i = 0; i < 3 * n; i++) out[i] += out_h[i];
+  if (flags) *flags = *I->h_flag | (H ? *H->h_flag : 0);
+  for (int i = 0; i < 3 * n; i++) out[i] = I->h_stage[i] + (H ? H->h_stage[i] : 0.0);
   if (scatter) {
     const size_t Pt = I->light_idx.size() + I->heavy_idx.size();
     for (int k = 0; k < nsec; k++) {
@@ -2499,12 +2529,12 @@ int engine_tick_collect(Instance *I, int sidx, double *res, int *flags) {
   return rc;
 }
 
-int engine_edges(Instance *I, const DevEdgeOp *ops, int n, double *out, double *site_lnl) {
-  return edges_impl(I, ops, n, out, site_lnl, 1);
+int engine_edges(Instance *I, const DevEdgeOp *ops, int n, double *out, double *site_lnl, int *flags) {
+  return edges_impl(I, ops, n, out, site_lnl, 1, flags);
 }
 
-int engine_edge_sites(Instance *I, const DevEdgeOp &op, double *out3, double *site5) {
-  return edges_impl(I, &op, 1, out3, site5, 5);
+int engine_edge_sites(Instance *I, const DevEdgeOp &op, double *out3, double *site5, int *flags) {
+  return edges_impl(I, &op, 1, out3, site5, 5, flags);
 }
 
 // Rescale flags of an instance and its heavy part.  flags may be null (reset
@@ -2599,6 +2629,7 @@ static int destroy(Instance *I) {
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (I->h_stage) (void)hipHostFree(I->h_stage);
+  if (I->h_flag) (void)hipHostFree(I->h_flag);
   for (auto &st : I->stage) {
     if (st.h) (void)hipHostFree(st.h);
     if (st.d) (void)hipFree(st.d);
@@ -2663,6 +2694,8 @@ static int create_internal(const gle_instance_desc *d, std::unique_ptr<Instance>
   } while (0)
   if (hipStreamCreateWithFlags(&I->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(set_error(GLE_ERR_DEVICE, "stream creation failed"));
+  if (hipHostMalloc(reinterpret_cast<void **>(&I->h_flag), 64) != hipSuccess)
+    return fail(set_error(GLE_ERR_NOMEM, "pinned allocation failed"));
   GLE_ALLOC(I->d_tips, size_t(I->N) * I->Ppad);
   GLE_ALLOC(I->d_counts, sizeof(int) * I->Ppad);
   GLE_ALLOC(I->d_cst, sizeof(int) * I->Ppad);

@@ -63,14 +63,18 @@ int instance_precision(const Instance *);
 unsigned long long instance_generation(const Instance *);
 int instance_cla_slots(const Instance *);
 int instance_pmat_slots(const Instance *);
-int engine_update_pmats(Instance *, const int *idx, const double *t, int n, int derivs);
+// wait = false: queue the work and return (the next evaluation on the
+// instance's streams runs after it; errors of the queued kernels surface at
+// that evaluation's wait)
+int engine_update_pmats(Instance *, const int *idx, const double *t, int n, int derivs, bool wait = true);
 // ops in level order; level_start[n_levels+1]; fused = one launch for all ops
 int engine_update_partials(Instance *, const DevClaOp *ops, int n_ops, const int *level_start, int n_levels,
-                           bool fused);
-int engine_edges(Instance *, const DevEdgeOp *ops, int n, double *out, double *site_lnl);
+                           bool fused, bool wait = true);
+// flags (optional): the rescale flags, read back with the results in the same wait
+int engine_edges(Instance *, const DevEdgeOp *ops, int n, double *out, double *site_lnl, int *flags = nullptr);
 // one edge with its per-pattern values: site5 = 5 x patterns (lnL, d1, d2,
 // partial-side and child-side rescale counts, -1 for a tip child)
-int engine_edge_sites(Instance *, const DevEdgeOp &op, double *out3, double *site5);
+int engine_edge_sites(Instance *, const DevEdgeOp &op, double *out3, double *site5, int *flags = nullptr);
 int engine_flags(Instance *, int *flags, int reset);
 // Staging areas per instance for serving ticks in flight at once.
 constexpr int kTickStages = 3;

@@ -83,6 +83,13 @@ static void dirty_all(Tree &t) {
     if (!is_tip(t, a))
       for (int k = 0; k < t.nnb[a]; k++) release(t, a, k);
 }
+// After a device error reported at a wait: nothing queued before it counts as
+// computed, so every CLA and P(t) slot of the tree is recomputed next time.
+static int forget_device_state(Tree &t, int rc) {
+  dirty_all(t);
+  std::fill(t.pm_len.begin(), t.pm_len.end(), std::nan(""));
+  return rc;
+}
 // SweepDirtynessOverTree: every X(x|y) whose side holds edge (n, parent(n))
 static void dirty_from(Tree &t, int from, int came) {
   for (int j = 0; j < t.nnb[from]; j++) {
@@ -184,7 +191,10 @@ static int collect(Tree &t, int a, int b, std::vector<std::array<int, 3>> &want)
 }
 
 // Pmat bookkeeping: make sure the given edges hold P (and P', P'' if derivs).
-static int refresh_pmats(Tree &t, const std::vector<int> &edges, bool derivs) {
+// wait = false: queued only (see engine_update_pmats); the slots count as
+// current at once, and forget_device_state() takes that back if the wait
+// that follows reports an error.
+static int refresh_pmats(Tree &t, const std::vector<int> &edges, bool derivs, bool wait = true) {
   std::vector<int> idx, nodes;
   std::vector<double> len;
   for (int n : edges) {
@@ -195,7 +205,7 @@ static int refresh_pmats(Tree &t, const std::vector<int> &edges, bool derivs) {
     len.push_back(t.blen[n]);
   }
   if (idx.empty()) return GLE_SUCCESS;
-  int rc = engine_update_pmats(t.I, idx.data(), len.data(), int(idx.size()), derivs ? 1 : 0);
+  int rc = engine_update_pmats(t.I, idx.data(), len.data(), int(idx.size()), derivs ? 1 : 0, wait);
   if (rc) return rc;
   for (int n : nodes) {
     t.pm_len[n] = t.blen[n];
@@ -207,8 +217,9 @@ static int refresh_pmats(Tree &t, const std::vector<int> &edges, bool derivs) {
 // Compute the listed directional CLAs (already in dependency order with levels).
 // On any failure the slots handed out here go back to the pool and the CLAs
 // stay dirty, so a later call recomputes them; the levels collect() recorded
-// are cleared on every path.
-static int run_ops(Tree &t, std::vector<std::array<int, 3>> &want) {
+// are cleared on every path.  wait = false only queues the updates (the edge
+// evaluation that follows on the same streams waits for them, once).
+static int run_ops(Tree &t, std::vector<std::array<int, 3>> &want, bool wait = true) {
   if (want.empty()) return GLE_SUCCESS;
   std::vector<std::array<int, 2>> assigned;  // (a, k) whose slot this call set
   auto fail = [&](int rc) {
@@ -279,7 +290,7 @@ static int run_ops(Tree &t, std::vector<std::array<int, 3>> &want) {
   }
   level_start.push_back(int(ops.size()));
   rc = engine_update_partials(t.I, ops.data(), int(ops.size()), level_start.data(), int(level_start.size()) - 1,
-                              fused);
+                              fused, wait);
   if (rc) return fail(rc);
   for (auto &w : want) t.level[size_t(w[0]) * 3 + w[1]] = -1;
   t.cla_updates += static_cast<long long>(ops.size());
@@ -293,26 +304,27 @@ static int child_code(Tree &t, int c, int a) {
 
 // One attempt of an edge evaluation.  Returns GLE_RESCALE_RETRY when the
 // reference would have thrown from RescaleRateHet.
+// The CLA updates, the edge's P(t) and the edge are queued back to back and
+// waited for once, with the rescale flags read in the same wait (a wait per
+// step had left the device idle between them).
 static int edge_attempt(Tree &t, int pa, int pb, bool derivs, double *res, double *site, double *site5) {
-  int flags = 0, rc = engine_flags(t.I, nullptr, 1);
+  int flags = 0, rc = engine_flags(t.I, nullptr, 1);  // reset, queued ahead of the updates
   if (rc) return rc;
   std::vector<std::array<int, 3>> want;  // collect() levels are cleared by run_ops on every path
   collect(t, pa, pb, want);
   collect(t, pb, pa, want);
-  rc = run_ops(t, want);
-  if (rc) return rc;
+  rc = run_ops(t, want, false);
+  if (rc) return forget_device_state(t, rc);
   const int en = edge_node(t, pa, pb);
-  rc = refresh_pmats(t, {en}, derivs);
-  if (rc) return rc;
+  rc = refresh_pmats(t, {en}, derivs, false);
+  if (rc) return forget_device_state(t, rc);
   DevEdgeOp op;
   op.partial = t.slot[pa][nb_index(t, pa, pb)];
   op.child = child_code(t, pb, pa);
   op.pm = t.pm[en];
   op.derivs = derivs ? 1 : 0;
-  rc = site5 ? engine_edge_sites(t.I, op, res, site5) : engine_edges(t.I, &op, 1, res, site);
-  if (rc) return rc;
-  rc = engine_flags(t.I, &flags, 1);
-  if (rc) return rc;
+  rc = site5 ? engine_edge_sites(t.I, op, res, site5, &flags) : engine_edges(t.I, &op, 1, res, site, &flags);
+  if (rc) return forget_device_state(t, rc);
   if (instance_precision(t.I) == GLE_PREC_FP64) {
     const bool thrown = ((flags & GLE_FLAG_REDUCE_RESCALE) && t.rescale_every > 2) ||
                         ((flags & GLE_FLAG_BAIL_OUT) && t.rescale_every <= 2);
@@ -417,27 +429,25 @@ static int optimize_all(Tree &t, double prec, double *improve, double *lnl) {
 static int sweep(Tree &t, double *out) {
   for (int attempt = 0;; attempt++) {
     dirty_all(t);
-    int flags = 0, rc = engine_flags(t.I, nullptr, 1);
+    int flags = 0, rc = engine_flags(t.I, nullptr, 1);  // reset, queued ahead of the updates
     if (rc) return rc;
     std::vector<std::array<int, 3>> want;
     for (int a = 0; a < t.nn; a++)
       if (!is_tip(t, a))
         for (int k = 0; k < t.nnb[a]; k++) collect(t, a, t.nb[a][k], want);
-    rc = run_ops(t, want);
-    if (rc) return rc;
+    rc = run_ops(t, want, false);  // queued; the edges below wait for everything once
+    if (rc) return forget_device_state(t, rc);
     std::vector<int> edges;
     for (int n = 1; n < t.nn; n++) edges.push_back(n);
-    rc = refresh_pmats(t, edges, true);
-    if (rc) return rc;
+    rc = refresh_pmats(t, edges, true, false);
+    if (rc) return forget_device_state(t, rc);
     std::vector<DevEdgeOp> ops(t.nn - 1);
     for (int n = 1; n < t.nn; n++) {
       const int a = t.parent[n];
       ops[n - 1] = {t.slot[a][nb_index(t, a, n)], child_code(t, n, a), t.pm[n], 1};
     }
-    rc = engine_edges(t.I, ops.data(), int(ops.size()), out, nullptr);
-    if (rc) return rc;
-    rc = engine_flags(t.I, &flags, 1);
-    if (rc) return rc;
+    rc = engine_edges(t.I, ops.data(), int(ops.size()), out, nullptr, &flags);
+    if (rc) return forget_device_state(t, rc);
     const bool thrown = instance_precision(t.I) == GLE_PREC_FP64 &&
                         (((flags & GLE_FLAG_REDUCE_RESCALE) && t.rescale_every > 2) ||
                          ((flags & GLE_FLAG_BAIL_OUT) && t.rescale_every <= 2));
