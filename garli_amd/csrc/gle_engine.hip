@@ -134,6 +134,8 @@ struct Instance {
   double *h_stage = nullptr;  // pinned
   size_t h_stage_cap = 0;
   int *h_flag = nullptr;  // pinned: the rescale flags read back with an edge evaluation's results
+  std::vector<void *> retired;       // device buffers outgrown by serving ticks, freed with the instance
+  std::vector<void *> retired_host;  // pinned host buffers likewise
   double pinv = 0.0;
   int pinv_in_model = 0, last_const = -1;
   // kernel timing (HIP events on this instance's stream): 0 CLA, 1 edge, 2 pmat
@@ -984,14 +986,16 @@ __device__ __forceinline__ void codon_child_into(T *__restrict__ cla, const doub
   };
   if (msg) {
     const T *src = cla + c * slot;
+    T m[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) m[t][v] = src[(size_t(p0 + g + 4 * v) * R + r) * 64 + 16 * t + lo];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
       gle_doublex4 d;
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int p = p0 + g + 4 * v;
-        d[v] = p < P ? double(src[(size_t(p) * R + r) * 64 + 16 * t + lo]) : 0.0;
-      }
+      for (int v = 0; v < 4; v++) d[v] = p0 + g + 4 * v < P ? double(m[t][v]) : 0.0;
       put(t, d);
     }
     return;
@@ -1080,6 +1084,18 @@ __global__ __launch_bounds__(64 * R, 3) void k_cla_codon(T *__restrict__ cla, in
           mx[v] = fmax(mx[v], o);
         }
     }
+    // the children's rescale counts for wave 0's stores, loaded back to back
+    // before the rescale exchange (rows p0 .. p0+15 lie inside the Ppad rows
+    // of a slot's counts; rows past P are not stored)
+    int u1[4] = {0, 0, 0, 0}, u2[4] = {0, 0, 0, 0};
+    if (r == 0) {
+      if (op.c1 >= 0)
+#pragma unroll
+        for (int v = 0; v < 4; v++) u1[v] = ua[size_t(op.c1) * Ppad + p0 + g + 4 * v];
+      if (op.c2 >= 0)
+#pragma unroll
+        for (int v = 0; v < 4; v++) u2[v] = ua[size_t(op.c2) * Ppad + p0 + g + 4 * v];
+    }
     if (op.flags & kOpRescale) {
 #pragma unroll
       for (int v = 0; v < 4; v++) {
@@ -1099,11 +1115,7 @@ __global__ __launch_bounds__(64 * R, 3) void k_cla_codon(T *__restrict__ cla, in
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int p = p0 + g + 4 * v;
-      int u = 0;
-      if (p < P && r == 0) {
-        if (op.c1 >= 0) u += ua[size_t(op.c1) * Ppad + p];
-        if (op.c2 >= 0) u += ua[size_t(op.c2) * Ppad + p];
-      }
+      int u = u1[v] + u2[v];
       // the pattern's rescale (over its R x 64 values) is applied as the values are stored
       int e = 0;
       double mult = 1.0;
@@ -1140,8 +1152,8 @@ __global__ __launch_bounds__(64 * R, 3) void k_cla_codon(T *__restrict__ cla, in
       if (p < P && lo == 0 && r == 0) {
         ua[size_t(op.dest) * Ppad + p] = u;
         if constexpr (MSG) {  // a kept message carries its child's rescale counts
-          if (op.m1 >= 0 && op.c1 >= 0) ua[size_t(op.m1) * Ppad + p] = ua[size_t(op.c1) * Ppad + p];
-          if (op.m2 >= 0 && op.c2 >= 0) ua[size_t(op.m2) * Ppad + p] = ua[size_t(op.c2) * Ppad + p];
+          if (op.m1 >= 0 && op.c1 >= 0) ua[size_t(op.m1) * Ppad + p] = u1[v];
+          if (op.m2 >= 0 && op.c2 >= 0) ua[size_t(op.m2) * Ppad + p] = u2[v];
         }
       }
     }
@@ -1471,10 +1483,13 @@ __global__ __launch_bounds__(64 * R) void k_edge_codon(const T *__restrict__ cla
     auto tile = [&](int t) {
       const int st = 16 * t + lo;
       double xw[4];  // x * pi of the partial side for this lane's cells
+      const double ps = st < 61 ? pi[st < 61 ? st : 0] : 0.0;
 #pragma unroll
       for (int v = 0; v < 4; v++) {
+        // unconditional load (rows up to p0+15 lie inside the slot's Ppad rows)
         const int p = p0 + g + 4 * v;
-        xw[v] = p < P && st < 61 ? double(cla[op.partial * slot + (size_t(p) * R + r) * 64 + st]) * pi[st] : 0.0;
+        const double xv = double(cla[op.partial * slot + (size_t(p) * R + r) * 64 + st]);
+        xw[v] = p < P ? xv * ps : 0.0;
       }
 #pragma unroll
       for (int mi = 0; mi < 3; mi++) {
@@ -1511,14 +1526,21 @@ __global__ __launch_bounds__(64 * R) void k_edge_codon(const T *__restrict__ cla
 #pragma unroll
         for (int t = 0; t < 4; t++) Y[t] = codon_tip_tile(Pm, ts, t, lo, 1.0);
       }
+      // the partial side's 16 values are loaded unconditionally and back to
+      // back (rows up to p0+15 lie inside the slot's Ppad rows); rows past P
+      // and the padding states weigh zero
+      T xp[4][4];
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) xp[t][v] = cla[op.partial * slot + (size_t(p0 + g + 4 * v) * R + r) * 64 + 16 * t + lo];
 #pragma unroll
       for (int t = 0; t < 4; t++) {
         const int st = 16 * t + lo;
+        const double ps = st < 61 ? pi[st < 61 ? st : 0] : 0.0;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-          const int p = p0 + g + 4 * v;
-          const double xw = p < P && st < 61 ? double(cla[op.partial * slot + (size_t(p) * R + r) * 64 + st]) * pi[st]
-                                             : 0.0;
+          const double xw = p0 + g + 4 * v < P ? double(xp[t][v]) * ps : 0.0;
           acc[0][v] += Y[t][v] * xw;
         }
       }
@@ -1549,17 +1571,28 @@ __global__ __launch_bounds__(64 * R) void k_edge_codon(const T *__restrict__ cla
       }
   double lnl = 0, g1 = 0, g2 = 0;
   if (r == 0 && lo == 0) {
+    // the four patterns' counts, loaded back to back (every array here has
+    // Ppad rows)
+    int u1v[4], u2v[4], csv[4], cnv[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int p = p0 + g + 4 * v;
+      u1v[v] = ua[size_t(op.partial) * Ppad + p];
+      u2v[v] = op.child >= 0 ? ua[size_t(op.child) * Ppad + p] : -1;
+      csv[v] = cst[p];
+      cnv[v] = counts[p];
+    }
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int p = p0 + g + 4 * v;
       if (p >= P) continue;
-      const int u1 = ua[size_t(op.partial) * Ppad + p];
-      const int u2 = op.child >= 0 ? ua[size_t(op.child) * Ppad + p] : -1;
-      double pt = 0.0;
-      const int cs = cst[p];
-      if (pinv_in_model && cs >= 0 && cs < 61) pt = pinv * pi[cs];
+      const int u1 = u1v[v], u2 = u2v[v];
+      const int cs = csv[v];
+      const bool inv = pinv_in_model && cs >= 0 && cs < 61;
+      const double pcs = pi[inv ? cs : 0];
+      const double pt = inv ? pinv * pcs : 0.0;
       double l1, d1, d2, ul, r1, r2;
-      site_terms<sizeof(T) == 4>(sL[v], s1[v], s2[v], u1 + (u2 > 0 ? u2 : 0), pt, true, counts[p], l1, d1, d2, ul,
+      site_terms<sizeof(T) == 4>(sL[v], s1[v], s2[v], u1 + (u2 > 0 ? u2 : 0), pt, true, cnv[v], l1, d1, d2, ul,
                                  r1, r2);
       lnl += l1;
       g1 += d1;
@@ -2360,6 +2393,11 @@ static int edges_impl(Instance *I, const DevEdgeOp *ops, int n, double *out, dou
 }
 
 // ---------------------------------------------------------------- serving ticks
+// The edge-partials capacity a tick grows from: what the instance holds, or 64 edges' worth.
+static size_t part_cap_floor(const Instance *X) {
+  return std::max(X->part_cap, size_t(64) * edge_blocks(X) * 3);
+}
+
 // One tick of the serving layer on one part (the instance or its heavy
 // part), enqueued without any wait: P, P', P'' for the new lengths, the
 // rescale-flag reset, the CLA levels, the edges and their reduction, then the
@@ -2402,8 +2440,10 @@ static int tick_enqueue_part(Instance *X, int sidx, const int *pm_idx, const dou
                o_res = o_flags + 8, need = o_res + sizeof(double) * 3 * n_e;
   if (need > st.cap) {
     const size_t cap = std::max(need + need / 2, size_t(64) << 10);
-    if (st.h) GLE_HIP(hipHostFree(st.h));
-    if (st.d) GLE_HIP(hipFree(st.d));
+    // retired rather than freed: either free would wait for the other
+    // stage's tick still in flight
+    if (st.h) X->retired_host.push_back(st.h);
+    if (st.d) X->retired.push_back(st.d);
     st.h = st.d = nullptr;
     st.cap = 0;
     GLE_HIP(hipHostMalloc(reinterpret_cast<void **>(&st.h), cap));
@@ -2452,11 +2492,15 @@ static int tick_enqueue_part(Instance *X, int sidx, const int *pm_idx, const dou
     const int nblk = edge_blocks(X);
     const size_t part = size_t(n_e) * nblk * 3;
     if (part > X->part_cap) {
-      if (X->d_part) GLE_HIP(hipFree(X->d_part));
+      // a tick grows the buffer without hipFree, which would wait for the
+      // whole device (the tick in flight included) before this tick's edges
+      // could be queued; the old buffer is freed with the instance
+      if (X->d_part) X->retired.push_back(X->d_part);
       X->d_part = nullptr;
       X->part_cap = 0;
-      GLE_HIP(hipMalloc(&X->d_part, sizeof(double) * part));
-      X->part_cap = part;
+      const size_t cap = std::max(part, 2 * part_cap_floor(X));
+      GLE_HIP(hipMalloc(&X->d_part, sizeof(double) * cap));
+      X->part_cap = cap;
     }
     bool derivs = false;
     double eb = 0.0;
@@ -2630,6 +2674,10 @@ static int destroy(Instance *I) {
     if (p) (void)hipFree(p);
   if (I->h_stage) (void)hipHostFree(I->h_stage);
   if (I->h_flag) (void)hipHostFree(I->h_flag);
+  for (void *p : I->retired) (void)hipFree(p);
+  for (void *p : I->retired_host) (void)hipHostFree(p);
+  I->retired.clear();
+  I->retired_host.clear();
   for (auto &st : I->stage) {
     if (st.h) (void)hipHostFree(st.h);
     if (st.d) (void)hipFree(st.d);
