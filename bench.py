@@ -369,7 +369,7 @@ def _serve_rate(ds, E, pop, prec, slots):
         srv = E.Server()
         a = time.perf_counter()
         ids = srv.submit_batch(inst, pop)
-        res = [srv.wait(i)[0] for i in ids]
+        res = list(srv.wait_all(ids))
         dt = time.perf_counter() - a
         st = srv.stats()
         st.update(srv.timing())

@@ -2760,6 +2760,20 @@ static int create_internal(const gle_instance_desc *d, std::unique_ptr<Instance>
   GLE_ALLOC(I->d_lam, sizeof(double) * size_t(I->n_eig) * S);
   GLE_ALLOC(I->d_model, sizeof(double) * (S + 2 * R));
   GLE_ALLOC(I->d_site, sizeof(double) * 5 * size_t(I->Ppad));
+  // Serving ticks' staging (both stages), their events and edge partials
+  // for 64 edges, made here so that an instance's first served requests do
+  // not wait for pinned allocations; a larger tick grows them.
+  for (auto &st : I->stage) {
+    const size_t cap = size_t(256) << 10;
+    if (hipHostMalloc(reinterpret_cast<void **>(&st.h), cap) != hipSuccess)
+      return fail(set_error(GLE_ERR_NOMEM, "pinned allocation failed"));
+    GLE_ALLOC(st.d, cap);
+    st.cap = cap;
+    if (hipEventCreateWithFlags(&st.done, hipEventDisableTiming) != hipSuccess)
+      return fail(set_error(GLE_ERR_DEVICE, "event creation failed"));
+  }
+  I->part_cap = size_t(64) * edge_blocks(I.get()) * 3;
+  GLE_ALLOC(I->d_part, sizeof(double) * I->part_cap);
 #undef GLE_ALLOC
   // zero everything the kernels may read before the caller sets it
   const unsigned char missing = (unsigned char)(S == 4 ? 15 : S);

@@ -893,11 +893,11 @@ static void worker_loop(Server *sv) {
         if (flights.size() < 2) {
           // One tick runs while the next is built and queued.  A large burst
           // (a generation's offspring) reaching an idle device starts with a
-          // quarter of it, so the device starts early and the rest is built
-          // while that quarter runs.
+          // quarter of it, at most 8 requests, so the device starts early and
+          // the rest is built while that part runs.
           int ready = 0;
           for (Req *r : sv->active) ready += !(r->in_flight || r->done);
-          const int chunk = (flights.empty() && ready >= 16) ? (ready + 3) / 4 : ready;
+          const int chunk = (flights.empty() && ready >= 16) ? std::min((ready + 3) / 4, 8) : ready;
           int taken = 0;
           // the instances are looked up here, under the lock that submissions
           // insert under (map references stay valid across later inserts)
@@ -1172,6 +1172,31 @@ extern "C" int gle_serve_wait(int h, int req, double *lnl, double *blen_out, dou
   if (blen_out) std::copy(r->blen.begin(), r->blen.end(), blen_out);
   if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
   sv->reqs.erase(it);
+  return rc;
+}
+
+extern "C" int gle_serve_wait_all(int h, int n, const int *reqs, double *lnl) {
+  Server *sv = get_server(h);
+  if (!sv || n < 0 || (n > 0 && (!reqs || !lnl))) return set_error(GLE_ERR_ARG, "bad argument");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  std::vector<Req *> rs(size_t(n), nullptr);
+  for (int i = 0; i < n; i++) {
+    auto it = sv->reqs.find(reqs[i]);
+    if (it == sv->reqs.end()) return set_error(GLE_ERR_ARG, "unknown request %d", reqs[i]);
+    rs[size_t(i)] = it->second.get();
+  }
+  sv->cv_done.wait(lk, [&] {
+    for (const Req *r : rs)
+      if (!r->finished) return false;
+    return true;
+  });
+  int rc = GLE_SUCCESS;
+  for (int i = 0; i < n; i++) {
+    const Req *r = rs[size_t(i)];
+    lnl[i] = r->lnl;
+    if (!rc && r->rc) rc = r->rc;
+  }
+  for (int i = 0; i < n; i++) sv->reqs.erase(reqs[i]);
   return rc;
 }
 

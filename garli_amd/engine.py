@@ -141,6 +141,7 @@ SIGNATURES = {
     "gle_serve_submit_batch": ([C.c_int, C.c_int, C.c_int, _I, _D, C.c_int, C.c_double, _I], C.c_int),
     "gle_serve_poll": ([C.c_int, C.c_int, _I], C.c_int),
     "gle_serve_wait": ([C.c_int, C.c_int, _D, _D, _D], C.c_int),
+    "gle_serve_wait_all": ([C.c_int, C.c_int, _I, _D], C.c_int),
     "gle_serve_stats": ([C.c_int, C.POINTER(C.c_longlong), _D, _D, _D, C.c_int], C.c_int),
     "gle_serve_set_batch_window": ([C.c_int, C.c_int], C.c_int),
     "gle_serve_timing": ([C.c_int, _D, _D, C.c_int], C.c_int),
@@ -928,6 +929,13 @@ class Server:
         _check(lib().gle_serve_wait(self.h, rid, C.byref(lnl), _p(bl, C.c_double) if n_nodes else None,
                                     C.byref(lat)))
         return (lnl.value, lat.value, bl) if n_nodes else (lnl.value, lat.value)
+
+    def wait_all(self, rids):
+        """Wait for every request in rids; returns their lnL values in order."""
+        ids = np.ascontiguousarray(rids, dtype=np.int32)
+        out = np.zeros(len(ids))
+        _check(lib().gle_serve_wait_all(self.h, len(ids), _p(ids, C.c_int), _p(out, C.c_double)))
+        return out
 
     def timing(self, reset=False):
         """(host ms building ticks, ms from first launch to results) so far."""

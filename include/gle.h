@@ -452,6 +452,11 @@ int gle_serve_submit_batch(int server, int inst, int n, const int *parents, cons
 int gle_serve_poll(int server, int req, int *done);
 /* blocks until done; blen_out (optional) receives the final branch lengths */
 int gle_serve_wait(int server, int req, double *lnl, double *blen_out, double *latency_ms);
+/* Blocks until all n requests are done and writes lnl[i] for reqs[i] (a GA
+ * generation's offspring collected in one call); returns the first failed
+ * request's error after every request was collected.  No reference
+ * counterpart (GARLI scores one individual at a time). */
+int gle_serve_wait_all(int server, int n, const int *reqs, double *lnl);
 /* latency percentiles over completed requests; mean requests per batched step */
 int gle_serve_stats(int server, long long *completed, double *p50_ms, double *p99_ms, double *mean_batch, int reset);
 /* An idle server (nothing in flight) lets a burst of requests (a GA
