@@ -1054,13 +1054,19 @@ __global__ __launch_bounds__(64 * R, 3) void k_cla_codon(T *__restrict__ cla, in
                                                          int Ppad, int P, int fused_n) {
   constexpr bool kF32 = sizeof(T) == 4;
   __shared__ double smax[R][16];
-  const int l = threadIdx.x & 63, lo = l & 15, g = l >> 4;
   const int r = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int p0 = blockIdx.x * 16;
   const size_t slot = size_t(Ppad) * R * 64;
   const int first = fused_n ? 0 : blockIdx.y, last = fused_n ? fused_n : blockIdx.y + 1;
   for (int oi = first; oi < last; ++oi) {
     if (oi != first) __syncthreads();  // the previous op's stores are read by other lanes
+    // The lane index is made opaque to the compiler in each op (an empty asm,
+    // no instruction), so the per-lane offsets and dither keys derived from
+    // it are formed where they are used instead of being hoisted out of the
+    // op loop and spilled at the register cap.
+    int l = int(threadIdx.x & 63);
+    asm volatile("" : "+v"(l));
+    const int lo = l & 15, g = l >> 4;
     const DevClaOp op = ops[oi];
     T *dst = cla + op.dest * slot;
     int ts1[4], ts2[4];
@@ -1085,17 +1091,21 @@ __global__ __launch_bounds__(64 * R, 3) void k_cla_codon(T *__restrict__ cla, in
         }
     }
     // the children's rescale counts for wave 0's stores, loaded back to back
-    // before the rescale exchange (rows p0 .. p0+15 lie inside the Ppad rows
-    // of a slot's counts; rows past P are not stored)
+    // (rows p0 .. p0+15 lie inside the Ppad rows of a slot's counts; rows past
+    // P are not stored): before the rescale exchange when messages keep them
+    // separately, after it otherwise (fewer registers held across it)
     int u1[4] = {0, 0, 0, 0}, u2[4] = {0, 0, 0, 0};
-    if (r == 0) {
-      if (op.c1 >= 0)
+    auto load_counts = [&] {
+      if (r == 0) {
+        if (op.c1 >= 0)
 #pragma unroll
-        for (int v = 0; v < 4; v++) u1[v] = ua[size_t(op.c1) * Ppad + p0 + g + 4 * v];
-      if (op.c2 >= 0)
+          for (int v = 0; v < 4; v++) u1[v] = ua[size_t(op.c1) * Ppad + p0 + g + 4 * v];
+        if (op.c2 >= 0)
 #pragma unroll
-        for (int v = 0; v < 4; v++) u2[v] = ua[size_t(op.c2) * Ppad + p0 + g + 4 * v];
-    }
+          for (int v = 0; v < 4; v++) u2[v] = ua[size_t(op.c2) * Ppad + p0 + g + 4 * v];
+      }
+    };
+    if constexpr (MSG) load_counts();
     if (op.flags & kOpRescale) {
 #pragma unroll
       for (int v = 0; v < 4; v++) {
@@ -1112,6 +1122,7 @@ __global__ __launch_bounds__(64 * R, 3) void k_cla_codon(T *__restrict__ cla, in
         mx[v] = m;
       }
     }
+    if constexpr (!MSG) load_counts();
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int p = p0 + g + 4 * v;
