@@ -618,11 +618,17 @@ __global__ __launch_bounds__(256) void k_cla_nuc(T *__restrict__ cla, int *__res
                                                  const double *__restrict__ pm, const DevClaOp *__restrict__ ops,
                                                  int *__restrict__ flags, int Ppad, int P, int fused_n) {
   constexpr int kPmStride = 3 * R * 16;
-  const int p = blockIdx.x * 256 + threadIdx.x;
-  if (p >= P) return;  // no cross-lane work below
-  const size_t slot = size_t(Ppad) * R * 4, off = nuc_off(p, 0, R);
+  if (int(blockIdx.x * 256 + threadIdx.x) >= P) return;  // no cross-lane work below
+  const size_t slot = size_t(Ppad) * R * 4;
   const int first = fused_n ? 0 : blockIdx.y, last = fused_n ? fused_n : blockIdx.y + 1;
   for (int oi = first; oi < last; ++oi) {
+    // the thread's pattern and offset, formed again in each op from an index
+    // the compiler cannot see through (an empty asm), so they are not hoisted
+    // out of the op loop and held across it
+    int t = int(threadIdx.x);
+    asm volatile("" : "+v"(t));
+    const int p = blockIdx.x * 256 + t;
+    const size_t off = nuc_off(p, 0, R);
     const DevClaOp op = ops[oi];
     T x1[R][4], x2[R][4];
     int m1 = 15, m2 = 15, u = 0;
@@ -765,12 +771,15 @@ __global__ __launch_bounds__(64 * R, sizeof(T) == 4 ? 1 : 3) void k_cla_aa(T *__
   constexpr bool kLdsP = !kF32;
   __shared__ double smax[R][64];
   __shared__ double sPT[kLdsP ? R : 1][2][kLdsP ? S * S : 2];  // FP64: the wave's P^T of both children
-  const int lane = threadIdx.x & 63;
   const int r = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
-  const int p = blockIdx.x * 64 + lane;
-  const bool live = p < P;
-  const size_t slot = size_t(Ppad) * R * S, off = aa_off(p, r, R);
+  const size_t slot = size_t(Ppad) * R * S;
   const int first = fused_n ? 0 : blockIdx.y, last = fused_n ? fused_n : blockIdx.y + 1;
+  // the lane's pattern and offset, formed again in each op from a lane index
+  // the compiler cannot see through (an empty asm), so they are not hoisted
+  // out of the op loop and held or spilled across it
+  int lane = 0, p = 0;
+  bool live = false;
+  size_t off = 0;
   // acc[f] = (P x)[f] for one child: a CLA (input state by input state) or a tip
   auto child = [&](int c, int pmslot, int ci, double(&acc)[S]) {
     // PT[q * S + f] = P[f][q]
@@ -812,6 +821,11 @@ __global__ __launch_bounds__(64 * R, sizeof(T) == 4 ? 1 : 3) void k_cla_aa(T *__
   };
   for (int oi = first; oi < last; ++oi) {
     if (oi != first) __syncthreads();  // smax of the previous op has been read
+    lane = int(threadIdx.x & 63);
+    asm volatile("" : "+v"(lane));
+    p = blockIdx.x * 64 + lane;
+    live = p < P;
+    off = aa_off(p, r, R);
     const DevClaOp op = ops[oi];
     if constexpr (kLdsP) {
       const double2 *g1 = reinterpret_cast<const double2 *>(pmT + (size_t(op.pm1) * 3 * R + r) * S * S);
@@ -1044,9 +1058,11 @@ __device__ __forceinline__ void codon_child_into(T *__restrict__ cla, const doub
 // The products are held until the op's rescale decision (each pattern's
 // maximum over its 16 lanes and over the R waves, through LDS) and stored
 // once, scaled where needed.
-// MSG (serving ticks): ops may read children as kept messages and keep the
-// messages they form (DevClaOp::m1, m2).
-template <typename T, int R, bool MSG>
+// Ops may read children as kept messages and keep the messages they form
+// (DevClaOp::m1, m2: serving ticks); the tree layer's ops set neither, and
+// its sweeps run this same kernel (a form without the message paths took
+// more registers, three waves per SIMD against four, and was slower).
+template <typename T, int R>
 __global__ __launch_bounds__(64 * R, 3) void k_cla_codon(T *__restrict__ cla, int *__restrict__ ua,
                                                          const unsigned char *__restrict__ tips,
                                                          const double *__restrict__ pmpad,
@@ -1076,11 +1092,11 @@ __global__ __launch_bounds__(64 * R, 3) void k_cla_codon(T *__restrict__ cla, in
     double ov[4][4];  // [pattern row v][state tile t], held until the rescale decision
     {
       gle_doublex4 a1[4];  // child 1's tiles, then the product
-      const bool g1 = MSG && (op.flags & kOpMsg1), g2 = MSG && (op.flags & kOpMsg2);
+      const bool g1 = (op.flags & kOpMsg1) != 0, g2 = (op.flags & kOpMsg2) != 0;
       codon_child_into<T, false>(cla, g1 ? nullptr : pmpad + (size_t(op.pm1) * 3 * R + r) * 4096, op.c1, g1,
-                                 MSG ? op.m1 : -1, ts1, slot, R, r, l, p0, P, a1);
+                                 op.m1, ts1, slot, R, r, l, p0, P, a1);
       codon_child_into<T, true>(cla, g2 ? nullptr : pmpad + (size_t(op.pm2) * 3 * R + r) * 4096, op.c2, g2,
-                                MSG ? op.m2 : -1, ts2, slot, R, r, l, p0, P, a1);
+                                op.m2, ts2, slot, R, r, l, p0, P, a1);
 #pragma unroll
       for (int v = 0; v < 4; v++)
 #pragma unroll
@@ -1091,21 +1107,17 @@ __global__ __launch_bounds__(64 * R, 3) void k_cla_codon(T *__restrict__ cla, in
         }
     }
     // the children's rescale counts for wave 0's stores, loaded back to back
-    // (rows p0 .. p0+15 lie inside the Ppad rows of a slot's counts; rows past
-    // P are not stored): before the rescale exchange when messages keep them
-    // separately, after it otherwise (fewer registers held across it)
+    // before the rescale exchange (rows p0 .. p0+15 lie inside the Ppad rows
+    // of a slot's counts; rows past P are not stored)
     int u1[4] = {0, 0, 0, 0}, u2[4] = {0, 0, 0, 0};
-    auto load_counts = [&] {
-      if (r == 0) {
-        if (op.c1 >= 0)
+    if (r == 0) {
+      if (op.c1 >= 0)
 #pragma unroll
-          for (int v = 0; v < 4; v++) u1[v] = ua[size_t(op.c1) * Ppad + p0 + g + 4 * v];
-        if (op.c2 >= 0)
+        for (int v = 0; v < 4; v++) u1[v] = ua[size_t(op.c1) * Ppad + p0 + g + 4 * v];
+      if (op.c2 >= 0)
 #pragma unroll
-          for (int v = 0; v < 4; v++) u2[v] = ua[size_t(op.c2) * Ppad + p0 + g + 4 * v];
-      }
-    };
-    if constexpr (MSG) load_counts();
+        for (int v = 0; v < 4; v++) u2[v] = ua[size_t(op.c2) * Ppad + p0 + g + 4 * v];
+    }
     if (op.flags & kOpRescale) {
 #pragma unroll
       for (int v = 0; v < 4; v++) {
@@ -1122,7 +1134,6 @@ __global__ __launch_bounds__(64 * R, 3) void k_cla_codon(T *__restrict__ cla, in
         mx[v] = m;
       }
     }
-    if constexpr (!MSG) load_counts();
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int p = p0 + g + 4 * v;
@@ -1162,10 +1173,9 @@ __global__ __launch_bounds__(64 * R, 3) void k_cla_codon(T *__restrict__ cla, in
       }
       if (p < P && lo == 0 && r == 0) {
         ua[size_t(op.dest) * Ppad + p] = u;
-        if constexpr (MSG) {  // a kept message carries its child's rescale counts
-          if (op.m1 >= 0 && op.c1 >= 0) ua[size_t(op.m1) * Ppad + p] = u1[v];
-          if (op.m2 >= 0 && op.c2 >= 0) ua[size_t(op.m2) * Ppad + p] = u2[v];
-        }
+        // a kept message carries its child's rescale counts
+        if (op.m1 >= 0 && op.c1 >= 0) ua[size_t(op.m1) * Ppad + p] = u1[v];
+        if (op.m2 >= 0 && op.c2 >= 0) ua[size_t(op.m2) * Ppad + p] = u2[v];
       }
     }
   }
@@ -1981,28 +1991,23 @@ static int launch_nuc_r(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
 }
 
 template <typename T, int R>
-static void launch_codon(Instance *I, const DevClaOp *d_ops, int n, bool fused, bool msg) {
-  if (msg)
-    GLE_LAUNCH(I, (k_cla_codon<T, R, true>), dim3(I->Ppad / 16, fused ? 1 : n), dim3(64 * R), 0,
-               reinterpret_cast<T *>(I->d_cla), I->d_u, I->d_tips, I->d_pmpad, d_ops, I->d_flags, I->Ppad, I->P,
-               fused ? n : 0);
-  else
-    GLE_LAUNCH(I, (k_cla_codon<T, R, false>), dim3(I->Ppad / 16, fused ? 1 : n), dim3(64 * R), 0,
-               reinterpret_cast<T *>(I->d_cla), I->d_u, I->d_tips, I->d_pmpad, d_ops, I->d_flags, I->Ppad, I->P,
-               fused ? n : 0);
+static void launch_codon(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
+  GLE_LAUNCH(I, (k_cla_codon<T, R>), dim3(I->Ppad / 16, fused ? 1 : n), dim3(64 * R), 0,
+             reinterpret_cast<T *>(I->d_cla), I->d_u, I->d_tips, I->d_pmpad, d_ops, I->d_flags, I->Ppad, I->P,
+             fused ? n : 0);
 }
 
 template <typename T>
-static int launch_codon_r(Instance *I, const DevClaOp *d_ops, int n, bool fused, bool msg) {
+static int launch_codon_r(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
   switch (I->R) {
-    case 1: launch_codon<T, 1>(I, d_ops, n, fused, msg); break;
-    case 2: launch_codon<T, 2>(I, d_ops, n, fused, msg); break;
-    case 3: launch_codon<T, 3>(I, d_ops, n, fused, msg); break;
-    case 4: launch_codon<T, 4>(I, d_ops, n, fused, msg); break;
-    case 5: launch_codon<T, 5>(I, d_ops, n, fused, msg); break;
-    case 6: launch_codon<T, 6>(I, d_ops, n, fused, msg); break;
-    case 7: launch_codon<T, 7>(I, d_ops, n, fused, msg); break;
-    case 8: launch_codon<T, 8>(I, d_ops, n, fused, msg); break;
+    case 1: launch_codon<T, 1>(I, d_ops, n, fused); break;
+    case 2: launch_codon<T, 2>(I, d_ops, n, fused); break;
+    case 3: launch_codon<T, 3>(I, d_ops, n, fused); break;
+    case 4: launch_codon<T, 4>(I, d_ops, n, fused); break;
+    case 5: launch_codon<T, 5>(I, d_ops, n, fused); break;
+    case 6: launch_codon<T, 6>(I, d_ops, n, fused); break;
+    case 7: launch_codon<T, 7>(I, d_ops, n, fused); break;
+    case 8: launch_codon<T, 8>(I, d_ops, n, fused); break;
     default: return set_error(GLE_ERR_UNSUPPORTED, "rate categories must be 1..8");
   }
   return GLE_SUCCESS;
@@ -2031,13 +2036,12 @@ static int launch_aa_r(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
   return GLE_SUCCESS;
 }
 
-// msg: a serving tick's ops, which may read and keep codon messages
 template <typename T>
-static int launch_cla(Instance *I, const DevClaOp *d_ops, int n, bool fused, bool msg = false) {
+static int launch_cla(Instance *I, const DevClaOp *d_ops, int n, bool fused) {
   switch (I->S) {
     case 4: return launch_nuc_r<T>(I, d_ops, n, fused);
     case 20: return launch_aa_r<T>(I, d_ops, n, fused);
-    case 61: return launch_codon_r<T>(I, d_ops, n, fused, msg);
+    case 61: return launch_codon_r<T>(I, d_ops, n, fused);
     default: return set_error(GLE_ERR_UNSUPPORTED, "state count %d has no kernel", I->S);
   }
 }
@@ -2493,8 +2497,8 @@ static int tick_enqueue_part(Instance *X, int sidx, const int *pm_idx, const dou
     double bytes = 0.0;
     for (int i = b; i < e; i++) bytes += cla_op_bytes(X, ops[i]);
     timed_begin(X, 0, bytes);
-    rc = X->prec == GLE_PREC_FP64 ? launch_cla<double>(X, d_ops + b, e - b, false, true)
-                                  : launch_cla<float>(X, d_ops + b, e - b, false, true);
+    rc = X->prec == GLE_PREC_FP64 ? launch_cla<double>(X, d_ops + b, e - b, false)
+                                  : launch_cla<float>(X, d_ops + b, e - b, false);
     timed_end(X);
   }
   X->d_flags = flags_save;
