@@ -1443,9 +1443,10 @@ __global__ __launch_bounds__(256) void k_edge_nuc(const T *__restrict__ cla, con
 // per-category row sums meet in LDS, where wave 0 weights them in category
 // order and finishes the sites.  D = false: no op of the launch wants
 // derivatives, and the four tiles are unrolled so that a tile's P fragments
-// load during the previous tile's MFMAs.
+// load during the previous tile's MFMAs; that form fits four waves per SIMD
+// (its products in VGPRs rather than accumulation registers).
 template <typename T, int R, bool D>
-__global__ __launch_bounds__(64 * R) void k_edge_codon(const T *__restrict__ cla, const int *__restrict__ ua,
+__global__ __launch_bounds__(64 * R, D ? 3 : 4) void k_edge_codon(const T *__restrict__ cla, const int *__restrict__ ua,
                                                    const unsigned char *__restrict__ tips,
                                                    const double *__restrict__ pmpad, const double *__restrict__ pdiag,
                                                    const double *__restrict__ eigfrag,
