@@ -1470,7 +1470,7 @@ __global__ __launch_bounds__(64 * R, D ? 3 : 4) void k_edge_codon(const T *__res
   {
     double acc[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};  // [matrix][v]: sum over this lane's states
     double y[16];
-    if (op.child >= 0) codon_load_x(cla, op.child, slot, R, r, lo, g, p0, P, y);
+    if (op.child >= 0 && !op.msg) codon_load_x(cla, op.child, slot, R, r, lo, g, p0, P, y);
     // FP32 form, derivatives on an internal child: spectral route.  With
     // P = V diag(ex) V^-1, z = V^-1 y and a = V^T (x * pi) are two MFMA
     // contractions per tile (instead of P y, P' y and P'' y), and each cell
@@ -1532,7 +1532,22 @@ __global__ __launch_bounds__(64 * R, D ? 3 : 4) void k_edge_codon(const T *__res
       // them, then the x * pi weighting
       const double *Pm = pmpad + (size_t(op.pm) * 3 * R + r) * 4096;
       gle_doublex4 Y[4];
-      if (op.child >= 0) {
+      if (op.child >= 0 && op.msg) {
+        // the child is a kept message P(t) . x (a serving tick's score whose
+        // child side a CLA op of the tick contracted at this length): read,
+        // not contracted again; rows p0 .. p0+15 lie inside the slot's Ppad
+        // rows, and rows past P weigh zero
+        const T *src = cla + op.child * slot;
+        T m[4][4];
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) m[t][v] = src[(size_t(p0 + g + 4 * v) * R + r) * 64 + 16 * t + lo];
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) Y[t][v] = p0 + g + 4 * v < P ? double(m[t][v]) : 0.0;
+      } else if (op.child >= 0) {
 #pragma unroll
         for (int t = 0; t < 4; t++) {
           const double *B = Pm + t * 1024 + l;
@@ -2440,12 +2455,17 @@ static int tick_enqueue_part(Instance *X, int sidx, const int *pm_idx, const dou
         ((g1 || g2 || o.m1 >= 0 || o.m2 >= 0) && X->S != 61))
       return set_error(GLE_ERR_ARG, "tick cla op %d out of range", i);
   }
+  bool any_derivs = false, any_msg = false;
   for (int i = 0; i < n_e; i++) {
     const DevEdgeOp &o = eops[i];
     if (o.partial < 0 || o.partial >= X->n_slots || o.child >= X->n_slots || (o.child < 0 && -o.child - 1 >= X->N) ||
-        o.pm < 0 || o.pm >= X->n_pm || o.child == o.partial)
+        o.pm < 0 || o.pm >= X->n_pm || o.child == o.partial || (o.msg && (o.child < 0 || X->S != 61)))
       return set_error(GLE_ERR_ARG, "tick edge op %d out of range", i);
+    any_derivs = any_derivs || o.derivs;
+    any_msg = any_msg || o.msg;
   }
+  // a message child is read only by the derivative-free edge kernel
+  if (any_msg && any_derivs) return set_error(GLE_ERR_ARG, "tick edges read messages in a launch with derivatives");
   // one staging layout in pinned host memory and on the device:
   // [P(t) slots | lengths | CLA ops | edge ops | flags | results]; the
   // upload carries everything up to the (zeroed) flags, the download the

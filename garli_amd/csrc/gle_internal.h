@@ -48,6 +48,7 @@ struct DevClaOp {
 // One edge integration.
 struct DevEdgeOp {
   int partial, child, pm, derivs;
+  int msg = 0;  // codon, no derivatives: child is a kept message P(t) . x (pm unused)
 };
 enum : int { kOpRescale = 1, kOpMsg1 = 2, kOpMsg2 = 4 };
 

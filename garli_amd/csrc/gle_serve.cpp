@@ -53,6 +53,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -420,16 +421,17 @@ struct TickBuilder {
   // After every request of the tick is laid out: a child the tick contracts
   // keeps its message in a free slot (or one an unused message frees) when
   // its subtree is shared -- cached before this tick (the unmodified part of
-  // a parent), or contracted more than once in it -- and the message is not
-  // kept already.  An offspring's own new subtrees are seldom met again, and
-  // their messages would only cost writes.  Messages never take a slot from
-  // a CLA, so admission's working sets are unaffected.
-  void keep_messages() {
+  // a parent), or contracted more than once in it -- or when a score edge of
+  // the tick will read it (`want`: (subtree id, length bits)), and the
+  // message is not kept already.  An offspring's own new subtrees are seldom
+  // met again, and their messages would only cost writes.  Messages never
+  // take a slot from a CLA, so admission's working sets are unaffected.
+  void keep_messages(const std::set<std::pair<uint64_t, uint64_t>> &want) {
     std::unordered_map<uint64_t, int> kept;
     for (size_t i = 0; i < ops.size(); i++)
       for (int k = 0; k < 2; k++) {
         const Kid &kd = kids[i][k];
-        if (!kd.id || !(kd.cached || uses[kd.id] > 1)) continue;
+        if (!kd.id || !(kd.cached || uses[kd.id] > 1 || want.count({kd.id, dbits(kd.len)}))) continue;
         const uint64_t name = si->intern_id(msg_key(kd.id, kd.len));
         if (si->msg.has(name) || kept.count(name)) continue;
         if (!pool->has_free()) {
@@ -705,6 +707,12 @@ static int build_part(Server &sv, ServeInstance &si, std::vector<Req *> &batch, 
     si.epoch++;
   }
   TickBuilder tb{&si, pool, tick, {}, {}, {}, {}, {}, {}, GLE_SUCCESS};
+  // each edge's two sides: subtree ids (0 for a tip) and the edge's length
+  struct EdgeSides {
+    uint64_t p, c;
+    double len;
+  };
+  std::vector<EdgeSides> sides;
   part.si = &si;
   part.epoch = si.epoch;
   for (Req *r : batch) {
@@ -738,6 +746,9 @@ static int build_part(Server &sv, ServeInstance &si, std::vector<Req *> &batch, 
       continue;
     }
     eops.push_back(e);
+    sides.push_back({req_tip(*r, r->edge_a) ? 0 : r->sid[r->edge_a][req_nbk(*r, r->edge_a, r->edge_b)],
+                     req_tip(*r, r->edge_b) ? 0 : r->sid[r->edge_b][req_nbk(*r, r->edge_b, r->edge_a)],
+                     req_len(*r, r->edge_a, r->edge_b)});
     part.reqs.push_back(r);
     r->waits = 0;
   }
@@ -745,7 +756,56 @@ static int build_part(Server &sv, ServeInstance &si, std::vector<Req *> &batch, 
     host_ms += std::chrono::duration<double, std::milli>(clock::now() - t_build).count();
     return deferred;
   }
-  if (si.msgs) tb.keep_messages();
+  // Codon scores (a tick without derivatives): an edge whose child side, or
+  // else whose partial side, a CLA op of the tick contracts at the edge's
+  // length reads that message instead of contracting the side again (the
+  // edge sum is symmetric in its two sides for a reversible model, so the
+  // partial side's message serves with the sides swapped).
+  const bool edge_msgs = si.msgs && !tb.derivs;
+  std::set<std::pair<uint64_t, uint64_t>> want;
+  if (edge_msgs) {
+    std::set<std::pair<uint64_t, uint64_t>> formed;  // (child id, length bits) the tick contracts
+    for (const auto &kd : tb.kids)
+      for (const auto &k : kd)
+        if (k.id) formed.insert({k.id, dbits(k.len)});
+    for (const EdgeSides &e : sides) {
+      const std::pair<uint64_t, uint64_t> c{e.c, dbits(e.len)}, p{e.p, dbits(e.len)};
+      if (e.c && formed.count(c)) want.insert(c);
+      else if (e.c && e.p && formed.count(p)) want.insert(p);
+    }
+  }
+  if (si.msgs) tb.keep_messages(want);
+  if (edge_msgs) {
+    std::unordered_map<uint64_t, int> fresh;
+    for (const auto &fm : tb.fresh_msgs) fresh[fm.first] = fm.second.slot;
+    auto msg_slot = [&](uint64_t id, double len) {
+      const uint64_t name = id ? si.find_id(msg_key(id, len)) : 0;
+      if (!name) return -1;
+      const auto it = fresh.find(name);
+      if (it != fresh.end()) return it->second;
+      if (CacheEntry *c = si.msg.find(name)) {
+        si.msg.touch(c, tick);
+        return c->slot;
+      }
+      return -1;
+    };
+    for (size_t i = 0; i < eops.size(); i++) {
+      const EdgeSides &e = sides[i];
+      if (!e.c) continue;  // a tip child is a gather, not a contraction
+      const int sc = msg_slot(e.c, e.len);
+      const int sp = sc >= 0 || !e.p ? -1 : msg_slot(e.p, e.len);
+      if (sc >= 0) {
+        eops[i].child = sc;
+      } else if (sp >= 0) {
+        eops[i].partial = eops[i].child;
+        eops[i].child = sp;
+      } else {
+        continue;
+      }
+      eops[i].msg = 1;
+      work.msg_read++;
+    }
+  }
   for (size_t i = 0; i < tb.ops.size(); i++) {
     const DevClaOp &o = tb.ops[i].second;
     work.ops++;

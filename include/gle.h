@@ -469,11 +469,11 @@ int gle_serve_set_batch_window(int server, int microseconds);
  * a tick's first launch until its results were back, since the last reset. */
 int gle_serve_timing(int server, double *host_ms, double *device_ms, int reset);
 /* CLA work of the ticks built since the last reset: CLA ops; children
- * contracted on the device; children read from a kept message instead (a
- * codon instance keeps a child's term P(t) . x, named by the child's subtree
- * and branch length, in a free slot and reads it in place of contracting that
- * child again); and messages kept.  Serving statistics, no reference
- * counterpart. */
+ * contracted on the device by CLA ops; children of CLA ops and sides of score
+ * edges read from a kept message instead (a codon instance keeps a child's
+ * term P(t) . x, named by the child's subtree and branch length, in a free
+ * slot and reads it in place of contracting that child again); and messages
+ * kept.  Serving statistics, no reference counterpart. */
 int gle_serve_work(int server, long long *cla_ops, long long *contractions, long long *msg_reads,
                    long long *msg_keeps, int reset);
 

@@ -174,6 +174,41 @@ def test_serving_state_spaces(gpu, kind):
     srv.close()
 
 
+def test_codon_score_edges_read_kept_messages(gpu):
+    """A score edge whose child side (or, with the sides swapped, partial side)
+    a CLA op of the same tick contracts at the edge's length reads that kept
+    message instead of contracting the side again.  A cold server's single
+    tick reads no message in its CLA ops (kept messages reach the cache only
+    after the tick), so every message read it counts is an edge's.  Results
+    equal the FP64 tree layer to 1e-12 relative and the oracle to 1e-9 in
+    FP64, and stay within 1e-3 of FP64 in the FP32 form."""
+    ds = sim.codon_m3(n_tips=12, n_patterns=240, seed=137)
+    pop = _population(ds, 12, 19)
+    o = O.Oracle(ds)
+    inst64 = ds.make_instance(E.PREC_FP64)
+    ref64 = []
+    for p, b in pop:
+        t = E.Tree(inst64, p, b)
+        ref64.append(t.score())
+        t.close()
+    inst64.close()
+    for prec in (E.PREC_FP64, E.PREC_FP32):
+        inst = ds.make_instance(prec, n_cla_slots=400, n_pmat_slots=600)
+        srv = E.Server()
+        got = srv.wait_all(srv.submit_batch(inst, pop))
+        w, st = srv.work(), srv.stats()
+        srv.close()
+        inst.close()
+        assert st["mean_batch"] == len(pop), st  # one tick
+        assert w["msg_reads"] > 0, w
+        for k in range(len(pop)):
+            tol = 1e-12 * abs(ref64[k]) if prec == E.PREC_FP64 else 1e-3
+            assert abs(got[k] - ref64[k]) <= tol, (prec, k, got[k], ref64[k])
+        for (p, b), g in zip(pop[::4], got[::4]):
+            ref = o.score(parent=p, blen=b)
+            assert abs(g - ref) < (1e-9 * abs(ref) if prec == E.PREC_FP64 else 1e-3)
+
+
 def test_codon_messages_are_kept_and_read(gpu):
     """Codon serving keeps a contracted child's message P(t) . x when its
     subtree is shared, and later CLAs read it instead of contracting the child
