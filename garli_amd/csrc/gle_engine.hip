@@ -980,11 +980,14 @@ __device__ __forceinline__ gle_doublex4 codon_tip_tile(const double *__restrict_
 // column for a tip child, or (msg) the product an earlier op kept in slot c.
 // ov[t] = tile (MUL false) or ov[t] *= tile (MUL true), so the second child's
 // tiles are never all held.  mkeep >= 0 keeps an internal child's tiles in
-// that slot (a message, read back by later ops with kOpMsg).  Tile t+1's B
-// fragments load while tile t's MFMA chain runs (the loads had been waited for
-// one tile at a time, leaving the MFMA unit idle for most of each wave); the
-// FP32 form holds x in FP32 and widens it at each MFMA, which with the
-// streamed second child makes room for the second buffer.  The FP32 form keeps
+// that slot (a message, read back by later ops with kOpMsg).  Each tile's 16
+// B fragments are loaded and its 16-step MFMA chain runs on them (the
+// compiler interleaves a tile's loads with the chain's first steps; loading
+// the next tile's fragments ahead of the chain spilled).  The FP32 form holds
+// x in FP32 and widens it at each MFMA.  The message reads are issued
+// unconditionally and back to back (rows p0 .. p0+15 lie inside the slot's
+// Ppad rows): guarded, they had compiled to one branch and one wait per
+// load.  The FP32 form keeps
 // a message rounded to nearest without the dither: a message is a contraction
 // of stored (dithered) CLA values, so its values differ from pattern to
 // pattern and their roundings are already independent; the dither is for
