@@ -267,6 +267,10 @@ struct Server {
   std::deque<Req *> pending;
   std::vector<Req *> active;
   std::unordered_map<int, std::unique_ptr<Req>> reqs;
+  // requests their clients collected, freed by the worker while a tick it
+  // queued runs (freeing a generation's requests inline cost its client
+  // ~50 us after the last result)
+  std::vector<std::unique_ptr<Req>> collected;
   std::map<int, ServeInstance> insts;
   std::thread worker;
   bool stop = false;
@@ -707,6 +711,17 @@ static int build_part(Server &sv, ServeInstance &si, std::vector<Req *> &batch, 
     si.epoch++;
   }
   TickBuilder tb{&si, pool, tick, {}, {}, {}, {}, {}, {}, GLE_SUCCESS};
+  {
+    // room for every request's directional CLAs, so a cold tick's maps do
+    // not rehash as they grow
+    const size_t cap = batch.size() * size_t(instance_tips(si.I));
+    tb.fresh.reserve(cap);
+    tb.uses.reserve(cap);
+    tb.pm_fresh.reserve(cap);
+    tb.fresh_order.reserve(cap);
+    tb.ops.reserve(cap);
+    tb.kids.reserve(cap);
+  }
   // each edge's two sides: subtree ids (0 for a tip) and the edge's length
   struct EdgeSides {
     uint64_t p, c;
@@ -999,6 +1014,13 @@ static void worker_loop(Server *sv) {
       if (!f.parts.empty()) {
         flights.push_back(std::move(f));
         next_stage = (next_stage + 1) % kTickStages;
+        // a tick is queued: free the requests clients have collected while
+        // the device runs it
+        std::vector<std::unique_ptr<Req>> done_reqs;
+        {
+          std::lock_guard<std::mutex> lk(sv->mu);
+          done_reqs.swap(sv->collected);
+        }
       }
     } else if (!flights.empty()) {
       Flight f = std::move(flights.front());
@@ -1231,6 +1253,7 @@ extern "C" int gle_serve_wait(int h, int req, double *lnl, double *blen_out, dou
   *lnl = r->lnl;
   if (blen_out) std::copy(r->blen.begin(), r->blen.end(), blen_out);
   if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+  sv->collected.push_back(std::move(it->second));
   sv->reqs.erase(it);
   return rc;
 }
@@ -1256,7 +1279,12 @@ extern "C" int gle_serve_wait_all(int h, int n, const int *reqs, double *lnl) {
     lnl[i] = r->lnl;
     if (!rc && r->rc) rc = r->rc;
   }
-  for (int i = 0; i < n; i++) sv->reqs.erase(reqs[i]);
+  for (int i = 0; i < n; i++) {
+    auto it = sv->reqs.find(reqs[i]);
+    if (it == sv->reqs.end()) continue;  // listed twice
+    sv->collected.push_back(std::move(it->second));
+    sv->reqs.erase(it);
+  }
   return rc;
 }
 
