@@ -2987,6 +2987,12 @@ extern "C" int gle_create_instance(const gle_instance_desc *d, int *handle) {
   std::unique_ptr<Instance> I;
   const int rc = create_internal(d, I);
   if (rc) return rc;
+  {
+    // the CLA slot pool (its free list holds every slot) is made with the
+    // instance rather than by its first tree or served tick
+    std::lock_guard<std::mutex> pk(pool_mutex());
+    pool_for(I.get());
+  }
   std::lock_guard<std::mutex> lk(g_mu);
   g_instances.push_back(std::move(I));
   *handle = int(g_instances.size()) - 1;

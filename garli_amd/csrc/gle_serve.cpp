@@ -308,10 +308,15 @@ static int tree_topology(Req &r) {
   }
   for (int n = 0; n < r.nn; n++)
     if (r.nnb[n] != ((n >= 1 && n <= r.N) ? 1 : 3)) return set_error(GLE_ERR_ARG, "node %d: wrong degree", n);
+  // every node reaches the root: walk up until a node known to reach it,
+  // then mark the walked path (linear overall; a cycle exceeds nn steps)
+  std::vector<char> reaches(size_t(r.nn), 0);
+  reaches[0] = 1;
   for (int n = 1; n < r.nn; n++) {
     int x = n, steps = 0;
-    while (x != 0 && steps++ <= r.nn) x = r.parent[x];
-    if (x != 0) return set_error(GLE_ERR_ARG, "node %d does not reach the root", n);
+    while (!reaches[size_t(x)] && steps++ <= r.nn) x = r.parent[x];
+    if (!reaches[size_t(x)]) return set_error(GLE_ERR_ARG, "node %d does not reach the root", n);
+    for (int y = n; !reaches[size_t(y)]; y = r.parent[y]) reaches[size_t(y)] = 1;
   }
   return GLE_SUCCESS;
 }

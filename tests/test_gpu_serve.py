@@ -111,6 +111,24 @@ def test_wait_all_collects_a_burst(gpu):
     srv.close()
 
 
+def test_submit_rejects_a_detached_cycle(gpu):
+    """A parent array whose degrees are all right but whose nodes 10..13 form
+    a cycle detached from the root (each with one tip) is refused at
+    submission; the server then serves a valid tree."""
+    ds = sim.gtr_gamma(n_tips=8, n_sites=200, seed=111)
+    parent = np.array([-1, 0, 0, 9, 9, 10, 11, 12, 13, 0, 11, 12, 13, 10], dtype=np.int32)
+    inst = ds.make_instance(E.PREC_FP64, n_cla_slots=40, n_pmat_slots=100)
+    srv = E.Server()
+    with pytest.raises(E.GleError):
+        srv.submit(inst, parent, ds.blen)
+    with pytest.raises(E.GleError):
+        srv.submit_batch(inst, [(ds.parent, ds.blen), (parent, ds.blen)])
+    got = srv.wait(srv.submit(inst, ds.parent, ds.blen))[0]
+    ref = O.Oracle(ds).score(parent=ds.parent, blen=ds.blen)
+    assert abs(got - ref) < 1e-9 * abs(ref)
+    srv.close()
+
+
 def test_admission_at_minimum_budget(gpu):
     """A score touches at most the N - 2 CLAs that point at its edge, so an
     instance with exactly N - 2 slots serves scores one at a time."""
