@@ -435,12 +435,15 @@ struct TickBuilder {
   // message is not kept already.  An offspring's own new subtrees are seldom
   // met again, and their messages would only cost writes.  Messages never
   // take a slot from a CLA, so admission's working sets are unaffected.
-  void keep_messages(const std::set<std::pair<uint64_t, uint64_t>> &want) {
+  void keep_messages(const std::vector<std::pair<uint64_t, uint64_t>> &want) {  // want: sorted
     std::unordered_map<uint64_t, int> kept;
     for (size_t i = 0; i < ops.size(); i++)
       for (int k = 0; k < 2; k++) {
         const Kid &kd = kids[i][k];
-        if (!kd.id || !(kd.cached || uses[kd.id] > 1 || want.count({kd.id, dbits(kd.len)}))) continue;
+        if (!kd.id ||
+            !(kd.cached || uses[kd.id] > 1 ||
+              std::binary_search(want.begin(), want.end(), std::make_pair(kd.id, dbits(kd.len)))))
+          continue;
         const uint64_t name = si->intern_id(msg_key(kd.id, kd.len));
         if (si->msg.has(name) || kept.count(name)) continue;
         if (!pool->has_free()) {
@@ -782,17 +785,23 @@ static int build_part(Server &sv, ServeInstance &si, std::vector<Req *> &batch, 
   // edge sum is symmetric in its two sides for a reversible model, so the
   // partial side's message serves with the sides swapped).
   const bool edge_msgs = si.msgs && !tb.derivs;
-  std::set<std::pair<uint64_t, uint64_t>> want;
+  std::vector<std::pair<uint64_t, uint64_t>> want;  // sorted
   if (edge_msgs) {
-    std::set<std::pair<uint64_t, uint64_t>> formed;  // (child id, length bits) the tick contracts
+    std::vector<std::pair<uint64_t, uint64_t>> formed;  // (child id, length bits) the tick contracts
+    formed.reserve(2 * tb.kids.size());
     for (const auto &kd : tb.kids)
       for (const auto &k : kd)
-        if (k.id) formed.insert({k.id, dbits(k.len)});
+        if (k.id) formed.push_back({k.id, dbits(k.len)});
+    std::sort(formed.begin(), formed.end());
+    auto has = [&](const std::pair<uint64_t, uint64_t> &x) {
+      return std::binary_search(formed.begin(), formed.end(), x);
+    };
     for (const EdgeSides &e : sides) {
       const std::pair<uint64_t, uint64_t> c{e.c, dbits(e.len)}, p{e.p, dbits(e.len)};
-      if (e.c && formed.count(c)) want.insert(c);
-      else if (e.c && e.p && formed.count(p)) want.insert(p);
+      if (e.c && has(c)) want.push_back(c);
+      else if (e.c && e.p && has(p)) want.push_back(p);
     }
+    std::sort(want.begin(), want.end());
   }
   if (si.msgs) tb.keep_messages(want);
   if (edge_msgs) {

@@ -910,8 +910,9 @@ class Server:
     def submit_batch(self, inst, trees, kind=REQ_SCORE, precision=0.0):
         """Submit [(parent, blen), ...] on one instance in one call; returns the ids."""
         nn = 2 * inst.n_tips - 2
-        par = np.ascontiguousarray(np.concatenate([np.asarray(p, np.int32) for p, _ in trees]), dtype=np.int32)
-        bl = _d(np.concatenate([np.asarray(b, np.float64) for _, b in trees]))
+        ps, bs = zip(*trees) if len(trees) else ((), ())
+        par = np.ascontiguousarray(np.concatenate(ps) if ps else np.zeros(0), dtype=np.int32)
+        bl = _d(np.concatenate(bs) if bs else np.zeros(0))
         assert len(par) == nn * len(trees)
         ids = np.zeros(len(trees), dtype=np.int32)
         _check(lib().gle_serve_submit_batch(self.h, inst.h, len(trees), _p(par, C.c_int), _p(bl, C.c_double),
