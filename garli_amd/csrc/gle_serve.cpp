@@ -255,6 +255,7 @@ struct Req {
   bool finished = false;  // published under Server::mu once the worker let go
   std::chrono::steady_clock::time_point t_submit, t_done;
   std::vector<std::array<uint64_t, 3>> sid;  // X(a|b) id memo, sid[a][index of b among a's neighbours]
+  std::vector<std::array<uint64_t, 3>> shape;  // X(a|b) content hash (the tick's sharing count), same indexing
   unsigned long long sid_names = ~0ull;      // the instance's name-table generation sid belongs to
   // per-tick scratch
   int edge_a = 0, edge_b = 0;
@@ -351,6 +352,39 @@ static uint64_t known_id(Req &r, const ServeInstance &si, int a, int b) {
   return m;
 }
 
+// Scores in a tick from which their edge choice weighs subtrees by sharing.
+constexpr int kShareMinScores = 8;
+
+// Counts of 64-bit keys (nonzero), open addressing with linear probing.
+class KeyCount {
+  std::vector<std::pair<uint64_t, int>> t_;
+  size_t mask_ = 0;
+
+ public:
+  void reset(size_t n) {
+    size_t cap = 64;
+    while (cap < 2 * n) cap <<= 1;
+    t_.assign(cap, {0, 0});
+    mask_ = cap - 1;
+  }
+  void add(uint64_t k) {
+    if (t_.empty()) return;
+    for (size_t i = size_t(mix64(k)) & mask_;; i = (i + 1) & mask_)
+      if (t_[i].first == k || t_[i].first == 0) {
+        t_[i].first = k;
+        t_[i].second++;
+        return;
+      }
+  }
+  int get(uint64_t k) const {
+    if (t_.empty()) return 0;
+    for (size_t i = size_t(mix64(k)) & mask_;; i = (i + 1) & mask_) {
+      if (t_[i].first == k) return t_[i].second;
+      if (t_[i].first == 0) return 0;
+    }
+  }
+};
+
 // ------------------------------------------------------------ one tick
 struct TickNew {
   int slot, rank, level;
@@ -379,6 +413,7 @@ struct TickBuilder {
   };
   std::vector<std::array<Kid, 2>> kids;
   std::unordered_map<uint64_t, int> uses;  // per subtree id: children contracted from it this tick
+  KeyCount sharing;  // per subtree content hash: the tick's scores whose trees contain it
   std::vector<std::pair<uint64_t, TickNew>> fresh_msgs;  // messages kept this tick
 
   struct Mark {
@@ -578,36 +613,58 @@ struct TickBuilder {
   }
 };
 
+// Content hash of X(a|b) in request r (tips, topology, branch-length bits),
+// memoised in r.shape; a key for counting how many of a tick's requests
+// contain the same subtree, not a name (names are interned exactly).
+static uint64_t shape_hash(Req &r, int a, int b) {
+  if (req_tip(r, a)) return mix64(uint64_t(a) * 0x9E3779B97F4A7C15ull);
+  uint64_t &m = r.shape[a][req_nbk(r, a, b)];
+  if (m) return m;
+  int x, y;
+  req_children(r, a, b, x, y);
+  uint64_t px = mix64(shape_hash(r, x, a) ^ dbits(req_len(r, a, x)) * 0xC2B2AE3D27D4EB4Full);
+  uint64_t py = mix64(shape_hash(r, y, a) ^ dbits(req_len(r, a, y)) * 0xC2B2AE3D27D4EB4Full);
+  if (px > py) std::swap(px, py);
+  m = mix64(px * 0x165667B19E3779F9ull + py) | 1;
+  return m;
+}
+
 // Scoring edge of a request.  lnL is the same on every edge (the pruning
 // algorithm's root invariance; the reference's RunTests checks it,
-// population.cpp:924-954), so a score is taken where the fewest CLAs are
-// missing from the cache and this tick: next to the perturbation, for an
-// offspring whose parent's CLAs are cached.  Ties keep the reference's edge
-// (root, middle child).  The choice depends on what is cached, so a result
-// alone and in a mix agree to the form's rounding, not bit for bit.
-// CLAs missing for X(a|b), memoised in memo
-static int missing_clas(Req &r, const TickBuilder &tb, std::vector<std::array<int, 3>> &memo, int a, int b) {
-  if (req_tip(r, a)) return 0;
-  int &m = memo[a][req_nbk(r, a, b)];
-  if (m >= 0) return m;
+// population.cpp:924-954), so a score is taken where the fewest new CLAs are
+// needed: next to the perturbation, for an offspring whose parent's CLAs are
+// cached.  A missing CLA counts 1/k when k of the tick's scores contain the
+// same subtree (it is computed once for all of them), so an offspring does
+// not avoid the parent's uncached subtrees that the rest of the generation
+// needs too.  Ties keep the reference's edge (root, middle child).  The
+// choice depends on what is cached, so a result alone and in a mix agree to
+// the form's rounding, not bit for bit.
+// Weighted CLAs missing for X(a|b), memoised in memo (< 0: not yet)
+static double missing_clas(Req &r, const TickBuilder &tb, std::vector<std::array<double, 3>> &memo, int a, int b) {
+  if (req_tip(r, a)) return 0.0;
+  const int k = req_nbk(r, a, b);
+  double &m = memo[a][k];
+  if (m >= 0.0) return m;
   const uint64_t id = known_id(r, *tb.si, a, b);
-  if (id && (tb.si->cla.has(id) || tb.fresh.count(id))) return m = 0;
-  int c = 1;
+  if (id && (tb.si->cla.has(id) || tb.fresh.count(id))) return m = 0.0;
+  // (a lone score in its tick has no sharing counts: each missing CLA is 1)
+  const int shared = r.shape.size() == size_t(r.nn) ? tb.sharing.get(shape_hash(r, a, b)) : 1;
+  double c = 1.0 / std::max(1, shared);
   for (int j = 0; j < r.nnb[a]; j++)
     if (r.nb[a][j] != b) c += missing_clas(r, tb, memo, r.nb[a][j], a);
   return m = c;
 }
 
 static void choose_score_edge(Req &r, const TickBuilder &tb) {
-  static thread_local std::vector<std::array<int, 3>> memo;
-  memo.assign(r.nn, {-1, -1, -1});
+  static thread_local std::vector<std::array<double, 3>> memo;
+  memo.assign(r.nn, {-1.0, -1.0, -1.0});
   auto cost = [&](int a, int b) { return missing_clas(r, tb, memo, a, b); };
   r.edge_a = 0;
   r.edge_b = r.nb[0][1];
-  int best = cost(r.edge_a, r.edge_b) + cost(r.edge_b, r.edge_a);
-  for (int n = 1; n < r.nn && best > 0; n++) {
+  double best = cost(r.edge_a, r.edge_b) + cost(r.edge_b, r.edge_a);
+  for (int n = 1; n < r.nn && best > 0.0; n++) {
     const int a = r.parent[n];
-    const int c = cost(n, a) + cost(a, n);
+    const double c = cost(n, a) + cost(a, n);
     if (c < best) {
       best = c;
       r.edge_a = a;
@@ -729,6 +786,24 @@ static int build_part(Server &sv, ServeInstance &si, std::vector<Req *> &batch, 
     tb.fresh_order.reserve(cap);
     tb.ops.reserve(cap);
     tb.kids.reserve(cap);
+  }
+  {
+    // how many of the tick's scores contain each directional subtree: for a
+    // tick of many scores (a generation's offspring), where the counts steer
+    // edges towards subtrees computed once for all; a small tick (a request
+    // stream, Newton steps) skips the pass and its cost
+    int scores = 0;
+    for (const Req *r : batch) scores += r->scoring ? 1 : 0;
+    if (scores >= kShareMinScores) {
+      tb.sharing.reset(size_t(scores) * 2 * size_t(instance_tips(si.I)) * 2);
+      for (Req *r : batch) {
+        if (!r->scoring) continue;
+        r->shape.assign(r->nn, {0, 0, 0});
+        for (int a = 0; a < r->nn; a++)
+          if (!req_tip(*r, a))
+            for (int k = 0; k < r->nnb[a]; k++) tb.sharing.add(shape_hash(*r, a, r->nb[a][k]));
+      }
+    }
   }
   // each edge's two sides: subtree ids (0 for a tip) and the edge's length
   struct EdgeSides {
