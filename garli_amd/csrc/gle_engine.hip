@@ -788,10 +788,15 @@ __global__ __launch_bounds__(64 * R, sizeof(T) == 4 ? 1 : 3) void k_cla_aa(T *__
       T x[S];
 #pragma unroll
       for (int q = 0; q < S; q++) x[q] = cla[c * slot + off + 64 * q];  // p < Ppad: inside the slot
+      // FP32 form: lanes past the last pattern compute on whatever their
+      // padded rows hold (nothing of theirs is stored or compared with
+      // another pattern), and each sum starts from its first product
+      if constexpr (!kF32) {
 #pragma unroll
-      for (int q = 0; q < S; q++) x[q] = live ? x[q] : T(0);
+        for (int q = 0; q < S; q++) x[q] = live ? x[q] : T(0);
 #pragma unroll
-      for (int f = 0; f < S; f++) acc[f] = 0.0;
+        for (int f = 0; f < S; f++) acc[f] = 0.0;
+      }
 #pragma unroll
       for (int q = 0; q < S; q++) {
         const double xq = double(x[q]);
@@ -801,7 +806,7 @@ __global__ __launch_bounds__(64 * R, sizeof(T) == 4 ? 1 : 3) void k_cla_aa(T *__
 #pragma unroll
         for (int f = 0; f < S; f++) {
           if constexpr (kF32)
-            acc[f] = __builtin_fma(col[f], xq, acc[f]);
+            acc[f] = q == 0 ? col[f] * xq : __builtin_fma(col[f], xq, acc[f]);
           else
             acc[f] = add_nc(acc[f], mul_nc(col[f], xq));
         }
