@@ -891,8 +891,8 @@ __global__ __launch_bounds__(64 * R, sizeof(T) == 4 ? 1 : 3) void k_cla_aa(T *__
         const unsigned seed = mix32((unsigned(p) * R + unsigned(r)) * 0x9e3779b1u ^
                                     unsigned(static_cast<unsigned long long>(__double_as_longlong(o[0])) >> 29));
 #pragma unroll
-        for (int f = 0; f < S; f++)
-          dst[64 * f] = round_dither_bits(scale ? __builtin_amdgcn_ldexp(o[f], e) : o[f], dither_stream(seed, unsigned(f)));
+        for (int f = 0; f < S; f++)  // e = 0 where the pattern is not scaled
+          dst[64 * f] = round_dither_bits(__builtin_amdgcn_ldexp(o[f], e), dither_stream(seed, unsigned(f)));
       } else {
 #pragma unroll
         for (int f = 0; f < S; f++) dst[64 * f] = scale ? mul_nc(o[f], mult) : o[f];
