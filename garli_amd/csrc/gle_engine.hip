@@ -2617,6 +2617,18 @@ int engine_tick_collect(Instance *I, int sidx, double *res, int *flags) {
   return rc;
 }
 
+bool engine_tick_ready(Instance *I, int sidx) {
+  std::lock_guard<std::mutex> lk_(I->mu);
+  if (sidx < 0 || sidx >= kTickStages) return true;
+  for (Instance *X : {I, I->heavy.get()}) {
+    if (!X) continue;
+    const Instance::TickStage &st = X->stage[sidx];
+    // an error is reported by the collect, which then does not wait either
+    if (st.busy && hipEventQuery(st.done) == hipErrorNotReady) return false;
+  }
+  return true;
+}
+
 int engine_edges(Instance *I, const DevEdgeOp *ops, int n, double *out, double *site_lnl, int *flags) {
   return edges_impl(I, ops, n, out, site_lnl, 1, flags);
 }

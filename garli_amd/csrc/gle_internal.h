@@ -88,6 +88,9 @@ int engine_tick_enqueue(Instance *, int sidx, const int *pm_idx, const double *p
                         const DevClaOp *ops, int n_ops, const int *level_start, int n_levels, const DevEdgeOp *eops,
                         int n_e);
 int engine_tick_collect(Instance *, int sidx, double *res, int *flags);
+// True when the tick in stage sidx has finished on the device (both parts),
+// or none is in flight there: engine_tick_collect will then not wait.
+bool engine_tick_ready(Instance *, int sidx);
 int engine_get_partials(Instance *, int slot, double *out, int *u);
 int engine_sync(Instance *);
 bool engine_supports_fused(const Instance *);

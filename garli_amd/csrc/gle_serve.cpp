@@ -209,6 +209,8 @@ struct ServeInstance {
   LruCache pm;   // blen bits -> P, P', P'' slot
   bool msgs = false;  // codon instance: keep and read messages
   int reserved = 0;                              // working-set slots of admitted requests
+  int next_stage = 0;                            // staging buffers of the next part queued
+  int parts_in_flight = 0;                       // parts queued and not yet collected (worker only)
   unsigned long long gen = 0;                    // instance generation the caches belong to
   unsigned long long epoch = 0;                  // bumped whenever the CLA cache is dropped
   unsigned long long names = 0;                  // bumped whenever the name table is cleared
@@ -728,22 +730,18 @@ static void advance(Req &r, const double *res) {
 // One instance's share of a tick: the requests it carries, the CLAs it
 // computes (named, so a failed tick can take them back out of the cache) and
 // the cache epoch it was built on.
+// Up to two parts of an instance are in flight, so the host builds the next
+// while the device runs the current one.  Every part's work is on its
+// instance's own stream, so a later part that reads CLAs an earlier one
+// writes, or reuses a slot an earlier one read, runs after it; parts of
+// different instances run and are collected independently.
 struct TickPart {
   ServeInstance *si = nullptr;
   std::vector<Req *> reqs;
   std::vector<uint64_t> fresh_ids;
   unsigned long long epoch = 0;
   int n_edges = 0;
-};
-// A tick queued on the device: up to two are in flight, so the host builds
-// the next while the device runs the current one.  Every tick's work is on
-// the instances' own streams, so a later tick that reads CLAs an earlier one
-// writes, or reuses a slot an earlier one read, runs after it.
-struct Flight {
-  uint64_t tick = 0;
-  int stage = 0;
-  std::vector<TickPart> parts;
-  std::chrono::steady_clock::time_point t_launch;
+  int stage = 0;  // the instance's staging buffers it was queued with
 };
 
 // Builds one instance's part of a tick from `batch` and queues it on the
@@ -959,11 +957,12 @@ static int build_part(Server &sv, ServeInstance &si, std::vector<Req *> &batch, 
 // the reference's rescale throw (tree.cpp:3436-3449, FP64) drops the caches
 // and lowers rescaleEvery by 2 (Tree::Score's retry, tree.cpp:4077-4090) and
 // the requests run their step again.
-static void collect_part(Flight &f, TickPart &part) {
+static void collect_part(TickPart &part) {
   ServeInstance &si = *part.si;
+  si.parts_in_flight--;
   std::vector<double> res(3 * size_t(std::max(part.n_edges, 1)));
   int flags = 0;
-  const int rc = engine_tick_collect(si.I, f.stage, res.data(), &flags);
+  const int rc = engine_tick_collect(si.I, part.stage, res.data(), &flags);
   for (Req *r : part.reqs) r->in_flight = false;
   const bool stale = part.epoch != si.epoch;
   const bool thrown = !rc && instance_precision(si.I) == GLE_PREC_FP64 &&
@@ -1021,8 +1020,7 @@ static void admit(Server &sv) {
 }
 
 static void worker_loop(Server *sv) {
-  std::deque<Flight> flights;
-  int next_stage = 0;
+  std::deque<TickPart> inflight;  // parts queued on the device, in the order they were queued
   for (;;) {
     std::vector<std::pair<ServeInstance *, std::vector<Req *>>> by_inst;
     bool stop = false;
@@ -1033,9 +1031,9 @@ static void worker_loop(Server *sv) {
           if (!r->in_flight && !r->done) return true;
         return false;
       };
-      sv->cv_work.wait(lk, [&] { return sv->stop || !sv->pending.empty() || !flights.empty() || has_ready(); });
+      sv->cv_work.wait(lk, [&] { return sv->stop || !sv->pending.empty() || !inflight.empty() || has_ready(); });
       stop = sv->stop;
-      if (!stop && sv->active.empty() && flights.empty() && sv->batch_window_us > 0) {
+      if (!stop && sv->active.empty() && inflight.empty() && sv->batch_window_us > 0) {
         // idle until now: let a burst of submissions gather into the first
         // tick, until no request has arrived for a fifth of the window (at
         // least 20 us) or the window has passed
@@ -1054,32 +1052,30 @@ static void worker_loop(Server *sv) {
       }
       if (!stop) {
         admit(*sv);
-        if (flights.size() < 2) {
-          // One tick runs while the next is built and queued.  A large burst
-          // (a generation's offspring) reaching an idle device starts with a
-          // quarter of it, at most 8 requests, so the device starts early and
-          // the rest is built while that part runs.
-          int ready = 0;
-          for (Req *r : sv->active) ready += !(r->in_flight || r->done);
-          const int chunk = (flights.empty() && ready >= 16) ? std::min((ready + 3) / 4, 8) : ready;
-          int taken = 0;
-          // the instances are looked up here, under the lock that submissions
-          // insert under (map references stay valid across later inserts)
-          for (Req *r : sv->active) {
-            if (r->in_flight || r->done || taken >= chunk) continue;
-            taken++;
-            ServeInstance *si = &sv->insts[r->inst];
-            auto it = std::find_if(by_inst.begin(), by_inst.end(), [&](const auto &e) { return e.first == si; });
-            if (it == by_inst.end()) by_inst.push_back({si, {r}});
-            else it->second.push_back(r);
-          }
+        // An instance runs one part while its next is built and queued.  A
+        // large burst (a generation's offspring) reaching an idle device
+        // starts with a quarter of it, at most 8 requests, so the device
+        // starts early and the rest is built while that part runs.
+        int ready = 0;
+        for (Req *r : sv->active) ready += !(r->in_flight || r->done);
+        const int chunk = (inflight.empty() && ready >= 16) ? std::min((ready + 3) / 4, 8) : ready;
+        int taken = 0;
+        // the instances are looked up here, under the lock that submissions
+        // insert under (map references stay valid across later inserts)
+        for (Req *r : sv->active) {
+          if (r->in_flight || r->done || taken >= chunk) continue;
+          ServeInstance *si = &sv->insts[r->inst];
+          if (si->parts_in_flight >= 2) continue;
+          taken++;
+          auto it = std::find_if(by_inst.begin(), by_inst.end(), [&](const auto &e) { return e.first == si; });
+          if (it == by_inst.end()) by_inst.push_back({si, {r}});
+          else it->second.push_back(r);
         }
         if (!by_inst.empty()) sv->tick++;
       }
     }
     if (stop) {  // let the queued work finish before the caches are released
-      for (Flight &f : flights)
-        for (TickPart &p : f.parts) collect_part(f, p);
+      for (TickPart &p : inflight) collect_part(p);
       return;
     }
     double host_ms = 0.0, device_ms = 0.0;
@@ -1087,35 +1083,50 @@ static void worker_loop(Server *sv) {
     long long in_tick = 0;
     int deferred = 0;
     if (!by_inst.empty()) {
-      Flight f;
-      f.tick = sv->tick;
-      f.stage = next_stage;
-      f.t_launch = std::chrono::steady_clock::now();
+      const uint64_t tick = sv->tick;  // written by this thread only
       for (auto &kv : by_inst) {
         // requests deferred last time go first, so none waits behind the same others forever
         std::stable_sort(kv.second.begin(), kv.second.end(),
                          [](const Req *a, const Req *b) { return a->waits > b->waits; });
         TickPart part;
-        deferred += build_part(*sv, *kv.first, kv.second, f.tick, f.stage, part, host_ms, work);
+        part.stage = kv.first->next_stage;
+        deferred += build_part(*sv, *kv.first, kv.second, tick, part.stage, part, host_ms, work);
         in_tick += long(part.reqs.size());
-        if (!part.reqs.empty()) f.parts.push_back(std::move(part));
+        if (!part.reqs.empty()) {
+          kv.first->next_stage = (part.stage + 1) % kTickStages;
+          kv.first->parts_in_flight++;
+          inflight.push_back(std::move(part));
+        }
       }
-      if (!f.parts.empty()) {
-        flights.push_back(std::move(f));
-        next_stage = (next_stage + 1) % kTickStages;
-        // a tick is queued: free the requests clients have collected while
-        // the device runs it
+      if (in_tick) {
+        // parts are queued: free the requests clients have collected while
+        // the device runs them
         std::vector<std::unique_ptr<Req>> done_reqs;
         {
           std::lock_guard<std::mutex> lk(sv->mu);
           done_reqs.swap(sv->collected);
         }
       }
-    } else if (!flights.empty()) {
-      Flight f = std::move(flights.front());
-      flights.pop_front();
+    } else if (!inflight.empty()) {
+      // collect every part the device has finished, each instance's parts in
+      // the order they were queued, so an instance whose part is done goes
+      // on without waiting for other instances' larger parts; with none
+      // finished, look again (new requests are picked up meanwhile)
       const auto t0 = std::chrono::steady_clock::now();
-      for (TickPart &p : f.parts) collect_part(f, p);
+      std::vector<const ServeInstance *> waiting;  // instances with an unfinished earlier part
+      bool any = false;
+      for (auto it = inflight.begin(); it != inflight.end();) {
+        const ServeInstance *si = it->si;
+        if (std::find(waiting.begin(), waiting.end(), si) != waiting.end() || !engine_tick_ready(si->I, it->stage)) {
+          waiting.push_back(si);
+          ++it;
+          continue;
+        }
+        collect_part(*it);
+        it = inflight.erase(it);
+        any = true;
+      }
+      if (!any) std::this_thread::yield();
       device_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     {
