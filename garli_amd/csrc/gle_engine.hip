@@ -125,6 +125,10 @@ struct Instance {
   int eops_cap = 0;
   double *d_part = nullptr;
   size_t part_cap = 0;
+  // a heavy part's edge partials in serving ticks, which run on the main
+  // part's stream (d_part stays with the work on the heavy part's own stream)
+  double *d_tpart = nullptr;
+  size_t tpart_cap = 0;
   double *d_eres = nullptr;
   int eres_cap = 0;
   double *d_site = nullptr;
@@ -196,8 +200,9 @@ struct Instance {
     size_t cap = 0;
     hipEvent_t done = nullptr;
     int n_edges = 0;
-    double *res = nullptr;
-    int *flags = nullptr;
+    double *res = nullptr;  // n_edges x 3, then the heavy part's n_edges x 3 (with_heavy)
+    int *flags = nullptr;   // this part's rescale flags, then the heavy part's
+    bool with_heavy = false;
     bool busy = false;
   } stage[kTickStages];
   std::unique_ptr<Instance> heavy;
@@ -2437,12 +2442,72 @@ static size_t part_cap_floor(const Instance *X) {
   return std::max(X->part_cap, size_t(64) * edge_blocks(X) * 3);
 }
 
-// One tick of the serving layer on one part (the instance or its heavy
-// part), enqueued without any wait: P, P', P'' for the new lengths, the
-// rescale-flag reset, the CLA levels, the edges and their reduction, then the
-// results and flags copied into pinned memory, and an event.  The lists are
-// staged in pinned memory first, so every copy is asynchronous.
-static int tick_enqueue_part(Instance *X, int sidx, const int *pm_idx, const double *pm_len, int n_pm,
+// The CLA levels, the edges and their reduction of one part of a tick, on
+// X's stream, reading the staged op lists; the rescale flags go to d_flags and
+// the (lnL, d1, d2) of the edges to d_res.  The edge partials go to the
+// part's own buffer, or (heavy) to the one kept for its serving ticks.
+static int tick_part_kernels(Instance *X, bool heavy, const DevClaOp *ops, const DevClaOp *d_ops,
+                             const int *level_start, int n_levels, const DevEdgeOp *eops, const DevEdgeOp *d_eops,
+                             int n_e, int *d_flags, double *d_res) {
+  int *flags_save = X->d_flags;
+  X->d_flags = d_flags;
+  int rc = GLE_SUCCESS;
+  for (int l = 0; l < n_levels && !rc; l++) {
+    const int b = level_start[l], e = level_start[l + 1];
+    if (e <= b) continue;
+    double bytes = 0.0;
+    for (int i = b; i < e; i++) bytes += cla_op_bytes(X, ops[i]);
+    timed_begin(X, 0, bytes);
+    rc = X->prec == GLE_PREC_FP64 ? launch_cla<double>(X, d_ops + b, e - b, false)
+                                  : launch_cla<float>(X, d_ops + b, e - b, false);
+    timed_end(X);
+  }
+  X->d_flags = flags_save;
+  if (rc || n_e == 0) return rc;
+  const int nblk = edge_blocks(X);
+  const size_t part = size_t(n_e) * nblk * 3;
+  double *&d_part = heavy ? X->d_tpart : X->d_part;
+  size_t &part_cap = heavy ? X->tpart_cap : X->part_cap;
+  if (part > part_cap) {
+    // a tick grows the buffer without hipFree, which would wait for the
+    // whole device (the tick in flight included) before this tick's edges
+    // could be queued; the old buffer is freed with the instance
+    if (d_part) X->retired.push_back(d_part);
+    d_part = nullptr;
+    part_cap = 0;
+    const size_t cap = std::max(part, 2 * part_cap_floor(X));
+    GLE_HIP(hipMalloc(&d_part, sizeof(double) * cap));
+    part_cap = cap;
+  }
+  bool derivs = false;
+  double eb = 0.0;
+  for (int i = 0; i < n_e; i++) {
+    derivs = derivs || eops[i].derivs;
+    eb += edge_op_bytes(X, eops[i]);
+  }
+  double *site_save = X->d_site;
+  X->d_site = nullptr;
+  timed_begin(X, 1, eb);
+  rc = X->prec == GLE_PREC_FP64 ? launch_edges<double>(X, d_eops, n_e, nblk, derivs, d_part)
+                                : launch_edges<float>(X, d_eops, n_e, nblk, derivs, d_part);
+  timed_end(X);
+  X->d_site = site_save;
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_edge_reduce, dim3(n_e), dim3(64), 0, X->stream, d_part, nblk, d_res);
+  return GLE_SUCCESS;
+}
+
+// One tick of the serving layer on an instance and its heavy part (H, or
+// null), enqueued without any wait on the instance's stream: P, P', P'' for
+// the new lengths, the CLA levels, the edges and their reduction of the
+// instance, then those of the heavy part, which read the same staged lists
+// and the instance's FP64 P slots (the same values: one P(t) kernel writes
+// both parts' layouts), and one copy of both parts' results and rescale flags
+// into pinned memory, and an event.  The lists are staged in pinned memory
+// first, so every copy is asynchronous.  Queuing the heavy part on its own
+// stream took a second upload, P(t) launch, download and event per tick
+// (about 2.4 us of host time each) for its few patterns.
+static int tick_enqueue_part(Instance *X, Instance *H, int sidx, const int *pm_idx, const double *pm_len, int n_pm,
                              int pm_derivs, const DevClaOp *ops, int n_ops, const int *level_start, int n_levels,
                              const DevEdgeOp *eops, int n_e) {
   Instance::TickStage &st = X->stage[sidx];
@@ -2475,13 +2540,14 @@ static int tick_enqueue_part(Instance *X, int sidx, const int *pm_idx, const dou
   // a message child is read only by the derivative-free edge kernel
   if (any_msg && any_derivs) return set_error(GLE_ERR_ARG, "tick edges read messages in a launch with derivatives");
   // one staging layout in pinned host memory and on the device:
-  // [P(t) slots | lengths | CLA ops | edge ops | flags | results]; the
-  // upload carries everything up to the (zeroed) flags, the download the
-  // flags and the results
+  // [P(t) slots | lengths | CLA ops | edge ops | flags (2 ints) | results |
+  // heavy part's results]; the upload carries everything up to the (zeroed)
+  // flags, the download the flags and the results
   auto a8 = [](size_t b) { return (b + 7) & ~size_t(7); };
+  const int parts = H ? 2 : 1;
   const size_t o_len = a8(sizeof(int) * n_pm), o_ops = o_len + sizeof(double) * n_pm,
                o_eops = o_ops + a8(sizeof(DevClaOp) * n_ops), o_flags = o_eops + a8(sizeof(DevEdgeOp) * n_e),
-               o_res = o_flags + 8, need = o_res + sizeof(double) * 3 * n_e;
+               o_res = o_flags + 8, need = o_res + sizeof(double) * 3 * n_e * parts;
   if (need > st.cap) {
     const size_t cap = std::max(need + need / 2, size_t(64) << 10);
     // retired rather than freed: either free would wait for the other
@@ -2516,58 +2582,30 @@ static int tick_enqueue_part(Instance *X, int sidx, const int *pm_idx, const dou
                X->d_pdiag, pm_derivs, S == 20 ? kPadTransposed : S == 61 ? kPadCodonFrag : kPadRowMajor);
     timed_end(X);
   }
-  // the CLA kernels raise their rescale flags in this tick's own word
-  int *flags_save = X->d_flags;
-  X->d_flags = d_flags;
-  int rc = GLE_SUCCESS;
-  for (int l = 0; l < n_levels && !rc; l++) {
-    const int b = level_start[l], e = level_start[l + 1];
-    if (e <= b) continue;
-    double bytes = 0.0;
-    for (int i = b; i < e; i++) bytes += cla_op_bytes(X, ops[i]);
-    timed_begin(X, 0, bytes);
-    rc = X->prec == GLE_PREC_FP64 ? launch_cla<double>(X, d_ops + b, e - b, false)
-                                  : launch_cla<float>(X, d_ops + b, e - b, false);
-    timed_end(X);
-  }
-  X->d_flags = flags_save;
+  // the CLA kernels raise their rescale flags in this tick's own words
+  double *d_res = reinterpret_cast<double *>(st.d + o_res);
+  int rc = tick_part_kernels(X, false, ops, d_ops, level_start, n_levels, eops, d_eops, n_e, d_flags, d_res);
   if (rc) return rc;
-  if (n_e > 0) {
-    const int nblk = edge_blocks(X);
-    const size_t part = size_t(n_e) * nblk * 3;
-    if (part > X->part_cap) {
-      // a tick grows the buffer without hipFree, which would wait for the
-      // whole device (the tick in flight included) before this tick's edges
-      // could be queued; the old buffer is freed with the instance
-      if (X->d_part) X->retired.push_back(X->d_part);
-      X->d_part = nullptr;
-      X->part_cap = 0;
-      const size_t cap = std::max(part, 2 * part_cap_floor(X));
-      GLE_HIP(hipMalloc(&X->d_part, sizeof(double) * cap));
-      X->part_cap = cap;
-    }
-    bool derivs = false;
-    double eb = 0.0;
-    for (int i = 0; i < n_e; i++) {
-      derivs = derivs || eops[i].derivs;
-      eb += edge_op_bytes(X, eops[i]);
-    }
-    double *site_save = X->d_site;
-    X->d_site = nullptr;
-    timed_begin(X, 1, eb);
-    rc = X->prec == GLE_PREC_FP64 ? launch_edges<double>(X, d_eops, n_e, nblk, derivs, X->d_part)
-                                  : launch_edges<float>(X, d_eops, n_e, nblk, derivs, X->d_part);
-    timed_end(X);
-    X->d_site = site_save;
+  if (H) {
+    // the heavy part's kernels on this stream, reading this part's P slots
+    hipStream_t hs = H->stream;
+    double *hpm64 = H->d_pm64, *hpmpad = H->d_pmpad;
+    H->stream = X->stream;
+    H->d_pm64 = X->d_pm64;
+    H->d_pmpad = X->d_pmpad;
+    rc = tick_part_kernels(H, true, ops, d_ops, level_start, n_levels, eops, d_eops, n_e, d_flags + 1,
+                           d_res + 3 * n_e);
+    H->stream = hs;
+    H->d_pm64 = hpm64;
+    H->d_pmpad = hpmpad;
     if (rc) return rc;
-    hipLaunchKernelGGL(k_edge_reduce, dim3(n_e), dim3(64), 0, X->stream, X->d_part, nblk,
-                       reinterpret_cast<double *>(st.d + o_res));
   }
   GLE_HIP(hipGetLastError());
-  GLE_HIP(hipMemcpyAsync(st.h + o_flags, st.d + o_flags, 8 + sizeof(double) * 3 * n_e, hipMemcpyDeviceToHost,
+  GLE_HIP(hipMemcpyAsync(st.h + o_flags, st.d + o_flags, 8 + sizeof(double) * 3 * n_e * parts, hipMemcpyDeviceToHost,
                          X->stream));
   GLE_HIP(hipEventRecord(st.done, X->stream));
   st.n_edges = n_e;
+  st.with_heavy = H != nullptr;
   st.busy = true;
   return GLE_SUCCESS;
 }
@@ -2577,15 +2615,11 @@ int engine_tick_enqueue(Instance *I, int sidx, const int *pm_idx, const double *
                         const DevEdgeOp *eops, int n_e) {
   std::lock_guard<std::mutex> lk_(I->mu);
   if (sidx < 0 || sidx >= kTickStages) return set_error(GLE_ERR_ARG, "tick stage %d", sidx);
-  int rc = tick_enqueue_part(I, sidx, pm_idx, pm_len, n_pm, pm_derivs, ops, n_ops, level_start, n_levels, eops, n_e);
-  if (!rc && I->heavy)
-    rc = tick_enqueue_part(I->heavy.get(), sidx, pm_idx, pm_len, n_pm, pm_derivs, ops, n_ops, level_start, n_levels,
-                           eops, n_e);
-  if (rc) {  // drain what was queued so the stages can be reused
+  const int rc = tick_enqueue_part(I, I->heavy.get(), sidx, pm_idx, pm_len, n_pm, pm_derivs, ops, n_ops, level_start,
+                                   n_levels, eops, n_e);
+  if (rc) {  // drain what was queued so the stage can be reused
     (void)hipStreamSynchronize(I->stream);
-    if (I->heavy) (void)hipStreamSynchronize(I->heavy->stream);
     I->stage[sidx].busy = false;
-    if (I->heavy) I->heavy->stage[sidx].busy = false;
   }
   return rc;
 }
@@ -2595,19 +2629,23 @@ int engine_tick_enqueue(Instance *I, int sidx, const int *pm_idx, const double *
 int engine_tick_collect(Instance *I, int sidx, double *res, int *flags) {
   std::lock_guard<std::mutex> lk_(I->mu);
   int rc = GLE_SUCCESS, f = 0;
-  for (Instance *X : {I, I->heavy.get()}) {
-    if (!X) continue;
-    Instance::TickStage &st = X->stage[sidx];
-    if (!st.busy) continue;
+  Instance::TickStage &st = I->stage[sidx];
+  if (st.busy) {
     const hipError_t e = hipEventSynchronize(st.done);
     st.busy = false;
     if (e != hipSuccess) {
       rc = set_error(GLE_ERR_DEVICE, "serving tick failed: %s", hipGetErrorString(e));
-      continue;
+    } else {
+      timed_collect(I);
+      const int n3 = 3 * st.n_edges;
+      for (int i = 0; i < n3; i++) res[i] = st.res[i];
+      f = st.flags[0];
+      if (st.with_heavy) {
+        if (I->heavy) timed_collect(I->heavy.get());
+        for (int i = 0; i < n3; i++) res[i] += st.res[n3 + i];
+        f |= st.flags[1];
+      }
     }
-    timed_collect(X);
-    for (int i = 0; i < 3 * st.n_edges; i++) res[i] = X == I ? st.res[i] : res[i] + st.res[i];
-    f |= st.flags[0];
   }
   if (flags) *flags = f;
   if (!rc) {
@@ -2620,13 +2658,10 @@ int engine_tick_collect(Instance *I, int sidx, double *res, int *flags) {
 bool engine_tick_ready(Instance *I, int sidx) {
   std::lock_guard<std::mutex> lk_(I->mu);
   if (sidx < 0 || sidx >= kTickStages) return true;
-  for (Instance *X : {I, I->heavy.get()}) {
-    if (!X) continue;
-    const Instance::TickStage &st = X->stage[sidx];
-    // an error is reported by the collect, which then does not wait either
-    if (st.busy && hipEventQuery(st.done) == hipErrorNotReady) return false;
-  }
-  return true;
+  const Instance::TickStage &st = I->stage[sidx];
+  // (both parts' work is on the instance's stream; an error is reported by
+  // the collect, which then does not wait either)
+  return !(st.busy && hipEventQuery(st.done) == hipErrorNotReady);
 }
 
 int engine_edges(Instance *I, const DevEdgeOp *ops, int n, double *out, double *site_lnl, int *flags) {
@@ -2725,7 +2760,7 @@ static int destroy(Instance *I) {
   void *ptrs[] = {I->d_tips, I->d_counts, I->d_cst, I->d_u, I->d_flags, I->d_eig_of_rate, I->d_cla,
                   I->d_pm64, I->d_pmpad,  I->d_pdiag, I->d_eigfrag, I->d_V, I->d_Vinv, I->d_lam, I->d_model,
                   I->d_ops,
-                  I->d_eops, I->d_part,  I->d_eres, I->d_site, I->d_pmidx, I->d_pmt};
+                  I->d_eops, I->d_part,  I->d_tpart, I->d_eres, I->d_site, I->d_pmidx, I->d_pmt};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (I->h_stage) (void)hipHostFree(I->h_stage);
