@@ -1858,6 +1858,35 @@ __global__ __launch_bounds__(64) void k_edge_reduce(const double *__restrict__ p
   }
 }
 
+// The edges of both parts of a serving tick reduced in one launch: block
+// op < n reduces op of part a (na partials each), block n + k op k of part b
+// (nb each), in k_edge_reduce's order; results to out[3 * block].
+__global__ __launch_bounds__(64) void k_edge_reduce2(const double *__restrict__ pa, int na,
+                                                     const double *__restrict__ pb, int nb, int n,
+                                                     double *__restrict__ out) {
+  const int blk = blockIdx.x, l = threadIdx.x;
+  const bool second = blk >= n;
+  const double *part = second ? pb : pa;
+  const int op = second ? blk - n : blk, nblk = second ? nb : na;
+  double a = 0, b = 0, c = 0;
+  for (int k = l; k < nblk; k += 64) {
+    const double *q = part + (size_t(op) * nblk + k) * 3;
+    a += q[0];
+    b += q[1];
+    c += q[2];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_down(a, o);
+    b += __shfl_down(b, o);
+    c += __shfl_down(c, o);
+  }
+  if (l == 0) {
+    out[3 * blk] = a;
+    out[3 * blk + 1] = b;
+    out[3 * blk + 2] = c;
+  }
+}
+
 // Plain device copy of known size: calibrates the HBM byte counters
 // (FETCH_SIZE / WRITE_SIZE) against the CLA kernels' measured traffic.
 __global__ __launch_bounds__(256) void k_copy_calibrate(const float4 *__restrict__ src, float4 *__restrict__ dst,
@@ -2442,13 +2471,13 @@ static size_t part_cap_floor(const Instance *X) {
   return std::max(X->part_cap, size_t(64) * edge_blocks(X) * 3);
 }
 
-// The CLA levels, the edges and their reduction of one part of a tick, on
-// X's stream, reading the staged op lists; the rescale flags go to d_flags and
-// the (lnL, d1, d2) of the edges to d_res.  The edge partials go to the
-// part's own buffer, or (heavy) to the one kept for its serving ticks.
+// The CLA levels and the edges of one part of a tick, on X's stream, reading
+// the staged op lists; the rescale flags go to d_flags.  The edge partials go
+// to the part's own buffer, or (heavy) to the one kept for its serving ticks,
+// returned with their count per edge for the reduction.
 static int tick_part_kernels(Instance *X, bool heavy, const DevClaOp *ops, const DevClaOp *d_ops,
                              const int *level_start, int n_levels, const DevEdgeOp *eops, const DevEdgeOp *d_eops,
-                             int n_e, int *d_flags, double *d_res) {
+                             int n_e, int *d_flags, const double *&parts_out, int &nblk_out) {
   int *flags_save = X->d_flags;
   X->d_flags = d_flags;
   int rc = GLE_SUCCESS;
@@ -2492,9 +2521,9 @@ static int tick_part_kernels(Instance *X, bool heavy, const DevClaOp *ops, const
                                 : launch_edges<float>(X, d_eops, n_e, nblk, derivs, d_part);
   timed_end(X);
   X->d_site = site_save;
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_edge_reduce, dim3(n_e), dim3(64), 0, X->stream, d_part, nblk, d_res);
-  return GLE_SUCCESS;
+  parts_out = d_part;
+  nblk_out = nblk;
+  return rc;
 }
 
 // One tick of the serving layer on an instance and its heavy part (H, or
@@ -2584,7 +2613,9 @@ static int tick_enqueue_part(Instance *X, Instance *H, int sidx, const int *pm_i
   }
   // the CLA kernels raise their rescale flags in this tick's own words
   double *d_res = reinterpret_cast<double *>(st.d + o_res);
-  int rc = tick_part_kernels(X, false, ops, d_ops, level_start, n_levels, eops, d_eops, n_e, d_flags, d_res);
+  const double *pa = nullptr, *pb = nullptr;
+  int na = 0, nb = 0;
+  int rc = tick_part_kernels(X, false, ops, d_ops, level_start, n_levels, eops, d_eops, n_e, d_flags, pa, na);
   if (rc) return rc;
   if (H) {
     // the heavy part's kernels on this stream, reading this part's P slots
@@ -2593,12 +2624,18 @@ static int tick_enqueue_part(Instance *X, Instance *H, int sidx, const int *pm_i
     H->stream = X->stream;
     H->d_pm64 = X->d_pm64;
     H->d_pmpad = X->d_pmpad;
-    rc = tick_part_kernels(H, true, ops, d_ops, level_start, n_levels, eops, d_eops, n_e, d_flags + 1,
-                           d_res + 3 * n_e);
+    rc = tick_part_kernels(H, true, ops, d_ops, level_start, n_levels, eops, d_eops, n_e, d_flags + 1, pb, nb);
     H->stream = hs;
     H->d_pm64 = hpm64;
     H->d_pmpad = hpmpad;
     if (rc) return rc;
+  }
+  // both parts' edges reduced by one launch, the heavy part's results after
+  if (n_e > 0) {
+    if (H)
+      hipLaunchKernelGGL(k_edge_reduce2, dim3(2 * n_e), dim3(64), 0, X->stream, pa, na, pb, nb, n_e, d_res);
+    else
+      hipLaunchKernelGGL(k_edge_reduce, dim3(n_e), dim3(64), 0, X->stream, pa, na, d_res);
   }
   GLE_HIP(hipGetLastError());
   GLE_HIP(hipMemcpyAsync(st.h + o_flags, st.d + o_flags, 8 + sizeof(double) * 3 * n_e * parts, hipMemcpyDeviceToHost,
