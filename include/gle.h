@@ -465,8 +465,9 @@ int gle_serve_stats(int server, long long *completed, double *p50_ms, double *p9
  * 20 us) or `microseconds` have passed; 0 (the default) starts at once.  No
  * reference counterpart (GARLI scores one individual at a time). */
 int gle_serve_set_batch_window(int server, int microseconds);
-/* Milliseconds the worker spent building tick op lists on the host, and from
- * a tick's first launch until its results were back, since the last reset. */
+/* Milliseconds the worker spent building tick op lists on the host, and
+ * collecting finished parts or polling for them while it had nothing to
+ * build (time the host waited on the device), since the last reset. */
 int gle_serve_timing(int server, double *host_ms, double *device_ms, int reset);
 /* CLA work of the ticks built since the last reset: CLA ops; children
  * contracted on the device by CLA ops; children of CLA ops and sides of score
