@@ -327,3 +327,44 @@ def test_batch_submit_equals_single_submits(gpu):
     for (p, b), got in zip(pop[::5], many[::5]):
         assert abs(got - o.score(parent=p, blen=b)) < 1e-9 * abs(got)
     srv.close()
+
+
+@pytest.mark.parametrize("kind", ["gtr", "aa"])
+def test_fp32_heavy_part_in_ticks(gpu, kind):
+    """FP32 instances with an FP64 heavy part (patterns of count > 1 here, so
+    it is a large share): a serving tick queues the heavy part's levels and
+    edges after the FP32 part's on one stream, reading the same staged lists
+    and P slots, and sums both parts' results.  Scores and a branch-length
+    pass must match the FP64 drop-in on the same trees."""
+    if kind == "gtr":
+        ds = sim.gtr_gamma(n_tips=16, n_sites=3000, pinv=0.1, seed=131)
+    else:
+        ds = sim.aa_gamma(n_tips=12, n_patterns=600, seed=132)
+    inst = ds.make_instance(E.PREC_FP32, n_cla_slots=400, n_pmat_slots=800)
+    inst.set_heavy_split(1)
+    assert 0 < inst.heavy_patterns() < ds.n_patterns
+    inst64 = ds.make_instance(E.PREC_FP64)
+    srv = E.Server()
+    pop = _population(ds, 12, 7)
+    ids = [srv.submit(inst, p, b) for p, b in pop]
+    opt = srv.submit(inst, ds.parent, ds.blen, kind=E.REQ_OPTIMIZE, precision=0.01)
+    got = [srv.wait(i)[0] for i in ids]
+    lnl_opt, _, bl = srv.wait(opt, n_nodes=2 * ds.n_tips - 2)
+    srv.close()
+    for (p, b), g in zip(pop, got):
+        t = E.Tree(inst64, p, b)
+        ref = t.score()
+        t.close()
+        assert abs(g - ref) < 1e-3
+    # the served pass's score at its own lengths, and its end against the
+    # FP64 drop-in's own pass from the same start
+    t = E.Tree(inst64, ds.parent, bl)
+    at_served = t.score()
+    t.close()
+    assert abs(lnl_opt - at_served) < 1e-3
+    t = E.Tree(inst64, ds.parent, ds.blen)
+    _, pass64 = t.optimize_all_branches(0.01)
+    t.close()
+    assert abs(lnl_opt - pass64) < 1e-3
+    inst.close()
+    inst64.close()
